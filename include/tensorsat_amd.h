@@ -1,0 +1,147 @@
+/*
+ * tensorsat_amd.h — C-ABI of the MI355X (gfx950) executor, libtensorsat_amd.so.
+ *
+ * Device tensors are Boolean relations (uint8 0/1) or counts (uint64) whose axes all
+ * have extent 2 ("bit" axes; wider FinSet axes are padded to powers of two by the host
+ * layer).  A tensor of rank r is 2^r contiguous elements in row-major order; axis i is
+ * bit (r-1-i) of the flat index.  Wires are small non-negative integers naming the axes
+ * of an operation, as in the reference's einsum-style calls.
+ *
+ * Every function returns TS_OK (0) or a negative TS_E* code; the host layer maps
+ *   TS_EINVAL -> ValueError, TS_EUNSUPPORTED -> NotImplementedError,
+ *   TS_EOVERFLOW -> OverflowError, TS_EHIP -> RuntimeError.
+ * Work is enqueued on `stream` (NULL = the null stream) and is asynchronous unless
+ * stated otherwise.  Not thread-safe per stream.
+ */
+#ifndef TENSORSAT_AMD_H
+#define TENSORSAT_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "tensorsat_prog.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TS_OK 0
+#define TS_EINVAL (-1)
+#define TS_ENOMEM (-2)
+#define TS_EUNSUPPORTED (-3)
+#define TS_EOVERFLOW (-4)
+#define TS_EHIP (-5)
+
+/* Epilogue of a Boolean step on the exact integer count c of an output entry. */
+#define TS_MODE_GT0 0     /* c > 0 : relational composition (existential sum) */
+#define TS_MODE_MOD256 1  /* (c mod 256) != 0 : what the reference's numpy code computes,
+                             fin_rel.py:279,287,392 (np.sign(..., dtype=uint8) after a
+                             wrapping integer einsum) */
+
+typedef void* ts_stream; /* hipStream_t */
+
+/* Library version / device check. Returns TS_OK if a gfx950 device is usable. */
+int ts_init(int device);
+const char* ts_version(void);
+
+/* ---- Box algebra (one kernel launch each) --------------------------------------- */
+
+/* FinRel._contract2 (tensorsat/lang/fin_rel.py:255-288), reached from
+ * tensorsat/_autoray.py:31-62 (einsum) and Box.contract2 (diagrams.py:811-866).
+ * out[o] = epilogue(sum_c lhs[..] * rhs[..]); out wires must be unique (the host layer
+ * handles repeated out wires with ts_rewire); lhs/rhs wires may repeat (diagonals). */
+int ts_contract2(const uint8_t* lhs, int32_t lhs_rank, const int32_t* lhs_wires,
+                 const uint8_t* rhs, int32_t rhs_rank, const int32_t* rhs_wires,
+                 uint8_t* out, int32_t out_rank, const int32_t* out_wires, int32_t mode,
+                 ts_stream stream);
+
+/* FinRel._rewire (fin_rel.py:379-393) and rewire_array (_utils/misc.py:104-123):
+ * out axes are input ports, possibly repeated (diagonal embedding) or absent
+ * (existential sum).  Pure permutations run the permute path. */
+int ts_rewire(const uint8_t* in, int32_t in_rank, uint8_t* out, int32_t out_rank,
+              const int32_t* out_ports, int32_t mode, ts_stream stream);
+
+/* FinRel._spider (fin_rel.py:290-298): copy tensor of `num_ports` axes, each of
+ * `bits` bits, ones where all ports hold the same value v < dim. */
+int ts_spider(uint8_t* out, int32_t num_ports, int32_t bits, int32_t dim, ts_stream stream);
+
+/* FinRel._union (fin_rel.py:345-346): out = a | b over n bytes. */
+int ts_union(const uint8_t* a, const uint8_t* b, uint8_t* out, int64_t n, ts_stream stream);
+
+/* FinRel._stack (fin_rel.py:304-308): out has rank in_rank + axis_bits; input k fills
+ * the slice whose new axis (at position `port`) equals k; slices k >= num are zero. */
+int ts_stack(const uint8_t* const* ins, int32_t num, int32_t in_rank, int32_t port,
+             int32_t axis_bits, uint8_t* out, ts_stream stream);
+
+/* FinRel._getitem (fin_rel.py:342-343): fix axes to values; fixed[i] is -1 (keep) or
+ * the bit value of axis i.  out has rank = number of kept axes. */
+int ts_select(const uint8_t* in, int32_t in_rank, const int32_t* fixed, uint8_t* out,
+              ts_stream stream);
+
+/* FinRel._as_scalar (fin_rel.py:472-473): all(tensor == 1) over n bytes; synchronous. */
+int ts_as_scalar(const uint8_t* x, int64_t n, int32_t* result, ts_stream stream);
+
+/* Diagonal extraction for operands with repeated wires: out axis j of rank out_rank;
+ * in axis i takes the bit of out axis axis_map[i] (several in axes may share one). */
+int ts_diagonal(const uint8_t* in, int32_t in_rank, const int32_t* axis_map, uint8_t* out,
+                int32_t out_rank, ts_stream stream);
+
+/* ---- Counting semiring (uint64, exact; #SAT) ------------------------------------ */
+
+/* Same index semantics as ts_contract2 with (+, x) over uint64.  Any entry >= 2^64 sets
+ * *overflow (device int32, must be zeroed by the caller) to 1; values are then invalid. */
+int ts_count_contract2(const uint64_t* lhs, int32_t lhs_rank, const int32_t* lhs_wires,
+                       const uint64_t* rhs, int32_t rhs_rank, const int32_t* rhs_wires,
+                       uint64_t* out, int32_t out_rank, const int32_t* out_wires,
+                       int32_t* overflow, ts_stream stream);
+
+/* Widen a 0/1 byte tensor to uint64 counts. */
+int ts_bytes_to_u64(const uint8_t* in, uint64_t* out, int64_t n, ts_stream stream);
+
+/* ---- Compiled programs (single instance, grid-wide steps) ----------------------- */
+
+/* Run one job of a compiled program with every step launched grid-wide on `stream`.
+ * steps_host is the host copy of the program's step array (records are passed by
+ * value), leaves_dev the device copy of its leaf bytes; arena holds
+ * jobs[job].arena_elems elements (uint8 for Boolean modes).  When `inter` is non-NULL,
+ * step s's output is also copied to inter + inter_offsets[s] (intermediate dumps). */
+int ts_program_run(const ts_job* job_host, const ts_step* steps_host, const uint8_t* leaves_dev,
+                   uint8_t* arena, int32_t mode, uint8_t* inter, const int64_t* inter_offsets,
+                   ts_stream stream);
+
+/* Counting variant: arena of uint64; result scaled by 2^free_vars is added to *count
+ * (device uint64); overflow sets *overflow. */
+int ts_program_count(const ts_job* job_host, const ts_step* steps_host, const uint8_t* leaves_dev,
+                     uint64_t* arena, uint64_t* count, int32_t* overflow, ts_stream stream);
+
+/* ---- Batched persistent executor ------------------------------------------------ */
+
+typedef struct {
+  const ts_job* jobs;       /* device, num_jobs records */
+  const int32_t* order;     /* device, dispatch order (num_jobs job indices) */
+  int32_t num_jobs;
+  const ts_step* steps;     /* device */
+  const uint8_t* leaves;    /* device */
+  uint8_t* arenas;          /* device, num_workgroups * arena_stride bytes */
+  int64_t arena_stride;     /* bytes per workgroup arena (>= max arena_elems, 256-aligned) */
+  int32_t num_workgroups;
+  int32_t mode;             /* TS_MODE_GT0 / TS_MODE_MOD256 */
+  uint32_t* verdicts;       /* device, per instance, zeroed by the caller: OR of job results */
+  int32_t* counter;         /* device scratch int32, zeroed by the caller */
+  int32_t early_exit;       /* 1: skip jobs of instances already decided SAT */
+  uint32_t* job_done;       /* device, per job (optional): 1 ran, 2 skipped */
+} ts_batch;
+
+/* One persistent launch: each workgroup takes the next job in `order`, copies its leaves
+ * into its arena and runs every step with the workgroup, OR-ing the scalar result into
+ * verdicts[instance]. */
+int ts_batch_run(const ts_batch* b, ts_stream stream);
+
+/* Maximum resident workgroups of the batched kernel on the current device. */
+int ts_batch_max_workgroups(int32_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TENSORSAT_AMD_H */

@@ -1,0 +1,113 @@
+/*
+ * tensorsat_prog.h — C-ABI of the host-side contraction-program compiler (libtsplan.so).
+ *
+ * A *program* is what the device executors run: one or more *jobs*, each a list of
+ * leaf tensors plus a list of pairwise steps over an arena.  It is the compiled form of
+ *   tensorsat/contractions/simple.py:134-194   SimpleContraction._new (contract2 args)
+ *   tensorsat/contractions/cotengra.py:120-126 tree.contract(...) step loop
+ * with every intermediate's axis order and arena offset fixed on the host.
+ *
+ * Tensors are Boolean/count tensors whose axes all have extent 2 ("bit" wires); the
+ * flat index of a rank-r tensor with axes (a_0..a_{r-1}) is sum_i bit(a_i) << (r-1-i).
+ * A step computes, for every out index o,
+ *     acc(o) = sum over summed index c of lhs[L(o,c)] * rhs[R(o,c)]
+ * where L/R place the bits of o and c at the positions given by the step record, and
+ * writes epilogue(acc(o)): (acc > 0) for GT0, (acc mod 256) != 0 for MOD256, acc for
+ * the counting semiring.
+ */
+#ifndef TENSORSAT_PROG_H
+#define TENSORSAT_PROG_H
+
+#include <stdint.h>
+
+#include "tensorsat_plan.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TS_MAX_BITS 27  /* max out bits and max summed bits of one step (128-byte records) */
+#define TS_NOPOS 0xFF   /* operand does not hold this bit */
+
+/* One pairwise step (128 bytes). Offsets are element offsets in the job arena. */
+typedef struct {
+  uint32_t lhs_off, rhs_off, out_off;
+  uint8_t lhs_rank, rhs_rank, out_rank, sum_rank;
+  uint8_t flags;   /* TS_STEP_* */
+  uint8_t pad[3];
+  /* For out bit j (LSB-first, j < out_rank): position of that bit in lhs / rhs. */
+  uint8_t o_lpos[TS_MAX_BITS], o_rpos[TS_MAX_BITS];
+  /* For summed bit j (j < sum_rank): position in lhs / rhs. */
+  uint8_t c_lpos[TS_MAX_BITS], c_rpos[TS_MAX_BITS];
+} ts_step;
+
+#define TS_STEP_KINNER 1  /* summed bits are the lowest sum_rank bits of both operands, in order */
+#define TS_STEP_NORHS 2   /* unary step (rewire/reduce): rhs is the constant 1 */
+
+/* One job: a contraction of its leaves down to one tensor. */
+typedef struct {
+  int32_t instance;      /* instance index (batched decision: verdicts are OR-ed per instance) */
+  int32_t cube;          /* index of this job among its instance's jobs */
+  int32_t step_begin;    /* first step record */
+  int32_t num_steps;
+  int64_t leaf_begin;    /* first byte in the leaf buffer */
+  int32_t num_leaf_bytes;/* leaves are copied to arena[0 .. num_leaf_bytes) at job start */
+  int32_t result_off;    /* element offset of the result tensor in the arena */
+  int32_t result_rank;
+  int32_t free_vars;     /* counting: result is multiplied by 2^free_vars */
+  int64_t arena_elems;   /* arena size in elements */
+  int32_t max_rank;      /* largest tensor rank of the job */
+  int32_t flags;
+} ts_job;
+
+/* ---- compiled programs (opaque handle) ---------------------------------------- */
+typedef struct ts_program ts_program;
+
+/* Compile one network + SSA path into a single job.  Wires must all have extent 2 and
+ * be unique within each slot; leaves[s] holds 2^rank(s) bytes in slot wire order.
+ * Out wires not held by any slot are added as all-ones rank-1 leaves (the
+ * spider(t, 1) products of SimpleContraction.contract, simple.py:281-285). */
+int tsc_compile(const tsp_network* net, const int32_t* path, const uint8_t* const* leaves,
+                ts_program** out);
+
+/* Elimination-order planner: `trials` randomised min-fill/min-degree orders turned into
+ * trees; returns the cheapest SSA path (max size first, then flops). */
+int tsp_elim(const tsp_network* net, int32_t trials, uint64_t seed, int32_t* path_out,
+             tsp_cost* cost_out);
+
+typedef struct {
+  int32_t width;        /* cube width W: contract once the residual's width estimate <= W */
+  int32_t max_cubes;    /* per-instance cap on emitted jobs (0 = unlimited) */
+  int32_t plan_trials;  /* elimination-order trials per cube */
+  int32_t threads;      /* host threads (<= 0: hardware concurrency) */
+  uint64_t seed;
+  int32_t counting;     /* reserved: cube enumeration is always complete unless capped */
+} tsc_sat_opts;
+
+#define TSC_NSTAT 8
+/* Per-instance statistics copied out by tsc_program_copy:
+ * [0] jobs  [1] branch nodes  [2] pruned branches  [3] truncated (cap reached)
+ * [4] max tensor rank  [5] decided on the host (-1 no, 0 UNSAT: every branch pruned)
+ * [6] host planning microseconds  [7] cubes whose residual is empty (trivially SAT) */
+
+/* Batched SAT front-end: conditioned slicing of each CNF instance into cubes (branch on
+ * variables, propagate unit clauses, prune falsified branches) until the residual
+ * formula's width estimate is <= width; each cube's residual clause network is planned
+ * and compiled into one job.  Literals are +-(v+1).  Clauses of instance i are
+ * clause_offsets[i] .. clause_offsets[i+1]; literals of clause c are lits[lit_offsets[c]
+ * .. lit_offsets[c+1]]. */
+int tsc_sat_batch(int32_t num_inst, const int32_t* nvars, const int64_t* clause_offsets,
+                  const int64_t* lit_offsets, const int32_t* lits, const tsc_sat_opts* opts,
+                  ts_program** out);
+
+int tsc_program_sizes(const ts_program* p, int64_t* num_jobs, int64_t* num_steps,
+                      int64_t* num_leaf_bytes, int64_t* num_instances);
+int tsc_program_copy(const ts_program* p, ts_job* jobs, ts_step* steps, uint8_t* leaf_bytes,
+                     int32_t* inst_stats);
+void tsc_program_free(ts_program* p);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TENSORSAT_PROG_H */

@@ -1,0 +1,126 @@
+"""
+ctypes binding of ``libtensorsat_amd.so`` (``include/tensorsat_amd.h``).
+
+The HIP library is the only compute path: there is no CPU or PyTorch fallback.  If the
+library is missing or no gfx950 device is usable, :func:`lib` raises.  PyTorch is used
+only to allocate device memory and to name the current HIP stream.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libtensorsat_amd.so"
+
+TS_OK = 0
+TS_EINVAL = -1
+TS_ENOMEM = -2
+TS_EUNSUPPORTED = -3
+TS_EOVERFLOW = -4
+TS_EHIP = -5
+
+MODE_GT0 = 0
+MODE_MOD256 = 1
+MODES = {"GT0": MODE_GT0, "MOD256": MODE_MOD256}
+
+# Every symbol declared in include/tensorsat_amd.h.
+EXPORTS = (
+    "ts_init", "ts_version", "ts_contract2", "ts_rewire", "ts_spider", "ts_union", "ts_stack",
+    "ts_select", "ts_as_scalar", "ts_diagonal", "ts_count_contract2", "ts_bytes_to_u64",
+    "ts_program_run", "ts_program_count", "ts_batch_run", "ts_batch_max_workgroups",
+)
+
+_lib = None
+_ready = False
+
+vp = ctypes.c_void_p
+i32 = ctypes.c_int32
+i64 = ctypes.c_int64
+P = ctypes.POINTER
+
+
+class Batch(ctypes.Structure):
+    _fields_ = [
+        ("jobs", vp), ("order", vp), ("num_jobs", i32), ("steps", vp), ("leaves", vp),
+        ("arenas", vp), ("arena_stride", i64), ("num_workgroups", i32), ("mode", i32),
+        ("verdicts", vp), ("counter", vp), ("early_exit", i32), ("job_done", vp),
+    ]
+
+
+def load() -> ctypes.CDLL:
+    """Load the shared library without touching the GPU (symbol checks on CPU)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -m tensorsat_amd.build hip`"
+            )
+        L = ctypes.CDLL(str(LIB_PATH))
+        L.ts_init.argtypes = [ctypes.c_int]
+        L.ts_version.restype = ctypes.c_char_p
+        L.ts_contract2.argtypes = [vp, i32, P(i32), vp, i32, P(i32), vp, i32, P(i32), i32, vp]
+        L.ts_count_contract2.argtypes = [vp, i32, P(i32), vp, i32, P(i32), vp, i32, P(i32), vp, vp]
+        L.ts_rewire.argtypes = [vp, i32, vp, i32, P(i32), i32, vp]
+        L.ts_diagonal.argtypes = [vp, i32, P(i32), vp, i32, vp]
+        L.ts_spider.argtypes = [vp, i32, i32, i32, vp]
+        L.ts_union.argtypes = [vp, vp, vp, i64, vp]
+        L.ts_stack.argtypes = [P(vp), i32, i32, i32, i32, vp, vp]
+        L.ts_select.argtypes = [vp, i32, P(i32), vp, vp]
+        L.ts_as_scalar.argtypes = [vp, i64, P(i32), vp]
+        L.ts_bytes_to_u64.argtypes = [vp, vp, i64, vp]
+        L.ts_program_run.argtypes = [vp, vp, vp, vp, i32, vp, vp, vp]
+        L.ts_program_count.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+        L.ts_batch_run.argtypes = [P(Batch), vp]
+        L.ts_batch_max_workgroups.argtypes = [P(i32)]
+        _lib = L
+    return _lib
+
+
+def lib() -> ctypes.CDLL:
+    """Load the library and initialise the current torch device (must be gfx950)."""
+    global _ready
+    L = load()
+    if not _ready:
+        import torch
+
+        if not torch.cuda.is_available():
+            raise RuntimeError("tensorsat_amd needs a ROCm GPU (gfx950); none is visible")
+        check(L.ts_init(torch.cuda.current_device()), "ts_init")
+        _ready = True
+    return L
+
+
+def check(rc: int, what: str) -> None:
+    if rc == TS_OK:
+        return
+    if rc == TS_EINVAL:
+        raise ValueError(f"{what}: invalid arguments")
+    if rc == TS_EUNSUPPORTED:
+        raise NotImplementedError(f"{what}: unsupported case")
+    if rc == TS_EOVERFLOW:
+        raise OverflowError(f"{what}: count exceeds 2^64")
+    if rc == TS_ENOMEM:
+        raise MemoryError(f"{what}: out of memory")
+    raise RuntimeError(f"{what}: HIP error ({rc})")
+
+
+def stream_ptr():
+    import torch
+
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def i32arr(xs) -> ctypes.Array:
+    xs = list(xs)
+    return (i32 * max(1, len(xs)))(*xs)
+
+
+def dptr(t) -> ctypes.c_void_p:
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def host_ptr(a: np.ndarray) -> ctypes.c_void_p:
+    return ctypes.c_void_p(a.ctypes.data)

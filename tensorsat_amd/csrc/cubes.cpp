@@ -1,0 +1,485 @@
+// Batched SAT front-end: conditioned slicing of CNF instances into contraction jobs.
+//
+// Slicing a variable wire fixes its value in every leaf that holds it (the reference's
+// Box.__getitem__, tensorsat/diagrams.py:972-996 -> FinRel._getitem fin_rel.py:342-343);
+// slice results are OR-ed for decisions (Box.__add__, diagrams.py:965-970) and summed for
+// counts.  Two exact tensor identities prune the slices: a clause whose literals are all
+// false is the all-zero tensor, so its slice contributes nothing; a clause with one
+// unassigned literal left is a one-hot rank-1 tensor, so only one value of that variable
+// contributes and it is fixed too (unit propagation).  Slicing stops once the residual
+// clause network's width estimate is <= W; the residual is planned (elimination order)
+// and compiled into one job.  The enumeration does not depend on contraction results.
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cstring>
+#include <stdexcept>
+#include <thread>
+#include <vector>
+
+#include "planner_internal.h"
+
+static_assert(sizeof(ts_step) == 128, "ts_step must be 128 bytes");
+static_assert(sizeof(ts_job) == 56, "ts_job layout");
+
+struct ts_program {
+  std::vector<ts_job> jobs;
+  std::vector<ts_step> steps;
+  std::vector<uint8_t> leaves;
+  std::vector<int32_t> stats;  // TSC_NSTAT per instance
+  int64_t num_instances = 0;
+};
+
+namespace tsp {
+namespace {
+
+struct Formula {
+  int n = 0;
+  std::vector<std::vector<int>> clauses;  // literals +-(v+1), distinct vars, no tautologies
+  bool has_empty = false;
+};
+
+Formula normalise(int n, const int32_t* lits, const int64_t* loff, int64_t c0, int64_t c1) {
+  Formula f;
+  f.n = n;
+  for (int64_t c = c0; c < c1; ++c) {
+    std::vector<int> cl(lits + loff[c], lits + loff[c + 1]);
+    std::sort(cl.begin(), cl.end());
+    cl.erase(std::unique(cl.begin(), cl.end()), cl.end());
+    bool taut = false;
+    for (size_t i = 0; i < cl.size(); ++i)
+      for (size_t j = i + 1; j < cl.size(); ++j)
+        if (cl[i] == -cl[j]) taut = true;
+    if (taut) continue;
+    if (cl.empty()) f.has_empty = true;
+    // Keep the caller's literal order for the clause tensor axes.
+    std::vector<int> ordered;
+    for (int64_t i = loff[c]; i < loff[c + 1]; ++i) {
+      int l = lits[i];
+      if (std::find(ordered.begin(), ordered.end(), l) == ordered.end()) ordered.push_back(l);
+    }
+    f.clauses.push_back(ordered);
+  }
+  return f;
+}
+
+struct Enumerator {
+  const Formula& f;
+  const tsc_sat_opts& o;
+  std::vector<int8_t> val;                 // per var: 0 free, +1 true, -1 false
+  std::vector<std::vector<int>> occ;       // literal index 2v+(neg) -> clauses
+  std::vector<int> nsat, nfalse;
+  std::vector<int> trail;
+  std::vector<int> queue;
+  int64_t nodes = 0, pruned = 0, empty_residual = 0;
+  bool truncated = false;
+  int emitted = 0;
+  int max_rank = 0;
+  uint64_t seed;
+  std::vector<Job>* jobs_out;
+  std::vector<int32_t>* free_out;
+
+  Enumerator(const Formula& f_, const tsc_sat_opts& o_, uint64_t sd, std::vector<Job>* jobs,
+             std::vector<int32_t>* fr)
+      : f(f_), o(o_), seed(sd), jobs_out(jobs), free_out(fr) {
+    val.assign(f.n, 0);
+    occ.assign(2 * f.n, {});
+    for (size_t c = 0; c < f.clauses.size(); ++c)
+      for (int l : f.clauses[c]) occ[lit_index(l)].push_back((int)c);
+    nsat.assign(f.clauses.size(), 0);
+    nfalse.assign(f.clauses.size(), 0);
+  }
+  static int lit_index(int l) { return 2 * (std::abs(l) - 1) + (l < 0 ? 1 : 0); }
+  bool lit_true(int l) const {
+    int8_t v = val[std::abs(l) - 1];
+    return l > 0 ? v > 0 : v < 0;
+  }
+
+  // Assign literal l true; push units; false on conflict.
+  bool assign(int l) {
+    int v = std::abs(l) - 1;
+    val[v] = l > 0 ? 1 : -1;
+    trail.push_back(l);
+    bool ok = true;
+    for (int c : occ[lit_index(l)]) nsat[c]++;
+    for (int c : occ[lit_index(-l)]) {
+      nfalse[c]++;
+      if (nsat[c] == 0) {
+        int len = (int)f.clauses[c].size();
+        if (nfalse[c] == len) ok = false;
+        else if (nfalse[c] == len - 1) {
+          for (int x : f.clauses[c])
+            if (val[std::abs(x) - 1] == 0) {
+              queue.push_back(x);
+              break;
+            }
+        }
+      }
+    }
+    return ok;
+  }
+  bool propagate() {
+    while (!queue.empty()) {
+      int l = queue.back();
+      queue.pop_back();
+      int8_t cur = val[std::abs(l) - 1];
+      if (cur != 0) {
+        if (!lit_true(l)) {
+          queue.clear();
+          return false;
+        }
+        continue;
+      }
+      if (!assign(l)) {
+        queue.clear();
+        return false;
+      }
+    }
+    return true;
+  }
+  void undo(size_t mark) {
+    while (trail.size() > mark) {
+      int l = trail.back();
+      trail.pop_back();
+      for (int c : occ[lit_index(l)]) nsat[c]--;
+      for (int c : occ[lit_index(-l)]) nfalse[c]--;
+      val[std::abs(l) - 1] = 0;
+    }
+  }
+  bool initial_units() {
+    for (size_t c = 0; c < f.clauses.size(); ++c)
+      if (f.clauses[c].size() == 1) queue.push_back(f.clauses[c][0]);
+    return propagate();
+  }
+
+  void emit(const std::vector<int>& rclauses_idx) {
+    // Residual: unsatisfied clauses restricted to free variables.
+    std::vector<int> vmap(f.n, -1);
+    int nv = 0;
+    BitNet net;
+    std::vector<std::vector<uint8_t>> leaf_data;
+    for (int c : rclauses_idx) {
+      std::vector<int> ws;
+      std::vector<uint8_t> neg;
+      for (int l : f.clauses[c]) {
+        int v = std::abs(l) - 1;
+        if (val[v] != 0) continue;
+        if (vmap[v] < 0) vmap[v] = nv++;
+        ws.push_back(vmap[v]);
+        neg.push_back(l < 0);
+      }
+      std::vector<uint8_t> t((size_t)1 << ws.size(), 1);
+      uint32_t zero = 0;
+      for (size_t i = 0; i < ws.size(); ++i)
+        if (neg[i]) zero |= 1u << (ws.size() - 1 - i);
+      t[zero] = 0;
+      net.slot_wires.push_back(ws);
+      leaf_data.push_back(std::move(t));
+    }
+    int free_vars = 0;
+    for (int v = 0; v < f.n; ++v)
+      if (val[v] == 0 && vmap[v] < 0) ++free_vars;
+    net.S = (int)net.slot_wires.size();
+    net.W = nv;
+    net.is_out.assign(nv, 0);
+    std::vector<int32_t> path;
+    elim_plan(net, o.plan_trials, seed + (uint64_t)emitted * 7919u, path, nullptr);
+    std::vector<const uint8_t*> leaves;
+    for (auto& t : leaf_data) leaves.push_back(t.data());
+    Job job;
+    if (compile_job(net, path, leaves, job) != TSP_OK) throw std::runtime_error("compile");
+    max_rank = std::max(max_rank, job.max_rank);
+    jobs_out->push_back(std::move(job));
+    free_out->push_back(free_vars);
+    ++emitted;
+    if (rclauses_idx.empty()) ++empty_residual;
+    if (o.max_cubes > 0 && emitted >= o.max_cubes) truncated = true;
+  }
+
+  void search() {
+    if (truncated) return;
+    std::vector<int> rc;
+    for (size_t c = 0; c < f.clauses.size(); ++c)
+      if (nsat[c] == 0) rc.push_back((int)c);
+    if (rc.empty()) {
+      emit(rc);
+      return;
+    }
+    // Residual variables and degrees.
+    std::vector<int> score(f.n, 0), occ_pos(f.n, 0), occ_neg(f.n, 0);
+    std::vector<int> rv;
+    for (int c : rc) {
+      int free_in = 0;
+      for (int l : f.clauses[c])
+        if (val[std::abs(l) - 1] == 0) ++free_in;
+      for (int l : f.clauses[c]) {
+        int v = std::abs(l) - 1;
+        if (val[v] != 0) continue;
+        if (score[v] == 0 && occ_pos[v] == 0 && occ_neg[v] == 0) rv.push_back(v);
+        score[v] += free_in - 1;
+        if (l > 0) occ_pos[v]++;
+        else occ_neg[v]++;
+      }
+    }
+    bool leaf = (int)rv.size() <= o.width + 1;
+    if (!leaf && (int)rv.size() <= 4 * o.width + 8) {
+      std::vector<int> vmap(f.n, -1);
+      for (size_t i = 0; i < rv.size(); ++i) vmap[rv[i]] = (int)i;
+      std::vector<std::vector<int>> cl;
+      cl.reserve(rc.size());
+      for (int c : rc) {
+        std::vector<int> ws;
+        for (int l : f.clauses[c]) {
+          int v = std::abs(l) - 1;
+          if (val[v] == 0) ws.push_back(vmap[v]);
+        }
+        cl.push_back(std::move(ws));
+      }
+      leaf = mindeg_width((int)rv.size(), cl) <= o.width;
+    }
+    if (leaf) {
+      emit(rc);
+      return;
+    }
+    ++nodes;
+    int bv = rv[0];
+    for (int v : rv)
+      if (score[v] > score[bv] || (score[v] == score[bv] && v < bv)) bv = v;
+    int first = occ_pos[bv] >= occ_neg[bv] ? bv + 1 : -(bv + 1);
+    for (int l : {first, -first}) {
+      size_t mark = trail.size();
+      if (assign(l) && propagate()) search();
+      else ++pruned;
+      queue.clear();
+      undo(mark);
+      if (truncated) return;
+    }
+  }
+};
+
+}  // namespace
+}  // namespace tsp
+
+using namespace tsp;
+
+extern "C" {
+
+int tsp_elim(const tsp_network* n, int32_t trials, uint64_t seed, int32_t* path_out,
+             tsp_cost* cost_out) {
+  if (!n || n->num_slots < 0 || n->num_wires < 0) return TSP_EINVAL;
+  BitNet net;
+  net.S = n->num_slots;
+  net.W = n->num_wires;
+  net.is_out.assign(net.W, 0);
+  for (int i = 0; i < n->num_out; ++i) {
+    int w = n->out_wires[i];
+    if (w < 0 || w >= net.W) return TSP_EINVAL;
+    net.is_out[w] = 1;
+  }
+  for (int s = 0; s < net.S; ++s) {
+    std::vector<int> ws(n->slot_wires + n->slot_offsets[s], n->slot_wires + n->slot_offsets[s + 1]);
+    for (int w : ws)
+      if (w < 0 || w >= net.W) return TSP_EINVAL;
+    std::sort(ws.begin(), ws.end());
+    ws.erase(std::unique(ws.begin(), ws.end()), ws.end());
+    net.slot_wires.push_back(ws);
+  }
+  std::vector<int32_t> path;
+  TreeCost tc;
+  try {
+    elim_plan(net, trials, seed, path, &tc);
+  } catch (const std::bad_alloc&) {
+    return TSP_ENOMEM;
+  }
+  if (!path.empty()) {
+    if (!path_out) return TSP_EINVAL;
+    std::memcpy(path_out, path.data(), path.size() * sizeof(int32_t));
+  }
+  if (cost_out) {
+    cost_out->log2_flops = tc.log2_flops;
+    cost_out->log2_write = tc.log2_write;
+    cost_out->log2_max_size = tc.log2_max;
+    cost_out->log2_peak = tc.log2_peak;
+    cost_out->num_slices = 1;
+  }
+  return TSP_OK;
+}
+
+int tsc_compile(const tsp_network* n, const int32_t* path, const uint8_t* const* leaves,
+                ts_program** out) {
+  if (!n || !out || n->num_slots < 0 || n->num_wires < 0 || n->num_out < 0) return TSP_EINVAL;
+  BitNet net;
+  net.S = n->num_slots;
+  net.W = n->num_wires;
+  net.is_out.assign(net.W, 0);
+  for (int w = 0; w < net.W; ++w)
+    if (n->wire_dims[w] != 2) return TSP_EINVAL;
+  for (int i = 0; i < n->num_out; ++i) {
+    int w = n->out_wires[i];
+    if (w < 0 || w >= net.W) return TSP_EINVAL;
+    net.is_out[w] = 1;
+    net.out_wires.push_back(w);
+  }
+  for (int s = 0; s < net.S; ++s) {
+    std::vector<int> ws(n->slot_wires + n->slot_offsets[s], n->slot_wires + n->slot_offsets[s + 1]);
+    for (int w : ws)
+      if (w < 0 || w >= net.W) return TSP_EINVAL;
+    net.slot_wires.push_back(ws);
+  }
+  std::vector<int32_t> p;
+  if (net.S > 1) {
+    if (!path) return TSP_EINVAL;
+    p.assign(path, path + 2 * (net.S - 1));
+  }
+  std::vector<const uint8_t*> lv(leaves, leaves + net.S);
+  auto* prog = new (std::nothrow) ts_program();
+  if (!prog) return TSP_ENOMEM;
+  try {
+    Job job;
+    int rc = compile_job(net, p, lv, job);
+    if (rc) {
+      delete prog;
+      return rc;
+    }
+    ts_job j{};
+    j.instance = 0;
+    j.cube = 0;
+    j.step_begin = 0;
+    j.num_steps = (int32_t)job.steps.size();
+    j.leaf_begin = 0;
+    j.num_leaf_bytes = (int32_t)job.leaf_bytes.size();
+    j.result_off = job.result_off;
+    j.result_rank = job.result_rank;
+    j.free_vars = 0;
+    j.arena_elems = job.arena_elems;
+    j.max_rank = job.max_rank;
+    prog->jobs.push_back(j);
+    prog->steps = std::move(job.steps);
+    prog->leaves = std::move(job.leaf_bytes);
+    prog->num_instances = 1;
+    prog->stats.assign(TSC_NSTAT, 0);
+    prog->stats[0] = 1;
+    prog->stats[4] = j.max_rank;
+    prog->stats[5] = -1;
+  } catch (...) {
+    delete prog;
+    return TSP_ENOMEM;
+  }
+  *out = prog;
+  return TSP_OK;
+}
+
+int tsc_sat_batch(int32_t num_inst, const int32_t* nvars, const int64_t* clause_offsets,
+                  const int64_t* lit_offsets, const int32_t* lits, const tsc_sat_opts* opts,
+                  ts_program** out) {
+  if (num_inst < 0 || !opts || !out || (num_inst > 0 && (!nvars || !clause_offsets))) return TSP_EINVAL;
+  if (opts->width < 0 || opts->width > TS_MAX_BITS) return TSP_EINVAL;
+  struct PerInst {
+    std::vector<Job> jobs;
+    std::vector<int32_t> free_vars;
+    int32_t stats[TSC_NSTAT] = {0};
+    bool ok = true;
+  };
+  std::vector<PerInst> res(num_inst);
+  std::atomic<int> next{0};
+  std::atomic<int> bad{0};
+  auto worker = [&]() {
+    for (;;) {
+      int i = next.fetch_add(1);
+      if (i >= num_inst) break;
+      auto t0 = std::chrono::steady_clock::now();
+      PerInst& r = res[i];
+      try {
+        Formula f = normalise(nvars[i], lits, lit_offsets, clause_offsets[i], clause_offsets[i + 1]);
+        for (auto& c : f.clauses)
+          for (int l : c)
+            if (l == 0 || std::abs(l) > f.n) throw std::invalid_argument("literal");
+        Enumerator e(f, *opts, opts->seed + (uint64_t)i * 1000003u, &r.jobs, &r.free_vars);
+        bool alive = !f.has_empty && e.initial_units();
+        if (alive) e.search();
+        r.stats[0] = (int32_t)r.jobs.size();
+        r.stats[1] = (int32_t)e.nodes;
+        r.stats[2] = (int32_t)e.pruned + (alive ? 0 : 1);
+        r.stats[3] = e.truncated ? 1 : 0;
+        r.stats[4] = e.max_rank;
+        r.stats[5] = r.jobs.empty() ? 0 : -1;
+        r.stats[7] = (int32_t)e.empty_residual;
+      } catch (const std::invalid_argument&) {
+        r.ok = false;
+        bad++;
+      } catch (...) {
+        r.ok = false;
+        bad++;
+      }
+      auto t1 = std::chrono::steady_clock::now();
+      r.stats[6] = (int32_t)std::chrono::duration_cast<std::chrono::microseconds>(t1 - t0).count();
+    }
+  };
+  int threads = opts->threads > 0 ? opts->threads : (int)std::thread::hardware_concurrency();
+  threads = std::max(1, std::min(threads, std::max(1, num_inst)));
+  std::vector<std::thread> pool;
+  for (int t = 1; t < threads; ++t) pool.emplace_back(worker);
+  worker();
+  for (auto& th : pool) th.join();
+  if (bad) return TSP_EINVAL;
+  auto* prog = new (std::nothrow) ts_program();
+  if (!prog) return TSP_ENOMEM;
+  try {
+    prog->num_instances = num_inst;
+    prog->stats.resize((size_t)num_inst * TSC_NSTAT);
+    for (int i = 0; i < num_inst; ++i) {
+      std::memcpy(&prog->stats[(size_t)i * TSC_NSTAT], res[i].stats, sizeof(res[i].stats));
+      for (size_t c = 0; c < res[i].jobs.size(); ++c) {
+        Job& jb = res[i].jobs[c];
+        ts_job j{};
+        j.instance = i;
+        j.cube = (int32_t)c;
+        j.step_begin = (int32_t)prog->steps.size();
+        j.num_steps = (int32_t)jb.steps.size();
+        j.leaf_begin = (int64_t)prog->leaves.size();
+        j.num_leaf_bytes = (int32_t)jb.leaf_bytes.size();
+        j.result_off = jb.result_off;
+        j.result_rank = jb.result_rank;
+        j.free_vars = res[i].free_vars[c];
+        j.arena_elems = jb.arena_elems;
+        j.max_rank = jb.max_rank;
+        prog->jobs.push_back(j);
+        prog->steps.insert(prog->steps.end(), jb.steps.begin(), jb.steps.end());
+        prog->leaves.insert(prog->leaves.end(), jb.leaf_bytes.begin(), jb.leaf_bytes.end());
+      }
+      res[i].jobs.clear();
+      res[i].jobs.shrink_to_fit();
+    }
+  } catch (...) {
+    delete prog;
+    return TSP_ENOMEM;
+  }
+  *out = prog;
+  return TSP_OK;
+}
+
+int tsc_program_sizes(const ts_program* p, int64_t* num_jobs, int64_t* num_steps,
+                      int64_t* num_leaf_bytes, int64_t* num_instances) {
+  if (!p) return TSP_EINVAL;
+  if (num_jobs) *num_jobs = (int64_t)p->jobs.size();
+  if (num_steps) *num_steps = (int64_t)p->steps.size();
+  if (num_leaf_bytes) *num_leaf_bytes = (int64_t)p->leaves.size();
+  if (num_instances) *num_instances = p->num_instances;
+  return TSP_OK;
+}
+
+int tsc_program_copy(const ts_program* p, ts_job* jobs, ts_step* steps, uint8_t* leaf_bytes,
+                     int32_t* inst_stats) {
+  if (!p) return TSP_EINVAL;
+  if (jobs && !p->jobs.empty()) std::memcpy(jobs, p->jobs.data(), p->jobs.size() * sizeof(ts_job));
+  if (steps && !p->steps.empty()) std::memcpy(steps, p->steps.data(), p->steps.size() * sizeof(ts_step));
+  if (leaf_bytes && !p->leaves.empty()) std::memcpy(leaf_bytes, p->leaves.data(), p->leaves.size());
+  if (inst_stats && !p->stats.empty())
+    std::memcpy(inst_stats, p->stats.data(), p->stats.size() * sizeof(int32_t));
+  return TSP_OK;
+}
+
+void tsc_program_free(ts_program* p) { delete p; }
+
+}  // extern "C"

@@ -1,0 +1,209 @@
+// Device-side step engine for gfx950, shared by the grid-wide step kernels and the
+// batched persistent executor.
+//
+// A step (ts_step, include/tensorsat_prog.h) computes for every out index o
+//     acc(o) = sum_c lhs[L(o, c)] * rhs[R(o, c)]
+// where L/R scatter the bits of o and of the summed index c to operand bit positions.
+// L(o, c) = Lo(o) + Lc(c) with both parts linear in the bits, evaluated with byte-chunk
+// lookup tables held in LDS: Lo(o) = T0[o & 255] + T1[(o >> 8) & 255] + ...
+// Out indices are walked in blocks of 256 consecutive values, one per thread, so the
+// low chunk is a per-thread constant and the higher chunks are workgroup-uniform
+// (LDS broadcast); stores are byte-coalesced.
+//
+// Paths:
+//  * sum_rank == 0: product (or copy for unary steps);
+//  * TS_STEP_KINNER: the summed bits are the lowest bits of both operands in the same
+//    order, so each operand's 2^sum_rank terms are contiguous: 2/4/8/16-byte vector
+//    loads and popcount(lhs & rhs) on 0/1 bytes;
+//  * otherwise: loop over c with LDS tables for Lc/Rc.
+// Epilogues: GT0 (acc > 0) and MOD256 ((acc mod 256) != 0) for 0/1 bytes; exact uint64
+// with overflow detection for the counting semiring.
+
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/tensorsat_amd.h"
+
+namespace tsd {
+
+constexpr int WG = 256;
+
+struct Lut {
+  uint32_t ol[4][256];
+  uint32_t orr[4][256];
+  uint32_t cl[4][256];
+  uint32_t cr[4][256];
+};
+
+__device__ __forceinline__ int nchunks(int bits) { return (bits + 7) >> 3; }
+
+// Build the chunk tables of one step (all threads of the workgroup participate; the
+// caller synchronises afterwards).
+__device__ inline void build_lut(const ts_step& st, Lut& L, bool need_c, int tid) {
+  const int ro = st.out_rank;
+  // Chunk 0 is always built: a scalar step (out_rank 0) reads entry [0][0], which must
+  // be 0 and not a value left in LDS by an earlier step.
+  const int nqo = nchunks(ro) > 0 ? nchunks(ro) : 1;
+  for (int e = tid; e < nqo * 256; e += WG) {
+    const int q = e >> 8, v = e & 255;
+    uint32_t al = 0, ar = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int j = q * 8 + i;
+      if (j < ro && ((v >> i) & 1)) {
+        const uint8_t pl = st.o_lpos[j], pr = st.o_rpos[j];
+        if (pl != TS_NOPOS) al |= 1u << pl;
+        if (pr != TS_NOPOS) ar |= 1u << pr;
+      }
+    }
+    L.ol[q][v] = al;
+    L.orr[q][v] = ar;
+  }
+  if (need_c) {
+    const int nc = st.sum_rank;
+    const int nqc = nchunks(nc) > 0 ? nchunks(nc) : 1;
+    for (int e = tid; e < nqc * 256; e += WG) {
+      const int q = e >> 8, v = e & 255;
+      uint32_t al = 0, ar = 0;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const int j = q * 8 + i;
+        if (j < nc && ((v >> i) & 1)) {
+          const uint8_t pl = st.c_lpos[j], pr = st.c_rpos[j];
+          if (pl != TS_NOPOS) al |= 1u << pl;
+          if (pr != TS_NOPOS) ar |= 1u << pr;
+        }
+      }
+      L.cl[q][v] = al;
+      L.cr[q][v] = ar;
+    }
+  }
+}
+
+// High-chunk part of an out offset (o >> 8), workgroup-uniform in the block loops.
+__device__ __forceinline__ uint32_t hi_off(const uint32_t (*T)[256], uint32_t hi, int nq) {
+  uint32_t r = 0;
+  if (nq > 1) r += T[1][hi & 255];
+  if (nq > 2) r += T[2][(hi >> 8) & 255];
+  if (nq > 3) r += T[3][hi >> 16];
+  return r;
+}
+
+__device__ __forceinline__ uint32_t c_off(const uint32_t (*T)[256], uint32_t c, int nq) {
+  uint32_t r = T[0][c & 255];
+  if (nq > 1) r += T[1][(c >> 8) & 255];
+  if (nq > 2) r += T[2][(c >> 16) & 255];
+  if (nq > 3) r += T[3][c >> 24];
+  return r;
+}
+
+__device__ __forceinline__ uint32_t popc_and(const uint8_t* a, const uint8_t* b, int nc,
+                                             bool norhs) {
+  // 2^nc contiguous 0/1 bytes in each operand, aligned to min(16, 2^nc).
+  switch (nc) {
+    case 1: {
+      uint32_t x = *reinterpret_cast<const uint16_t*>(a);
+      if (!norhs) x &= *reinterpret_cast<const uint16_t*>(b);
+      return __popc(x);
+    }
+    case 2: {
+      uint32_t x = *reinterpret_cast<const uint32_t*>(a);
+      if (!norhs) x &= *reinterpret_cast<const uint32_t*>(b);
+      return __popc(x);
+    }
+    case 3: {
+      uint64_t x = *reinterpret_cast<const uint64_t*>(a);
+      if (!norhs) x &= *reinterpret_cast<const uint64_t*>(b);
+      return __popcll(x);
+    }
+    default: {
+      uint32_t acc = 0;
+      const int n16 = 1 << (nc - 4);
+      for (int i = 0; i < n16; ++i) {
+        uint4 x = reinterpret_cast<const uint4*>(a)[i];
+        if (!norhs) {
+          const uint4 y = reinterpret_cast<const uint4*>(b)[i];
+          x.x &= y.x;
+          x.y &= y.y;
+          x.z &= y.z;
+          x.w &= y.w;
+        }
+        acc += __popc(x.x) + __popc(x.y) + __popc(x.z) + __popc(x.w);
+      }
+      return acc;
+    }
+  }
+}
+
+// Boolean step over out indices o = first + k * stride (stride a multiple of 256 and
+// first % 256 == tid % 256 whenever out_rank >= 8).
+__device__ inline void step_u8(const ts_step& st, const Lut& L, const uint8_t* __restrict__ lhs,
+                               const uint8_t* __restrict__ rhs, uint8_t* __restrict__ out,
+                               uint64_t first, uint64_t stride, int mode) {
+  const int ro = st.out_rank, nc = st.sum_rank;
+  const bool norhs = st.flags & TS_STEP_NORHS;
+  const bool kinner = st.flags & TS_STEP_KINNER;
+  const uint64_t n = 1ull << ro;
+  const int nqo = nchunks(ro), nqc = nchunks(nc);
+  if (first >= n) return;
+  const uint32_t lo_l = L.ol[0][first & 255];
+  const uint32_t lo_r = L.orr[0][first & 255];
+  for (uint64_t o = first; o < n; o += stride) {
+    const uint32_t hi = (uint32_t)(o >> 8);
+    const uint32_t offl = lo_l + hi_off(L.ol, hi, nqo);
+    const uint32_t offr = lo_r + hi_off(L.orr, hi, nqo);
+    uint32_t acc;
+    if (nc == 0) {
+      acc = lhs[offl];
+      if (!norhs) acc &= rhs[offr];
+    } else if (kinner) {
+      acc = popc_and(lhs + offl, rhs + offr, nc, norhs);
+    } else {
+      acc = 0;
+      const uint32_t nterm = 1u << nc;
+      for (uint32_t c = 0; c < nterm; ++c) {
+        const uint8_t a = lhs[offl + c_off(L.cl, c, nqc)];
+        const uint8_t b = norhs ? (uint8_t)1 : rhs[offr + c_off(L.cr, c, nqc)];
+        acc += a & b;
+      }
+    }
+    out[o] = (uint8_t)(mode == TS_MODE_MOD256 ? ((acc & 255u) != 0) : (acc != 0));
+  }
+}
+
+// Counting step (uint64 exact); returns true if any value of this thread overflowed.
+__device__ inline bool step_u64(const ts_step& st, const Lut& L, const uint64_t* __restrict__ lhs,
+                                const uint64_t* __restrict__ rhs, uint64_t* __restrict__ out,
+                                uint64_t first, uint64_t stride) {
+  const int ro = st.out_rank, nc = st.sum_rank;
+  const bool norhs = st.flags & TS_STEP_NORHS;
+  const bool kinner = st.flags & TS_STEP_KINNER;
+  const uint64_t n = 1ull << ro;
+  const int nqo = nchunks(ro), nqc = nchunks(nc);
+  bool ovf = false;
+  if (first >= n) return false;
+  const uint32_t lo_l = L.ol[0][first & 255];
+  const uint32_t lo_r = L.orr[0][first & 255];
+  for (uint64_t o = first; o < n; o += stride) {
+    const uint32_t hi = (uint32_t)(o >> 8);
+    const uint32_t offl = lo_l + hi_off(L.ol, hi, nqo);
+    const uint32_t offr = lo_r + hi_off(L.orr, hi, nqo);
+    uint64_t acc = 0;
+    const uint32_t nterm = 1u << nc;
+    for (uint32_t c = 0; c < nterm; ++c) {
+      const uint32_t cl = kinner ? c : c_off(L.cl, c, nqc);
+      const uint32_t cr = kinner ? c : c_off(L.cr, c, nqc);
+      const uint64_t a = lhs[offl + cl];
+      const uint64_t b = norhs ? 1ull : rhs[offr + cr];
+      uint64_t p;
+      ovf |= __builtin_mul_overflow(a, b, &p);
+      ovf |= __builtin_add_overflow(acc, p, &acc);
+    }
+    out[o] = acc;
+  }
+  return ovf;
+}
+
+}  // namespace tsd

@@ -1,0 +1,492 @@
+// Grid-wide kernels and C-ABI entry points of libtensorsat_amd.so (gfx950):
+// Box algebra (contract2, rewire, spider, union, stack, select, diagonal, as_scalar),
+// the counting semiring, and single-instance program execution.
+//
+// Reference anchors: tensorsat/lang/fin_rel.py:255-393,472 (FinRel box semantics),
+// tensorsat/_utils/misc.py:104-123 (rewire_array), tensorsat/_autoray.py:31-80 (the
+// einsum/transpose/stack seam), tensorsat/contractions/simple.py:251-292 (step loop).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "device.h"
+
+using namespace tsd;
+
+namespace {
+
+inline hipStream_t S(ts_stream s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline int hip_rc(hipError_t e) { return e == hipSuccess ? TS_OK : TS_EHIP; }
+
+int grid_for(uint64_t n) {
+  uint64_t b = (n + WG - 1) / WG;
+  return (int)std::max<uint64_t>(1, std::min<uint64_t>(b, 8192));
+}
+
+// ---- kernels -------------------------------------------------------------------------
+
+__global__ __launch_bounds__(WG) void k_step_u8(const uint8_t* __restrict__ lhs,
+                                                const uint8_t* __restrict__ rhs,
+                                                uint8_t* __restrict__ out, ts_step st, int mode) {
+  __shared__ Lut L;
+  const bool need_c = !(st.flags & TS_STEP_KINNER) && st.sum_rank > 0;
+  build_lut(st, L, need_c, threadIdx.x);
+  __syncthreads();
+  step_u8(st, L, lhs, rhs, out, (uint64_t)blockIdx.x * WG + threadIdx.x, (uint64_t)gridDim.x * WG, mode);
+}
+
+__global__ __launch_bounds__(WG) void k_step_u64(const uint64_t* __restrict__ lhs,
+                                                 const uint64_t* __restrict__ rhs,
+                                                 uint64_t* __restrict__ out, ts_step st,
+                                                 int32_t* __restrict__ overflow) {
+  __shared__ Lut L;
+  const bool need_c = !(st.flags & TS_STEP_KINNER) && st.sum_rank > 0;
+  build_lut(st, L, need_c, threadIdx.x);
+  __syncthreads();
+  bool ovf = step_u64(st, L, lhs, rhs, out, (uint64_t)blockIdx.x * WG + threadIdx.x, (uint64_t)gridDim.x * WG);
+  if (ovf) atomicOr(overflow, 1);
+}
+
+// General rewire (permute + duplicate + discard): out axis j holds input port ports[j].
+// Discarded ports are summed (existential OR with the chosen epilogue); output entries
+// whose duplicated axes disagree are 0.
+struct RewireArgs {
+  int32_t in_rank, out_rank, num_disc;
+  uint8_t out_port[TS_MAX_BITS];   // input axis for each out axis
+  uint8_t disc_port[TS_MAX_BITS];  // discarded input axes
+};
+
+__global__ __launch_bounds__(WG) void k_rewire(const uint8_t* __restrict__ in, uint8_t* __restrict__ out,
+                                               RewireArgs a, int mode) {
+  const uint64_t n = 1ull << a.out_rank;
+  for (uint64_t o = (uint64_t)blockIdx.x * WG + threadIdx.x; o < n; o += (uint64_t)gridDim.x * WG) {
+    uint32_t base = 0, seen = 0, val = 0;
+    bool ok = true;
+    for (int j = 0; j < a.out_rank; ++j) {
+      const uint32_t bit = (uint32_t)(o >> (a.out_rank - 1 - j)) & 1u;
+      const int p = a.out_port[j];
+      const uint32_t m = 1u << (a.in_rank - 1 - p);
+      if (seen & m) {
+        if (((val & m) != 0) != (bit != 0)) ok = false;
+      } else {
+        seen |= m;
+        if (bit) val |= m;
+      }
+    }
+    base = val;
+    uint32_t acc = 0;
+    if (ok) {
+      const uint32_t nt = 1u << a.num_disc;
+      for (uint32_t c = 0; c < nt; ++c) {
+        uint32_t off = base;
+        for (int i = 0; i < a.num_disc; ++i)
+          if ((c >> i) & 1) off |= 1u << (a.in_rank - 1 - a.disc_port[i]);
+        acc += in[off];
+      }
+    }
+    out[o] = (uint8_t)(mode == TS_MODE_MOD256 ? ((acc & 255u) != 0) : (acc != 0));
+  }
+}
+
+// Permutation path of ts_rewire (kernel (a)): out[o] = in[pi(o)], with the input tile of
+// each workgroup staged through LDS so both the gather and the store are coalesced when
+// the low bits of in and out overlap.  Here: plain coalesced-store gather with byte LUTs.
+__global__ __launch_bounds__(WG) void k_permute(const uint8_t* __restrict__ in, uint8_t* __restrict__ out,
+                                                ts_step st) {
+  __shared__ Lut L;
+  build_lut(st, L, false, threadIdx.x);
+  __syncthreads();
+  step_u8(st, L, in, nullptr, out, (uint64_t)blockIdx.x * WG + threadIdx.x, (uint64_t)gridDim.x * WG,
+          TS_MODE_GT0);
+}
+
+__global__ __launch_bounds__(WG) void k_diagonal(const uint8_t* __restrict__ in, uint8_t* __restrict__ out,
+                                                 int in_rank, int out_rank, RewireArgs a) {
+  // a.out_port[i] = out axis feeding input axis i
+  const uint64_t n = 1ull << out_rank;
+  for (uint64_t o = (uint64_t)blockIdx.x * WG + threadIdx.x; o < n; o += (uint64_t)gridDim.x * WG) {
+    uint32_t off = 0;
+    for (int i = 0; i < in_rank; ++i) {
+      const int j = a.out_port[i];
+      if ((o >> (out_rank - 1 - j)) & 1) off |= 1u << (in_rank - 1 - i);
+    }
+    out[o] = in[off];
+  }
+}
+
+__global__ __launch_bounds__(WG) void k_spider(uint8_t* __restrict__ out, int num_ports, int bits, int dim) {
+  const int rank = num_ports * bits;
+  const uint64_t n = 1ull << rank;
+  const uint32_t mask = (1u << bits) - 1u;
+  for (uint64_t o = (uint64_t)blockIdx.x * WG + threadIdx.x; o < n; o += (uint64_t)gridDim.x * WG) {
+    const uint32_t v0 = (uint32_t)(o >> (rank - bits)) & mask;
+    bool eq = v0 < (uint32_t)dim;
+    for (int p = 1; p < num_ports && eq; ++p)
+      eq = (((uint32_t)(o >> (rank - (p + 1) * bits)) & mask) == v0);
+    out[o] = eq ? 1 : 0;
+  }
+}
+
+__global__ __launch_bounds__(WG) void k_union(const uint8_t* __restrict__ a, const uint8_t* __restrict__ b,
+                                              uint8_t* __restrict__ out, int64_t n) {
+  const int64_t n16 = n / 16;
+  const uint4* a4 = reinterpret_cast<const uint4*>(a);
+  const uint4* b4 = reinterpret_cast<const uint4*>(b);
+  uint4* o4 = reinterpret_cast<uint4*>(out);
+  const int64_t tid = (int64_t)blockIdx.x * WG + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * WG;
+  for (int64_t i = tid; i < n16; i += stride) {
+    uint4 x = a4[i], y = b4[i];
+    x.x |= y.x;
+    x.y |= y.y;
+    x.z |= y.z;
+    x.w |= y.w;
+    o4[i] = x;
+  }
+  for (int64_t i = n16 * 16 + tid; i < n; i += stride) out[i] = a[i] | b[i];
+}
+
+// Copy `in` (rank r) into out (rank r + bits) at new-axis value k placed at axis `port`.
+__global__ __launch_bounds__(WG) void k_embed(const uint8_t* __restrict__ in, uint8_t* __restrict__ out,
+                                              int in_rank, int port, int bits, uint32_t k) {
+  const uint64_t n = 1ull << in_rank;
+  const int low = in_rank - port;  // input axes after the port stay the low bits
+  for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < n; i += (uint64_t)gridDim.x * WG) {
+    const uint64_t lo = i & ((1ull << low) - 1);
+    const uint64_t hi = i >> low;
+    const uint64_t o = (((hi << bits) | k) << low) | lo;
+    out[o] = in[i];
+  }
+}
+
+__global__ __launch_bounds__(WG) void k_fill(uint8_t* __restrict__ out, uint8_t v, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * WG + threadIdx.x; i < n; i += (int64_t)gridDim.x * WG) out[i] = v;
+}
+
+__global__ __launch_bounds__(WG) void k_widen(const uint8_t* __restrict__ in, uint64_t* __restrict__ out, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * WG + threadIdx.x; i < n; i += (int64_t)gridDim.x * WG) out[i] = in[i];
+}
+
+__global__ void k_scale_add(const uint64_t* __restrict__ val, int shift, uint64_t* __restrict__ acc,
+                            int32_t* __restrict__ overflow) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    const uint64_t v = *val;
+    if (shift >= 64 ? v != 0 : (shift > 0 && (v >> (64 - shift)) != 0)) {
+      atomicOr(overflow, 1);
+      return;
+    }
+    const uint64_t s = shift >= 64 ? 0 : (v << shift);
+    const unsigned long long old = atomicAdd(reinterpret_cast<unsigned long long*>(acc), (unsigned long long)s);
+    if (old + s < old) atomicOr(overflow, 1);
+  }
+}
+
+// ---- host helpers ----------------------------------------------------------------------
+
+// Build a step record from wire lists; summed wires ordered LSB-first by their lhs
+// position, so K-innermost layouts get the vector path.
+int make_step(int rl, const int32_t* lw, int rr, const int32_t* rw, int ro, const int32_t* ow,
+              bool norhs, ts_step& st) {
+  if (rl < 0 || rr < 0 || ro < 0 || rl > TS_MAX_BITS || rr > TS_MAX_BITS || ro > TS_MAX_BITS)
+    return TS_EINVAL;
+  auto uniq = [](int n, const int32_t* w) {
+    for (int i = 0; i < n; ++i)
+      for (int j = i + 1; j < n; ++j)
+        if (w[i] == w[j]) return false;
+    return true;
+  };
+  if (!uniq(rl, lw) || (!norhs && !uniq(rr, rw))) return TS_EUNSUPPORTED;
+  if (!uniq(ro, ow)) return TS_EUNSUPPORTED;
+  auto find = [](int n, const int32_t* w, int32_t x) {
+    for (int i = 0; i < n; ++i)
+      if (w[i] == x) return n - 1 - i;  // LSB-first bit position
+    return -1;
+  };
+  std::memset(&st, 0, sizeof(st));
+  std::memset(st.o_lpos, TS_NOPOS, sizeof(st.o_lpos));
+  std::memset(st.o_rpos, TS_NOPOS, sizeof(st.o_rpos));
+  std::memset(st.c_lpos, TS_NOPOS, sizeof(st.c_lpos));
+  std::memset(st.c_rpos, TS_NOPOS, sizeof(st.c_rpos));
+  st.lhs_rank = (uint8_t)rl;
+  st.rhs_rank = norhs ? 0 : (uint8_t)rr;
+  st.out_rank = (uint8_t)ro;
+  for (int j = 0; j < ro; ++j) {
+    const int32_t w = ow[ro - 1 - j];
+    const int pl = find(rl, lw, w);
+    const int pr = norhs ? -1 : find(rr, rw, w);
+    if (pl < 0 && pr < 0) return TS_EINVAL;
+    st.o_lpos[j] = pl < 0 ? TS_NOPOS : (uint8_t)pl;
+    st.o_rpos[j] = pr < 0 ? TS_NOPOS : (uint8_t)pr;
+  }
+  // Summed wires: lhs wires not in out (LSB-first), then rhs-only wires not in out.
+  std::vector<int32_t> sum;
+  for (int i = rl - 1; i >= 0; --i)
+    if (find(ro, ow, lw[i]) < 0) sum.push_back(lw[i]);
+  if (!norhs)
+    for (int i = rr - 1; i >= 0; --i)
+      if (find(ro, ow, rw[i]) < 0 && find(rl, lw, rw[i]) < 0) sum.push_back(rw[i]);
+  if ((int)sum.size() > TS_MAX_BITS) return TS_EUNSUPPORTED;
+  st.sum_rank = (uint8_t)sum.size();
+  bool kinner = true;
+  for (size_t j = 0; j < sum.size(); ++j) {
+    const int pl = find(rl, lw, sum[j]);
+    const int pr = norhs ? -1 : find(rr, rw, sum[j]);
+    st.c_lpos[j] = pl < 0 ? TS_NOPOS : (uint8_t)pl;
+    st.c_rpos[j] = pr < 0 ? TS_NOPOS : (uint8_t)pr;
+    if (pl != (int)j || (!norhs && pr != (int)j)) kinner = false;
+  }
+  st.flags = (uint8_t)((kinner ? TS_STEP_KINNER : 0) | (norhs ? TS_STEP_NORHS : 0));
+  return TS_OK;
+}
+
+int launch_step_u8(const uint8_t* lhs, const uint8_t* rhs, uint8_t* out, const ts_step& st, int mode,
+                   hipStream_t s) {
+  hipLaunchKernelGGL(k_step_u8, dim3(grid_for(1ull << st.out_rank)), dim3(WG), 0, s, lhs, rhs, out, st, mode);
+  return hip_rc(hipGetLastError());
+}
+
+int launch_step_u64(const uint64_t* lhs, const uint64_t* rhs, uint64_t* out, const ts_step& st,
+                    int32_t* ovf, hipStream_t s) {
+  hipLaunchKernelGGL(k_step_u64, dim3(grid_for(1ull << st.out_rank)), dim3(WG), 0, s, lhs, rhs, out, st, ovf);
+  return hip_rc(hipGetLastError());
+}
+
+}  // namespace
+
+extern "C" {
+
+int ts_init(int device) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= device || device < 0) return TS_EHIP;
+  if (hipSetDevice(device) != hipSuccess) return TS_EHIP;
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, device) != hipSuccess) return TS_EHIP;
+  if (std::strncmp(p.gcnArchName, "gfx950", 6) != 0) return TS_EUNSUPPORTED;
+  return TS_OK;
+}
+
+const char* ts_version(void) { return "tensorsat_amd 0.1.0 (gfx950)"; }
+
+int ts_contract2(const uint8_t* lhs, int32_t lhs_rank, const int32_t* lhs_wires, const uint8_t* rhs,
+                 int32_t rhs_rank, const int32_t* rhs_wires, uint8_t* out, int32_t out_rank,
+                 const int32_t* out_wires, int32_t mode, ts_stream stream) {
+  if (!lhs || !rhs || !out || (mode != TS_MODE_GT0 && mode != TS_MODE_MOD256)) return TS_EINVAL;
+  ts_step st;
+  int rc = make_step(lhs_rank, lhs_wires, rhs_rank, rhs_wires, out_rank, out_wires, false, st);
+  if (rc) return rc;
+  if ((reinterpret_cast<uintptr_t>(lhs) | reinterpret_cast<uintptr_t>(rhs)) & 15) st.flags &= ~TS_STEP_KINNER;
+  return launch_step_u8(lhs, rhs, out, st, mode, S(stream));
+}
+
+int ts_count_contract2(const uint64_t* lhs, int32_t lhs_rank, const int32_t* lhs_wires, const uint64_t* rhs,
+                       int32_t rhs_rank, const int32_t* rhs_wires, uint64_t* out, int32_t out_rank,
+                       const int32_t* out_wires, int32_t* overflow, ts_stream stream) {
+  if (!lhs || !rhs || !out || !overflow) return TS_EINVAL;
+  ts_step st;
+  int rc = make_step(lhs_rank, lhs_wires, rhs_rank, rhs_wires, out_rank, out_wires, false, st);
+  if (rc) return rc;
+  return launch_step_u64(lhs, rhs, out, st, overflow, S(stream));
+}
+
+int ts_rewire(const uint8_t* in, int32_t in_rank, uint8_t* out, int32_t out_rank, const int32_t* out_ports,
+              int32_t mode, ts_stream stream) {
+  if (!in || !out || in_rank < 0 || in_rank > TS_MAX_BITS || out_rank < 0 || out_rank > TS_MAX_BITS)
+    return TS_EINVAL;
+  if (mode != TS_MODE_GT0 && mode != TS_MODE_MOD256) return TS_EINVAL;
+  RewireArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.in_rank = in_rank;
+  a.out_rank = out_rank;
+  std::vector<char> used(in_rank, 0);
+  bool dup = false;
+  for (int j = 0; j < out_rank; ++j) {
+    const int p = out_ports[j];
+    if (p < 0 || p >= in_rank) return TS_EINVAL;
+    if (used[p]) dup = true;
+    used[p] = 1;
+    a.out_port[j] = (uint8_t)p;
+  }
+  for (int p = 0; p < in_rank; ++p)
+    if (!used[p]) a.disc_port[a.num_disc++] = (uint8_t)p;
+  hipStream_t s = S(stream);
+  if (!dup) {
+    // Permutation / reduction: a unary step (kernel (a) when nothing is discarded,
+    // the reduction path of the step engine otherwise).
+    std::vector<int32_t> iw(in_rank), ow(out_rank);
+    for (int p = 0; p < in_rank; ++p) iw[p] = p;
+    for (int j = 0; j < out_rank; ++j) ow[j] = out_ports[j];
+    ts_step st;
+    int rc = make_step(in_rank, iw.data(), 0, nullptr, out_rank, ow.data(), true, st);
+    if (rc) return rc;
+    if (a.num_disc == 0) {
+      hipLaunchKernelGGL(k_permute, dim3(grid_for(1ull << out_rank)), dim3(WG), 0, s, in, out, st);
+      return hip_rc(hipGetLastError());
+    }
+    return launch_step_u8(in, in, out, st, mode, s);  // unary: rhs is never read
+  }
+  hipLaunchKernelGGL(k_rewire, dim3(grid_for(1ull << out_rank)), dim3(WG), 0, s, in, out, a, mode);
+  return hip_rc(hipGetLastError());
+}
+
+int ts_diagonal(const uint8_t* in, int32_t in_rank, const int32_t* axis_map, uint8_t* out, int32_t out_rank,
+                ts_stream stream) {
+  if (!in || !out || in_rank < 0 || in_rank > TS_MAX_BITS || out_rank < 0 || out_rank > TS_MAX_BITS)
+    return TS_EINVAL;
+  RewireArgs a;
+  std::memset(&a, 0, sizeof(a));
+  for (int i = 0; i < in_rank; ++i) {
+    if (axis_map[i] < 0 || axis_map[i] >= out_rank) return TS_EINVAL;
+    a.out_port[i] = (uint8_t)axis_map[i];
+  }
+  hipLaunchKernelGGL(k_diagonal, dim3(grid_for(1ull << out_rank)), dim3(WG), 0, S(stream), in, out, in_rank,
+                     out_rank, a);
+  return hip_rc(hipGetLastError());
+}
+
+int ts_spider(uint8_t* out, int32_t num_ports, int32_t bits, int32_t dim, ts_stream stream) {
+  if (!out || num_ports <= 0 || bits < 0 || dim <= 0 || (int64_t)num_ports * bits > TS_MAX_BITS ||
+      (bits < 31 && dim > (1 << bits)))
+    return TS_EINVAL;
+  hipLaunchKernelGGL(k_spider, dim3(grid_for(1ull << (num_ports * bits))), dim3(WG), 0, S(stream), out, num_ports,
+                     bits, dim);
+  return hip_rc(hipGetLastError());
+}
+
+int ts_union(const uint8_t* a, const uint8_t* b, uint8_t* out, int64_t n, ts_stream stream) {
+  if (!a || !b || !out || n < 0) return TS_EINVAL;
+  if (n == 0) return TS_OK;
+  if ((reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(b) | reinterpret_cast<uintptr_t>(out)) & 15)
+    return TS_EINVAL;
+  hipLaunchKernelGGL(k_union, dim3(grid_for((uint64_t)(n + 15) / 16)), dim3(WG), 0, S(stream), a, b, out, n);
+  return hip_rc(hipGetLastError());
+}
+
+int ts_stack(const uint8_t* const* ins, int32_t num, int32_t in_rank, int32_t port, int32_t axis_bits,
+             uint8_t* out, ts_stream stream) {
+  if (!ins || !out || num <= 0 || in_rank < 0 || port < 0 || port > in_rank || axis_bits < 0 ||
+      in_rank + axis_bits > TS_MAX_BITS || num > (1 << axis_bits))
+    return TS_EINVAL;
+  hipStream_t s = S(stream);
+  const int64_t total = int64_t(1) << (in_rank + axis_bits);
+  if (num < (1 << axis_bits)) {
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(total)), dim3(WG), 0, s, out, (uint8_t)0, total);
+  }
+  for (int k = 0; k < num; ++k) {
+    if (!ins[k]) return TS_EINVAL;
+    hipLaunchKernelGGL(k_embed, dim3(grid_for(1ull << in_rank)), dim3(WG), 0, s, ins[k], out, in_rank, port,
+                       axis_bits, (uint32_t)k);
+  }
+  return hip_rc(hipGetLastError());
+}
+
+int ts_select(const uint8_t* in, int32_t in_rank, const int32_t* fixed, uint8_t* out, ts_stream stream) {
+  if (!in || !out || in_rank < 0 || in_rank > TS_MAX_BITS) return TS_EINVAL;
+  uint32_t base = 0;
+  std::vector<int32_t> ow;
+  int kept = 0;
+  for (int i = 0; i < in_rank; ++i) {
+    if (fixed[i] < 0) {
+      ++kept;
+    } else if (fixed[i] > 1) {
+      return TS_EINVAL;
+    } else if (fixed[i] == 1) {
+      base |= 1u << (in_rank - 1 - i);
+    }
+  }
+  // The kept axes, read at the offset of the fixed values: a unary copy.
+  for (int i = 0; i < in_rank; ++i)
+    if (fixed[i] < 0) ow.push_back(i);
+  ts_step st;
+  std::memset(&st, 0, sizeof(st));
+  std::memset(st.o_lpos, TS_NOPOS, sizeof(st.o_lpos));
+  std::memset(st.o_rpos, TS_NOPOS, sizeof(st.o_rpos));
+  std::memset(st.c_lpos, TS_NOPOS, sizeof(st.c_lpos));
+  std::memset(st.c_rpos, TS_NOPOS, sizeof(st.c_rpos));
+  st.lhs_rank = (uint8_t)in_rank;
+  st.out_rank = (uint8_t)kept;
+  st.flags = TS_STEP_NORHS;
+  for (int j = 0; j < kept; ++j) {
+    const int axis = ow[kept - 1 - j];
+    st.o_lpos[j] = (uint8_t)(in_rank - 1 - axis);
+  }
+  hipLaunchKernelGGL(k_permute, dim3(grid_for(1ull << kept)), dim3(WG), 0, S(stream), in + base, out, st);
+  return hip_rc(hipGetLastError());
+}
+
+int ts_as_scalar(const uint8_t* x, int64_t n, int32_t* result, ts_stream stream) {
+  if (!x || !result || n < 0) return TS_EINVAL;
+  std::vector<uint8_t> h((size_t)n);
+  if (n > 0) {
+    if (hipMemcpyAsync(h.data(), x, (size_t)n, hipMemcpyDeviceToHost, S(stream)) != hipSuccess) return TS_EHIP;
+    if (hipStreamSynchronize(S(stream)) != hipSuccess) return TS_EHIP;
+  }
+  int all = 1;
+  for (uint8_t v : h)
+    if (v != 1) all = 0;
+  *result = all;
+  return TS_OK;
+}
+
+int ts_bytes_to_u64(const uint8_t* in, uint64_t* out, int64_t n, ts_stream stream) {
+  if (!in || !out || n < 0) return TS_EINVAL;
+  if (n == 0) return TS_OK;
+  hipLaunchKernelGGL(k_widen, dim3(grid_for(n)), dim3(WG), 0, S(stream), in, out, n);
+  return hip_rc(hipGetLastError());
+}
+
+int ts_program_run(const ts_job* job, const ts_step* steps, const uint8_t* leaves, uint8_t* arena, int32_t mode,
+                   uint8_t* inter, const int64_t* inter_offsets, ts_stream stream) {
+  if (!job || !arena || !leaves || (job->num_steps > 0 && !steps)) return TS_EINVAL;
+  if (mode != TS_MODE_GT0 && mode != TS_MODE_MOD256) return TS_EINVAL;
+  hipStream_t s = S(stream);
+  if (job->num_leaf_bytes > job->arena_elems) return TS_EINVAL;
+  if (hipMemcpyAsync(arena, leaves + job->leaf_begin, (size_t)job->num_leaf_bytes, hipMemcpyDeviceToDevice, s) !=
+      hipSuccess)
+    return TS_EHIP;
+  for (int i = 0; i < job->num_steps; ++i) {
+    const ts_step& st = steps[job->step_begin + i];
+    const uint64_t hi = std::max<uint64_t>({st.lhs_off + (1ull << st.lhs_rank),
+                                             (st.flags & TS_STEP_NORHS) ? 0 : st.rhs_off + (1ull << st.rhs_rank),
+                                             st.out_off + (1ull << st.out_rank)});
+    if (hi > (uint64_t)job->arena_elems) return TS_EINVAL;
+    const uint8_t* rhs = arena + ((st.flags & TS_STEP_NORHS) ? st.lhs_off : st.rhs_off);
+    int rc = launch_step_u8(arena + st.lhs_off, rhs, arena + st.out_off, st, mode, s);
+    if (rc) return rc;
+    if (inter) {
+      if (hipMemcpyAsync(inter + inter_offsets[i], arena + st.out_off, (size_t)1 << st.out_rank,
+                         hipMemcpyDeviceToDevice, s) != hipSuccess)
+        return TS_EHIP;
+    }
+  }
+  return TS_OK;
+}
+
+int ts_program_count(const ts_job* job, const ts_step* steps, const uint8_t* leaves, uint64_t* arena,
+                     uint64_t* count, int32_t* overflow, ts_stream stream) {
+  if (!job || !arena || !leaves || !count || !overflow || (job->num_steps > 0 && !steps)) return TS_EINVAL;
+  hipStream_t s = S(stream);
+  if (job->num_leaf_bytes > job->arena_elems) return TS_EINVAL;
+  if (job->num_leaf_bytes > 0) {
+    hipLaunchKernelGGL(k_widen, dim3(grid_for(job->num_leaf_bytes)), dim3(WG), 0, s, leaves + job->leaf_begin,
+                       arena, (int64_t)job->num_leaf_bytes);
+  }
+  for (int i = 0; i < job->num_steps; ++i) {
+    const ts_step& st = steps[job->step_begin + i];
+    const uint64_t hi = std::max<uint64_t>({st.lhs_off + (1ull << st.lhs_rank),
+                                             (st.flags & TS_STEP_NORHS) ? 0 : st.rhs_off + (1ull << st.rhs_rank),
+                                             st.out_off + (1ull << st.out_rank)});
+    if (hi > (uint64_t)job->arena_elems) return TS_EINVAL;
+    const uint64_t* rhs = arena + ((st.flags & TS_STEP_NORHS) ? st.lhs_off : st.rhs_off);
+    int rc = launch_step_u64(arena + st.lhs_off, rhs, arena + st.out_off, st, overflow, s);
+    if (rc) return rc;
+  }
+  if (job->result_rank != 0) return TS_EINVAL;
+  hipLaunchKernelGGL(k_scale_add, dim3(1), dim3(64), 0, s, arena + job->result_off, job->free_vars, count, overflow);
+  return hip_rc(hipGetLastError());
+}
+
+}  // extern "C"

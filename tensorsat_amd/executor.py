@@ -1,0 +1,181 @@
+"""
+Device executors for compiled programs (``tensorsat_amd.program``).
+
+* :func:`run_job` / :func:`count_job` — one job, every step a grid-wide kernel on the
+  current stream (single large instance: the n=120 decision and the #SAT counts).
+* :class:`BatchExecutor` — all jobs of a program in one persistent launch
+  (``ts_batch_run``), one workgroup per job at a time, verdicts OR-ed per instance.
+
+These replace the reference's per-step loop (``contractions/cotengra.py:120-126``,
+``contractions/simple.py:251-292``) and final ``bool()`` (``diagrams.py:949-958``).
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _native as N
+from .program import JOB_DTYPE, Program, validate
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+class DeviceProgram:
+    """Device copies of a program's step, job and leaf arrays."""
+
+    def __init__(self, prog: Program, device: str = "cuda") -> None:
+        torch = _torch()
+        N.lib()
+        validate(prog)
+        self.prog = prog
+        self.device = device
+        self.steps_host = np.ascontiguousarray(prog.steps)
+        self.jobs_host = np.ascontiguousarray(prog.jobs)
+        leaves = prog.leaves if prog.leaves.size else np.zeros(16, np.uint8)
+        self.leaves = torch.from_numpy(np.ascontiguousarray(leaves)).to(device)
+        steps_bytes = self.steps_host.view(np.uint8) if self.steps_host.size else np.zeros(128, np.uint8)
+        self.steps = torch.from_numpy(np.ascontiguousarray(steps_bytes)).to(device)
+        jobs_bytes = self.jobs_host.view(np.uint8) if self.jobs_host.size else np.zeros(JOB_DTYPE.itemsize, np.uint8)
+        self.jobs = torch.from_numpy(np.ascontiguousarray(jobs_bytes)).to(device)
+
+    def job_ptr(self, j: int) -> ctypes.c_void_p:
+        return ctypes.c_void_p(self.jobs_host.ctypes.data + j * JOB_DTYPE.itemsize)
+
+
+def run_job(dp: DeviceProgram, j: int, mode: str = "GT0", *, keep_intermediates: bool = False):
+    """Run job ``j`` grid-wide; returns (result uint8 tensor, list of step outputs)."""
+    torch = _torch()
+    L = N.lib()
+    job = dp.jobs_host[j]
+    arena = torch.empty(max(16, int(job["arena_elems"])), dtype=torch.uint8, device=dp.device)
+    inter = None
+    offsets = None
+    sizes: list[int] = []
+    if keep_intermediates:
+        steps = dp.prog.job_steps(j)
+        sizes = [1 << int(r) for r in steps["out_rank"]]
+        offsets = np.zeros(max(1, len(sizes)), dtype=np.int64)
+        if sizes:
+            offsets[1:] = np.cumsum(sizes)[:-1] if len(sizes) > 1 else []
+        inter = torch.empty(max(16, sum(sizes)), dtype=torch.uint8, device=dp.device)
+    N.check(
+        L.ts_program_run(
+            dp.job_ptr(j), N.host_ptr(dp.steps_host), N.dptr(dp.leaves), N.dptr(arena), N.MODES[mode],
+            N.dptr(inter) if inter is not None else None,
+            N.host_ptr(offsets) if offsets is not None else None, N.stream_ptr(),
+        ),
+        "ts_program_run",
+    )
+    r0, rr = int(job["result_off"]), int(job["result_rank"])
+    result = arena[r0 : r0 + (1 << rr)].clone()
+    outs = []
+    if inter is not None:
+        for off, sz in zip(offsets.tolist(), sizes):
+            outs.append(inter[off : off + sz])
+    return result, outs
+
+
+def count_job(dp: DeviceProgram, j: int, acc=None, overflow=None):
+    """Exact uint64 count of job ``j`` (times 2^free_vars) added into ``acc``."""
+    torch = _torch()
+    L = N.lib()
+    job = dp.jobs_host[j]
+    arena = torch.empty(max(2, int(job["arena_elems"])), dtype=torch.int64, device=dp.device)
+    if acc is None:
+        acc = torch.zeros(1, dtype=torch.int64, device=dp.device)
+    if overflow is None:
+        overflow = torch.zeros(1, dtype=torch.int32, device=dp.device)
+    N.check(
+        L.ts_program_count(dp.job_ptr(j), N.host_ptr(dp.steps_host), N.dptr(dp.leaves), N.dptr(arena),
+                           N.dptr(acc), N.dptr(overflow), N.stream_ptr()),
+        "ts_program_count",
+    )
+    return acc, overflow
+
+
+def count_program(dp: DeviceProgram) -> list[int]:
+    """Per-instance exact model counts; raises OverflowError if any count >= 2^64."""
+    torch = _torch()
+    n = dp.prog.num_instances
+    accs = torch.zeros(n, dtype=torch.int64, device=dp.device)
+    ovf = torch.zeros(n, dtype=torch.int32, device=dp.device)
+    for j in range(len(dp.jobs_host)):
+        i = int(dp.jobs_host["instance"][j])
+        count_job(dp, j, accs[i : i + 1], ovf[i : i + 1])
+    torch.cuda.synchronize()
+    if int(ovf.max().item()) if n else 0:
+        raise OverflowError("model count exceeds 2^64")
+    vals = accs.cpu().numpy().view(np.uint64)
+    return [int(v) for v in vals]
+
+
+def dispatch_order(prog: Program) -> np.ndarray:
+    """
+    Job order for the persistent queue: by cube index within the instance (so early
+    exit can skip later cubes of decided instances), then largest job first.
+    """
+    jobs = prog.jobs
+    if len(jobs) == 0:
+        return np.zeros(0, dtype=np.int32)
+    steps = prog.steps
+    # Work estimate: sum over steps of 2^(out_rank + sum_rank).
+    work = np.zeros(len(jobs), dtype=np.float64)
+    if len(steps):
+        per = np.ldexp(1.0, steps["out_rank"].astype(np.int64) + steps["sum_rank"].astype(np.int64))
+        csum = np.concatenate([[0.0], np.cumsum(per)])
+        b = jobs["step_begin"].astype(np.int64)
+        e = b + jobs["num_steps"].astype(np.int64)
+        work = csum[e] - csum[b]
+    return np.lexsort((-work, jobs["cube"])).astype(np.int32)
+
+
+class BatchExecutor:
+    """All jobs of a program in one persistent launch (``ts_batch_run``)."""
+
+    def __init__(self, prog: Program, *, num_workgroups: int | None = None, device: str = "cuda",
+                 order: np.ndarray | None = None, track_jobs: bool = False) -> None:
+        torch = _torch()
+        L = N.lib()
+        self.dp = DeviceProgram(prog, device)
+        self.prog = prog
+        nj = len(prog.jobs)
+        if num_workgroups is None:
+            mx = N.i32(0)
+            N.check(L.ts_batch_max_workgroups(ctypes.byref(mx)), "ts_batch_max_workgroups")
+            num_workgroups = int(mx.value)
+        self.num_workgroups = max(1, min(int(num_workgroups), max(1, nj)))
+        max_arena = int(prog.jobs["arena_elems"].max()) if nj else 16
+        self.arena_stride = (max_arena + 255) // 256 * 256
+        self.arenas = torch.empty(self.num_workgroups * self.arena_stride, dtype=torch.uint8, device=device)
+        self.order = torch.from_numpy(dispatch_order(prog) if order is None else order.astype(np.int32)).to(device)
+        self.verdicts = torch.zeros(max(1, prog.num_instances), dtype=torch.int32, device=device)
+        self.counter = torch.zeros(1, dtype=torch.int32, device=device)
+        self.job_done = torch.zeros(max(1, nj), dtype=torch.int32, device=device) if track_jobs else None
+
+    def launch(self, mode: str = "GT0", early_exit: bool = True) -> None:
+        """Enqueue one full run on the current stream (verdicts reset first)."""
+        L = N.lib()
+        self.verdicts.zero_()
+        self.counter.zero_()
+        if self.job_done is not None:
+            self.job_done.zero_()
+        b = N.Batch(
+            jobs=self.dp.jobs.data_ptr(), order=self.order.data_ptr(), num_jobs=len(self.prog.jobs),
+            steps=self.dp.steps.data_ptr(), leaves=self.dp.leaves.data_ptr(), arenas=self.arenas.data_ptr(),
+            arena_stride=self.arena_stride, num_workgroups=self.num_workgroups, mode=N.MODES[mode],
+            verdicts=self.verdicts.data_ptr(), counter=self.counter.data_ptr(), early_exit=int(early_exit),
+            job_done=self.job_done.data_ptr() if self.job_done is not None else None,
+        )
+        N.check(L.ts_batch_run(ctypes.byref(b), N.stream_ptr()), "ts_batch_run")
+
+    def run(self, mode: str = "GT0", early_exit: bool = True) -> np.ndarray:
+        torch = _torch()
+        self.launch(mode, early_exit)
+        torch.cuda.synchronize()
+        return self.verdicts.cpu().numpy()[: self.prog.num_instances].astype(np.uint8)

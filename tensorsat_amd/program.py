@@ -1,0 +1,267 @@
+"""
+Compiled contraction programs (``include/tensorsat_prog.h``): jobs of pairwise steps
+with every intermediate's axis order and arena offset fixed on the host.
+
+A program is the compiled form of the contract2 argument lists that the reference's
+``SimpleContraction._new`` precomputes (``contractions/simple.py:134-194``) and of the
+pairwise loop cotengra runs (``contractions/cotengra.py:120-126``).  The same records
+drive the GPU executors and the CPU checker.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import plan as _plan
+from .network import Wiring
+from .sat import CNFInstance
+
+MAX_BITS = 27
+NOPOS = 0xFF
+STEP_KINNER = 1
+STEP_NORHS = 2
+NSTAT = 8
+
+STEP_DTYPE = np.dtype(
+    [
+        ("lhs_off", "<u4"), ("rhs_off", "<u4"), ("out_off", "<u4"),
+        ("lhs_rank", "u1"), ("rhs_rank", "u1"), ("out_rank", "u1"), ("sum_rank", "u1"),
+        ("flags", "u1"), ("pad", "u1", (3,)),
+        ("o_lpos", "u1", (MAX_BITS,)), ("o_rpos", "u1", (MAX_BITS,)),
+        ("c_lpos", "u1", (MAX_BITS,)), ("c_rpos", "u1", (MAX_BITS,)),
+    ]
+)
+assert STEP_DTYPE.itemsize == 128
+
+JOB_DTYPE = np.dtype(
+    [
+        ("instance", "<i4"), ("cube", "<i4"), ("step_begin", "<i4"), ("num_steps", "<i4"),
+        ("leaf_begin", "<i8"), ("num_leaf_bytes", "<i4"), ("result_off", "<i4"),
+        ("result_rank", "<i4"), ("free_vars", "<i4"), ("arena_elems", "<i8"),
+        ("max_rank", "<i4"), ("flags", "<i4"),
+    ]
+)
+assert JOB_DTYPE.itemsize == 56
+
+STAT_NAMES = ("jobs", "branch_nodes", "pruned", "truncated", "max_rank", "host_verdict",
+              "plan_us", "empty_residuals")
+
+
+class _SatOpts(ctypes.Structure):
+    _fields_ = [
+        ("width", ctypes.c_int32), ("max_cubes", ctypes.c_int32), ("plan_trials", ctypes.c_int32),
+        ("threads", ctypes.c_int32), ("seed", ctypes.c_uint64), ("counting", ctypes.c_int32),
+    ]
+
+
+_bound = False
+
+
+def _lib():
+    global _bound
+    L = _plan.lib()
+    if not _bound:
+        P = ctypes.POINTER
+        vp = ctypes.c_void_p
+        L.tsc_compile.argtypes = [P(_plan._Net), P(ctypes.c_int32), P(ctypes.c_void_p), P(vp)]
+        L.tsc_sat_batch.argtypes = [ctypes.c_int32, P(ctypes.c_int32), P(ctypes.c_int64), P(ctypes.c_int64),
+                                    P(ctypes.c_int32), P(_SatOpts), P(vp)]
+        L.tsc_program_sizes.argtypes = [vp, P(ctypes.c_int64), P(ctypes.c_int64), P(ctypes.c_int64), P(ctypes.c_int64)]
+        L.tsc_program_copy.argtypes = [vp, vp, vp, vp, vp]
+        L.tsc_program_free.argtypes = [vp]
+        L.tsc_program_free.restype = None
+        L.tsp_elim.argtypes = [P(_plan._Net), ctypes.c_int32, ctypes.c_uint64, P(ctypes.c_int32), P(_plan._Cost)]
+        for f in (L.tsc_compile, L.tsc_sat_batch, L.tsc_program_sizes, L.tsc_program_copy, L.tsp_elim):
+            f.restype = ctypes.c_int
+        _bound = True
+    return L
+
+
+@dataclass
+class Program:
+    """Jobs, steps and leaf bytes of a compiled program, plus per-instance statistics."""
+
+    jobs: np.ndarray     # JOB_DTYPE
+    steps: np.ndarray    # STEP_DTYPE
+    leaves: np.ndarray   # uint8
+    stats: np.ndarray    # int32 [num_instances, NSTAT]
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def num_instances(self) -> int:
+        return int(self.stats.shape[0])
+
+    def job_steps(self, j: int) -> np.ndarray:
+        b = int(self.jobs["step_begin"][j])
+        return self.steps[b : b + int(self.jobs["num_steps"][j])]
+
+    def job_leaves(self, j: int) -> np.ndarray:
+        b = int(self.jobs["leaf_begin"][j])
+        return self.leaves[b : b + int(self.jobs["num_leaf_bytes"][j])]
+
+
+def _take(handle) -> Program:
+    L = _lib()
+    nj, ns, nl, ni = (ctypes.c_int64() for _ in range(4))
+    try:
+        _plan._check(L.tsc_program_sizes(handle, ctypes.byref(nj), ctypes.byref(ns), ctypes.byref(nl),
+                                         ctypes.byref(ni)), "tsc_program_sizes")
+        jobs = np.zeros(nj.value, dtype=JOB_DTYPE)
+        steps = np.zeros(ns.value, dtype=STEP_DTYPE)
+        leaves = np.zeros(nl.value, dtype=np.uint8)
+        stats = np.zeros((ni.value, NSTAT), dtype=np.int32)
+        _plan._check(L.tsc_program_copy(handle, jobs.ctypes.data, steps.ctypes.data, leaves.ctypes.data,
+                                        stats.ctypes.data), "tsc_program_copy")
+    finally:
+        L.tsc_program_free(handle)
+    return Program(jobs, steps, leaves, stats)
+
+
+def elim(wiring: Wiring, *, trials: int = 8, seed: int = 0) -> _plan.Plan:
+    """Elimination-order planner (min-fill / min-degree trials)."""
+    L = _lib()
+    h = _plan._NetHolder(wiring)
+    S = wiring.num_slots
+    path = np.zeros(max(0, 2 * (S - 1)), dtype=np.int32)
+    c = _plan._Cost()
+    _plan._check(L.tsp_elim(ctypes.byref(h.net), trials, seed, _plan._ptr(path), ctypes.byref(c)), "tsp_elim")
+    return _plan.Plan(S, path.reshape(-1, 2), cost=_plan.Cost._from(c))
+
+
+def compile_network(wiring: Wiring, leaves, plan: _plan.Plan) -> Program:
+    """
+    Compile a bit network (every wire extent 2, unique wires per slot, unique out wires)
+    with host leaf tensors (uint8, slot wire order) along ``plan``.
+    """
+    if any(d != 2 for d in wiring.wire_dims):
+        raise ValueError("compile_network needs extent-2 wires")
+    if len(leaves) != wiring.num_slots:
+        raise ValueError("one leaf per slot is required")
+    arrs = [np.ascontiguousarray(np.asarray(t, dtype=np.uint8)) for t in leaves]
+    for s, (a, ws) in enumerate(zip(arrs, wiring.slot_wires_list)):
+        if a.size != 1 << len(ws):
+            raise ValueError(f"leaf {s} has {a.size} entries, expected {1 << len(ws)}")
+    ptrs = (ctypes.c_void_p * max(1, len(arrs)))(*[a.ctypes.data for a in arrs])
+    h = _plan._NetHolder(wiring)
+    path = np.ascontiguousarray(plan.path.reshape(-1), dtype=np.int32)
+    handle = ctypes.c_void_p()
+    L = _lib()
+    _plan._check(L.tsc_compile(ctypes.byref(h.net), _plan._ptr(path), ptrs, ctypes.byref(handle)), "tsc_compile")
+    prog = _take(handle)
+    prog.meta["plan"] = plan
+    return prog
+
+
+def sat_batch(
+    cnfs: list[CNFInstance],
+    *,
+    width: int = 18,
+    max_cubes: int = 0,
+    plan_trials: int = 2,
+    threads: int = 0,
+    seed: int = 0,
+) -> Program:
+    """
+    Conditioned-slicing front-end for a batch of CNF instances: one job per cube, in
+    depth-first order per instance (``tsc_sat_batch``).
+    """
+    nv = np.asarray([c.num_vars for c in cnfs], dtype=np.int32)
+    clause_counts = np.asarray([len(c.clauses) for c in cnfs], dtype=np.int64)
+    coff = np.zeros(len(cnfs) + 1, dtype=np.int64)
+    coff[1:] = np.cumsum(clause_counts)
+    lens = np.fromiter((len(cl) for c in cnfs for cl in c.clauses), dtype=np.int64, count=int(coff[-1]))
+    loff = np.zeros(int(coff[-1]) + 1, dtype=np.int64)
+    loff[1:] = np.cumsum(lens)
+    lits = np.fromiter((l for c in cnfs for cl in c.clauses for l in cl), dtype=np.int32, count=int(loff[-1]))
+    opts = _SatOpts(width, max_cubes, plan_trials, threads, seed, 0)
+    handle = ctypes.c_void_p()
+    L = _lib()
+    P = ctypes.POINTER
+    rc = L.tsc_sat_batch(
+        len(cnfs), nv.ctypes.data_as(P(ctypes.c_int32)), coff.ctypes.data_as(P(ctypes.c_int64)),
+        loff.ctypes.data_as(P(ctypes.c_int64)), lits.ctypes.data_as(P(ctypes.c_int32)),
+        ctypes.byref(opts), ctypes.byref(handle),
+    )
+    _plan._check(rc, "tsc_sat_batch")
+    prog = _take(handle)
+    prog.meta.update(width=width, max_cubes=max_cubes, plan_trials=plan_trials)
+    return prog
+
+
+def step_labels(st) -> tuple[list[int], list[int], list[int]]:
+    """
+    Einsum labels of one step record: out bit j is label j, summed bit j is label
+    ``out_rank + j``; operand axes are listed MSB first (row-major order).
+    """
+    ro, nc = int(st["out_rank"]), int(st["sum_rank"])
+    rl, rr = int(st["lhs_rank"]), int(st["rhs_rank"])
+    lab_l = [-1] * rl
+    lab_r = [-1] * rr
+    for j in range(ro):
+        p = int(st["o_lpos"][j])
+        if p != NOPOS:
+            lab_l[rl - 1 - p] = j
+        p = int(st["o_rpos"][j])
+        if p != NOPOS:
+            lab_r[rr - 1 - p] = j
+    for j in range(nc):
+        p = int(st["c_lpos"][j])
+        if p != NOPOS:
+            lab_l[rl - 1 - p] = ro + j
+        p = int(st["c_rpos"][j])
+        if p != NOPOS:
+            lab_r[rr - 1 - p] = ro + j
+    out = [ro - 1 - i for i in range(ro)]
+    if -1 in lab_l or -1 in lab_r:
+        raise ValueError("malformed step record")
+    return lab_l, lab_r, out
+
+
+def validate(prog: Program) -> None:
+    """
+    Host-side bounds check of every record before any device launch: operand and out
+    ranges inside the job arena, ranks within limits, leaves inside the arena and the
+    leaf buffer, and summed/out bit positions inside each operand.
+    """
+    jobs, steps = prog.jobs, prog.steps
+    if len(jobs) == 0:
+        return
+    if np.any(jobs["num_leaf_bytes"].astype(np.int64) > jobs["arena_elems"]):
+        raise ValueError("leaves exceed the job arena")
+    if np.any(jobs["leaf_begin"] + jobs["num_leaf_bytes"] > prog.leaves.size):
+        raise ValueError("leaf range outside the leaf buffer")
+    if np.any(jobs["leaf_begin"] % 16):
+        raise ValueError("leaf blocks must be 16-byte aligned")
+    if np.any(jobs["step_begin"] + jobs["num_steps"] > len(steps)) or np.any(jobs["step_begin"] < 0):
+        raise ValueError("step range outside the step array")
+    if np.any(jobs["result_off"].astype(np.int64) + (1 << jobs["result_rank"].astype(np.int64)) > jobs["arena_elems"]):
+        raise ValueError("result outside the job arena")
+    if len(steps) == 0:
+        return
+    owner = np.repeat(np.arange(len(jobs)), jobs["num_steps"])
+    order = np.concatenate([np.arange(b, b + n) for b, n in zip(jobs["step_begin"], jobs["num_steps"])]) \
+        if len(owner) else np.zeros(0, dtype=np.int64)
+    st = steps[order]
+    arena = jobs["arena_elems"][owner].astype(np.int64)
+    for name in ("lhs_rank", "rhs_rank", "out_rank", "sum_rank"):
+        if np.any(st[name] > MAX_BITS):
+            raise ValueError(f"{name} above {MAX_BITS}")
+    norhs = (st["flags"] & STEP_NORHS) != 0
+    one = np.int64(1)
+    if np.any(st["lhs_off"].astype(np.int64) + (one << st["lhs_rank"].astype(np.int64)) > arena):
+        raise ValueError("lhs outside the job arena")
+    if np.any(~norhs & (st["rhs_off"].astype(np.int64) + (one << st["rhs_rank"].astype(np.int64)) > arena)):
+        raise ValueError("rhs outside the job arena")
+    if np.any(st["out_off"].astype(np.int64) + (one << st["out_rank"].astype(np.int64)) > arena):
+        raise ValueError("out outside the job arena")
+    for pos, rank in (("o_lpos", "lhs_rank"), ("c_lpos", "lhs_rank"), ("o_rpos", "rhs_rank"), ("c_rpos", "rhs_rank")):
+        p = st[pos].astype(np.int64)
+        lim = st[rank].astype(np.int64)[:, None]
+        valid = (p == NOPOS) | (p < lim)
+        if not np.all(valid):
+            raise ValueError(f"{pos} outside its operand")
+    if np.any(st["lhs_off"] % 16) or np.any(~norhs & (st["rhs_off"] % 16 != 0)) or np.any(st["out_off"] % 16):
+        raise ValueError("tensor offsets must be 16-element aligned")

@@ -1,0 +1,170 @@
+"""
+CNF instances: random k-SAT generation, DIMACS input/output and input selections.
+
+Mirrors ``CNFInstance`` of the reference (``tensorsat/lib/sat.py:40-275``): the same
+constructor names, argument meaning, validation errors and random stream, so that a
+seed produces the same clauses as the reference generator.  Diagram building is done
+by :mod:`tensorsat_amd.network`, which emits flat index lists directly instead of a
+Python object graph.
+"""
+
+from __future__ import annotations
+
+import re
+from collections.abc import Sequence
+from math import comb
+
+import numpy as np
+
+Clause = tuple[int, ...]
+"""A clause: non-zero literals, sign = polarity, magnitude = 1-based variable."""
+
+
+class CNFInstance:
+    """A SAT instance in conjunctive normal form (``lib/sat.py:40``)."""
+
+    __slots__ = ("_num_vars", "_clauses")
+
+    def __init__(self, num_vars: int, clauses: Sequence[Sequence[int]]) -> None:
+        """Public constructor, validating literals (``lib/sat.py:111-121``)."""
+        clauses = tuple(tuple(int(l) for l in c) for c in clauses)
+        lits = [abs(l) for c in clauses for l in c]
+        if lits and num_vars < max(lits):
+            raise ValueError("Clauses contain invalid variables.")
+        if any(l == 0 for c in clauses for l in c):
+            raise ValueError("Literals must be non-zero.")
+        self._num_vars = int(num_vars)
+        self._clauses = clauses
+
+    @classmethod
+    def _new(cls, num_vars: int, clauses: tuple[Clause, ...]) -> "CNFInstance":
+        self = object.__new__(cls)
+        self._num_vars = num_vars
+        self._clauses = clauses
+        return self
+
+    @classmethod
+    def random(
+        cls, k: int, n: int, m: int, *, rng: int | np.random.Generator | None = None
+    ) -> "CNFInstance":
+        """
+        Random k-SAT with ``n`` variables and ``m`` distinct clauses.
+
+        Follows ``lib/sat.py:43-78``: per clause, ``k`` distinct variables drawn by
+        ``rng.choice(range(n), size=k, replace=False)``, then ``k`` signs drawn by
+        ``rng.choice(range(2), size=k)``; literal ``v+1`` for sign 0 and ``-(v+1)``
+        for sign 1; literals sorted; a clause equal to an earlier one is redrawn.
+        The two ``choice`` calls are made in the same order with the same arguments,
+        so a given seed yields the same clause sequence.
+        """
+        if k <= 0:
+            raise ValueError("Clause size 'k' must be positive.")
+        if n < k:
+            raise ValueError(f"Number of variables 'n' must be at least {k = }.")
+        if m <= 0:
+            raise ValueError("Number of clauses 'm' must be positive.")
+        if m > comb(n, k) * 2**k:
+            raise ValueError(f"Number of clauses 'm' cannot exceed {comb(n, k)*2**k = }")
+        if not isinstance(rng, np.random.Generator):
+            rng = np.random.default_rng(rng)
+        clauses: list[Clause] = []
+        seen: set[Clause] = set()
+        pool = range(n)
+        while len(clauses) < m:
+            vs = rng.choice(pool, size=k, replace=False)
+            signs = rng.choice(range(2), size=k)
+            clause = tuple(
+                sorted(int(v) + 1 if int(s) == 0 else -int(v) - 1 for v, s in zip(vs, signs))
+            )
+            if clause not in seen:
+                seen.add(clause)
+                clauses.append(clause)
+        return cls._new(n, tuple(clauses))
+
+    @classmethod
+    def from_dimacs(cls, dimacs: str) -> "CNFInstance":
+        """
+        Parses DIMACS text (``lib/sat.py:80-100``): comment lines start with ``c``,
+        the header is ``p cnf <vars> <clauses>``, then one clause per line with its
+        terminating ``0`` dropped.
+        """
+        lines = [
+            s for line in dimacs.strip().split("\n") if (s := line.strip()) and not s.startswith("c")
+        ]
+        if not lines:
+            raise ValueError("DIMACS code must start with 'p cnf <num vars> <num clauses>'.")
+        match = re.compile(r"p cnf ([0-9]+) ([0-9]+)").match(lines[0])
+        if not match:
+            raise ValueError("DIMACS code must start with 'p cnf <num vars> <num clauses>'.")
+        num_vars, num_clauses = map(int, match.groups())
+        clauses = tuple(tuple(map(int, line.split()))[:-1] for line in lines[1:])
+        if len(clauses) != num_clauses:
+            raise ValueError("Number of clauses does not match the specified number.")
+        lits = [abs(l) for c in clauses for l in c]
+        if lits and num_vars < max(lits):
+            raise ValueError("Clauses contain invalid variables.")
+        return cls._new(num_vars, clauses)
+
+    @property
+    def num_vars(self) -> int:
+        return self._num_vars
+
+    @property
+    def clauses(self) -> tuple[Clause, ...]:
+        return self._clauses
+
+    def to_dimacs(self) -> str:
+        """DIMACS text (``lib/sat.py:135-142``)."""
+        lines = [f"p cnf {self.num_vars} {len(self.clauses)}"]
+        lines += [f"{' '.join(map(str, c))} 0" for c in self.clauses]
+        return "\n".join(lines)
+
+    def normalize_inputs(self, inputs: str | int | None) -> str:
+        """
+        Input selection string over ``0``, ``1``, ``?`` (unknown) and ``_`` (open),
+        as ``lib/sat.py:144-163``: an int is written in binary, zero-padded to ``n``.
+        """
+        n = self.num_vars
+        if isinstance(inputs, (int, np.integer)) and not isinstance(inputs, bool):
+            if inputs < 0:
+                raise ValueError("Inputs must be specified by a non-negative number.")
+            inputs = f"{int(inputs):0>{n}b}"
+        elif inputs is None:
+            inputs = "?" * n
+        else:
+            if not isinstance(inputs, str):
+                raise TypeError("Inputs must be a string, an int or None.")
+            if not all(b in "01?_" for b in inputs):
+                raise ValueError("Input specification characters must be '0', '1', '?' or '_'.")
+            if len(inputs) != n:
+                raise ValueError(
+                    f"Expected input specification string to be {n} characters long,"
+                    f" found {len(inputs)} characters instead."
+                )
+        return inputs
+
+    def clause_matrix(self) -> np.ndarray:
+        """Clauses as a dense ``(m, k_max)`` int32 array, zero-padded on the right."""
+        m = len(self.clauses)
+        kmax = max((len(c) for c in self.clauses), default=0)
+        out = np.zeros((m, kmax), dtype=np.int32)
+        for i, c in enumerate(self.clauses):
+            out[i, : len(c)] = c
+        return out
+
+    def __eq__(self, other: object) -> bool:
+        if not isinstance(other, CNFInstance):
+            return NotImplemented
+        return self._num_vars == other._num_vars and self._clauses == other._clauses
+
+    def __hash__(self) -> int:
+        return hash((self._num_vars, self._clauses))
+
+    def __repr__(self) -> str:
+        attrs = [f"{self.num_vars} vars", f"{len(self.clauses)} clauses"]
+        ks = {len(c) for c in self.clauses}
+        if len(ks) == 1:
+            attrs.append(f"k={ks.pop()}")
+        elif ks:
+            attrs.append(f"k in [{min(ks)}..{max(ks)}]")
+        return f"<CNFInstance: {', '.join(attrs)}>"

@@ -1,0 +1,139 @@
+"""
+CPU emulation of the device step engine's index math (``csrc/device.h``).
+
+Rebuilds the byte-chunk tables exactly as ``build_lut`` does, carrying the tables from
+one step to the next the way the persistent kernel keeps them in LDS (entries a step does
+not rebuild keep stale values, here filled with large garbage first), and evaluates the
+offsets ``step_u8`` computes for every out index and summed index.  Every offset must lie
+inside its operand, and the values must match the oracle's einsum on the same records.
+"""
+
+import numpy as np
+import pytest
+
+from tensorsat_amd import program as PG
+from tensorsat_amd.network import bintree_network, clause_network
+from tensorsat_amd.sat import CNFInstance
+
+NOPOS = PG.NOPOS
+
+
+def _build(st, tables, need_c):
+    ro, nc = int(st["out_rank"]), int(st["sum_rank"])
+    nqo = max(1, (ro + 7) // 8)
+    v = np.arange(256)
+    for q in range(nqo):
+        al = np.zeros(256, np.int64)
+        ar = np.zeros(256, np.int64)
+        for i in range(8):
+            j = q * 8 + i
+            if j < ro:
+                on = (v >> i) & 1
+                pl, pr = int(st["o_lpos"][j]), int(st["o_rpos"][j])
+                if pl != NOPOS:
+                    al |= on << pl
+                if pr != NOPOS:
+                    ar |= on << pr
+        tables["ol"][q] = al
+        tables["or"][q] = ar
+    if need_c:
+        nqc = max(1, (nc + 7) // 8)
+        for q in range(nqc):
+            al = np.zeros(256, np.int64)
+            ar = np.zeros(256, np.int64)
+            for i in range(8):
+                j = q * 8 + i
+                if j < nc:
+                    on = (v >> i) & 1
+                    pl, pr = int(st["c_lpos"][j]), int(st["c_rpos"][j])
+                    if pl != NOPOS:
+                        al |= on << pl
+                    if pr != NOPOS:
+                        ar |= on << pr
+            tables["cl"][q] = al
+            tables["cr"][q] = ar
+
+
+def _offsets(T, idx, nbits, low_from_table=True):
+    nq = (nbits + 7) // 8
+    r = T[0][idx & 255].copy()
+    for q in range(1, 4):
+        if nq > q:
+            r += T[q][(idx >> (8 * q)) & 255]
+    return r
+
+
+def emulate_job(prog, j, mode):
+    job = prog.jobs[j]
+    arena = np.zeros(int(job["arena_elems"]), np.uint8)
+    lv = prog.job_leaves(j)
+    arena[: lv.size] = lv
+    rng = np.random.default_rng(j)
+    tables = {k: rng.integers(1 << 28, 1 << 30, size=(4, 256)) for k in ("ol", "or", "cl", "cr")}
+    for st in prog.job_steps(j):
+        ro, nc = int(st["out_rank"]), int(st["sum_rank"])
+        rl, rr = int(st["lhs_rank"]), int(st["rhs_rank"])
+        norhs = bool(int(st["flags"]) & PG.STEP_NORHS)
+        kinner = bool(int(st["flags"]) & PG.STEP_KINNER)
+        need_c = (not kinner) and nc > 0
+        _build(st, tables, need_c)
+        o = np.arange(1 << ro, dtype=np.int64)
+        offl = _offsets(tables["ol"], o, ro)
+        offr = _offsets(tables["or"], o, ro)
+        c = np.arange(1 << nc, dtype=np.int64)
+        if kinner:
+            cl = c
+            cr = c
+        elif nc > 0:
+            cl = _offsets(tables["cl"], c, nc)
+            cr = _offsets(tables["cr"], c, nc)
+        else:
+            cl = cr = np.zeros(1, np.int64)
+        L = offl[:, None] + cl[None, :]
+        assert L.min() >= 0 and L.max() < (1 << rl), "lhs offset out of bounds"
+        lhs = arena[int(st["lhs_off"]) : int(st["lhs_off"]) + (1 << rl)]
+        a = lhs[L].astype(np.int64)
+        if not norhs:
+            R = offr[:, None] + cr[None, :]
+            assert R.min() >= 0 and R.max() < (1 << rr), "rhs offset out of bounds"
+            rhs = arena[int(st["rhs_off"]) : int(st["rhs_off"]) + (1 << rr)]
+            a = a & rhs[R].astype(np.int64)
+        acc = a.sum(axis=1)
+        out = ((acc % 256) != 0) if mode == "MOD256" else (acc != 0)
+        oo = int(st["out_off"])
+        arena[oo : oo + (1 << ro)] = out.astype(np.uint8)
+    r0, rr_ = int(job["result_off"]), int(job["result_rank"])
+    return arena[r0 : r0 + (1 << rr_)].copy()
+
+
+@pytest.mark.parametrize("mode", ["GT0", "MOD256"])
+def test_batch_program_indexing(mode):
+    from oracle import executor as OX
+
+    rng = np.random.default_rng(5)
+    cnfs = []
+    for i in range(16):
+        n = int(rng.integers(14, 21))
+        ratio = 4.26 if i % 2 == 0 else 3.5
+        cnfs.append(CNFInstance.random(3, n, int(round(ratio * n)), rng=1000 + i))
+    prog = PG.sat_batch(cnfs, width=8)
+    PG.validate(prog)
+    for j in range(len(prog.jobs)):
+        got = emulate_job(prog, j, mode)
+        ref, _ = OX.run_job(prog, j, mode)
+        np.testing.assert_array_equal(got, ref)
+
+
+@pytest.mark.parametrize("enc", [bintree_network, clause_network])
+def test_whole_network_indexing(enc):
+    from oracle import executor as OX
+
+    for seed in range(4):
+        n = 9 + seed
+        cnf = CNFInstance.random(3, n, int(round(4.26 * n)), rng=seed)
+        d = enc(cnf)
+        prog = PG.compile_network(d.wiring, d.blocks, PG.elim(d.wiring, trials=2))
+        PG.validate(prog)
+        got = emulate_job(prog, 0, "GT0")
+        ref, _ = OX.run_job(prog, 0, "GT0")
+        np.testing.assert_array_equal(got, ref)
