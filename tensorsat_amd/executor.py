@@ -158,13 +158,18 @@ class BatchExecutor:
         self.counter = torch.zeros(1, dtype=torch.int32, device=device)
         self.job_done = torch.zeros(max(1, nj), dtype=torch.int32, device=device) if track_jobs else None
 
-    def launch(self, mode: str = "GT0", early_exit: bool = True) -> None:
-        """Enqueue one full run on the current stream (verdicts reset first)."""
+    def launch(self, mode: str = "GT0", early_exit: bool = True, events=None) -> None:
+        """
+        Enqueue one full run on the current stream (verdicts reset first).  ``events``
+        (start, end) are recorded on the same stream right around the persistent kernel.
+        """
         L = N.lib()
         self.verdicts.zero_()
         self.counter.zero_()
         if self.job_done is not None:
             self.job_done.zero_()
+        if events is not None:
+            events[0].record()
         b = N.Batch(
             jobs=self.dp.jobs.data_ptr(), order=self.order.data_ptr(), num_jobs=len(self.prog.jobs),
             steps=self.dp.steps.data_ptr(), leaves=self.dp.leaves.data_ptr(), arenas=self.arenas.data_ptr(),
@@ -173,6 +178,8 @@ class BatchExecutor:
             job_done=self.job_done.data_ptr() if self.job_done is not None else None,
         )
         N.check(L.ts_batch_run(ctypes.byref(b), N.stream_ptr()), "ts_batch_run")
+        if events is not None:
+            events[1].record()
 
     def run(self, mode: str = "GT0", early_exit: bool = True) -> np.ndarray:
         torch = _torch()
