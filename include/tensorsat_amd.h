@@ -130,12 +130,20 @@ typedef struct {
   int32_t* counter;         /* device scratch int32, zeroed by the caller */
   int32_t early_exit;       /* 1: skip jobs of instances already decided SAT */
   uint32_t* job_done;       /* device, per job (optional): 1 ran, 2 skipped */
+  const int32_t* num_jobs_dev; /* optional device count overriding num_jobs (compacted lists) */
 } ts_batch;
 
 /* One persistent launch: each workgroup takes the next job in `order`, copies its leaves
  * into its arena and runs every step with the workgroup, OR-ing the scalar result into
  * verdicts[instance]. */
 int ts_batch_run(const ts_batch* b, ts_stream stream);
+
+/* Keep the entries of order[0..n) whose instance is not yet decided SAT: they are written
+ * to out[0..*out_count) (*out_count must be zeroed by the caller; relative order within a
+ * wave is kept).  Used between dispatch rounds so the persistent kernel only walks live
+ * jobs. */
+int ts_batch_compact(const int32_t* order, int32_t n, const ts_job* jobs, const uint32_t* verdicts,
+                     int32_t* out, int32_t* out_count, ts_stream stream);
 
 /* Maximum resident workgroups of the batched kernel on the current device. */
 int ts_batch_max_workgroups(int32_t* out);

@@ -81,7 +81,8 @@ typedef struct {
   int32_t plan_trials;  /* elimination-order trials per cube */
   int32_t threads;      /* host threads (<= 0: hardware concurrency) */
   uint64_t seed;
-  int32_t counting;     /* reserved: cube enumeration is always complete unless capped */
+  int32_t cube_begin;   /* skip (do not compile) the first cube_begin cubes of each instance;
+                           max_cubes then counts cubes from cube_begin on */
 } tsc_sat_opts;
 
 #define TSC_NSTAT 8

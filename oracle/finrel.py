@@ -127,12 +127,13 @@ def contract2(lhs, lhs_wires, rhs, rhs_wires, out_wires=None, mode: str = "GT0")
     if set(out_wires) - set(lhs_wires) - set(rhs_wires):
         raise ValueError("Every output wire must appear in LHR or RHS.")
     if mode == "GT0":
+        if len(set(out_wires)) == len(out_wires):
+            cnt = _count_einsum(lhs, lhs_wires, rhs, rhs_wires, out_wires)
+            return (cnt > 0).astype(np.uint8)
         uniq = sorted(set(out_wires))
         cnt = _count_einsum(lhs, lhs_wires, rhs, rhs_wires, uniq)
         res = (cnt > 0).astype(np.uint8)
-        if len(uniq) != len(out_wires):
-            res = rewire_exact(res, [uniq.index(w) for w in out_wires]).astype(np.uint8)
-        return res
+        return rewire_exact(res, [uniq.index(w) for w in out_wires]).astype(np.uint8)
     if mode != "MOD256":
         raise ValueError(f"unknown mode {mode!r}")
     common = set(lhs_wires) & set(rhs_wires)

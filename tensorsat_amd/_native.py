@@ -31,6 +31,7 @@ EXPORTS = (
     "ts_init", "ts_version", "ts_contract2", "ts_rewire", "ts_spider", "ts_union", "ts_stack",
     "ts_select", "ts_as_scalar", "ts_diagonal", "ts_count_contract2", "ts_bytes_to_u64",
     "ts_program_run", "ts_program_count", "ts_batch_run", "ts_batch_max_workgroups",
+    "ts_batch_compact",
 )
 
 _lib = None
@@ -47,6 +48,7 @@ class Batch(ctypes.Structure):
         ("jobs", vp), ("order", vp), ("num_jobs", i32), ("steps", vp), ("leaves", vp),
         ("arenas", vp), ("arena_stride", i64), ("num_workgroups", i32), ("mode", i32),
         ("verdicts", vp), ("counter", vp), ("early_exit", i32), ("job_done", vp),
+        ("num_jobs_dev", vp),
     ]
 
 
@@ -74,6 +76,7 @@ def load() -> ctypes.CDLL:
         L.ts_program_run.argtypes = [vp, vp, vp, vp, i32, vp, vp, vp]
         L.ts_program_count.argtypes = [vp, vp, vp, vp, vp, vp, vp]
         L.ts_batch_run.argtypes = [P(Batch), vp]
+        L.ts_batch_compact.argtypes = [vp, i32, vp, vp, vp, vp, vp]
         L.ts_batch_max_workgroups.argtypes = [P(i32)]
         _lib = L
     return _lib

@@ -1,18 +1,26 @@
-// Batched persistent executor (gfx950): one launch walks a device-resident queue of
-// contraction jobs.  Each workgroup repeatedly takes the next job index from a global
-// counter, copies the job's leaves into its private arena, runs the job's steps with the
-// whole workgroup (one __syncthreads between steps) and ORs the scalar result into the
-// job's instance verdict.  With early exit, a job whose instance is already decided SAT
-// is skipped: the verdict of an instance is the OR over its jobs either way.
+// Batched persistent executor (gfx950): launches that walk a device-resident queue of
+// contraction jobs.
+//
+// Dispatch runs in rounds by cube index (host-chosen boundaries): round 0 holds the first
+// cube of every instance; before each later round, k_compact keeps only the jobs whose
+// instance is still undecided, and the persistent kernel reads that count from device
+// memory, so no host synchronisation is needed between rounds.  Inside a round every
+// workgroup takes one queue entry per atomic, re-checks the instance verdict (early
+// exit), stages the job's step records in LDS (one coalesced copy per 64 steps), copies
+// the leaves into its private arena, runs each step with all 256 threads and ORs the
+// scalar result into the instance verdict.  The verdict of an instance is the OR over
+// its jobs whether or not later jobs are skipped.
 //
 // This replaces, for many instances at once, the reference's per-instance Python loop
 // (tensorsat/contractions/cotengra.py:120-126 -> _autoray.einsum -> FinRel._contract2,
 // fin_rel.py:255-288) and the final bool() (Box.__bool__, diagrams.py:949-958).
 //
-// Every loop bound comes from the job/step records; the dispatch loop ends when the
-// counter passes num_jobs, which every workgroup reaches.
+// Every loop bound comes from the job/step records or the queue length; the dispatch loop
+// ends once the counter passes the queue length, which every workgroup reaches.
 
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "device.h"
 
@@ -20,35 +28,35 @@ using namespace tsd;
 
 namespace {
 
+constexpr int SREC = 64;  // step records staged in LDS at a time
+
+__device__ __forceinline__ bool decided(const uint32_t* verdicts, int instance) {
+  return __hip_atomic_load(&verdicts[instance], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+}
+
 __global__ __launch_bounds__(WG) void k_batch(ts_batch b) {
   __shared__ Lut L;
+  __shared__ ts_step s_steps[SREC];
   __shared__ int s_job;
-  __shared__ int s_skip;
   const int tid = threadIdx.x;
   uint8_t* __restrict__ arena = b.arenas + (int64_t)blockIdx.x * b.arena_stride;
+  const int num_jobs = b.num_jobs_dev ? *b.num_jobs_dev : b.num_jobs;
   for (;;) {
     if (tid == 0) {
       const int k = atomicAdd(b.counter, 1);
-      int j = k < b.num_jobs ? b.order[k] : -1;
-      int skip = 0;
-      if (j >= 0 && b.early_exit) {
-        const uint32_t v = __hip_atomic_load(&b.verdicts[b.jobs[j].instance], __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT);
-        skip = v != 0;
+      int j = k < num_jobs ? b.order[k] : -1;
+      if (j >= 0 && b.early_exit && decided(b.verdicts, b.jobs[j].instance)) {
+        if (b.job_done) b.job_done[j] = 2;
+        j = -2;  // skipped: take the next entry
       }
       s_job = j;
-      s_skip = skip;
     }
     __syncthreads();
     const int j = s_job;
-    const int skip = s_skip;
     __syncthreads();
-    if (j < 0) break;
+    if (j == -1) break;
+    if (j < 0) continue;
     const ts_job job = b.jobs[j];
-    if (skip) {
-      if (tid == 0 && b.job_done) b.job_done[j] = 2;
-      continue;
-    }
     // Leaves -> arena (16-byte aligned prefix).
     {
       const uint8_t* src = b.leaves + job.leaf_begin;
@@ -58,26 +66,54 @@ __global__ __launch_bounds__(WG) void k_batch(ts_batch b) {
         reinterpret_cast<uint4*>(arena)[i] = reinterpret_cast<const uint4*>(src)[i];
       for (int i = (n16 << 4) + tid; i < nbytes; i += WG) arena[i] = src[i];
     }
-    __syncthreads();
-    for (int s = 0; s < job.num_steps; ++s) {
-      const ts_step& st = b.steps[job.step_begin + s];
-      const bool need_c = !(st.flags & TS_STEP_KINNER) && st.sum_rank > 0;
-      build_lut(st, L, need_c, tid);
+    for (int s0 = 0; s0 < job.num_steps; s0 += SREC) {
+      const int ns = min(SREC, job.num_steps - s0);
+      // Stage up to 64 step records (128 bytes = 8 x uint4 each).
+      const uint4* src = reinterpret_cast<const uint4*>(b.steps + job.step_begin + s0);
+      for (int i = tid; i < ns * 8; i += WG) reinterpret_cast<uint4*>(s_steps)[i] = src[i];
       __syncthreads();
-      const uint8_t* rhs = arena + ((st.flags & TS_STEP_NORHS) ? st.lhs_off : st.rhs_off);
-      step_u8(st, L, arena + st.lhs_off, rhs, arena + st.out_off, (uint64_t)tid, (uint64_t)WG, b.mode);
-      __syncthreads();
+      for (int s = 0; s < ns; ++s) {
+        const ts_step& st = s_steps[s];
+        const bool need_c = !(st.flags & TS_STEP_KINNER) && st.sum_rank > 0;
+        build_lut(st, L, need_c, tid);
+        __syncthreads();
+        const uint8_t* rhs = arena + ((st.flags & TS_STEP_NORHS) ? st.lhs_off : st.rhs_off);
+        step_u8(st, L, arena + st.lhs_off, rhs, arena + st.out_off, (uint64_t)tid, (uint64_t)WG, b.mode);
+        __syncthreads();
+      }
     }
     if (tid == 0) {
       // Decision jobs end in a scalar; a non-scalar result counts as satisfiable when
       // any entry is set.
       uint32_t v = 0;
-      const int64_t n = int64_t(1) << job.result_rank;
-      for (int64_t i = 0; i < n && !v; ++i) v = arena[job.result_off + i] != 0;
+      const int64_t nr = int64_t(1) << job.result_rank;
+      for (int64_t i = 0; i < nr && !v; ++i) v = arena[job.result_off + i] != 0;
       if (v) atomicOr(&b.verdicts[job.instance], 1u);
       if (b.job_done) b.job_done[j] = 1;
     }
     __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(WG) void k_compact(const int32_t* __restrict__ order, int n,
+                                                const ts_job* __restrict__ jobs,
+                                                const uint32_t* __restrict__ verdicts,
+                                                int32_t* __restrict__ out, int32_t* __restrict__ count) {
+  const int lane = threadIdx.x & 63;
+  for (int base = blockIdx.x * WG; base < n; base += gridDim.x * WG) {
+    const int k = base + threadIdx.x;
+    int j = -1;
+    bool keep = false;
+    if (k < n) {
+      j = order[k];
+      keep = !decided(verdicts, jobs[j].instance);
+    }
+    const uint64_t mask = __ballot(keep);
+    const int cnt = __popcll(mask);
+    int first = 0;
+    if (lane == 0 && cnt) first = atomicAdd(count, cnt);
+    first = __shfl(first, 0);
+    if (keep) out[first + __popcll(mask & ((1ull << lane) - 1ull))] = j;
   }
 }
 
@@ -104,6 +140,16 @@ int ts_batch_run(const ts_batch* b, ts_stream stream) {
   if (b->mode != TS_MODE_GT0 && b->mode != TS_MODE_MOD256) return TS_EINVAL;
   if (b->num_jobs == 0) return TS_OK;
   hipLaunchKernelGGL(k_batch, dim3(b->num_workgroups), dim3(WG), 0, reinterpret_cast<hipStream_t>(stream), *b);
+  return hipGetLastError() == hipSuccess ? TS_OK : TS_EHIP;
+}
+
+int ts_batch_compact(const int32_t* order, int32_t n, const ts_job* jobs, const uint32_t* verdicts, int32_t* out,
+                     int32_t* out_count, ts_stream stream) {
+  if (n < 0 || (n > 0 && (!order || !jobs || !verdicts || !out || !out_count))) return TS_EINVAL;
+  if (n == 0) return TS_OK;
+  const int grid = (int)std::min<int64_t>((n + WG - 1) / WG, 4096);
+  hipLaunchKernelGGL(k_compact, dim3(grid), dim3(WG), 0, reinterpret_cast<hipStream_t>(stream), order, n, jobs,
+                     verdicts, out, out_count);
   return hipGetLastError() == hipSuccess ? TS_OK : TS_EHIP;
 }
 

@@ -75,6 +75,7 @@ struct Enumerator {
   int64_t nodes = 0, pruned = 0, empty_residual = 0;
   bool truncated = false;
   int emitted = 0;
+  int seen = 0;  // cubes reached in depth-first order (compiled or skipped)
   int max_rank = 0;
   uint64_t seed;
   std::vector<Job>* jobs_out;
@@ -154,6 +155,7 @@ struct Enumerator {
   }
 
   void emit(const std::vector<int>& rclauses_idx) {
+    if (seen++ < o.cube_begin) return;  // compiled by an earlier pass
     // Residual: unsatisfied clauses restricted to free variables.
     std::vector<int> vmap(f.n, -1);
     int nv = 0;
@@ -403,7 +405,7 @@ int tsc_sat_batch(int32_t num_inst, const int32_t* nvars, const int64_t* clause_
         r.stats[2] = (int32_t)e.pruned + (alive ? 0 : 1);
         r.stats[3] = e.truncated ? 1 : 0;
         r.stats[4] = e.max_rank;
-        r.stats[5] = r.jobs.empty() ? 0 : -1;
+        r.stats[5] = e.seen == 0 ? 0 : -1;  // no cube at all: every branch pruned (UNSAT)
         r.stats[7] = (int32_t)e.empty_residual;
       } catch (const std::invalid_argument&) {
         r.ok = false;
@@ -434,7 +436,7 @@ int tsc_sat_batch(int32_t num_inst, const int32_t* nvars, const int64_t* clause_
         Job& jb = res[i].jobs[c];
         ts_job j{};
         j.instance = i;
-        j.cube = (int32_t)c;
+        j.cube = (int32_t)(opts->cube_begin + (int)c);
         j.step_begin = (int32_t)prog->steps.size();
         j.num_steps = (int32_t)jb.steps.size();
         j.leaf_begin = (int64_t)prog->leaves.size();

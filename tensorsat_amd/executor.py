@@ -135,11 +135,20 @@ def dispatch_order(prog: Program) -> np.ndarray:
     return np.lexsort((-work, jobs["cube"])).astype(np.int32)
 
 
+ROUND_BOUNDS = (0, 1, 2, 4, 8, 16, 64, 1 << 30)
+"""Cube-index boundaries of the dispatch rounds: [0,1), [1,2), [2,4), ... ."""
+
+
 class BatchExecutor:
-    """All jobs of a program in one persistent launch (``ts_batch_run``)."""
+    """
+    All jobs of a program in a few persistent launches (``ts_batch_run``): one round per
+    range of cube indices; from the second round on, ``ts_batch_compact`` first drops the
+    jobs of instances already decided SAT, on the device.
+    """
 
     def __init__(self, prog: Program, *, num_workgroups: int | None = None, device: str = "cuda",
-                 order: np.ndarray | None = None, track_jobs: bool = False) -> None:
+                 order: np.ndarray | None = None, track_jobs: bool = False,
+                 round_bounds=ROUND_BOUNDS) -> None:
         torch = _torch()
         L = N.lib()
         self.dp = DeviceProgram(prog, device)
@@ -153,31 +162,58 @@ class BatchExecutor:
         max_arena = int(prog.jobs["arena_elems"].max()) if nj else 16
         self.arena_stride = (max_arena + 255) // 256 * 256
         self.arenas = torch.empty(self.num_workgroups * self.arena_stride, dtype=torch.uint8, device=device)
-        self.order = torch.from_numpy(dispatch_order(prog) if order is None else order.astype(np.int32)).to(device)
+        full = dispatch_order(prog) if order is None else np.asarray(order, dtype=np.int32)
+        cubes = prog.jobs["cube"][full] if nj else np.zeros(0, np.int32)
+        self.rounds: list[tuple[int, int]] = []
+        if order is not None:
+            self.rounds = [(0, len(full))]
+        else:
+            for lo, hi in zip(round_bounds[:-1], round_bounds[1:]):
+                a = int(np.searchsorted(cubes, lo, side="left"))
+                b = int(np.searchsorted(cubes, hi, side="left"))
+                if b > a:
+                    self.rounds.append((a, b))
+        self.order = torch.from_numpy(np.ascontiguousarray(full, dtype=np.int32)).to(device)
+        longest = max((b - a for a, b in self.rounds), default=1)
+        self.worklist = torch.empty(max(1, longest), dtype=torch.int32, device=device)
         self.verdicts = torch.zeros(max(1, prog.num_instances), dtype=torch.int32, device=device)
-        self.counter = torch.zeros(1, dtype=torch.int32, device=device)
+        # counters[2r] dispatch counter of round r, counters[2r+1] its compacted length
+        self.counters = torch.zeros(2 * max(1, len(self.rounds)), dtype=torch.int32, device=device)
         self.job_done = torch.zeros(max(1, nj), dtype=torch.int32, device=device) if track_jobs else None
 
     def launch(self, mode: str = "GT0", early_exit: bool = True, events=None) -> None:
         """
-        Enqueue one full run on the current stream (verdicts reset first).  ``events``
-        (start, end) are recorded on the same stream right around the persistent kernel.
+        Enqueue one full decision pass on the current stream (verdicts reset first).
+        ``events`` (start, end) are recorded right around the round kernels.
         """
         L = N.lib()
         self.verdicts.zero_()
-        self.counter.zero_()
+        self.counters.zero_()
         if self.job_done is not None:
             self.job_done.zero_()
         if events is not None:
             events[0].record()
-        b = N.Batch(
-            jobs=self.dp.jobs.data_ptr(), order=self.order.data_ptr(), num_jobs=len(self.prog.jobs),
-            steps=self.dp.steps.data_ptr(), leaves=self.dp.leaves.data_ptr(), arenas=self.arenas.data_ptr(),
-            arena_stride=self.arena_stride, num_workgroups=self.num_workgroups, mode=N.MODES[mode],
-            verdicts=self.verdicts.data_ptr(), counter=self.counter.data_ptr(), early_exit=int(early_exit),
-            job_done=self.job_done.data_ptr() if self.job_done is not None else None,
-        )
-        N.check(L.ts_batch_run(ctypes.byref(b), N.stream_ptr()), "ts_batch_run")
+        stream = N.stream_ptr()
+        cbase = self.counters.data_ptr()
+        for r, (a, b) in enumerate(self.rounds):
+            n = b - a
+            order_ptr = self.order.data_ptr() + 4 * a
+            num_dev = None
+            if r > 0 and early_exit:
+                N.check(L.ts_batch_compact(ctypes.c_void_p(order_ptr), n, N.dptr(self.dp.jobs),
+                                           N.dptr(self.verdicts), N.dptr(self.worklist),
+                                           ctypes.c_void_p(cbase + 4 * (2 * r + 1)), stream), "ts_batch_compact")
+                order_ptr = self.worklist.data_ptr()
+                num_dev = cbase + 4 * (2 * r + 1)
+            bt = N.Batch(
+                jobs=self.dp.jobs.data_ptr(), order=order_ptr, num_jobs=n,
+                steps=self.dp.steps.data_ptr(), leaves=self.dp.leaves.data_ptr(), arenas=self.arenas.data_ptr(),
+                arena_stride=self.arena_stride, num_workgroups=self.num_workgroups, mode=N.MODES[mode],
+                verdicts=self.verdicts.data_ptr(), counter=cbase + 4 * (2 * r), early_exit=int(early_exit),
+                job_done=self.job_done.data_ptr() if self.job_done is not None else None,
+                num_jobs_dev=num_dev,
+            )
+            N.check(L.ts_batch_run(ctypes.byref(bt), stream), "ts_batch_run")
         if events is not None:
             events[1].record()
 

@@ -53,7 +53,7 @@ STAT_NAMES = ("jobs", "branch_nodes", "pruned", "truncated", "max_rank", "host_v
 class _SatOpts(ctypes.Structure):
     _fields_ = [
         ("width", ctypes.c_int32), ("max_cubes", ctypes.c_int32), ("plan_trials", ctypes.c_int32),
-        ("threads", ctypes.c_int32), ("seed", ctypes.c_uint64), ("counting", ctypes.c_int32),
+        ("threads", ctypes.c_int32), ("seed", ctypes.c_uint64), ("cube_begin", ctypes.c_int32),
     ]
 
 
@@ -163,10 +163,13 @@ def sat_batch(
     plan_trials: int = 2,
     threads: int = 0,
     seed: int = 0,
+    cube_begin: int = 0,
 ) -> Program:
     """
     Conditioned-slicing front-end for a batch of CNF instances: one job per cube, in
-    depth-first order per instance (``tsc_sat_batch``).
+    depth-first order per instance (``tsc_sat_batch``).  ``cube_begin`` skips the first
+    cubes of each instance (compiled by an earlier pass) and ``max_cubes`` caps the jobs
+    compiled per instance from there; ``stats[:, 3]`` flags instances that were cut off.
     """
     nv = np.asarray([c.num_vars for c in cnfs], dtype=np.int32)
     clause_counts = np.asarray([len(c.clauses) for c in cnfs], dtype=np.int64)
@@ -176,7 +179,7 @@ def sat_batch(
     loff = np.zeros(int(coff[-1]) + 1, dtype=np.int64)
     loff[1:] = np.cumsum(lens)
     lits = np.fromiter((l for c in cnfs for cl in c.clauses for l in cl), dtype=np.int32, count=int(loff[-1]))
-    opts = _SatOpts(width, max_cubes, plan_trials, threads, seed, 0)
+    opts = _SatOpts(width, max_cubes, plan_trials, threads, seed, cube_begin)
     handle = ctypes.c_void_p()
     L = _lib()
     P = ctypes.POINTER
@@ -187,7 +190,7 @@ def sat_batch(
     )
     _plan._check(rc, "tsc_sat_batch")
     prog = _take(handle)
-    prog.meta.update(width=width, max_cubes=max_cubes, plan_trials=plan_trials)
+    prog.meta.update(width=width, max_cubes=max_cubes, plan_trials=plan_trials, cube_begin=cube_begin)
     return prog
 
 

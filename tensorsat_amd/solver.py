@@ -1,0 +1,108 @@
+"""
+Batched SAT decisions: conditioned slicing on the host, contraction on the device.
+
+:class:`BatchSolver` decides many CNF instances at once.  The host front-end
+(``program.sat_batch``) slices every instance into cubes of width <= W and compiles the
+first ``first_pass`` cubes of each into contraction jobs; the device runs them in
+persistent launches with early exit.  Instances still undecided whose cube list was cut
+off get a second pass over the rest of their cubes.  Both passes are prepared once
+(:meth:`prepare`), so :meth:`run` only launches kernels on device-resident jobs.
+
+The verdict of an instance is the OR over all its cubes; every pass keeps that OR, so
+the answer does not depend on the pass split.  This is the batched counterpart of the
+reference's per-instance ``bool(CotengraContraction(...).contract(diagram))``
+(``examples/test_sat_solver.py:5-9``).
+"""
+
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import program as PG
+from .executor import BatchExecutor
+from .sat import CNFInstance
+
+
+@dataclass
+class SolveStats:
+    plan_s: float = 0.0
+    pass1_jobs: int = 0
+    pass2_jobs: int = 0
+    pass2_instances: int = 0
+    host_unsat: int = 0
+    kernel_ms: list = field(default_factory=list)
+
+
+class BatchSolver:
+    def __init__(self, *, width: int = 10, first_pass: int = 4, mode: str = "GT0", threads: int = 0,
+                 plan_trials: int = 1, num_workgroups: int | None = None, track_jobs: bool = True) -> None:
+        self.width = width
+        self.first_pass = first_pass
+        self.mode = mode
+        self.threads = threads
+        self.plan_trials = plan_trials
+        self.num_workgroups = num_workgroups
+        self.track_jobs = track_jobs
+        self.stats = SolveStats()
+        self.progs: list[PG.Program] = []
+        self.execs: list[BatchExecutor] = []
+        self.subsets: list[np.ndarray] = []
+
+    def prepare(self, cnfs: list[CNFInstance]) -> None:
+        """Plan and compile both passes (runs the first pass once to select the second)."""
+        self.n = len(cnfs)
+        t0 = time.perf_counter()
+        p1 = PG.sat_batch(cnfs, width=self.width, max_cubes=self.first_pass, plan_trials=self.plan_trials,
+                          threads=self.threads)
+        self.stats.plan_s = time.perf_counter() - t0
+        self.stats.pass1_jobs = len(p1.jobs)
+        self.stats.host_unsat = int((p1.stats[:, 5] == 0).sum())
+        ex1 = BatchExecutor(p1, num_workgroups=self.num_workgroups, track_jobs=self.track_jobs)
+        v1 = ex1.run(self.mode)
+        self.progs = [p1]
+        self.execs = [ex1]
+        self.subsets = [np.arange(self.n)]
+        need = np.flatnonzero((v1 == 0) & (p1.stats[:, 3] == 1))
+        if need.size:
+            t0 = time.perf_counter()
+            p2 = PG.sat_batch([cnfs[i] for i in need], width=self.width, cube_begin=self.first_pass,
+                              plan_trials=self.plan_trials, threads=self.threads)
+            self.stats.plan_s += time.perf_counter() - t0
+            self.stats.pass2_jobs = len(p2.jobs)
+            self.stats.pass2_instances = int(need.size)
+            if len(p2.jobs):
+                self.progs.append(p2)
+                self.execs.append(BatchExecutor(p2, num_workgroups=self.num_workgroups,
+                                                track_jobs=self.track_jobs))
+                self.subsets.append(need)
+
+    def launch(self, events=None) -> None:
+        """Enqueue every pass on the current stream; ``events`` (start, end) bracket all
+        of them (the verdict/counter resets of each pass included)."""
+        if events is not None:
+            events[0].record()
+        for ex in self.execs:
+            ex.launch(self.mode)
+        if events is not None:
+            events[1].record()
+
+    def verdicts(self) -> np.ndarray:
+        out = np.zeros(self.n, dtype=np.uint8)
+        for ex, sub in zip(self.execs, self.subsets):
+            v = ex.verdicts.cpu().numpy()[: len(sub)].astype(np.uint8)
+            out[sub] |= v
+        return out
+
+    def run(self) -> np.ndarray:
+        import torch
+
+        self.launch()
+        torch.cuda.synchronize()
+        return self.verdicts()
+
+    def executed_jobs(self) -> list[np.ndarray]:
+        """Per pass: boolean mask of jobs that ran in the last launch (needs track_jobs)."""
+        return [ex.job_done.cpu().numpy()[: len(p.jobs)] == 1 for ex, p in zip(self.execs, self.progs)]
