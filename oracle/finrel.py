@@ -178,7 +178,7 @@ def rewire(t: np.ndarray, out_ports, mode: str = "MOD256") -> np.ndarray:
     if not all(0 <= p < t.ndim for p in out_ports):
         raise ValueError("Output ports must be valid for this box.")
     if len(set(out_ports)) == len(out_ports) == t.ndim:
-        return np.ascontiguousarray(np.transpose(t, out_ports))
+        return np.array(np.transpose(t, out_ports), order="C")  # keeps 0-d shapes
     if mode == "GT0":
         return (rewire_exact(t, out_ports) > 0).astype(np.uint8)
     discarded = set(range(t.ndim)) - set(out_ports)
@@ -217,7 +217,7 @@ def getitem(t: np.ndarray, idxs) -> np.ndarray:
     for idx in idxs:
         if isinstance(idx, slice) and idx != slice(None):
             raise NotImplementedError("Non-trivial slicing is not implemented.")
-    return np.ascontiguousarray(t[tuple(idxs)])
+    return np.array(t[tuple(idxs)], order="C")
 
 
 def as_scalar(t: np.ndarray) -> bool:

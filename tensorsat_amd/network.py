@@ -167,6 +167,7 @@ class Diagram:
     wiring: Wiring
     blocks: tuple = field(default=())
     leaf_kinds: tuple[str, ...] | None = None
+    _cnf_source: tuple | None = field(default=None, compare=False, repr=False)
 
     def __post_init__(self) -> None:
         if len(self.blocks) != self.wiring.num_slots:
@@ -191,6 +192,10 @@ class Diagram:
     def is_flat(self) -> bool:
         return True
 
+    def flatten(self) -> "Diagram":
+        """Flat already (``diagrams.py:1452-1481`` is the identity on flat diagrams)."""
+        return self
+
     def __repr__(self) -> str:
         w = self.wiring
         attrs = [f"{w.num_wires} wires", f"{w.num_slots} blocks"]
@@ -202,15 +207,25 @@ class Diagram:
 _INPUT_LEAF = {"0": ("bit_0", gates.bit_0), "1": ("bit_1", gates.bit_1), "?": ("bit_unk", gates.bit_unk)}
 
 
+def _spec(cnf: CNFInstance, inputs) -> str:
+    """Input selection: ``0``/``1``/``?`` leaves, ``_`` open, ``*`` summed without a leaf."""
+    if isinstance(inputs, str) and "*" in inputs:
+        if len(inputs) != cnf.num_vars or not all(b in "01?_*" for b in inputs):
+            raise ValueError("Input specification characters must be '0', '1', '?', '_' or '*'.")
+        return inputs
+    return cnf.normalize_inputs(inputs)
+
+
 def bintree_network(cnf: CNFInstance, inputs: str | int | None = None) -> Diagram:
     """
     ``(cnf.inputs(inputs) >> cnf.diagram()).flatten()`` as a flat diagram
     (``lib/sat.py:165-262``; composition ``diagrams.py:1592-1637``).
 
     ``'_'`` leaves a variable open: no input leaf, and its wire becomes an out wire,
-    as in contracting ``cnf.diagram()`` itself (its ports are the variables).
+    as in contracting ``cnf.diagram()`` itself (its ports are the variables); ``'*'``
+    sums it without a leaf (``inputs(..., discard_unk=True)``).
     """
-    spec = cnf.normalize_inputs(inputs)
+    spec = _spec(cnf, inputs)
     n = cnf.num_vars
     slot_wires: list[tuple[int, ...]] = []
     blocks: list[np.ndarray] = []
@@ -219,7 +234,7 @@ def bintree_network(cnf: CNFInstance, inputs: str | int | None = None) -> Diagra
     for i, b in enumerate(spec):
         if b == "_":
             out_wires.append(i)
-        else:
+        elif b != "*":
             kind, leaf = _INPUT_LEAF[b]
             slot_wires.append((i,))
             blocks.append(leaf)
@@ -273,7 +288,7 @@ def clause_network(cnf: CNFInstance, inputs: str | int | None = None) -> Diagram
     * ``'0'``/``'1'`` add a ``bit_0``/``bit_1`` leaf on the variable; ``'_'`` makes it an
       out wire.
     """
-    spec = cnf.normalize_inputs(inputs)
+    spec = _spec(cnf, inputs)
     n = cnf.num_vars
     slot_wires: list[tuple[int, ...]] = []
     blocks: list[np.ndarray] = []
@@ -308,8 +323,10 @@ def clause_network(cnf: CNFInstance, inputs: str | int | None = None) -> Diagram
     for i, b in enumerate(spec):
         if b == "_":
             out_wires.append(i)
+        elif b == "*" and used[i]:
+            continue
         elif b in "01" or not used[i]:
-            kind, leaf = _INPUT_LEAF[b]
+            kind, leaf = _INPUT_LEAF["?" if b == "*" else b]
             slot_wires.append((i,))
             blocks.append(leaf)
             kinds.append(kind)

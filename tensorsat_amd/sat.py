@@ -168,3 +168,59 @@ class CNFInstance:
         elif ks:
             attrs.append(f"k in [{min(ks)}..{max(ks)}]")
         return f"<CNFInstance: {', '.join(attrs)}>"
+
+
+class _Inputs:
+    """``cnf.inputs(x)``: an input selection waiting to be composed with ``cnf.diagram()``
+    (``lib/sat.py:165-206``; composition ``diagrams.py:1592-1637``)."""
+
+    __slots__ = ("cnf", "spec")
+
+    def __init__(self, cnf: CNFInstance, spec: str) -> None:
+        self.cnf = cnf
+        self.spec = spec
+
+    def __rshift__(self, other):
+        from .network import Diagram, sat_network
+
+        src = getattr(other, "_cnf_source", None) if isinstance(other, Diagram) else None
+        if src is None or src[0] != self.cnf:
+            raise ValueError("inputs(...) composes with the diagram of the same CNF instance")
+        return sat_network(self.cnf, self.spec, mode=src[1])
+
+    @property
+    def wiring(self):
+        from .network import Wiring
+
+        n = self.cnf.num_vars
+        slots = [(i,) for i, b in enumerate(self.spec) if b in "01?"]
+        return Wiring((2,) * n, slots, list(range(n)))
+
+
+def _cnf_diagram(self: CNFInstance, *, mode: str = "bintree"):
+    """``CNFInstance.diagram`` (``lib/sat.py:208-229``): every variable an open port."""
+    from .network import sat_network
+
+    d = sat_network(self, "_" * self.num_vars, mode=mode)
+    object.__setattr__(d, "_cnf_source", (self, mode))
+    return d
+
+
+def _cnf_inputs(self: CNFInstance, inputs: str | int | None, /, *, discard_unk: bool = False) -> _Inputs:
+    """``CNFInstance.inputs`` (``lib/sat.py:165-206``); ``discard_unk`` leaves unknown
+    variables as plain summed wires instead of ``bit_unk`` leaves."""
+    spec = self.normalize_inputs(inputs)
+    if discard_unk:
+        spec = spec.replace("?", "*")
+    return _Inputs(self, spec)
+
+
+def _contraction_wiring(self: CNFInstance, *, mode: str = "bintree"):
+    """``CNFInstance.contraction_wiring`` (``lib/sat.py:231-246``): the wiring shared by
+    every ``inputs(x) >> diagram()`` composition with 0/1/? inputs."""
+    return (self.inputs(0) >> self.diagram(mode=mode)).wiring
+
+
+CNFInstance.diagram = _cnf_diagram
+CNFInstance.inputs = _cnf_inputs
+CNFInstance.contraction_wiring = _contraction_wiring
