@@ -147,6 +147,12 @@ def main() -> int:
                       f"(oracle/executor.py) with early exit, {dt:.1f}s, single thread, {platform.processor() or 'host CPU'}",
         }
         log(f"cpu baseline {cpu['value']:.1f} inst/s on {n_done} instances; parity mismatches {mism}")
+        from oracle import dpll
+
+        dp = np.array([dpll.solve(c.num_vars, c.clauses) for c in cnfs], dtype=np.uint8)
+        parity["dpll_checked"] = len(cnfs)
+        parity["dpll_mismatches"] = int((dp != verdicts).sum())
+        log(f"independent DPLL cross-check: {parity['dpll_mismatches']} mismatches over {len(cnfs)}")
 
     out = {
         "metric": METRIC,
@@ -191,7 +197,7 @@ def main() -> int:
         "parity": parity,
     }
     print(json.dumps(out), flush=True)
-    return 0 if parity["mismatches"] == 0 else 1
+    return 0 if parity["mismatches"] == 0 and parity.get("dpll_mismatches", 0) == 0 else 1
 
 
 if __name__ == "__main__":
