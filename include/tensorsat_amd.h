@@ -138,6 +138,13 @@ typedef struct {
  * verdicts[instance]. */
 int ts_batch_run(const ts_batch* b, ts_stream stream);
 
+/* Counting variant (#SAT): arenas hold uint64 entries (arena_stride in bytes, >= 8 x the
+ * largest arena_elems), every job's scalar result times 2^free_vars is added to
+ * counts[instance] (zeroed by the caller); a product, sum or shift past 2^64 sets
+ * overflow[instance] to 1 (the count is then invalid).  mode/early_exit/verdicts are
+ * ignored. */
+int ts_batch_count(const ts_batch* b, uint64_t* counts, int32_t* overflow, ts_stream stream);
+
 /* Keep the entries of order[0..n) whose instance is not yet decided SAT: they are written
  * to out[0..*out_count) (*out_count must be zeroed by the caller; relative order within a
  * wave is kept).  Used between dispatch rounds so the persistent kernel only walks live

@@ -31,7 +31,7 @@ EXPORTS = (
     "ts_init", "ts_version", "ts_contract2", "ts_rewire", "ts_spider", "ts_union", "ts_stack",
     "ts_select", "ts_as_scalar", "ts_diagonal", "ts_count_contract2", "ts_bytes_to_u64",
     "ts_program_run", "ts_program_count", "ts_batch_run", "ts_batch_max_workgroups",
-    "ts_batch_compact",
+    "ts_batch_compact", "ts_batch_count",
 )
 
 _lib = None
@@ -77,6 +77,7 @@ def load() -> ctypes.CDLL:
         L.ts_program_count.argtypes = [vp, vp, vp, vp, vp, vp, vp]
         L.ts_batch_run.argtypes = [P(Batch), vp]
         L.ts_batch_compact.argtypes = [vp, i32, vp, vp, vp, vp, vp]
+        L.ts_batch_count.argtypes = [P(Batch), vp, vp, vp]
         L.ts_batch_max_workgroups.argtypes = [P(i32)]
         _lib = L
     return _lib

@@ -82,6 +82,7 @@ __global__ __launch_bounds__(WG) void k_batch(ts_batch b) {
         __syncthreads();
       }
     }
+    __syncthreads();  // leaves of step-less jobs were written by all threads
     if (tid == 0) {
       // Decision jobs end in a scalar; a non-scalar result counts as satisfiable when
       // any entry is set.
@@ -89,6 +90,73 @@ __global__ __launch_bounds__(WG) void k_batch(ts_batch b) {
       const int64_t nr = int64_t(1) << job.result_rank;
       for (int64_t i = 0; i < nr && !v; ++i) v = arena[job.result_off + i] != 0;
       if (v) atomicOr(&b.verdicts[job.instance], 1u);
+      if (b.job_done) b.job_done[j] = 1;
+    }
+    __syncthreads();
+  }
+}
+
+// Counting semiring: the same walk with uint64 arenas (arena_stride is in bytes) and
+// checked arithmetic; each job's scalar result times 2^free_vars is added to its
+// instance's count; any overflow sets the instance's flag.
+__global__ __launch_bounds__(WG) void k_batch_count(ts_batch b, unsigned long long* __restrict__ counts,
+                                                    int32_t* __restrict__ overflow) {
+  __shared__ Lut L;
+  __shared__ ts_step s_steps[SREC];
+  __shared__ int s_job;
+  __shared__ int s_ovf;
+  const int tid = threadIdx.x;
+  uint64_t* __restrict__ arena =
+      reinterpret_cast<uint64_t*>(b.arenas + (int64_t)blockIdx.x * b.arena_stride);
+  const int num_jobs = b.num_jobs_dev ? *b.num_jobs_dev : b.num_jobs;
+  for (;;) {
+    if (tid == 0) {
+      const int k = atomicAdd(b.counter, 1);
+      s_job = k < num_jobs ? b.order[k] : -1;
+      s_ovf = 0;
+    }
+    __syncthreads();
+    const int j = s_job;
+    if (j < 0) break;
+    const ts_job job = b.jobs[j];
+    {
+      const uint8_t* src = b.leaves + job.leaf_begin;
+      for (int i = tid; i < job.num_leaf_bytes; i += WG) arena[i] = src[i];
+    }
+    bool ovf = false;
+    for (int s0 = 0; s0 < job.num_steps; s0 += SREC) {
+      const int ns = min(SREC, job.num_steps - s0);
+      const uint4* src = reinterpret_cast<const uint4*>(b.steps + job.step_begin + s0);
+      for (int i = tid; i < ns * 8; i += WG) reinterpret_cast<uint4*>(s_steps)[i] = src[i];
+      __syncthreads();
+      for (int s = 0; s < ns; ++s) {
+        const ts_step& st = s_steps[s];
+        const bool need_c = !(st.flags & TS_STEP_KINNER) && st.sum_rank > 0;
+        build_lut(st, L, need_c, tid);
+        __syncthreads();
+        const uint64_t* rhs = arena + ((st.flags & TS_STEP_NORHS) ? st.lhs_off : st.rhs_off);
+        ovf |= step_u64(st, L, arena + st.lhs_off, rhs, arena + st.out_off, (uint64_t)tid, (uint64_t)WG);
+        __syncthreads();
+      }
+    }
+    if (ovf) atomicOr(&s_ovf, 1);
+    __syncthreads();
+    if (tid == 0) {
+      // Decision and counting jobs end in a scalar; a non-scalar result adds its entries.
+      unsigned long long total = 0;
+      bool bad = s_ovf != 0;
+      const int64_t nr = int64_t(1) << job.result_rank;
+      for (int64_t i = 0; i < nr; ++i) {
+        const unsigned long long v = arena[job.result_off + i];
+        if (total + v < total) bad = true;
+        total += v;
+      }
+      const int sh = job.free_vars;
+      if (sh >= 64 ? total != 0 : (sh > 0 && (total >> (64 - sh)) != 0)) bad = true;
+      const unsigned long long add = sh >= 64 ? 0ull : (total << sh);
+      const unsigned long long old = atomicAdd(&counts[job.instance], add);
+      if (old + add < old) bad = true;
+      if (bad) atomicOr(&overflow[job.instance], 1);
       if (b.job_done) b.job_done[j] = 1;
     }
     __syncthreads();
@@ -140,6 +208,16 @@ int ts_batch_run(const ts_batch* b, ts_stream stream) {
   if (b->mode != TS_MODE_GT0 && b->mode != TS_MODE_MOD256) return TS_EINVAL;
   if (b->num_jobs == 0) return TS_OK;
   hipLaunchKernelGGL(k_batch, dim3(b->num_workgroups), dim3(WG), 0, reinterpret_cast<hipStream_t>(stream), *b);
+  return hipGetLastError() == hipSuccess ? TS_OK : TS_EHIP;
+}
+
+int ts_batch_count(const ts_batch* b, uint64_t* counts, int32_t* overflow, ts_stream stream) {
+  if (!b || !counts || !overflow || b->num_jobs < 0 || b->num_workgroups <= 0 || !b->counter) return TS_EINVAL;
+  if (b->num_jobs > 0 && (!b->jobs || !b->order || !b->steps || !b->leaves || !b->arenas)) return TS_EINVAL;
+  if (b->arena_stride <= 0 || (b->arena_stride & 255)) return TS_EINVAL;
+  if (b->num_jobs == 0) return TS_OK;
+  hipLaunchKernelGGL(k_batch_count, dim3(b->num_workgroups), dim3(WG), 0, reinterpret_cast<hipStream_t>(stream),
+                     *b, reinterpret_cast<unsigned long long*>(counts), overflow);
   return hipGetLastError() == hipSuccess ? TS_OK : TS_EHIP;
 }
 

@@ -154,8 +154,11 @@ struct Enumerator {
     return propagate();
   }
 
-  void emit(const std::vector<int>& rclauses_idx) {
-    if (seen++ < o.cube_begin) return;  // compiled by an earlier pass
+  // Plan and compile the residual of the current node as one cube.  Returns false (the
+  // caller branches further) when the planned tree's largest tensor exceeds 2^width or a
+  // step does not fit a record.  Cubes before cube_begin are planned and counted but not
+  // kept, so every pass numbers the cubes identically.
+  bool emit(const std::vector<int>& rclauses_idx) {
     // Residual: unsatisfied clauses restricted to free variables.
     std::vector<int> vmap(f.n, -1);
     int nv = 0;
@@ -186,17 +189,21 @@ struct Enumerator {
     net.W = nv;
     net.is_out.assign(nv, 0);
     std::vector<int32_t> path;
-    elim_plan(net, o.plan_trials, seed + (uint64_t)emitted * 7919u, path, nullptr);
+    TreeCost tc;
+    elim_plan(net, o.plan_trials, seed + (uint64_t)seen * 7919u, path, &tc);
+    if (!rclauses_idx.empty() && tc.log2_max > std::min(o.width, TS_MAX_BITS)) return false;
     std::vector<const uint8_t*> leaves;
     for (auto& t : leaf_data) leaves.push_back(t.data());
     Job job;
-    if (compile_job(net, path, leaves, job) != TSP_OK) throw std::runtime_error("compile");
+    if (compile_job(net, path, leaves, job) != TSP_OK) return false;
+    if (seen++ < o.cube_begin) return true;  // kept by an earlier pass
     max_rank = std::max(max_rank, job.max_rank);
     jobs_out->push_back(std::move(job));
     free_out->push_back(free_vars);
     ++emitted;
     if (rclauses_idx.empty()) ++empty_residual;
     if (o.max_cubes > 0 && emitted >= o.max_cubes) truncated = true;
+    return true;
   }
 
   void search() {
@@ -224,7 +231,7 @@ struct Enumerator {
         else occ_neg[v]++;
       }
     }
-    bool leaf = (int)rv.size() <= o.width + 1;
+    bool leaf = (int)rv.size() <= o.width;
     if (!leaf && (int)rv.size() <= 4 * o.width + 8) {
       std::vector<int> vmap(f.n, -1);
       for (size_t i = 0; i < rv.size(); ++i) vmap[rv[i]] = (int)i;
@@ -238,12 +245,10 @@ struct Enumerator {
         }
         cl.push_back(std::move(ws));
       }
-      leaf = mindeg_width((int)rv.size(), cl) <= o.width;
+      // The estimate only filters; the planned tree decides (emit).
+      leaf = mindeg_width((int)rv.size(), cl) <= o.width + 1;
     }
-    if (leaf) {
-      emit(rc);
-      return;
-    }
+    if (leaf && emit(rc)) return;
     ++nodes;
     int bv = rv[0];
     for (int v : rv)

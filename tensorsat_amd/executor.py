@@ -222,3 +222,61 @@ class BatchExecutor:
         self.launch(mode, early_exit)
         torch.cuda.synchronize()
         return self.verdicts.cpu().numpy()[: self.prog.num_instances].astype(np.uint8)
+
+
+class BatchCounter:
+    """
+    Exact model counts of all jobs of a program in one persistent launch
+    (``ts_batch_count``): uint64 arenas, per-instance sums of result x 2^free_vars,
+    overflow flags raised instead of wrapping.
+    """
+
+    def __init__(self, prog: Program, *, num_workgroups: int | None = None, device: str = "cuda") -> None:
+        torch = _torch()
+        L = N.lib()
+        self.dp = DeviceProgram(prog, device)
+        self.prog = prog
+        nj = len(prog.jobs)
+        if num_workgroups is None:
+            mx = N.i32(0)
+            N.check(L.ts_batch_max_workgroups(ctypes.byref(mx)), "ts_batch_max_workgroups")
+            num_workgroups = int(mx.value)
+        self.num_workgroups = max(1, min(int(num_workgroups), max(1, nj)))
+        max_arena = int(prog.jobs["arena_elems"].max()) if nj else 16
+        self.arena_stride = (8 * max_arena + 255) // 256 * 256
+        self.arenas = torch.empty(self.num_workgroups * self.arena_stride, dtype=torch.uint8, device=device)
+        self.order = torch.from_numpy(dispatch_order(prog)).to(device)
+        n = max(1, prog.num_instances)
+        self.counts = torch.zeros(n, dtype=torch.int64, device=device)
+        self.overflow = torch.zeros(n, dtype=torch.int32, device=device)
+        self.counter = torch.zeros(1, dtype=torch.int32, device=device)
+        self.dummy = torch.zeros(n, dtype=torch.int32, device=device)
+
+    def launch(self, events=None) -> None:
+        L = N.lib()
+        self.counts.zero_()
+        self.overflow.zero_()
+        self.counter.zero_()
+        if events is not None:
+            events[0].record()
+        b = N.Batch(
+            jobs=self.dp.jobs.data_ptr(), order=self.order.data_ptr(), num_jobs=int(self.order.numel()),
+            steps=self.dp.steps.data_ptr(), leaves=self.dp.leaves.data_ptr(), arenas=self.arenas.data_ptr(),
+            arena_stride=self.arena_stride, num_workgroups=self.num_workgroups, mode=0,
+            verdicts=self.dummy.data_ptr(), counter=self.counter.data_ptr(), early_exit=0,
+            job_done=None, num_jobs_dev=None,
+        )
+        N.check(L.ts_batch_count(ctypes.byref(b), N.dptr(self.counts), N.dptr(self.overflow), N.stream_ptr()),
+                "ts_batch_count")
+        if events is not None:
+            events[1].record()
+
+    def run(self) -> list[int]:
+        torch = _torch()
+        self.launch()
+        torch.cuda.synchronize()
+        n = self.prog.num_instances
+        if n and int(self.overflow[:n].max().item()):
+            raise OverflowError("model count exceeds 2^64")
+        vals = self.counts.cpu().numpy()[:n].view(np.uint64)
+        return [int(v) for v in vals]
