@@ -155,6 +155,42 @@ int ts_batch_compact(const int32_t* order, int32_t n, const ts_job* jobs, const 
 /* Maximum resident workgroups of the batched kernel on the current device. */
 int ts_batch_max_workgroups(int32_t* out);
 
+/* ---- Learned planner: message-passing network kernels (bf16 in, fp32 accumulate) --
+ * No counterpart in the reference: the learned planner is an addition of this build.
+ * Matrices are row-major; bf16 buffers are passed as void*. */
+
+/* C (+)= A x B (+ bias) (ReLU): A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn] (bf16),
+ * C[m*ldc + n] fp32 or bf16 (out_bf16); v_mfma_f32_16x16x32_bf16 with fp32 accumulation. */
+int ts_gnn_gemm(const void* A, int64_t sam, int64_t sak, const void* B, int64_t sbk, int64_t sbn, void* C,
+                int64_t ldc, int32_t M, int32_t N, int32_t K, const float* bias, int32_t relu,
+                int32_t out_bf16, int32_t accumulate, ts_stream stream);
+
+/* Weighted signed neighbour sums over a CSR incidence (row r -> col[e], weight w[e] whose
+ * sign is the literal's): forward writes sum |w| X[col, xoff..] over positive and negative
+ * edges into bf16 column blocks off_pos / off_neg; with out_f32_accumulate both signs (read
+ * from xoff_pos / xoff_neg) add into the fp32 block off_pos (the transpose, for the
+ * backward pass, with the same symmetric weights).  F <= 128. */
+int ts_gnn_segsum(const int32_t* rowptr, const int32_t* col, const float* weight, int32_t nrows, const void* X,
+                  int64_t ldx, int32_t xoff_pos, int32_t xoff_neg, int32_t x_f32, void* out, int64_t ldo,
+                  int32_t off_pos, int32_t off_neg, int32_t F, int32_t out_f32_accumulate, ts_stream stream);
+
+/* dZ = dH * (H > 0) as bf16, column sums of dZ added to db (if non-NULL). */
+int ts_gnn_relu_back(const float* dH, int64_t ldd, const void* H, int64_t ldh, int32_t rows, int32_t F, void* dZ,
+                     int64_t ldz, float* db, ts_stream stream);
+
+/* Scalar head y = H w + b; in training also the masked squared error against target
+ * (added to *loss), dH = dy w^T (fp32 [rows, F]), dw, db. */
+int ts_gnn_head(const void* H, int64_t ldh, int32_t rows, const float* w, const float* b, const float* target,
+                const float* mask, float inv_count, float* y, float* loss, float* dH, float* dw, float* db,
+                int32_t F, int32_t train, ts_stream stream);
+
+/* Adam on fp32 master weights (bias-corrected; L2 term wd * p added to the gradient),
+ * refreshing the bf16 copy pbf (if non-NULL). */
+int ts_gnn_adam(float* p, const float* g, float* m, float* v, void* pbf, int64_t n, float lr, float b1, float b2,
+                float eps, int32_t step, float wd, ts_stream stream);
+
+int ts_gnn_cast_bf16(const float* x, void* y, int64_t n, ts_stream stream);
+
 #ifdef __cplusplus
 }
 #endif

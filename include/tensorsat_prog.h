@@ -83,7 +83,26 @@ typedef struct {
   uint64_t seed;
   int32_t cube_begin;   /* skip (do not compile) the first cube_begin cubes of each instance;
                            max_cubes then counts cubes from cube_begin on */
+  const float* priority;/* optional per-variable priorities, instances concatenated (NULL:
+                           plain heuristics).  Branching takes the free variable of highest
+                           priority; each cube's tree follows ascending priority as an
+                           elimination order, kept only if the exact cost model rates it no
+                           worse than the min-degree tree by more than gate_margin (log2
+                           flops); otherwise the min-degree tree is the fallback. */
+  float gate_margin;
+  int32_t max_nodes;    /* branch-node budget per instance (0: none); exceeding it stops the
+                           search with stats[3] = 2 */
+  int32_t stop_on_sat;  /* stop an instance at its first cube whose residual is empty
+                           (every clause satisfied): stats[5] = 1 */
 } tsc_sat_opts;
+
+/* Elimination order of the cheapest of `trials` min-fill/min-degree orders (labels for the
+ * learned planner).  order_out has num_wires entries (out wires last, unordered). */
+int tsp_elim_order(const tsp_network* net, int32_t trials, uint64_t seed, int32_t* order_out,
+                   tsp_cost* cost_out);
+
+/* Tree cost of an explicit elimination order (all num_wires wires, out wires ignored). */
+int tsp_order_cost(const tsp_network* net, const int32_t* order, int32_t* path_out, tsp_cost* cost_out);
 
 #define TSC_NSTAT 8
 /* Per-instance statistics copied out by tsc_program_copy:

@@ -31,7 +31,8 @@ EXPORTS = (
     "ts_init", "ts_version", "ts_contract2", "ts_rewire", "ts_spider", "ts_union", "ts_stack",
     "ts_select", "ts_as_scalar", "ts_diagonal", "ts_count_contract2", "ts_bytes_to_u64",
     "ts_program_run", "ts_program_count", "ts_batch_run", "ts_batch_max_workgroups",
-    "ts_batch_compact", "ts_batch_count",
+    "ts_batch_compact", "ts_batch_count", "ts_gnn_gemm", "ts_gnn_segsum", "ts_gnn_relu_back",
+    "ts_gnn_head", "ts_gnn_adam", "ts_gnn_cast_bf16",
 )
 
 _lib = None
@@ -78,6 +79,14 @@ def load() -> ctypes.CDLL:
         L.ts_batch_run.argtypes = [P(Batch), vp]
         L.ts_batch_compact.argtypes = [vp, i32, vp, vp, vp, vp, vp]
         L.ts_batch_count.argtypes = [P(Batch), vp, vp, vp]
+        i8 = ctypes.c_int64
+        f32 = ctypes.c_float
+        L.ts_gnn_gemm.argtypes = [vp, i8, i8, vp, i8, i8, vp, i8, i32, i32, i32, vp, i32, i32, i32, vp]
+        L.ts_gnn_segsum.argtypes = [vp, vp, vp, i32, vp, i8, i32, i32, i32, vp, i8, i32, i32, i32, i32, vp]
+        L.ts_gnn_relu_back.argtypes = [vp, i8, vp, i8, i32, i32, vp, i8, vp, vp]
+        L.ts_gnn_head.argtypes = [vp, i8, i32, vp, vp, vp, vp, f32, vp, vp, vp, vp, vp, i32, i32, vp]
+        L.ts_gnn_adam.argtypes = [vp, vp, vp, vp, vp, i8, f32, f32, f32, f32, i32, f32, vp]
+        L.ts_gnn_cast_bf16.argtypes = [vp, vp, i8, vp]
         L.ts_batch_max_workgroups.argtypes = [P(i32)]
         _lib = L
     return _lib

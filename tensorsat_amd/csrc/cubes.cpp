@@ -80,10 +80,13 @@ struct Enumerator {
   uint64_t seed;
   std::vector<Job>* jobs_out;
   std::vector<int32_t>* free_out;
+  const float* prio = nullptr;  // per-variable priorities of this instance (learned planner)
+  bool budget_hit = false, sat_found = false;
+  int64_t learned_trees = 0, fallback_trees = 0;
 
   Enumerator(const Formula& f_, const tsc_sat_opts& o_, uint64_t sd, std::vector<Job>* jobs,
-             std::vector<int32_t>* fr)
-      : f(f_), o(o_), seed(sd), jobs_out(jobs), free_out(fr) {
+             std::vector<int32_t>* fr, const float* pr)
+      : f(f_), o(o_), seed(sd), jobs_out(jobs), free_out(fr), prio(pr) {
     val.assign(f.n, 0);
     occ.assign(2 * f.n, {});
     for (size_t c = 0; c < f.clauses.size(); ++c)
@@ -190,7 +193,28 @@ struct Enumerator {
     net.is_out.assign(nv, 0);
     std::vector<int32_t> path;
     TreeCost tc;
-    elim_plan(net, o.plan_trials, seed + (uint64_t)seen * 7919u, path, &tc);
+    if (prio) {
+      // Learned proposal: eliminate the residual's variables by ascending priority;
+      // gate it against the min-degree tree with the exact cost model.
+      std::vector<int> rvars(nv);
+      for (int v = 0; v < f.n; ++v)
+        if (vmap[v] >= 0) rvars[vmap[v]] = v;
+      std::vector<int> order(nv);
+      for (int i = 0; i < nv; ++i) order[i] = i;
+      std::stable_sort(order.begin(), order.end(),
+                       [&](int a, int b) { return prio[rvars[a]] < prio[rvars[b]]; });
+      std::vector<int32_t> lpath, ppath;
+      TreeCost lc, pc;
+      order_plan(net, order, lpath, &lc);
+      elim_plan(net, 1, seed + (uint64_t)seen * 7919u, ppath, &pc);
+      const bool take = lc.log2_max <= pc.log2_max && lc.log2_flops <= pc.log2_flops + o.gate_margin;
+      path = take ? lpath : ppath;
+      tc = take ? lc : pc;
+      if (take) ++learned_trees;
+      else ++fallback_trees;
+    } else {
+      elim_plan(net, o.plan_trials, seed + (uint64_t)seen * 7919u, path, &tc);
+    }
     if (!rclauses_idx.empty() && tc.log2_max > std::min(o.width, TS_MAX_BITS)) return false;
     std::vector<const uint8_t*> leaves;
     for (auto& t : leaf_data) leaves.push_back(t.data());
@@ -201,7 +225,13 @@ struct Enumerator {
     jobs_out->push_back(std::move(job));
     free_out->push_back(free_vars);
     ++emitted;
-    if (rclauses_idx.empty()) ++empty_residual;
+    if (rclauses_idx.empty()) {
+      ++empty_residual;
+      if (o.stop_on_sat) {
+        sat_found = true;
+        truncated = true;
+      }
+    }
     if (o.max_cubes > 0 && emitted >= o.max_cubes) truncated = true;
     return true;
   }
@@ -249,10 +279,20 @@ struct Enumerator {
       leaf = mindeg_width((int)rv.size(), cl) <= o.width + 1;
     }
     if (leaf && emit(rc)) return;
+    if (o.max_nodes > 0 && nodes >= o.max_nodes) {
+      budget_hit = true;
+      truncated = true;
+      return;
+    }
     ++nodes;
     int bv = rv[0];
-    for (int v : rv)
-      if (score[v] > score[bv] || (score[v] == score[bv] && v < bv)) bv = v;
+    if (prio) {
+      for (int v : rv)
+        if (prio[v] > prio[bv] || (prio[v] == prio[bv] && v < bv)) bv = v;
+    } else {
+      for (int v : rv)
+        if (score[v] > score[bv] || (score[v] == score[bv] && v < bv)) bv = v;
+    }
     int first = occ_pos[bv] >= occ_neg[bv] ? bv + 1 : -(bv + 1);
     for (int l : {first, -first}) {
       size_t mark = trail.size();
@@ -310,6 +350,83 @@ int tsp_elim(const tsp_network* n, int32_t trials, uint64_t seed, int32_t* path_
     cost_out->log2_peak = tc.log2_peak;
     cost_out->num_slices = 1;
   }
+  return TSP_OK;
+}
+
+static int to_bitnet(const tsp_network* n, BitNet& net) {
+  if (!n || n->num_slots < 0 || n->num_wires < 0 || n->num_out < 0) return TSP_EINVAL;
+  net.S = n->num_slots;
+  net.W = n->num_wires;
+  net.is_out.assign(net.W, 0);
+  for (int i = 0; i < n->num_out; ++i) {
+    int w = n->out_wires[i];
+    if (w < 0 || w >= net.W) return TSP_EINVAL;
+    net.is_out[w] = 1;
+  }
+  for (int s = 0; s < net.S; ++s) {
+    std::vector<int> ws(n->slot_wires + n->slot_offsets[s], n->slot_wires + n->slot_offsets[s + 1]);
+    for (int w : ws)
+      if (w < 0 || w >= net.W) return TSP_EINVAL;
+    std::sort(ws.begin(), ws.end());
+    ws.erase(std::unique(ws.begin(), ws.end()), ws.end());
+    net.slot_wires.push_back(ws);
+  }
+  return TSP_OK;
+}
+
+static void fill_cost(const TreeCost& tc, tsp_cost* c) {
+  if (!c) return;
+  c->log2_flops = tc.log2_flops;
+  c->log2_write = tc.log2_write;
+  c->log2_max_size = tc.log2_max;
+  c->log2_peak = tc.log2_peak;
+  c->num_slices = 1;
+}
+
+int tsp_elim_order(const tsp_network* n, int32_t trials, uint64_t seed, int32_t* order_out, tsp_cost* cost_out) {
+  BitNet net;
+  int rc = to_bitnet(n, net);
+  if (rc) return rc;
+  if (!order_out) return TSP_EINVAL;
+  std::vector<int32_t> path;
+  std::vector<int> order;
+  TreeCost tc;
+  try {
+    elim_plan(net, trials, seed, path, &tc, &order);
+  } catch (const std::bad_alloc&) {
+    return TSP_ENOMEM;
+  }
+  std::vector<char> in(net.W, 0);
+  int k = 0;
+  for (int w : order) {
+    order_out[k++] = w;
+    in[w] = 1;
+  }
+  for (int w = 0; w < net.W; ++w)
+    if (!in[w]) order_out[k++] = w;
+  fill_cost(tc, cost_out);
+  return TSP_OK;
+}
+
+int tsp_order_cost(const tsp_network* n, const int32_t* order, int32_t* path_out, tsp_cost* cost_out) {
+  BitNet net;
+  int rc = to_bitnet(n, net);
+  if (rc) return rc;
+  if (!order) return TSP_EINVAL;
+  std::vector<int> ord;
+  for (int i = 0; i < net.W; ++i) {
+    if (order[i] < 0 || order[i] >= net.W) return TSP_EINVAL;
+    if (!net.is_out[order[i]]) ord.push_back(order[i]);
+  }
+  std::vector<int32_t> path;
+  TreeCost tc;
+  try {
+    order_plan(net, ord, path, &tc);
+  } catch (const std::bad_alloc&) {
+    return TSP_ENOMEM;
+  }
+  if (path_out && !path.empty()) std::memcpy(path_out, path.data(), path.size() * sizeof(int32_t));
+  fill_cost(tc, cost_out);
   return TSP_OK;
 }
 
@@ -389,6 +506,11 @@ int tsc_sat_batch(int32_t num_inst, const int32_t* nvars, const int64_t* clause_
     bool ok = true;
   };
   std::vector<PerInst> res(num_inst);
+  std::vector<int64_t> var_offset(num_inst + 1, 0);
+  for (int i = 0; i < num_inst; ++i) {
+    if (nvars[i] < 0) return TSP_EINVAL;
+    var_offset[i + 1] = var_offset[i] + nvars[i];
+  }
   std::atomic<int> next{0};
   std::atomic<int> bad{0};
   auto worker = [&]() {
@@ -402,15 +524,18 @@ int tsc_sat_batch(int32_t num_inst, const int32_t* nvars, const int64_t* clause_
         for (auto& c : f.clauses)
           for (int l : c)
             if (l == 0 || std::abs(l) > f.n) throw std::invalid_argument("literal");
-        Enumerator e(f, *opts, opts->seed + (uint64_t)i * 1000003u, &r.jobs, &r.free_vars);
+        const float* pr = opts->priority ? opts->priority + var_offset[i] : nullptr;
+        Enumerator e(f, *opts, opts->seed + (uint64_t)i * 1000003u, &r.jobs, &r.free_vars, pr);
         bool alive = !f.has_empty && e.initial_units();
         if (alive) e.search();
         r.stats[0] = (int32_t)r.jobs.size();
         r.stats[1] = (int32_t)e.nodes;
         r.stats[2] = (int32_t)e.pruned + (alive ? 0 : 1);
-        r.stats[3] = e.truncated ? 1 : 0;
+        r.stats[3] = e.budget_hit ? 2 : (e.truncated ? 1 : 0);
         r.stats[4] = e.max_rank;
-        r.stats[5] = e.seen == 0 ? 0 : -1;  // no cube at all: every branch pruned (UNSAT)
+        // host verdict: 0 every branch pruned (UNSAT), 1 an all-satisfied cube was reached
+        // with stop_on_sat, -1 undecided on the host
+        r.stats[5] = e.sat_found ? 1 : (e.seen == 0 && !e.budget_hit ? 0 : -1);
         r.stats[7] = (int32_t)e.empty_residual;
       } catch (const std::invalid_argument&) {
         r.ok = false;

@@ -1,0 +1,317 @@
+// Kernels of the learned planner's message-passing network (gfx950): bf16 operands,
+// fp32 accumulation.
+//
+//  * k_gemm: C = A x B (+ bias) (ReLU) with arbitrary strides, so one kernel covers the
+//    forward product X W and the backward products dX = dZ W^T and dW = X^T dZ.
+//    64x64 block tiles, 4 waves each computing 32x32 with 2x2 v_mfma_f32_16x16x32_bf16,
+//    K in steps of 32 staged through LDS.  Output fp32 or bf16, optionally accumulated.
+//  * k_segsum: out[r, off + f] = sum over incidences e of row r of |w[e]| X[col[e], f],
+//    positive and negative literals (the sign of w[e]) into separate column blocks: the
+//    normalised signed neighbour sums of the clause-variable incidence graph.  The edge
+//    weights are symmetric (1/sqrt(deg(c) deg(v))), so the transpose incidence with the
+//    same weights runs the backward pass.
+//  * elementwise: bias-gradient column sums, ReLU masks, the squared-error loss and its
+//    gradient, Adam updates on fp32 master weights with a bf16 copy.
+//
+// There is no counterpart in the reference (the north star adds the learned planner).
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+#include <stdint.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "../../include/tensorsat_amd.h"
+
+namespace {
+
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+constexpr int TM = 64, TN = 64, TK = 32;
+
+__device__ __forceinline__ float bf2f(__bf16 x) { return (float)x; }
+__device__ __forceinline__ __bf16 f2bf(float x) { return (__bf16)x; }
+
+// C[m, n] (+)= sum_k A(m, k) B(k, n); A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn],
+// C at C[m*ldc + n].  Bounds are checked per element.
+__global__ __launch_bounds__(256) void k_gemm(const __bf16* __restrict__ A, int64_t sam, int64_t sak,
+                                              const __bf16* __restrict__ B, int64_t sbk, int64_t sbn,
+                                              void* __restrict__ C, int64_t ldc, int M, int N, int K,
+                                              const float* __restrict__ bias, int relu, int out_bf16,
+                                              int accumulate, int kchunk) {
+  __shared__ __bf16 sA[TM][TK + 8];
+  __shared__ __bf16 sB[TN][TK + 8];  // stored transposed: sB[n][k]
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  // Split-K: block z covers k in [kb, ke); partial sums are added atomically (fp32 C).
+  const int kb = blockIdx.z * kchunk;
+  const int ke = min(K, kb + kchunk);
+  const bool a_kfast = sak == 1, b_kfast = sbk == 1;  // coalesce along the contiguous index
+  f4 acc[2][2];
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j) acc[i][j] = (f4){0.f, 0.f, 0.f, 0.f};
+  for (int k0 = kb; k0 < ke; k0 += TK) {
+    for (int e = tid; e < TM * TK; e += 256) {
+      const int r = a_kfast ? e / TK : e % TM, c = a_kfast ? e % TK : e / TM;
+      const int m = m0 + r, k = k0 + c;
+      sA[r][c] = (m < M && k < ke) ? A[(int64_t)m * sam + (int64_t)k * sak] : f2bf(0.f);
+    }
+    for (int e = tid; e < TN * TK; e += 256) {
+      const int r = b_kfast ? e / TK : e % TN, c = b_kfast ? e % TK : e / TN;  // r = n, c = k
+      const int n = n0 + r, k = k0 + c;
+      sB[r][c] = (n < N && k < ke) ? B[(int64_t)k * sbk + (int64_t)n * sbn] : f2bf(0.f);
+    }
+    __syncthreads();
+    // Lane l of a 16x16x32 tile holds row/col (l % 16) and k = 8*(l/16) .. +7 of both.
+    const int rr = lane & 15, kk = (lane >> 4) * 8;
+    bf8 a[2], b[2];
+    for (int i = 0; i < 2; ++i)
+      for (int t = 0; t < 8; ++t) a[i][t] = sA[wm + 16 * i + rr][kk + t];
+    for (int j = 0; j < 2; ++j)
+      for (int t = 0; t < 8; ++t) b[j][t] = sB[wn + 16 * j + rr][kk + t];
+    for (int i = 0; i < 2; ++i)
+      for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    __syncthreads();
+  }
+  // Accumulator layout: lane l holds rows 4*(l/16) + t (t = 0..3), column l % 16.
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j)
+      for (int t = 0; t < 4; ++t) {
+        const int m = m0 + wm + 16 * i + 4 * (lane >> 4) + t;
+        const int n = n0 + wn + 16 * j + (lane & 15);
+        if (m >= M || n >= N) continue;
+        float v = acc[i][j][t];
+        if (gridDim.z > 1) {  // split-K partial sum (fp32 output, no bias/ReLU)
+          atomicAdd(reinterpret_cast<float*>(C) + (int64_t)m * ldc + n, v);
+          continue;
+        }
+        if (bias) v += bias[n];
+        if (relu) v = v > 0.f ? v : 0.f;
+        if (out_bf16) {
+          __bf16* c = reinterpret_cast<__bf16*>(C) + (int64_t)m * ldc + n;
+          *c = f2bf(accumulate ? bf2f(*c) + v : v);
+        } else {
+          float* c = reinterpret_cast<float*>(C) + (int64_t)m * ldc + n;
+          *c = accumulate ? *c + v : v;
+        }
+      }
+}
+
+// One wave per row: lanes cover the F features (F <= 128, two per lane).  Positive and
+// negative incidences go to column blocks off_pos / off_neg of the row.
+__global__ __launch_bounds__(256) void k_segsum(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                                                const float* __restrict__ weight, int nrows,
+                                                const void* __restrict__ X, int64_t ldx, int xoff_pos,
+                                                int xoff_neg, int x_f32, void* __restrict__ out, int64_t ldo,
+                                                int off_pos, int off_neg, int F, int out_f32_accumulate) {
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (wave >= nrows) return;
+  const int r = wave;
+  float p0 = 0.f, p1 = 0.f, q0 = 0.f, q1 = 0.f;
+  const int f0 = lane, f1 = lane + 64;
+  for (int e = rowptr[r]; e < rowptr[r + 1]; ++e) {
+    const int c = col[e];
+    const float w = weight[e];
+    const bool pos = w > 0.f;
+    const float aw = fabsf(w);
+    const int64_t base = (int64_t)c * ldx + (pos ? xoff_pos : xoff_neg);
+    float v0 = 0.f, v1 = 0.f;
+    if (x_f32) {
+      const float* x = reinterpret_cast<const float*>(X) + base;
+      if (f0 < F) v0 = aw * x[f0];
+      if (f1 < F) v1 = aw * x[f1];
+    } else {
+      const __bf16* x = reinterpret_cast<const __bf16*>(X) + base;
+      if (f0 < F) v0 = aw * bf2f(x[f0]);
+      if (f1 < F) v1 = aw * bf2f(x[f1]);
+    }
+    if (pos) {
+      p0 += v0;
+      p1 += v1;
+    } else {
+      q0 += v0;
+      q1 += v1;
+    }
+  }
+  if (out_f32_accumulate) {
+    float* o = reinterpret_cast<float*>(out) + (int64_t)r * ldo;
+    if (f0 < F) o[off_pos + f0] += p0 + q0;
+    if (f1 < F) o[off_pos + f1] += p1 + q1;
+  } else {
+    __bf16* o = reinterpret_cast<__bf16*>(out) + (int64_t)r * ldo;
+    if (f0 < F) {
+      o[off_pos + f0] = f2bf(p0);
+      o[off_neg + f0] = f2bf(q0);
+    }
+    if (f1 < F) {
+      o[off_pos + f1] = f2bf(p1);
+      o[off_neg + f1] = f2bf(q1);
+    }
+  }
+}
+
+// dZ = dH * (H > 0) (ReLU backward), written as bf16 for the next products and fp32
+// column sums accumulated into db.
+__global__ __launch_bounds__(256) void k_relu_back(const float* __restrict__ dH, int64_t ldd,
+                                                   const __bf16* __restrict__ H, int64_t ldh, int rows, int F,
+                                                   __bf16* __restrict__ dZ, int64_t ldz, float* __restrict__ db) {
+  const int f = threadIdx.x % 128;
+  const int rgroup = (blockIdx.x * blockDim.x + threadIdx.x) / 128;
+  const int ngroups = (gridDim.x * blockDim.x) / 128;
+  float s = 0.f;
+  if (f < F) {
+    for (int r = rgroup; r < rows; r += ngroups) {
+      const float g = bf2f(H[(int64_t)r * ldh + f]) > 0.f ? dH[(int64_t)r * ldd + f] : 0.f;
+      dZ[(int64_t)r * ldz + f] = f2bf(g);
+      s += g;
+    }
+    if (db) atomicAdd(&db[f], s);
+  }
+}
+
+// y[r] = sum_f H[r, f] w[f] + b ; loss = sum_r mask[r] (y[r]-t[r])^2 / count ;
+// dy[r] = 2 mask[r] (y[r]-t[r]) / count ; dH[r, f] = dy[r] w[f] ; dw[f] += sum_r dy[r] H[r, f].
+__global__ __launch_bounds__(128) void k_head(const __bf16* __restrict__ H, int64_t ldh, int rows,
+                                              const float* __restrict__ w, const float* __restrict__ b,
+                                              const float* __restrict__ target, const float* __restrict__ mask,
+                                              float inv_count, float* __restrict__ y, float* __restrict__ loss,
+                                              float* __restrict__ dH, float* __restrict__ dw, float* __restrict__ db,
+                                              int F, int train) {
+  __shared__ float red[128];
+  const int f = threadIdx.x;
+  for (int r = blockIdx.x; r < rows; r += gridDim.x) {
+    const float h = f < F ? bf2f(H[(int64_t)r * ldh + f]) : 0.f;
+    red[f] = h * (f < F ? w[f] : 0.f);
+    __syncthreads();
+    for (int s = 64; s > 0; s >>= 1) {
+      if (f < s) red[f] += red[f + s];
+      __syncthreads();
+    }
+    const float yr = red[0] + b[0];
+    __syncthreads();
+    if (f == 0) y[r] = yr;
+    if (train) {
+      const float m = mask ? mask[r] : 1.f;
+      const float d = yr - target[r];
+      const float g = 2.f * m * d * inv_count;
+      if (f == 0) {
+        atomicAdd(loss, m * d * d * inv_count);
+        atomicAdd(db, g);
+      }
+      if (f < F) {
+        dH[(int64_t)r * F + f] = g * w[f];
+        atomicAdd(&dw[f], g * h);
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_adam(float* __restrict__ p, const float* __restrict__ g,
+                                              float* __restrict__ m, float* __restrict__ v,
+                                              __bf16* __restrict__ pbf, int64_t n, float lr, float b1, float b2,
+                                              float eps, float c1, float c2, float wd) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const float gi = g[i] + wd * p[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float upd = lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    p[i] -= upd;
+    if (pbf) pbf[i] = f2bf(p[i]);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_cast_bf16(const float* __restrict__ x, __bf16* __restrict__ y, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = f2bf(x[i]);
+}
+
+inline int blocks_for(int64_t n, int per) { return (int)std::min<int64_t>((n + per - 1) / per, 65535); }
+
+}  // namespace
+
+extern "C" {
+
+int ts_gnn_gemm(const void* A, int64_t sam, int64_t sak, const void* B, int64_t sbk, int64_t sbn, void* C,
+                int64_t ldc, int32_t M, int32_t N, int32_t K, const float* bias, int32_t relu, int32_t out_bf16,
+                int32_t accumulate, ts_stream stream) {
+  if (!A || !B || !C || M < 0 || N < 0 || K < 0) return TS_EINVAL;
+  if (M == 0 || N == 0) return TS_OK;
+  dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM, 1);
+  if (grid.y > 65535) return TS_EUNSUPPORTED;
+  // Few output tiles over a long K (weight gradients X^T dZ): split K across blocks and
+  // add fp32 partial sums into C, which the caller has zeroed (requires accumulate).
+  int kchunk = K > 0 ? K : 1;
+  const int tiles = (int)(grid.x * grid.y);
+  if (tiles < 256 && K >= 4096 && !out_bf16 && accumulate && !bias && !relu) {
+    int splits = std::min(256 / tiles + 1, (K + 2047) / 2048);
+    kchunk = ((K + splits - 1) / splits + TK - 1) / TK * TK;
+    grid.z = (K + kchunk - 1) / kchunk;
+  }
+  hipLaunchKernelGGL(k_gemm, grid, dim3(256), 0, reinterpret_cast<hipStream_t>(stream),
+                     reinterpret_cast<const __bf16*>(A), sam, sak, reinterpret_cast<const __bf16*>(B), sbk, sbn, C,
+                     ldc, M, N, K, bias, relu, out_bf16, accumulate, kchunk);
+  return hipGetLastError() == hipSuccess ? TS_OK : TS_EHIP;
+}
+
+int ts_gnn_segsum(const int32_t* rowptr, const int32_t* col, const float* weight, int32_t nrows, const void* X,
+                  int64_t ldx, int32_t xoff_pos, int32_t xoff_neg, int32_t x_f32, void* out, int64_t ldo,
+                  int32_t off_pos, int32_t off_neg, int32_t F, int32_t out_f32_accumulate, ts_stream stream) {
+  if (nrows < 0 || F < 0 || F > 128) return TS_EINVAL;
+  if (nrows == 0) return TS_OK;
+  if (!rowptr || !col || !weight || !X || !out) return TS_EINVAL;
+  const int threads = 256;
+  const int64_t waves = nrows;
+  hipLaunchKernelGGL(k_segsum, dim3(blocks_for(waves * 64, threads)), dim3(threads), 0,
+                     reinterpret_cast<hipStream_t>(stream), rowptr, col, weight, nrows, X, ldx, xoff_pos, xoff_neg,
+                     x_f32, out, ldo, off_pos, off_neg, F, out_f32_accumulate);
+  return hipGetLastError() == hipSuccess ? TS_OK : TS_EHIP;
+}
+
+int ts_gnn_relu_back(const float* dH, int64_t ldd, const void* H, int64_t ldh, int32_t rows, int32_t F, void* dZ,
+                     int64_t ldz, float* db, ts_stream stream) {
+  if (!dH || !H || !dZ || rows < 0 || F < 0 || F > 128) return TS_EINVAL;
+  if (rows == 0) return TS_OK;
+  const int grid = std::min(1024, std::max(1, rows / 16));
+  hipLaunchKernelGGL(k_relu_back, dim3(grid), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), dH, ldd,
+                     reinterpret_cast<const __bf16*>(H), ldh, rows, F, reinterpret_cast<__bf16*>(dZ), ldz, db);
+  return hipGetLastError() == hipSuccess ? TS_OK : TS_EHIP;
+}
+
+int ts_gnn_head(const void* H, int64_t ldh, int32_t rows, const float* w, const float* b, const float* target,
+                const float* mask, float inv_count, float* y, float* loss, float* dH, float* dw, float* db,
+                int32_t F, int32_t train, ts_stream stream) {
+  if (!H || !w || !b || !y || rows < 0 || F < 0 || F > 128) return TS_EINVAL;
+  if (train && (!target || !loss || !dH || !dw || !db)) return TS_EINVAL;
+  if (rows == 0) return TS_OK;
+  hipLaunchKernelGGL(k_head, dim3(std::min(rows, 8192)), dim3(128), 0, reinterpret_cast<hipStream_t>(stream),
+                     reinterpret_cast<const __bf16*>(H), ldh, rows, w, b, target, mask, inv_count, y, loss, dH, dw,
+                     db, F, train);
+  return hipGetLastError() == hipSuccess ? TS_OK : TS_EHIP;
+}
+
+int ts_gnn_adam(float* p, const float* g, float* m, float* v, void* pbf, int64_t n, float lr, float b1, float b2,
+                float eps, int32_t step, float wd, ts_stream stream) {
+  if (!p || !g || !m || !v || n < 0 || step < 1) return TS_EINVAL;
+  if (n == 0) return TS_OK;
+  const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
+  hipLaunchKernelGGL(k_adam, dim3(blocks_for(n, 256)), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), p, g, m,
+                     v, reinterpret_cast<__bf16*>(pbf), n, lr, b1, b2, eps, c1, c2, wd);
+  return hipGetLastError() == hipSuccess ? TS_OK : TS_EHIP;
+}
+
+int ts_gnn_cast_bf16(const float* x, void* y, int64_t n, ts_stream stream) {
+  if (!x || !y || n < 0) return TS_EINVAL;
+  if (n == 0) return TS_OK;
+  hipLaunchKernelGGL(k_cast_bf16, dim3(blocks_for(n, 256)), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), x,
+                     reinterpret_cast<__bf16*>(y), n);
+  return hipGetLastError() == hipSuccess ? TS_OK : TS_EHIP;
+}
+
+}  // extern "C"

@@ -33,7 +33,11 @@ TreeCost tree_cost(const BitNet& net, const std::vector<int32_t>& path);
 // (trial 0 deterministic min-fill), each turned into a tree by merging the tensors of
 // each eliminated wire smallest-first.  Returns the cheapest (max size, then flops).
 void elim_plan(const BitNet& net, int trials, uint64_t seed, std::vector<int32_t>& path,
-               TreeCost* cost);
+               TreeCost* cost, std::vector<int>* best_order = nullptr);
+
+// Tree and cost of a given elimination order (wires not listed are merged at the end).
+void order_plan(const BitNet& net, const std::vector<int>& order, std::vector<int32_t>& path,
+                TreeCost* cost);
 
 // Width estimate (min-degree heuristic) of the primal graph of a clause list over
 // variables 0..nv-1.

@@ -304,11 +304,23 @@ bool cost_less(const TreeCost& a, const TreeCost& b) {
 
 }  // namespace
 
+void order_plan(const BitNet& net, const std::vector<int>& order, std::vector<int32_t>& path,
+                TreeCost* cost) {
+  order_to_path(net, order, path);
+  if (cost) *cost = tree_cost(net, path);
+}
+
 void elim_plan(const BitNet& net, int trials, uint64_t seed, std::vector<int32_t>& path,
-               TreeCost* cost) {
+               TreeCost* cost, std::vector<int>* best_order) {
   path.clear();
+  if (best_order) best_order->clear();
   if (net.S <= 1) {
     if (cost) *cost = tree_cost(net, path);
+    if (best_order) {
+      std::vector<char> keep(net.W, 0);
+      for (int w = 0; w < net.W; ++w) keep[w] = net.is_out[w];
+      elim_order(net.W, net.slot_wires, keep, false, nullptr, *best_order);
+    }
     return;
   }
   std::vector<char> keep(net.W, 0);
@@ -328,6 +340,7 @@ void elim_plan(const BitNet& net, int trials, uint64_t seed, std::vector<int32_t
       best = c;
       path = cur;
       have = true;
+      if (best_order) *best_order = order;
     }
   }
   if (cost) *cost = best;

@@ -54,6 +54,8 @@ class _SatOpts(ctypes.Structure):
     _fields_ = [
         ("width", ctypes.c_int32), ("max_cubes", ctypes.c_int32), ("plan_trials", ctypes.c_int32),
         ("threads", ctypes.c_int32), ("seed", ctypes.c_uint64), ("cube_begin", ctypes.c_int32),
+        ("priority", ctypes.c_void_p), ("gate_margin", ctypes.c_float), ("max_nodes", ctypes.c_int32),
+        ("stop_on_sat", ctypes.c_int32),
     ]
 
 
@@ -74,7 +76,11 @@ def _lib():
         L.tsc_program_free.argtypes = [vp]
         L.tsc_program_free.restype = None
         L.tsp_elim.argtypes = [P(_plan._Net), ctypes.c_int32, ctypes.c_uint64, P(ctypes.c_int32), P(_plan._Cost)]
-        for f in (L.tsc_compile, L.tsc_sat_batch, L.tsc_program_sizes, L.tsc_program_copy, L.tsp_elim):
+        L.tsp_elim_order.argtypes = [P(_plan._Net), ctypes.c_int32, ctypes.c_uint64, P(ctypes.c_int32),
+                                     P(_plan._Cost)]
+        L.tsp_order_cost.argtypes = [P(_plan._Net), P(ctypes.c_int32), P(ctypes.c_int32), P(_plan._Cost)]
+        for f in (L.tsc_compile, L.tsc_sat_batch, L.tsc_program_sizes, L.tsc_program_copy, L.tsp_elim,
+                  L.tsp_elim_order, L.tsp_order_cost):
             f.restype = ctypes.c_int
         _bound = True
     return L
@@ -131,6 +137,33 @@ def elim(wiring: Wiring, *, trials: int = 8, seed: int = 0) -> _plan.Plan:
     return _plan.Plan(S, path.reshape(-1, 2), cost=_plan.Cost._from(c))
 
 
+def elim_order(wiring: Wiring, *, trials: int = 8, seed: int = 0) -> tuple[np.ndarray, _plan.Cost]:
+    """The cheapest of ``trials`` min-fill/min-degree elimination orders and its tree cost."""
+    L = _lib()
+    h = _plan._NetHolder(wiring)
+    order = np.zeros(max(1, wiring.num_wires), dtype=np.int32)
+    c = _plan._Cost()
+    _plan._check(L.tsp_elim_order(ctypes.byref(h.net), trials, seed, _plan._ptr(order), ctypes.byref(c)),
+                 "tsp_elim_order")
+    return order[: wiring.num_wires], _plan.Cost._from(c)
+
+
+def order_cost(wiring: Wiring, order) -> tuple[_plan.Plan, _plan.Cost]:
+    """Tree and exact cost of an explicit elimination order over all wires."""
+    L = _lib()
+    h = _plan._NetHolder(wiring)
+    order = np.ascontiguousarray(np.asarray(order, dtype=np.int32))
+    if order.shape != (wiring.num_wires,) or sorted(order.tolist()) != list(range(wiring.num_wires)):
+        raise ValueError("order must be a permutation of the wires")
+    S = wiring.num_slots
+    path = np.zeros(max(1, 2 * (S - 1)), dtype=np.int32)
+    c = _plan._Cost()
+    _plan._check(L.tsp_order_cost(ctypes.byref(h.net), _plan._ptr(order), _plan._ptr(path), ctypes.byref(c)),
+                 "tsp_order_cost")
+    cost = _plan.Cost._from(c)
+    return _plan.Plan(S, path[: max(0, 2 * (S - 1))].reshape(-1, 2), cost=cost), cost
+
+
 def compile_network(wiring: Wiring, leaves, plan: _plan.Plan) -> Program:
     """
     Compile a bit network (every wire extent 2, unique wires per slot, unique out wires)
@@ -164,6 +197,10 @@ def sat_batch(
     threads: int = 0,
     seed: int = 0,
     cube_begin: int = 0,
+    priority: np.ndarray | None = None,
+    gate_margin: float = 0.0,
+    max_nodes: int = 0,
+    stop_on_sat: bool = False,
 ) -> Program:
     """
     Conditioned-slicing front-end for a batch of CNF instances: one job per cube, in
@@ -179,7 +216,13 @@ def sat_batch(
     loff = np.zeros(int(coff[-1]) + 1, dtype=np.int64)
     loff[1:] = np.cumsum(lens)
     lits = np.fromiter((l for c in cnfs for cl in c.clauses for l in cl), dtype=np.int32, count=int(loff[-1]))
-    opts = _SatOpts(width, max_cubes, plan_trials, threads, seed, cube_begin)
+    prio = None
+    if priority is not None:
+        prio = np.ascontiguousarray(np.asarray(priority, dtype=np.float32))
+        if prio.shape != (int(nv.sum()),):
+            raise ValueError("priority needs one value per variable of every instance")
+    opts = _SatOpts(width, max_cubes, plan_trials, threads, seed, cube_begin,
+                    prio.ctypes.data if prio is not None else None, gate_margin, max_nodes, int(stop_on_sat))
     handle = ctypes.c_void_p()
     L = _lib()
     P = ctypes.POINTER
