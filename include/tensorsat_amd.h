@@ -81,6 +81,14 @@ int ts_select(const uint8_t* in, int32_t in_rank, const int32_t* fixed, uint8_t*
 /* FinRel._as_scalar (fin_rel.py:472-473): all(tensor == 1) over n bytes; synchronous. */
 int ts_as_scalar(const uint8_t* x, int64_t n, int32_t* result, ts_stream stream);
 
+/* Routing of dense steps (ts_contract2 and ts_program_run) to the int8 MFMA kernel
+ * (v_mfma_i32_16x16x64_i8, exact int32 counts): a step goes there when its summed wires
+ * are innermost in both operands, number at least max(6, min_sum_bits), and at least 4
+ * out wires belong to lhs only and 4 to rhs only; other steps run on the VALU path.
+ * Default: enabled, min_sum_bits = 6.  Results are identical on both paths; the switch
+ * exists to time one against the other.  Not thread-safe (process-wide setting). */
+int ts_dense_config(int32_t enable, int32_t min_sum_bits);
+
 /* Diagonal extraction for operands with repeated wires: out axis j of rank out_rank;
  * in axis i takes the bit of out axis axis_map[i] (several in axes may share one). */
 int ts_diagonal(const uint8_t* in, int32_t in_rank, const int32_t* axis_map, uint8_t* out,

@@ -32,7 +32,7 @@ EXPORTS = (
     "ts_select", "ts_as_scalar", "ts_diagonal", "ts_count_contract2", "ts_bytes_to_u64",
     "ts_program_run", "ts_program_count", "ts_batch_run", "ts_batch_max_workgroups",
     "ts_batch_compact", "ts_batch_count", "ts_gnn_gemm", "ts_gnn_segsum", "ts_gnn_relu_back",
-    "ts_gnn_head", "ts_gnn_adam", "ts_gnn_cast_bf16",
+    "ts_gnn_head", "ts_gnn_adam", "ts_gnn_cast_bf16", "ts_dense_config",
 )
 
 _lib = None
@@ -88,6 +88,7 @@ def load() -> ctypes.CDLL:
         L.ts_gnn_adam.argtypes = [vp, vp, vp, vp, vp, i8, f32, f32, f32, f32, i32, f32, vp]
         L.ts_gnn_cast_bf16.argtypes = [vp, vp, i8, vp]
         L.ts_batch_max_workgroups.argtypes = [P(i32)]
+        L.ts_dense_config.argtypes = [i32, i32]
         _lib = L
     return _lib
 

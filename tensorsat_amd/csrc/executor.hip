@@ -12,6 +12,7 @@
 #include <cstring>
 #include <vector>
 
+#include "dense.h"
 #include "device.h"
 
 using namespace tsd;
@@ -245,6 +246,7 @@ int make_step(int rl, const int32_t* lw, int rr, const int32_t* rw, int ro, cons
 
 int launch_step_u8(const uint8_t* lhs, const uint8_t* rhs, uint8_t* out, const ts_step& st, int mode,
                    hipStream_t s) {
+  if (tsd::dense_eligible(st, lhs, rhs)) return tsd::launch_dense(lhs, rhs, out, st, mode, s);
   hipLaunchKernelGGL(k_step_u8, dim3(grid_for(1ull << st.out_rank)), dim3(WG), 0, s, lhs, rhs, out, st, mode);
   return hip_rc(hipGetLastError());
 }
