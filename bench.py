@@ -2,6 +2,9 @@
 """
 bench.py — batched random 3-SAT decisions on one MI355X.
 
+``--workload dense`` instead measures one dense Boolean composition step on the int8 MFMA
+route (see ``bench_dense``).
+
 Workload: 4096 random 3-SAT instances (instance i from default_rng(10000 + i),
 n ~ U{20..80}, clause ratio 4.26 for even i and 3.5 for odd i).  The host front-end
 slices each instance into conditioned sub-networks (cubes) of width <= W and compiles
@@ -83,12 +86,18 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workgroups", type=int, default=0)
     ap.add_argument("--threads", type=int, default=0, help="host planning threads")
+    ap.add_argument("--workload", default="batched", choices=["batched", "dense"])
+    ap.add_argument("--dense-bits", type=int, nargs=3, default=[13, 13, 13], metavar=("M", "N", "K"),
+                    help="dense workload: log2 rows, columns and contracted extent")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(instances and cube slices would shard across GPUs with one final OR).", file=sys.stderr)
         return 2
+
+    if args.workload == "dense":
+        return bench_dense(args)
 
     import torch
 
@@ -198,6 +207,112 @@ def main() -> int:
     }
     print(json.dumps(out), flush=True)
     return 0 if parity["mismatches"] == 0 and parity.get("dpll_mismatches", 0) == 0 else 1
+
+
+def bench_dense(args) -> int:
+    """
+    One dense Boolean step: the relational composition out[m, n] = OR_k L[m, k] & R[n, k]
+    with 2^M rows, 2^N columns and 2^K contracted entries, in the compiled layout (summed
+    wires innermost in both operands), routed to the int8 MFMA kernel.  Operand density
+    2^(-K/2) keeps both output values frequent.  Algorithmic work: 2 * 2^(M+N+K) int8
+    ops; bytes 2^(M+K) + 2^(N+K) + 2^(M+N).  The VALU route is timed on the same step, and
+    rows of the result are checked against exact CPU counts.
+    """
+    import torch
+
+    from tensorsat_amd import _native as N
+
+    def log(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    mb, nb, kb = args.dense_bits
+    K = list(range(kb))[::-1]
+    lw = [100 + i for i in range(mb)] + K
+    rw = [200 + i for i in range(nb)] + K
+    ow = [100 + i for i in range(mb)] + [200 + i for i in range(nb)]
+    g = torch.Generator(device="cuda").manual_seed(1)
+    dens = 2.0 ** (-kb / 2)
+    lhs = (torch.rand(1 << (mb + kb), device="cuda", generator=g) < dens).to(torch.uint8)
+    rhs = (torch.rand(1 << (nb + kb), device="cuda", generator=g) < dens).to(torch.uint8)
+    out = torch.empty(1 << (mb + nb), dtype=torch.uint8, device="cuda")
+    mode = N.MODES[args.mode]
+    L = N.lib()
+
+    def step():
+        N.check(L.ts_contract2(N.dptr(lhs), len(lw), N.i32arr(lw), N.dptr(rhs), len(rw), N.i32arr(rw), N.dptr(out),
+                               len(ow), N.i32arr(ow), mode, N.stream_ptr()), "ts_contract2")
+
+    def timed(steps):
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        a.record()
+        for _ in range(steps):
+            step()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / steps, (time.perf_counter() - t0) * 1000.0 / steps
+
+    kernel_ms, ms_per_step = timed(args.steps)
+    res = out.cpu().numpy().reshape(1 << mb, 1 << nb)
+    N.check(L.ts_dense_config(0, 6), "ts_dense_config")
+    try:
+        valu_ms, _ = timed(1)
+        valu = out.cpu().numpy().reshape(1 << mb, 1 << nb)
+    finally:
+        N.check(L.ts_dense_config(1, 6), "ts_dense_config")
+    # Exact counts for a sample of rows (float32 products of 0/1 are exact below 2^24).
+    rows = np.random.default_rng(0).choice(1 << mb, size=min(64, 1 << mb), replace=False)
+    Lh = lhs.cpu().numpy().reshape(1 << mb, 1 << kb)
+    Rh = rhs.cpu().numpy().reshape(1 << nb, 1 << kb)
+    cnt = Lh[rows].astype(np.float32) @ Rh.astype(np.float32).T
+    want = (cnt > 0) if args.mode == "GT0" else ((cnt.astype(np.int64) % 256) != 0)
+    mism = int((res[rows] != want).sum())
+    valu_mism = int((valu != res).sum())
+    ops = 2.0 * 2.0 ** (mb + nb + kb)
+    nbytes = 2.0 ** (mb + kb) + 2.0 ** (nb + kb) + 2.0 ** (mb + nb)
+    tops = ops / (kernel_ms / 1000.0) / 1e12
+    log(f"MFMA {kernel_ms:.3f} ms ({tops:.1f} TOP/s), VALU {valu_ms:.3f} ms; sampled-row mismatches {mism}, "
+        f"MFMA vs VALU mismatches {valu_mism}; ones {float(res.mean()):.3f}")
+    cpu = None
+    if not args.no_cpu:
+        from oracle import finrel as F
+
+        # The reference-equivalent step (np.einsum with widening) on a bounded row sample:
+        # double the rows until one call takes a quarter of the CPU budget.
+        sb = 0
+        while True:
+            sub = 1 << sb
+            t0 = time.perf_counter()
+            F.contract2(Lh[:sub].reshape((2,) * (sb + kb)), [100 + i for i in range(sb)] + K,
+                        Rh.reshape((2,) * (nb + kb)), rw, [100 + i for i in range(sb)] + [200 + i for i in range(nb)],
+                        args.mode)
+            dt = time.perf_counter() - t0
+            if dt > args.cpu_seconds / 4 or sb >= mb:
+                break
+            sb += 1
+        cpu = {"value": 2.0 * sub * 2.0 ** (nb + kb) / dt / 1e12, "unit": "TOP/s", "cores": 1, "kind": "port",
+               "sample": f"{sub} of {1 << mb} rows through the oracle's contract2 (np.einsum with widening), "
+                         f"{dt:.1f}s, single thread"}
+    peak = 5000.0
+    line = {
+        "metric": "TOP/s dense Boolean composition step (int8 MFMA, exact counts)",
+        "value": tops, "unit": "TOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "i8", "data": "synthetic",
+        "config": {"workload": f"dense composition 2^{mb} x 2^{kb} by 2^{kb} x 2^{nb}, K innermost, "
+                               f"density 2^-{kb / 2:g}, {args.mode}",
+                   "kernel_ms": kernel_ms, "valu_ms": valu_ms, "bytes": nbytes,
+                   "hbm_gbs": nbytes / (kernel_ms / 1000.0) / 1e9},
+        "roofline": {"bound": "mfma", "achieved": tops, "peak": peak, "unit": "TFLOP/s", "frac": tops / peak,
+                     "traffic": None, "kernel": "k_step_mfma"},
+        "cpu_baseline": cpu,
+        "parity": {"sampled_rows": len(rows), "mismatches": mism, "valu_mismatches": valu_mism},
+    }
+    print(json.dumps(line), flush=True)
+    return 0 if mism == 0 and valu_mism == 0 else 1
 
 
 if __name__ == "__main__":

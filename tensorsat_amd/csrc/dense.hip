@@ -9,9 +9,10 @@
 //   batch   b : the out bits held by both        (one offset triple per batch index)
 //   count[b, m, n] = sum_k lhs[b, m, k] * rhs[b, n, k]      (0/1 int8, exact int32)
 //
-// Each workgroup computes a 64 x 64 tile of (m, n) for one batch index with 4 waves of
-// 2 x 2 v_mfma_i32_16x16x64_i8; K is staged through LDS in 64-byte chunks (16-byte vector
-// loads, four lanes per 64-byte row segment).  Row/column offsets are scattered by bit
+// Each workgroup computes a 128 x 128 (or, for narrow steps, 64 x 64) tile of (m, n) for
+// one batch index with 4 waves of 4 x 4 (2 x 2) v_mfma_i32_16x16x64_i8; K is staged
+// through double-buffered LDS in 64-byte chunks (16-byte vector loads, four lanes per
+// 64-byte row segment, the next chunk's loads in flight during the current MFMAs).  Row/column offsets are scattered by bit
 // position (any interleaving of m, n and b bits in the operands and the output), so they
 // are computed once per tile into LDS.  The epilogue is the step's: count > 0 (GT0) or
 // (count mod 256) != 0 (MOD256); counts are exact because sum_rank <= 27 < 31.
@@ -29,8 +30,7 @@ namespace {
 
 typedef int i4 __attribute__((ext_vector_type(4)));
 
-constexpr int TILE = 64;   // rows and columns per workgroup
-constexpr int KC = 64;     // bytes of K per LDS stage (one MFMA K step)
+constexpr int KC = 64;         // bytes of K per LDS stage (one MFMA K step)
 constexpr int LDSW = KC + 16;  // padded LDS row (bytes)
 
 struct DenseArgs {
@@ -43,17 +43,24 @@ struct DenseArgs {
   int64_t mtiles, ntiles;  // tiles along m and n
 };
 
-__device__ __forceinline__ uint64_t scatter(uint64_t idx, int nb, const uint8_t* pos) {
-  uint64_t off = 0;
-  for (int i = 0; i < nb; ++i) off |= ((idx >> i) & 1ull) << pos[i];
+__device__ __forceinline__ uint32_t scatter(uint64_t idx, int nb, const uint8_t* pos) {
+  uint32_t off = 0;
+  for (int i = 0; i < nb; ++i) off |= (uint32_t)((idx >> i) & 1ull) << pos[i];
   return off;
 }
 
+// Workgroup tile BM x BN = (32 TM) x (32 TN): 2 x 2 waves, each TM x TN MFMA tiles of
+// 16 x 16.  K advances 64 bytes per stage through two LDS buffers: the next stage's
+// global loads are issued before the current stage's MFMAs, and one barrier per stage
+// separates the LDS write of stage c+1 from the reads of stage c.
+template <int TM, int TN>
 __global__ __launch_bounds__(256) void k_step_mfma(const uint8_t* __restrict__ lhs, const uint8_t* __restrict__ rhs,
                                                    uint8_t* __restrict__ out, DenseArgs a) {
-  __shared__ __attribute__((aligned(16))) uint8_t sA[TILE][LDSW];
-  __shared__ __attribute__((aligned(16))) uint8_t sB[TILE][LDSW];
-  __shared__ uint64_t rowL[TILE], rowO[TILE], colR[TILE], colO[TILE];
+  constexpr int BM = 32 * TM, BN = 32 * TN;
+  constexpr int LA = BM * KC / 16 / 256, LB = BN * KC / 16 / 256;  // 16-byte loads per thread
+  __shared__ __attribute__((aligned(16))) uint8_t sA[2][BM][LDSW];
+  __shared__ __attribute__((aligned(16))) uint8_t sB[2][BN][LDSW];
+  __shared__ uint32_t rowL[BM], rowO[BM], colR[BN], colO[BN];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   // Tile order: n fastest, then m, then batch.  Consecutive workgroups land on different
   // XCDs, so regroup them: workgroup w runs on XCD w % 8; give each XCD a contiguous run
@@ -64,56 +71,78 @@ __global__ __launch_bounds__(256) void k_step_mfma(const uint8_t* __restrict__ l
   const uint64_t nt = w % a.ntiles;
   const uint64_t mt = (w / a.ntiles) % a.mtiles;
   const uint64_t b = w / (a.ntiles * a.mtiles);
-  const uint64_t m0 = mt * TILE, n0 = nt * TILE;
+  const uint64_t m0 = mt * BM, n0 = nt * BN;
   const uint64_t mcount = 1ull << a.mbits, ncount = 1ull << a.nbits;
-  if (tid < TILE) {
-    const uint64_t m = m0 + tid;
-    rowL[tid] = m < mcount ? scatter(m, a.mbits, a.m_lpos) : 0;
-    rowO[tid] = m < mcount ? scatter(m, a.mbits, a.m_opos) : 0;
-  } else if (tid < 2 * TILE) {
-    const uint64_t n = n0 + (tid - TILE);
-    colR[tid - TILE] = n < ncount ? scatter(n, a.nbits, a.n_rpos) : 0;
-    colO[tid - TILE] = n < ncount ? scatter(n, a.nbits, a.n_opos) : 0;
+  for (int i = tid; i < BM; i += 256) {
+    const uint64_t m = m0 + i;
+    rowL[i] = m < mcount ? scatter(m, a.mbits, a.m_lpos) : 0u;
+    rowO[i] = m < mcount ? scatter(m, a.mbits, a.m_opos) : 0u;
   }
-  const uint64_t bL = scatter(b, a.bbits, a.b_lpos), bR = scatter(b, a.bbits, a.b_rpos);
-  const uint64_t bO = scatter(b, a.bbits, a.b_opos);
+  for (int i = tid; i < BN; i += 256) {
+    const uint64_t n = n0 + i;
+    colR[i] = n < ncount ? scatter(n, a.nbits, a.n_rpos) : 0u;
+    colO[i] = n < ncount ? scatter(n, a.nbits, a.n_opos) : 0u;
+  }
+  const uint32_t bL = scatter(b, a.bbits, a.b_lpos), bR = scatter(b, a.bbits, a.b_rpos);
+  const uint32_t bO = scatter(b, a.bbits, a.b_opos);
   __syncthreads();
-  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-  i4 acc[2][2];
-  for (int i = 0; i < 2; ++i)
-    for (int j = 0; j < 2; ++j) acc[i][j] = (i4){0, 0, 0, 0};
-  // Loader: thread t moves 16 bytes of row t/4 (segment t%4) of both tiles.
-  const int lr = tid >> 2, ls = (tid & 3) * 16;
-  const bool lrow_ok = m0 + lr < mcount, lcol_ok = n0 + lr < ncount;
-  const uint8_t* pa = lhs + bL + rowL[lr] + ls;
-  const uint8_t* pb = rhs + bR + colR[lr] + ls;
-  const uint64_t K = 1ull << a.kbits;
-  for (uint64_t k0 = 0; k0 < K; k0 += KC) {
-    const i4 va = lrow_ok ? *reinterpret_cast<const i4*>(pa + k0) : (i4){0, 0, 0, 0};
-    const i4 vb = lcol_ok ? *reinterpret_cast<const i4*>(pb + k0) : (i4){0, 0, 0, 0};
-    *reinterpret_cast<i4*>(&sA[lr][ls]) = va;
-    *reinterpret_cast<i4*>(&sB[lr][ls]) = vb;
-    __syncthreads();
-    // Lane l feeds row / column l % 16 with 16 bytes of K at 16 * (l / 16); A and B use
-    // the same K assignment, so the sum over K does not depend on the hardware's order.
-    const int rr = lane & 15, kk = (lane >> 4) * 16;
-    i4 fa[2], fb[2];
-    for (int i = 0; i < 2; ++i) fa[i] = *reinterpret_cast<const i4*>(&sA[wm + 16 * i + rr][kk]);
-    for (int j = 0; j < 2; ++j) fb[j] = *reinterpret_cast<const i4*>(&sB[wn + 16 * j + rr][kk]);
-    for (int i = 0; i < 2; ++i)
-      for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[i], fb[j], acc[i][j], 0, 0, 0);
+  // Loader: load j of thread t moves 16 bytes (segment t % 4) of row t / 4 + 64 j.
+  const int ls = (tid & 3) * 16;
+  const uint8_t* pa[LA];
+  const uint8_t* pb[LB];
+  bool oka[LA], okb[LB];
+  for (int j = 0; j < LA; ++j) {
+    const int r = (tid >> 2) + 64 * j;
+    oka[j] = m0 + r < mcount;
+    pa[j] = lhs + bL + rowL[r] + ls;
+  }
+  for (int j = 0; j < LB; ++j) {
+    const int r = (tid >> 2) + 64 * j;
+    okb[j] = n0 + r < ncount;
+    pb[j] = rhs + bR + colR[r] + ls;
+  }
+  i4 ra[LA], rb[LB];
+  auto gload = [&](uint64_t k0) {
+    for (int j = 0; j < LA; ++j) ra[j] = oka[j] ? *reinterpret_cast<const i4*>(pa[j] + k0) : (i4){0, 0, 0, 0};
+    for (int j = 0; j < LB; ++j) rb[j] = okb[j] ? *reinterpret_cast<const i4*>(pb[j] + k0) : (i4){0, 0, 0, 0};
+  };
+  auto lstore = [&](int buf) {
+    for (int j = 0; j < LA; ++j) *reinterpret_cast<i4*>(&sA[buf][(tid >> 2) + 64 * j][ls]) = ra[j];
+    for (int j = 0; j < LB; ++j) *reinterpret_cast<i4*>(&sB[buf][(tid >> 2) + 64 * j][ls]) = rb[j];
+  };
+  const int wm = (wave >> 1) * 16 * TM, wn = (wave & 1) * 16 * TN;
+  i4 acc[TM][TN];
+  for (int i = 0; i < TM; ++i)
+    for (int j = 0; j < TN; ++j) acc[i][j] = (i4){0, 0, 0, 0};
+  const uint64_t nstages = (1ull << a.kbits) / KC;
+  gload(0);
+  lstore(0);
+  __syncthreads();
+  // Lane l feeds row / column l % 16 with 16 bytes of K at 16 * (l / 16); A and B use the
+  // same K assignment, so the sum over K does not depend on the hardware's order.
+  const int rr = lane & 15, kk = (lane >> 4) * 16;
+  for (uint64_t c = 0; c < nstages; ++c) {
+    const int buf = (int)(c & 1);
+    const bool more = c + 1 < nstages;
+    if (more) gload((c + 1) * KC);
+    i4 fa[TM], fb[TN];
+    for (int i = 0; i < TM; ++i) fa[i] = *reinterpret_cast<const i4*>(&sA[buf][wm + 16 * i + rr][kk]);
+    for (int j = 0; j < TN; ++j) fb[j] = *reinterpret_cast<const i4*>(&sB[buf][wn + 16 * j + rr][kk]);
+    for (int i = 0; i < TM; ++i)
+      for (int j = 0; j < TN; ++j) acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[i], fb[j], acc[i][j], 0, 0, 0);
+    if (more) lstore(buf ^ 1);
     __syncthreads();
   }
   // Accumulator layout: lane l holds rows 4 * (l / 16) + t, column l % 16.
-  for (int i = 0; i < 2; ++i)
-    for (int j = 0; j < 2; ++j)
+  for (int i = 0; i < TM; ++i)
+    for (int j = 0; j < TN; ++j)
       for (int t = 0; t < 4; ++t) {
         const int r = wm + 16 * i + 4 * (lane >> 4) + t;
-        const int c = wn + 16 * j + (lane & 15);
-        if (m0 + r >= mcount || n0 + c >= ncount) continue;
+        const int col = wn + 16 * j + (lane & 15);
+        if (m0 + r >= mcount || n0 + col >= ncount) continue;
         const int v = acc[i][j][t];
         const uint8_t bit = a.mode == TS_MODE_MOD256 ? (uint8_t)((v & 255) != 0) : (uint8_t)(v > 0);
-        out[bO + rowO[r] + colO[c]] = bit;
+        out[(uint64_t)bO + rowO[r] + colO[col]] = bit;
       }
 }
 
@@ -156,11 +185,17 @@ int launch_dense(const uint8_t* lhs, const uint8_t* rhs, uint8_t* out, const ts_
       a.n_opos[a.nbits++] = (uint8_t)j;
     }
   }
-  a.mtiles = ((1ll << a.mbits) + TILE - 1) / TILE;
-  a.ntiles = ((1ll << a.nbits) + TILE - 1) / TILE;
+  // 128 x 128 tiles when both sides have at least 128 entries, else 64 x 64.
+  const bool big = a.mbits >= 7 && a.nbits >= 7;
+  const int64_t BM = big ? 128 : 64, BN = big ? 128 : 64;
+  a.mtiles = ((1ll << a.mbits) + BM - 1) / BM;
+  a.ntiles = ((1ll << a.nbits) + BN - 1) / BN;
   const int64_t blocks = a.mtiles * a.ntiles * (1ll << a.bbits);
   if (blocks > (1ll << 24)) return TS_EUNSUPPORTED;  // grid.x * 256 threads < 2^32
-  hipLaunchKernelGGL(k_step_mfma, dim3((unsigned)blocks), dim3(256), 0, s, lhs, rhs, out, a);
+  if (big)
+    hipLaunchKernelGGL((k_step_mfma<4, 4>), dim3((unsigned)blocks), dim3(256), 0, s, lhs, rhs, out, a);
+  else
+    hipLaunchKernelGGL((k_step_mfma<2, 2>), dim3((unsigned)blocks), dim3(256), 0, s, lhs, rhs, out, a);
   return hipGetLastError() == hipSuccess ? TS_OK : TS_EHIP;
 }
 

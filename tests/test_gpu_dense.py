@@ -57,6 +57,9 @@ CASES = [
     (10, 8, 8, 0, 0.02),  # sparse: many zero counts
     (7, 4, 6, 4, 0.6),   # batch bits interleaved with row/column bits
     (9, 6, 7, 2, 0.97),  # counts near 2^9
+    (6, 7, 8, 1, 0.5),   # 128 x 128 workgroup tiles
+    (7, 8, 7, 0, 0.3),
+    (8, 7, 7, 0, 1.0),   # 128 x 128 tiles, every count 256
 ]
 
 
