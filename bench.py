@@ -3,7 +3,9 @@
 bench.py — batched random 3-SAT decisions on one MI355X.
 
 ``--workload dense`` instead measures one dense Boolean composition step on the int8 MFMA
-route (see ``bench_dense``).
+route (see ``bench_dense``); ``--workload learned`` compares time to solution of the
+learned planner against the plain planner on the learned-planner workload
+(see ``bench_learned``).
 
 Workload: 4096 random 3-SAT instances (instance i from default_rng(10000 + i),
 n ~ U{20..80}, clause ratio 4.26 for even i and 3.5 for odd i).  The host front-end
@@ -86,7 +88,11 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workgroups", type=int, default=0)
     ap.add_argument("--threads", type=int, default=0, help="host planning threads")
-    ap.add_argument("--workload", default="batched", choices=["batched", "dense"])
+    ap.add_argument("--workload", default="batched", choices=["batched", "dense", "learned"])
+    ap.add_argument("--learned-instances", type=int, default=8)
+    ap.add_argument("--learned-width", type=int, default=22, help="learned workload: cube width W")
+    ap.add_argument("--learned-k", type=int, default=0, help="learned workload: keep only clauses of this k (0: all)")
+    ap.add_argument("--budget", type=float, default=8.0, help="learned workload: seconds per instance and method")
     ap.add_argument("--dense-bits", type=int, nargs=3, default=[13, 13, 13], metavar=("M", "N", "K"),
                     help="dense workload: log2 rows, columns and contracted extent")
     args = ap.parse_args()
@@ -98,6 +104,8 @@ def main() -> int:
 
     if args.workload == "dense":
         return bench_dense(args)
+    if args.workload == "learned":
+        return bench_learned(args)
 
     import torch
 
@@ -207,6 +215,80 @@ def main() -> int:
     }
     print(json.dumps(out), flush=True)
     return 0 if parity["mismatches"] == 0 and parity.get("dpll_mismatches", 0) == 0 else 1
+
+
+def bench_learned(args) -> int:
+    """
+    Learned planner vs plain planner, end to end (host planning + GPU contraction), on the
+    learned-planner workload: k ~ U{3,4,5}, n ~ U[150, 300], m = 0.9 x threshold x n
+    (``workloads.learned_ksat`` with seeds 3e6 + i, disjoint from the training seeds).
+    Each instance is decided by ``single.decide_budget`` twice with the same budget:
+    plain (multi-trial planner trees, dynamic branching) and learned (network scores from
+    one batched GPU inference steer the trees and the branching order).  Reported: time to
+    solution of each, solved counts, and verdict identity where both decide.
+    """
+    import torch
+
+    from tensorsat_amd.learn.propose import LearnedPlanner
+    from tensorsat_amd.single import decide_budget
+    from tensorsat_amd.workloads import learned_ksat
+
+    def log(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    cnfs, seeds = [], []
+    i = 0
+    while len(cnfs) < args.learned_instances:
+        c = learned_ksat(i, base=3_000_000)
+        if args.learned_k == 0 or max(len(cl) for cl in c.clauses) == args.learned_k:
+            cnfs.append(c)
+            seeds.append(3_000_000 + i)
+        i += 1
+    lp = LearnedPlanner()
+    lp.scores(cnfs[:1])  # warm-up (kernel load)
+    scores = lp.scores(cnfs)
+    infer_s = lp.last_infer_s
+    log(f"network inference for {len(cnfs)} instances: {infer_s * 1000:.1f} ms")
+    rows = []
+    for i, cnf in enumerate(cnfs):
+        k = max(len(c) for c in cnf.clauses)
+        r = {"seed": seeds[i], "k": k, "n": cnf.num_vars, "m": len(cnf.clauses)}
+        torch.cuda.synchronize()
+        for name, kw in (("plain", {}), ("learned", {"priority": scores[i], "prior_branch": True})):
+            t0 = time.perf_counter()
+            res = decide_budget(cnf, width=args.learned_width, budget_s=args.budget, **kw)
+            dt = time.perf_counter() - t0 + (infer_s / len(cnfs) if name == "learned" else 0.0)
+            r[name] = {"verdict": res.value, "s": dt, "plan_s": res.plan_s, "gpu_s": res.gpu_s, "cubes": res.cubes,
+                       "decided": res.stats.get("decided", "budget"),
+                       "learned_trees": res.stats["learned_trees"], "fallback_trees": res.stats["fallback_trees"]}
+        log(f"seed={seeds[i]} k={k} n={cnf.num_vars}: plain {r['plain']['verdict']} in {r['plain']['s']:.2f}s "
+            f"({r['plain']['cubes']} cubes), learned {r['learned']['verdict']} in {r['learned']['s']:.2f}s "
+            f"({r['learned']['cubes']} cubes, trees {r['learned']['learned_trees']} learned / "
+            f"{r['learned']['fallback_trees']} fallback)")
+        rows.append(r)
+    both = [r for r in rows if r["plain"]["verdict"] >= 0 and r["learned"]["verdict"] >= 0]
+    mism = sum(int(r["plain"]["verdict"] != r["learned"]["verdict"]) for r in both)
+    tp = sum(r["plain"]["s"] if r["plain"]["verdict"] >= 0 else args.budget for r in rows)
+    tl = sum(r["learned"]["s"] if r["learned"]["verdict"] >= 0 else args.budget for r in rows)
+    solved_p = sum(r["plain"]["verdict"] >= 0 for r in rows)
+    solved_l = sum(r["learned"]["verdict"] >= 0 for r in rows)
+    speed = [r["plain"]["s"] / r["learned"]["s"] for r in both]
+    line = {
+        "metric": METRIC, "value": solved_l, "unit": "instances solved within budget (learned planner)",
+        "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": tl * 1000.0 / max(len(rows), 1),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+        "config": {"workload": "learned planner vs plain planner: random k-SAT, k~U{3,4,5}, n~U[150,300], "
+                               "0.9 x threshold, seeds 3e6+i, budgeted cube search + GPU contraction",
+                   "instances": len(rows), "k_filter": args.learned_k or None, "budget_s": args.budget, "cube_width": args.learned_width,
+                   "solved_plain": solved_p, "solved_learned": solved_l, "both_solved": len(both),
+                   "verdict_mismatches": mism, "total_s_plain": tp, "total_s_learned": tl,
+                   "geomean_speedup_both_solved": float(np.exp(np.mean(np.log(speed)))) if speed else None,
+                   "inference_ms": infer_s * 1000.0, "rows": rows},
+        "roofline": None, "cpu_baseline": None,
+    }
+    torch.cuda.synchronize()
+    print(json.dumps(line), flush=True)
+    return 0 if mism == 0 else 1
 
 
 def bench_dense(args) -> int:

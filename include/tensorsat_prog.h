@@ -83,17 +83,21 @@ typedef struct {
   uint64_t seed;
   int32_t cube_begin;   /* skip (do not compile) the first cube_begin cubes of each instance;
                            max_cubes then counts cubes from cube_begin on */
-  const float* priority;/* optional per-variable priorities, instances concatenated (NULL:
-                           plain heuristics).  Branching takes the free variable of highest
-                           priority; each cube's tree follows ascending priority as an
-                           elimination order, kept only if the exact cost model rates it no
-                           worse than the min-degree tree by more than gate_margin (log2
-                           flops); otherwise the min-degree tree is the fallback. */
+  const float* priority;/* optional per-variable priorities (learned scores), instances
+                           concatenated (NULL: plain planner).  Each cube's tree is proposed
+                           by one min-fill pass steered by the priorities (prior_lam), kept
+                           only if the exact cost model rates it no larger than one
+                           deterministic min-fill pass and no costlier in log2 flops by more
+                           than gate_margin; otherwise the plan_trials planner is the
+                           fallback.  With prior_branch, branching also takes the free
+                           variable of highest priority. */
   float gate_margin;
   int32_t max_nodes;    /* branch-node budget per instance (0: none); exceeding it stops the
                            search with stats[3] = 2 */
   int32_t stop_on_sat;  /* stop an instance at its first cube whose residual is empty
                            (every clause satisfied): stats[5] = 1 */
+  int32_t prior_branch; /* with priority: branch on the highest-priority free variable */
+  float prior_lam;      /* with priority: weight of the priority against fill (fill units) */
 } tsc_sat_opts;
 
 /* Elimination order of the cheapest of `trials` min-fill/min-degree orders (labels for the
@@ -101,14 +105,21 @@ typedef struct {
 int tsp_elim_order(const tsp_network* net, int32_t trials, uint64_t seed, int32_t* order_out,
                    tsp_cost* cost_out);
 
+/* One adaptive min-fill pass steered by a per-wire prior (a learned proposal; no
+ * counterpart in the reference): the next wire minimises fill + lam * prior[w], ties to the
+ * smaller prior.  Returns the order (all wires) and its tree cost. */
+int tsp_elim_order_prior(const tsp_network* net, const float* prior, float lam, int32_t* order_out,
+                         tsp_cost* cost_out);
+
 /* Tree cost of an explicit elimination order (all num_wires wires, out wires ignored). */
 int tsp_order_cost(const tsp_network* net, const int32_t* order, int32_t* path_out, tsp_cost* cost_out);
 
-#define TSC_NSTAT 8
+#define TSC_NSTAT 10
 /* Per-instance statistics copied out by tsc_program_copy:
  * [0] jobs  [1] branch nodes  [2] pruned branches  [3] truncated (cap reached)
  * [4] max tensor rank  [5] decided on the host (-1 no, 0 UNSAT: every branch pruned)
- * [6] host planning microseconds  [7] cubes whose residual is empty (trivially SAT) */
+ * [6] host planning microseconds  [7] cubes whose residual is empty (trivially SAT)
+ * [8] cube trees taken from the learned proposal  [9] cube trees from the fallback planner */
 
 /* Batched SAT front-end: conditioned slicing of each CNF instance into cubes (branch on
  * variables, propagate unit clauses, prune falsified branches) until the residual

@@ -194,24 +194,25 @@ struct Enumerator {
     std::vector<int32_t> path;
     TreeCost tc;
     if (prio) {
-      // Learned proposal: eliminate the residual's variables by ascending priority;
-      // gate it against the min-degree tree with the exact cost model.
-      std::vector<int> rvars(nv);
+      // Learned proposal: one min-fill pass over the residual steered by the priorities,
+      // gated by the exact cost model against one deterministic min-fill pass; the
+      // plan_trials planner is the fallback.
+      std::vector<float> rp(nv);
       for (int v = 0; v < f.n; ++v)
-        if (vmap[v] >= 0) rvars[vmap[v]] = v;
-      std::vector<int> order(nv);
-      for (int i = 0; i < nv; ++i) order[i] = i;
-      std::stable_sort(order.begin(), order.end(),
-                       [&](int a, int b) { return prio[rvars[a]] < prio[rvars[b]]; });
+        if (vmap[v] >= 0) rp[vmap[v]] = prio[v];
       std::vector<int32_t> lpath, ppath;
       TreeCost lc, pc;
-      order_plan(net, order, lpath, &lc);
+      prior_plan(net, rp.data(), o.prior_lam, lpath, &lc);
       elim_plan(net, 1, seed + (uint64_t)seen * 7919u, ppath, &pc);
       const bool take = lc.log2_max <= pc.log2_max && lc.log2_flops <= pc.log2_flops + o.gate_margin;
-      path = take ? lpath : ppath;
-      tc = take ? lc : pc;
-      if (take) ++learned_trees;
-      else ++fallback_trees;
+      if (take) {
+        path = lpath;
+        tc = lc;
+        ++learned_trees;
+      } else {
+        elim_plan(net, o.plan_trials, seed + (uint64_t)seen * 7919u, path, &tc);
+        ++fallback_trees;
+      }
     } else {
       elim_plan(net, o.plan_trials, seed + (uint64_t)seen * 7919u, path, &tc);
     }
@@ -286,7 +287,7 @@ struct Enumerator {
     }
     ++nodes;
     int bv = rv[0];
-    if (prio) {
+    if (prio && o.prior_branch) {
       for (int v : rv)
         if (prio[v] > prio[bv] || (prio[v] == prio[bv] && v < bv)) bv = v;
     } else {
@@ -393,6 +394,32 @@ int tsp_elim_order(const tsp_network* n, int32_t trials, uint64_t seed, int32_t*
   TreeCost tc;
   try {
     elim_plan(net, trials, seed, path, &tc, &order);
+  } catch (const std::bad_alloc&) {
+    return TSP_ENOMEM;
+  }
+  std::vector<char> in(net.W, 0);
+  int k = 0;
+  for (int w : order) {
+    order_out[k++] = w;
+    in[w] = 1;
+  }
+  for (int w = 0; w < net.W; ++w)
+    if (!in[w]) order_out[k++] = w;
+  fill_cost(tc, cost_out);
+  return TSP_OK;
+}
+
+int tsp_elim_order_prior(const tsp_network* n, const float* prior, float lam, int32_t* order_out,
+                         tsp_cost* cost_out) {
+  BitNet net;
+  int rc = to_bitnet(n, net);
+  if (rc) return rc;
+  if (!order_out || !prior) return TSP_EINVAL;
+  std::vector<int32_t> path;
+  std::vector<int> order;
+  TreeCost tc;
+  try {
+    prior_plan(net, prior, lam, path, &tc, &order);
   } catch (const std::bad_alloc&) {
     return TSP_ENOMEM;
   }
@@ -537,6 +564,8 @@ int tsc_sat_batch(int32_t num_inst, const int32_t* nvars, const int64_t* clause_
         // with stop_on_sat, -1 undecided on the host
         r.stats[5] = e.sat_found ? 1 : (e.seen == 0 && !e.budget_hit ? 0 : -1);
         r.stats[7] = (int32_t)e.empty_residual;
+        r.stats[8] = (int32_t)e.learned_trees;
+        r.stats[9] = (int32_t)e.fallback_trees;
       } catch (const std::invalid_argument&) {
         r.ok = false;
         bad++;

@@ -39,6 +39,11 @@ void elim_plan(const BitNet& net, int trials, uint64_t seed, std::vector<int32_t
 void order_plan(const BitNet& net, const std::vector<int>& order, std::vector<int32_t>& path,
                 TreeCost* cost);
 
+// One adaptive min-fill pass steered by a per-wire prior (a learned proposal): the next
+// wire minimises fill + lam * prior, ties to the smaller prior.
+void prior_plan(const BitNet& net, const float* prior, float lam, std::vector<int32_t>& path, TreeCost* cost,
+                std::vector<int>* order_out = nullptr);
+
 // Width estimate (min-degree heuristic) of the primal graph of a clause list over
 // variables 0..nv-1.
 int mindeg_width(int nv, const std::vector<std::vector<int>>& clauses_vars);

@@ -132,8 +132,11 @@ struct Bits {
 
 // Elimination order over `nv` vertices (cliques given), skipping `keep` vertices.
 // Returns the order and its width (max degree at elimination).
+// With `prior` (one value per vertex), the score is fill + lam * prior[v] and ties go to
+// the smaller prior: a learned proposal steering the adaptive heuristic.
 int elim_order(int nv, const std::vector<std::vector<int>>& cliques, const std::vector<char>& keep,
-               bool min_fill, std::mt19937_64* rng, std::vector<int>& order) {
+               bool min_fill, std::mt19937_64* rng, std::vector<int>& order, const float* prior = nullptr,
+               float lam = 0.f) {
   Bits adj;
   adj.init(nv, nv);
   for (const auto& c : cliques)
@@ -156,6 +159,8 @@ int elim_order(int nv, const std::vector<std::vector<int>>& cliques, const std::
   std::vector<uint64_t> tmp(adj.nw);
   while (remaining > 0) {
     long best = -1;
+    double bestp = 0.0;
+    int bestv = -1;
     cand.clear();
     for (int v = 0; v < nv; ++v) {
       if (gone[v] || keep[v]) continue;
@@ -180,6 +185,15 @@ int elim_order(int nv, const std::vector<std::vector<int>>& cliques, const std::
       } else {
         score = deg[v];
       }
+      if (prior) {
+        // score = fill * 4096 + deg (min-fill) or deg; steer by lam * prior in fill units.
+        const double p = (double)score / 4096.0 + (double)lam * prior[v];
+        if (bestv < 0 || p < bestp || (p == bestp && prior[v] < prior[bestv])) {
+          bestp = p;
+          bestv = v;
+        }
+        continue;
+      }
       if (best < 0 || score < best) {
         best = score;
         cand.clear();
@@ -188,6 +202,7 @@ int elim_order(int nv, const std::vector<std::vector<int>>& cliques, const std::
         cand.push_back(v);
       }
     }
+    if (prior) cand.assign(1, bestv);
     int v = cand[0];
     if (rng && cand.size() > 1) v = cand[(*rng)() % cand.size()];
     width = std::max(width, deg[v]);
@@ -344,6 +359,17 @@ void elim_plan(const BitNet& net, int trials, uint64_t seed, std::vector<int32_t
     }
   }
   if (cost) *cost = best;
+}
+
+void prior_plan(const BitNet& net, const float* prior, float lam, std::vector<int32_t>& path, TreeCost* cost,
+                std::vector<int>* order_out) {
+  std::vector<char> keep(net.W, 0);
+  for (int w = 0; w < net.W; ++w) keep[w] = net.is_out[w];
+  std::vector<int> order;
+  elim_order(net.W, net.slot_wires, keep, net.W <= 600, nullptr, order, prior, lam);
+  order_to_path(net, order, path);
+  if (cost) *cost = tree_cost(net, path);
+  if (order_out) *order_out = order;
 }
 
 int mindeg_width(int nv, const std::vector<std::vector<int>>& clauses_vars) {

@@ -94,6 +94,20 @@ def plan_order(inst: KSat, trials: int = 8, seed: int = 0) -> tuple[np.ndarray, 
     return order, _plan.Cost._from(c)
 
 
+def prior_order(inst: KSat, prior, lam: float = 0.0) -> tuple[np.ndarray, _plan.Cost]:
+    """One adaptive min-fill pass steered by per-variable ``prior`` (learned scores)."""
+    L = _prog._lib()
+    net, keep = _net(inst)
+    pr = np.ascontiguousarray(np.asarray(prior, np.float32))
+    if pr.shape != (inst.num_vars,):
+        raise ValueError("one prior value per variable")
+    order = np.zeros(inst.num_vars, np.int32)
+    c = _plan._Cost()
+    _plan._check(L.tsp_elim_order_prior(ctypes.byref(net), pr.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                        float(lam), _plan._ptr(order), ctypes.byref(c)), "tsp_elim_order_prior")
+    return order, _plan.Cost._from(c)
+
+
 def order_cost(inst: KSat, order) -> _plan.Cost:
     """Exact tree cost of eliminating the variables in ``order``."""
     L = _prog._lib()

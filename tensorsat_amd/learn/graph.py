@@ -43,24 +43,36 @@ class HostGraph:
 
 
 def build(instances) -> HostGraph:
-    """``instances``: sequence of (num_vars, clause list) or objects with ``num_vars`` and
-    ``clauses``."""
-    nvs, cls = [], []
+    """``instances``: sequence of (num_vars, clause list), objects with ``num_vars`` and
+    ``clauses``, or objects with ``num_vars`` and a 2-D literal array ``lits`` (one row per
+    clause, the vectorised path)."""
+    nvs, lens_l, lits_l = [], [], []
     for inst in instances:
-        if hasattr(inst, "num_vars"):
+        if hasattr(inst, "lits"):
+            a = np.asarray(inst.lits)
             nvs.append(int(inst.num_vars))
-            cls.append(inst.clauses)
+            lens_l.append(np.full(a.shape[0], a.shape[1], np.int64))
+            lits_l.append(a.reshape(-1).astype(np.int64))
+            continue
+        if hasattr(inst, "num_vars"):
+            n, cl = int(inst.num_vars), inst.clauses
         else:
-            nvs.append(int(inst[0]))
-            cls.append(inst[1])
+            n, cl = int(inst[0]), inst[1]
+        nvs.append(n)
+        lens_l.append(np.fromiter((len(c) for c in cl), dtype=np.int64, count=len(cl)))
+        lits_l.append(np.fromiter((l for c in cl for l in c), dtype=np.int64, count=int(lens_l[-1].sum())))
+    return _build(nvs, lens_l, lits_l)
+
+
+def _build(nvs, lens_l, lits_l) -> HostGraph:
     I = len(nvs)
     var_offset = np.zeros(I + 1, np.int64)
     var_offset[1:] = np.cumsum(nvs)
-    ncl = [len(c) for c in cls]
+    ncl = [len(x) for x in lens_l]
     clause_offset = np.zeros(I + 1, np.int64)
     clause_offset[1:] = np.cumsum(ncl)
-    lens = np.fromiter((len(c) for cs in cls for c in cs), dtype=np.int64, count=int(clause_offset[-1]))
-    lits = np.fromiter((l for cs in cls for c in cs for l in c), dtype=np.int64, count=int(lens.sum()))
+    lens = np.concatenate(lens_l) if lens_l else np.zeros(0, np.int64)
+    lits = np.concatenate(lits_l) if lits_l else np.zeros(0, np.int64)
     inst_of_clause = np.repeat(np.arange(I), ncl)
     inst_of_lit = np.repeat(inst_of_clause, lens)
     var = (np.abs(lits) - 1 + var_offset[inst_of_lit]).astype(np.int32)

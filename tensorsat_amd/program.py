@@ -23,7 +23,7 @@ MAX_BITS = 27
 NOPOS = 0xFF
 STEP_KINNER = 1
 STEP_NORHS = 2
-NSTAT = 8
+NSTAT = 10
 
 STEP_DTYPE = np.dtype(
     [
@@ -47,7 +47,7 @@ JOB_DTYPE = np.dtype(
 assert JOB_DTYPE.itemsize == 56
 
 STAT_NAMES = ("jobs", "branch_nodes", "pruned", "truncated", "max_rank", "host_verdict",
-              "plan_us", "empty_residuals")
+              "plan_us", "empty_residuals", "learned_trees", "fallback_trees")
 
 
 class _SatOpts(ctypes.Structure):
@@ -55,7 +55,7 @@ class _SatOpts(ctypes.Structure):
         ("width", ctypes.c_int32), ("max_cubes", ctypes.c_int32), ("plan_trials", ctypes.c_int32),
         ("threads", ctypes.c_int32), ("seed", ctypes.c_uint64), ("cube_begin", ctypes.c_int32),
         ("priority", ctypes.c_void_p), ("gate_margin", ctypes.c_float), ("max_nodes", ctypes.c_int32),
-        ("stop_on_sat", ctypes.c_int32),
+        ("stop_on_sat", ctypes.c_int32), ("prior_branch", ctypes.c_int32), ("prior_lam", ctypes.c_float),
     ]
 
 
@@ -79,8 +79,10 @@ def _lib():
         L.tsp_elim_order.argtypes = [P(_plan._Net), ctypes.c_int32, ctypes.c_uint64, P(ctypes.c_int32),
                                      P(_plan._Cost)]
         L.tsp_order_cost.argtypes = [P(_plan._Net), P(ctypes.c_int32), P(ctypes.c_int32), P(_plan._Cost)]
+        L.tsp_elim_order_prior.argtypes = [P(_plan._Net), P(ctypes.c_float), ctypes.c_float, P(ctypes.c_int32),
+                                           P(_plan._Cost)]
         for f in (L.tsc_compile, L.tsc_sat_batch, L.tsc_program_sizes, L.tsc_program_copy, L.tsp_elim,
-                  L.tsp_elim_order, L.tsp_order_cost):
+                  L.tsp_elim_order, L.tsp_order_cost, L.tsp_elim_order_prior):
             f.restype = ctypes.c_int
         _bound = True
     return L
@@ -201,6 +203,8 @@ def sat_batch(
     gate_margin: float = 0.0,
     max_nodes: int = 0,
     stop_on_sat: bool = False,
+    prior_branch: bool = False,
+    prior_lam: float = 16.0,
 ) -> Program:
     """
     Conditioned-slicing front-end for a batch of CNF instances: one job per cube, in
@@ -222,7 +226,8 @@ def sat_batch(
         if prio.shape != (int(nv.sum()),):
             raise ValueError("priority needs one value per variable of every instance")
     opts = _SatOpts(width, max_cubes, plan_trials, threads, seed, cube_begin,
-                    prio.ctypes.data if prio is not None else None, gate_margin, max_nodes, int(stop_on_sat))
+                    prio.ctypes.data if prio is not None else None, gate_margin, max_nodes, int(stop_on_sat),
+                    int(prior_branch), float(prior_lam))
     handle = ctypes.c_void_p()
     L = _lib()
     P = ctypes.POINTER

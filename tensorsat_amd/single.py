@@ -5,6 +5,12 @@ One large instance on the whole GPU.
   device (the n=120, m=510 workload: cubes of width up to 26, i.e. 2^26-entry tensors):
   cubes run one after the other in depth-first order, every step grid-wide, stopping at
   the first satisfiable cube.
+* :func:`decide_budget` — a SAT decision under a wall-clock budget for instances whose
+  cube enumeration cannot be completed (the learned-planner workload, n = 150..300):
+  cubes are compiled in chunks of doubling size in depth-first order and each chunk is
+  decided on the GPU by the batched executor with early exit; the first satisfiable cube
+  (or a residual with every clause satisfied, found on the host) decides SAT, a completed
+  enumeration without one decides UNSAT, and otherwise the answer is unknown (-1).
 * :func:`count` — the exact model count (#SAT) of one instance: all cubes of width W in
   one persistent launch of the counting kernel, result x 2^free summed per cube in
   checked uint64; raises ``OverflowError`` instead of returning a wrapped count.
@@ -60,6 +66,62 @@ def decide(cnf: CNFInstance, *, width: int = 26, mode: str = "GT0", plan_trials:
     torch.cuda.synchronize()
     return SingleResult(verdict, plan_s, time.perf_counter() - t0, len(prog.jobs), ran,
                         {"max_rank": int(prog.stats[0, 4]), "branch_nodes": int(prog.stats[0, 1])})
+
+
+def decide_budget(cnf: CNFInstance, *, width: int = 22, budget_s: float = 20.0, mode: str = "GT0",
+                  plan_trials: int = 8, priority: np.ndarray | None = None, prior_branch: bool = False,
+                  gate_margin: float = 0.5, first_chunk: int = 16, node_budget: int = 100_000,
+                  threads: int = 1) -> SingleResult:
+    """Budgeted decision (see the module docstring).  ``priority`` (learned scores) makes
+    the search use the learned proposals; ``plan_s`` includes every host planning call."""
+    import torch
+
+    from .executor import BatchExecutor
+
+    t_start = time.perf_counter()
+    plan_s = gpu_s = 0.0
+    begin, chunk, max_nodes = 0, first_chunk, node_budget
+    cubes = 0
+    stats = {"learned_trees": 0, "fallback_trees": 0, "branch_nodes": 0, "calls": 0}
+    rate = None  # branch nodes per second of the host search, measured on the first call
+    while True:
+        left = budget_s - (time.perf_counter() - t_start)
+        if left <= 0:
+            return SingleResult(-1, plan_s, gpu_s, cubes, cubes, stats)
+        if rate:
+            # keep one call inside the remaining budget (the search re-walks earlier nodes)
+            max_nodes = max(1000, min(max_nodes, int(rate * left)))
+        t0 = time.perf_counter()
+        prog = PG.sat_batch([cnf], width=width, cube_begin=begin, max_cubes=chunk, max_nodes=max_nodes,
+                            stop_on_sat=True, plan_trials=plan_trials, priority=priority, prior_branch=prior_branch,
+                            gate_margin=gate_margin, threads=threads)
+        dt = time.perf_counter() - t0
+        plan_s += dt
+        st = prog.stats[0]
+        if st[1] > 1000 and dt > 0:
+            rate = st[1] / dt
+        stats["calls"] += 1
+        stats["branch_nodes"] = max(stats["branch_nodes"], int(st[1]))
+        stats["learned_trees"] += int(st[8])
+        stats["fallback_trees"] += int(st[9])
+        cubes += len(prog.jobs)
+        if st[5] == 1:
+            return SingleResult(1, plan_s, gpu_s, cubes, cubes, dict(stats, decided="host: satisfied residual"))
+        if st[5] == 0:
+            return SingleResult(0, plan_s, gpu_s, cubes, cubes, dict(stats, decided="host: every branch refuted"))
+        if len(prog.jobs):
+            t0 = time.perf_counter()
+            v = int(BatchExecutor(prog).run(mode)[0])
+            torch.cuda.synchronize()
+            gpu_s += time.perf_counter() - t0
+            if v:
+                return SingleResult(1, plan_s, gpu_s, cubes, cubes, dict(stats, decided="gpu: satisfiable cube"))
+        if st[3] == 0:
+            return SingleResult(0, plan_s, gpu_s, cubes, cubes, dict(stats, decided="all cubes unsatisfiable"))
+        if st[3] == 2:
+            max_nodes *= 2  # the search ran out of branch nodes before filling the chunk
+        begin += len(prog.jobs)
+        chunk *= 2
 
 
 def count(cnf: CNFInstance, *, width: int = 18, plan_trials: int = 1,
