@@ -88,7 +88,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workgroups", type=int, default=0)
     ap.add_argument("--threads", type=int, default=0, help="host planning threads")
-    ap.add_argument("--workload", default="batched", choices=["batched", "dense", "learned"])
+    ap.add_argument("--workload", default="batched", choices=["batched", "dense", "learned", "copy"])
     ap.add_argument("--learned-instances", type=int, default=8)
     ap.add_argument("--learned-width", type=int, default=22, help="learned workload: cube width W")
     ap.add_argument("--learned-k", type=int, default=0, help="learned workload: keep only clauses of this k (0: all)")
@@ -106,6 +106,8 @@ def main() -> int:
         return bench_dense(args)
     if args.workload == "learned":
         return bench_learned(args)
+    if args.workload == "copy":
+        return bench_copy(args)
 
     import torch
 
@@ -129,10 +131,21 @@ def main() -> int:
     for _ in range(args.warmup):
         solver.run()
     torch.cuda.synchronize()
+    # Eager launches (per-pass kernel sequence from the host), for reference.
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        solver.launch()
+    torch.cuda.synchronize()
+    eager_ms = (time.perf_counter() - t_start) * 1000.0 / args.steps
+    # The timed steps: one HIP graph launch each (all passes captured once).
+    solver.capture()
+    for _ in range(args.warmup):
+        solver.replay()
+    torch.cuda.synchronize()
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
     t_start = time.perf_counter()
     for k in range(args.steps):
-        solver.launch(events=ev[k])
+        solver.replay(events=ev[k])
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_start
     kernel_ms = [a.elapsed_time(b) for a, b in ev]
@@ -199,6 +212,8 @@ def main() -> int:
             "end_to_end_s": st.plan_s + ms_per_step / 1000.0,
             "sat": int(verdicts.sum()),
             "kernel_ms": avg_kernel_ms,
+            "launch": "one hipGraph replay per step (all passes captured)",
+            "eager_ms_per_step": eager_ms,
         },
         "roofline": {
             "bound": "hbm",
@@ -217,20 +232,54 @@ def main() -> int:
     return 0 if parity["mismatches"] == 0 and parity.get("dpll_mismatches", 0) == 0 else 1
 
 
+def bench_copy(args) -> int:
+    """
+    Calibration for hardware traffic counters: a plain device-to-device copy of 1 GiB
+    (torch ``copy_``), --steps times.  Under ``rocprofv3 --pmc FETCH_SIZE`` (then
+    ``WRITE_SIZE``) each dispatch should read (write) 2^20 KiB; the measured ratio scales
+    the counters of the other kernels.
+    """
+    import torch
+
+    n = 1 << 30
+    src = torch.ones(n, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(args.warmup):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(args.steps):
+        dst.copy_(src)
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / args.steps
+    gbs = 2.0 * n / (ms / 1000.0) / 1e9
+    print(json.dumps({"metric": "GB/s device copy (counter calibration)", "value": gbs, "unit": "GB/s", "n_gpus": 1,
+                      "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+                      "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+                      "config": {"workload": "copy of 1 GiB", "bytes_read": n, "bytes_written": n},
+                      "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                   "frac": gbs / HBM_PEAK_GBS, "traffic": 2.0 * n, "kernel": "copy"},
+                      "cpu_baseline": None}), flush=True)
+    return 0
+
+
 def bench_learned(args) -> int:
     """
     Learned planner vs plain planner, end to end (host planning + GPU contraction), on the
     learned-planner workload: k ~ U{3,4,5}, n ~ U[150, 300], m = 0.9 x threshold x n
     (``workloads.learned_ksat`` with seeds 3e6 + i, disjoint from the training seeds).
-    Each instance is decided by ``single.decide_budget`` twice with the same budget:
-    plain (multi-trial planner trees, dynamic branching) and learned (network scores from
-    one batched GPU inference steer the trees and the branching order).  Reported: time to
+    Each instance is decided twice with the same budget: plain (``single.decide_budget``:
+    multi-trial planner trees, dynamic branching) and learned (``single.decide_learned``:
+    network scores from one batched GPU inference steer the trees and the branching order,
+    with the plain search as fallback in alternating time slices).  Reported: time to
     solution of each, solved counts, and verdict identity where both decide.
     """
     import torch
 
     from tensorsat_amd.learn.propose import LearnedPlanner
-    from tensorsat_amd.single import decide_budget
+    from tensorsat_amd.single import decide_budget, decide_learned
     from tensorsat_amd.workloads import learned_ksat
 
     def log(msg):
@@ -254,9 +303,12 @@ def bench_learned(args) -> int:
         k = max(len(c) for c in cnf.clauses)
         r = {"seed": seeds[i], "k": k, "n": cnf.num_vars, "m": len(cnf.clauses)}
         torch.cuda.synchronize()
-        for name, kw in (("plain", {}), ("learned", {"priority": scores[i], "prior_branch": True})):
+        for name in ("plain", "learned"):
             t0 = time.perf_counter()
-            res = decide_budget(cnf, width=args.learned_width, budget_s=args.budget, **kw)
+            if name == "plain":
+                res = decide_budget(cnf, width=args.learned_width, budget_s=args.budget)
+            else:
+                res = decide_learned(cnf, scores[i], width=args.learned_width, budget_s=args.budget)
             dt = time.perf_counter() - t0 + (infer_s / len(cnfs) if name == "learned" else 0.0)
             r[name] = {"verdict": res.value, "s": dt, "plan_s": res.plan_s, "gpu_s": res.gpu_s, "cubes": res.cubes,
                        "decided": res.stats.get("decided", "budget"),

@@ -139,11 +139,16 @@ typedef struct {
   int32_t early_exit;       /* 1: skip jobs of instances already decided SAT */
   uint32_t* job_done;       /* device, per job (optional): 1 ran, 2 skipped */
   const int32_t* num_jobs_dev; /* optional device count overriding num_jobs (compacted lists) */
+  int32_t lds_arena;        /* 1: each job runs on one wave with its arena in LDS (arena_stride
+                               bytes of LDS per workgroup, <= TS_BATCH_LDS_MAX; `arenas` unused;
+                               num_workgroups from ts_batch_max_workgroups_lds) */
 } ts_batch;
 
+#define TS_BATCH_LDS_MAX 65536
+
 /* One persistent launch: each workgroup takes the next job in `order`, copies its leaves
- * into its arena and runs every step with the workgroup, OR-ing the scalar result into
- * verdicts[instance]. */
+ * into its arena (device memory, or LDS with lds_arena) and runs every step with the
+ * workgroup, OR-ing the scalar result into verdicts[instance]. */
 int ts_batch_run(const ts_batch* b, ts_stream stream);
 
 /* Counting variant (#SAT): arenas hold uint64 entries (arena_stride in bytes, >= 8 x the
@@ -162,6 +167,10 @@ int ts_batch_compact(const int32_t* order, int32_t n, const ts_job* jobs, const 
 
 /* Maximum resident workgroups of the batched kernel on the current device. */
 int ts_batch_max_workgroups(int32_t* out);
+
+/* Maximum resident workgroups of the LDS-arena batched kernel for arenas of arena_bytes
+ * (a multiple of 256, at most TS_BATCH_LDS_MAX). */
+int ts_batch_max_workgroups_lds(int64_t arena_bytes, int32_t* out);
 
 /* ---- Learned planner: message-passing network kernels (bf16 in, fp32 accumulate) --
  * No counterpart in the reference: the learned planner is an addition of this build.

@@ -21,6 +21,7 @@ TS_ENOMEM = -2
 TS_EUNSUPPORTED = -3
 TS_EOVERFLOW = -4
 TS_EHIP = -5
+BATCH_LDS_MAX = 65536
 
 MODE_GT0 = 0
 MODE_MOD256 = 1
@@ -31,7 +32,7 @@ EXPORTS = (
     "ts_init", "ts_version", "ts_contract2", "ts_rewire", "ts_spider", "ts_union", "ts_stack",
     "ts_select", "ts_as_scalar", "ts_diagonal", "ts_count_contract2", "ts_bytes_to_u64",
     "ts_program_run", "ts_program_count", "ts_batch_run", "ts_batch_max_workgroups",
-    "ts_batch_compact", "ts_batch_count", "ts_gnn_gemm", "ts_gnn_segsum", "ts_gnn_relu_back",
+    "ts_batch_compact", "ts_batch_count", "ts_batch_max_workgroups_lds", "ts_gnn_gemm", "ts_gnn_segsum", "ts_gnn_relu_back",
     "ts_gnn_head", "ts_gnn_adam", "ts_gnn_cast_bf16", "ts_dense_config",
 )
 
@@ -49,7 +50,7 @@ class Batch(ctypes.Structure):
         ("jobs", vp), ("order", vp), ("num_jobs", i32), ("steps", vp), ("leaves", vp),
         ("arenas", vp), ("arena_stride", i64), ("num_workgroups", i32), ("mode", i32),
         ("verdicts", vp), ("counter", vp), ("early_exit", i32), ("job_done", vp),
-        ("num_jobs_dev", vp),
+        ("num_jobs_dev", vp), ("lds_arena", i32),
     ]
 
 
@@ -88,6 +89,7 @@ def load() -> ctypes.CDLL:
         L.ts_gnn_adam.argtypes = [vp, vp, vp, vp, vp, i8, f32, f32, f32, f32, i32, f32, vp]
         L.ts_gnn_cast_bf16.argtypes = [vp, vp, i8, vp]
         L.ts_batch_max_workgroups.argtypes = [P(i32)]
+        L.ts_batch_max_workgroups_lds.argtypes = [i64, P(i32)]
         L.ts_dense_config.argtypes = [i32, i32]
         _lib = L
     return _lib

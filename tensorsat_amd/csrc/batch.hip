@@ -15,6 +15,13 @@
 // (tensorsat/contractions/cotengra.py:120-126 -> _autoray.einsum -> FinRel._contract2,
 // fin_rel.py:255-288) and the final bool() (Box.__bool__, diagrams.py:949-958).
 //
+// Small jobs (arena up to 64 KiB: every job of cube widths up to 12 on the batched
+// workload) run on k_batch_lds instead: one wave per workgroup and per job, the job's
+// arena in LDS (leaves copied in, every intermediate written and read there), so a step
+// costs no L2 round trip and its barrier is a single-wave one.  Operand offsets come from
+// the step record's bit positions: a per-lane part for the 6 low out bits and a
+// wave-uniform part for the rest, with no lookup tables.
+//
 // Every loop bound comes from the job/step records or the queue length; the dispatch loop
 // ends once the counter passes the queue length, which every workgroup reaches.
 
@@ -93,6 +100,101 @@ __global__ __launch_bounds__(WG) void k_batch(ts_batch b) {
       if (b.job_done) b.job_done[j] = 1;
     }
     __syncthreads();
+  }
+}
+
+constexpr int LW = 64;  // k_batch_lds: one wave per workgroup
+
+// Offset of the bits of v (LSB-first, nbits of them) placed at positions pos[i];
+// TS_NOPOS entries contribute nothing.
+__device__ __forceinline__ uint32_t scatter_bits(uint32_t v, int nbits, const uint8_t* pos) {
+  uint32_t off = 0;
+  for (int i = 0; i < nbits; ++i) {
+    const uint32_t p = pos[i];
+    if (((v >> i) & 1u) && p != TS_NOPOS) off |= 1u << p;
+  }
+  return off;
+}
+
+// One Boolean step by one wave on an LDS arena: out index o = o0 + lane.
+__device__ inline void step_lds(const ts_step* __restrict__ sp, uint8_t* arena, int mode, int lane) {
+  const int ro = sp->out_rank, nc = sp->sum_rank, flags = sp->flags;
+  const bool norhs = flags & TS_STEP_NORHS, kinner = flags & TS_STEP_KINNER;
+  const uint8_t* lhs = arena + sp->lhs_off;
+  const uint8_t* rhs = arena + (norhs ? sp->lhs_off : sp->rhs_off);
+  uint8_t* out = arena + sp->out_off;
+  const int lowbits = ro < 6 ? ro : 6;
+  const uint32_t lo_l = scatter_bits((uint32_t)lane, lowbits, sp->o_lpos);
+  const uint32_t lo_r = scatter_bits((uint32_t)lane, lowbits, sp->o_rpos);
+  const uint32_t n = 1u << ro;
+  for (uint32_t o0 = 0; o0 < n; o0 += LW) {
+    const uint32_t hi = o0 >> 6;
+    const uint32_t hl = ro > 6 ? scatter_bits(hi, ro - 6, sp->o_lpos + 6) : 0u;
+    const uint32_t hr = ro > 6 ? scatter_bits(hi, ro - 6, sp->o_rpos + 6) : 0u;
+    const uint32_t o = o0 + (uint32_t)lane;
+    if (o >= n) continue;
+    const uint32_t offl = lo_l | hl, offr = lo_r | hr;
+    uint32_t acc;
+    if (nc == 0) {
+      acc = lhs[offl];
+      if (!norhs) acc &= rhs[offr];
+    } else if (kinner) {
+      acc = popc_and(lhs + offl, rhs + offr, nc, norhs);
+    } else {
+      acc = 0;
+      const uint32_t nterm = 1u << nc;
+      for (uint32_t c = 0; c < nterm; ++c) {
+        const uint8_t a = lhs[offl | scatter_bits(c, nc, sp->c_lpos)];
+        const uint8_t bb = norhs ? (uint8_t)1 : rhs[offr | scatter_bits(c, nc, sp->c_rpos)];
+        acc += a & bb;
+      }
+    }
+    out[o] = (uint8_t)(mode == TS_MODE_MOD256 ? ((acc & 255u) != 0) : (acc != 0));
+  }
+}
+
+__global__ __launch_bounds__(LW) void k_batch_lds(ts_batch b) {
+  extern __shared__ __attribute__((aligned(16))) uint8_t larena[];
+  const int lane = threadIdx.x;
+  const int num_jobs = b.num_jobs_dev ? *b.num_jobs_dev : b.num_jobs;
+  for (;;) {
+    int j = -1;
+    if (lane == 0) {
+      const int k = atomicAdd(b.counter, 1);
+      j = k < num_jobs ? b.order[k] : -1;
+      if (j >= 0 && b.early_exit && decided(b.verdicts, b.jobs[j].instance)) {
+        if (b.job_done) b.job_done[j] = 2;
+        j = -2;
+      }
+    }
+    j = __builtin_amdgcn_readfirstlane(__shfl(j, 0));
+    if (j == -1) break;
+    if (j < 0) continue;
+    const ts_job* __restrict__ jp = b.jobs + j;
+    const int nbytes = jp->num_leaf_bytes;
+    const uint8_t* src = b.leaves + jp->leaf_begin;
+    const int n16 = nbytes >> 4;
+    for (int i = lane; i < n16; i += LW)
+      reinterpret_cast<uint4*>(larena)[i] = reinterpret_cast<const uint4*>(src)[i];
+    for (int i = (n16 << 4) + lane; i < nbytes; i += LW) larena[i] = src[i];
+    __syncthreads();
+    const ts_step* __restrict__ sp = b.steps + jp->step_begin;
+    const int ns = jp->num_steps;
+    for (int s = 0; s < ns; ++s) {
+      step_lds(sp + s, larena, b.mode, lane);
+      __syncthreads();
+    }
+    // Decision jobs end in a scalar; a non-scalar result counts as satisfiable when any
+    // entry is set.
+    const int64_t nr = int64_t(1) << jp->result_rank;
+    uint32_t v = 0;
+    for (int64_t i = lane; i < nr; i += LW) v |= larena[jp->result_off + i];
+    const bool any = __ballot(v != 0) != 0;
+    if (lane == 0) {
+      if (any) atomicOr(&b.verdicts[jp->instance], 1u);
+      if (b.job_done) b.job_done[j] = 1;
+    }
+    __syncthreads();  // the next job's leaves overwrite the arena
   }
 }
 
@@ -201,13 +303,31 @@ int ts_batch_max_workgroups(int32_t* out) {
   return TS_OK;
 }
 
+int ts_batch_max_workgroups_lds(int64_t arena_bytes, int32_t* out) {
+  if (!out || arena_bytes <= 0 || arena_bytes > TS_BATCH_LDS_MAX || (arena_bytes & 255)) return TS_EINVAL;
+  int dev = 0, ncu = 0, per = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return TS_EHIP;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return TS_EHIP;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void*>(k_batch_lds), LW,
+                                                   (size_t)arena_bytes) != hipSuccess)
+    return TS_EHIP;
+  *out = ncu * (per > 0 ? per : 1);
+  return TS_OK;
+}
+
 int ts_batch_run(const ts_batch* b, ts_stream stream) {
   if (!b || b->num_jobs < 0 || b->num_workgroups <= 0 || !b->counter || !b->verdicts) return TS_EINVAL;
-  if (b->num_jobs > 0 && (!b->jobs || !b->order || !b->steps || !b->leaves || !b->arenas)) return TS_EINVAL;
+  if (b->num_jobs > 0 && (!b->jobs || !b->order || !b->steps || !b->leaves)) return TS_EINVAL;
+  if (b->num_jobs > 0 && !b->lds_arena && !b->arenas) return TS_EINVAL;
   if (b->arena_stride <= 0 || (b->arena_stride & 255)) return TS_EINVAL;
+  if (b->lds_arena && b->arena_stride > TS_BATCH_LDS_MAX) return TS_EINVAL;
   if (b->mode != TS_MODE_GT0 && b->mode != TS_MODE_MOD256) return TS_EINVAL;
   if (b->num_jobs == 0) return TS_OK;
-  hipLaunchKernelGGL(k_batch, dim3(b->num_workgroups), dim3(WG), 0, reinterpret_cast<hipStream_t>(stream), *b);
+  if (b->lds_arena)
+    hipLaunchKernelGGL(k_batch_lds, dim3(b->num_workgroups), dim3(LW), (size_t)b->arena_stride,
+                       reinterpret_cast<hipStream_t>(stream), *b);
+  else
+    hipLaunchKernelGGL(k_batch, dim3(b->num_workgroups), dim3(WG), 0, reinterpret_cast<hipStream_t>(stream), *b);
   return hipGetLastError() == hipSuccess ? TS_OK : TS_EHIP;
 }
 

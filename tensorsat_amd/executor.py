@@ -148,20 +148,31 @@ class BatchExecutor:
 
     def __init__(self, prog: Program, *, num_workgroups: int | None = None, device: str = "cuda",
                  order: np.ndarray | None = None, track_jobs: bool = False,
-                 round_bounds=ROUND_BOUNDS) -> None:
+                 round_bounds=ROUND_BOUNDS, lds: bool | None = None) -> None:
+        """``lds``: run every job on one wave with its arena in LDS (``k_batch_lds``);
+        default: whenever the largest arena fits (``BATCH_LDS_MAX`` bytes)."""
         torch = _torch()
         L = N.lib()
         self.dp = DeviceProgram(prog, device)
         self.prog = prog
         nj = len(prog.jobs)
-        if num_workgroups is None:
-            mx = N.i32(0)
-            N.check(L.ts_batch_max_workgroups(ctypes.byref(mx)), "ts_batch_max_workgroups")
-            num_workgroups = int(mx.value)
-        self.num_workgroups = max(1, min(int(num_workgroups), max(1, nj)))
         max_arena = int(prog.jobs["arena_elems"].max()) if nj else 16
         self.arena_stride = (max_arena + 255) // 256 * 256
-        self.arenas = torch.empty(self.num_workgroups * self.arena_stride, dtype=torch.uint8, device=device)
+        fits = self.arena_stride <= N.BATCH_LDS_MAX
+        self.lds = fits if lds is None else bool(lds)
+        if self.lds and not fits:
+            raise ValueError(f"job arenas of {self.arena_stride} bytes exceed the LDS limit")
+        if num_workgroups is None:
+            mx = N.i32(0)
+            if self.lds:
+                N.check(L.ts_batch_max_workgroups_lds(self.arena_stride, ctypes.byref(mx)),
+                        "ts_batch_max_workgroups_lds")
+            else:
+                N.check(L.ts_batch_max_workgroups(ctypes.byref(mx)), "ts_batch_max_workgroups")
+            num_workgroups = int(mx.value)
+        self.num_workgroups = max(1, min(int(num_workgroups), max(1, nj)))
+        n_arena = 16 if self.lds else self.num_workgroups * self.arena_stride
+        self.arenas = torch.empty(n_arena, dtype=torch.uint8, device=device)
         full = dispatch_order(prog) if order is None else np.asarray(order, dtype=np.int32)
         cubes = prog.jobs["cube"][full] if nj else np.zeros(0, np.int32)
         self.rounds: list[tuple[int, int]] = []
@@ -211,7 +222,7 @@ class BatchExecutor:
                 arena_stride=self.arena_stride, num_workgroups=self.num_workgroups, mode=N.MODES[mode],
                 verdicts=self.verdicts.data_ptr(), counter=cbase + 4 * (2 * r), early_exit=int(early_exit),
                 job_done=self.job_done.data_ptr() if self.job_done is not None else None,
-                num_jobs_dev=num_dev,
+                num_jobs_dev=num_dev, lds_arena=int(self.lds),
             )
             N.check(L.ts_batch_run(ctypes.byref(bt), stream), "ts_batch_run")
         if events is not None:

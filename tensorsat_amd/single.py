@@ -11,6 +11,8 @@ One large instance on the whole GPU.
   decided on the GPU by the batched executor with early exit; the first satisfiable cube
   (or a residual with every clause satisfied, found on the host) decides SAT, a completed
   enumeration without one decides UNSAT, and otherwise the answer is unknown (-1).
+* :func:`decide_learned` — the learned planner's search first, with the plain planner as
+  fallback, in alternating time slices of doubling length under one budget.
 * :func:`count` — the exact model count (#SAT) of one instance: all cubes of width W in
   one persistent launch of the counting kernel, result x 2^free summed per cube in
   checked uint64; raises ``OverflowError`` instead of returning a wrapped count.
@@ -122,6 +124,41 @@ def decide_budget(cnf: CNFInstance, *, width: int = 22, budget_s: float = 20.0, 
             max_nodes *= 2  # the search ran out of branch nodes before filling the chunk
         begin += len(prog.jobs)
         chunk *= 2
+
+
+def decide_learned(cnf: CNFInstance, priority: np.ndarray, *, width: int = 22, budget_s: float = 20.0,
+                   slice_s: float = 0.25, mode: str = "GT0", plan_trials: int = 8,
+                   gate_margin: float = 0.5) -> SingleResult:
+    """
+    The learned planner with the plain planner as fallback, under one budget: time
+    slices of ``decide_budget`` alternate between the learned search (learned branching
+    and tree proposals) and the plain search, each slice twice as long as the previous
+    pair's.  Every slice restarts its search, so the answer is whichever search decides
+    first; both are exact.
+    """
+    t_start = time.perf_counter()
+    plan_s = gpu_s = 0.0
+    cubes = 0
+    stats = {"learned_trees": 0, "fallback_trees": 0, "slices": 0}
+    s = slice_s
+    while True:
+        for learned in (True, False):
+            left = budget_s - (time.perf_counter() - t_start)
+            if left <= 0:
+                return SingleResult(-1, plan_s, gpu_s, cubes, cubes, stats)
+            kw = dict(priority=priority, prior_branch=True) if learned else {}
+            r = decide_budget(cnf, width=width, budget_s=min(s, left), mode=mode, plan_trials=plan_trials,
+                              gate_margin=gate_margin, **kw)
+            plan_s += r.plan_s
+            gpu_s += r.gpu_s
+            cubes += r.cubes
+            stats["slices"] += 1
+            stats["learned_trees"] += r.stats["learned_trees"]
+            stats["fallback_trees"] += r.stats["fallback_trees"]
+            if r.value >= 0:
+                return SingleResult(r.value, plan_s, gpu_s, cubes, cubes,
+                                    dict(stats, decided=r.stats.get("decided"), by="learned" if learned else "plain"))
+        s *= 2
 
 
 def count(cnf: CNFInstance, *, width: int = 18, plan_trials: int = 1,

@@ -89,6 +89,32 @@ class BatchSolver:
         if events is not None:
             events[1].record()
 
+    def capture(self) -> None:
+        """Record every pass (resets, compactions, persistent launches) into one HIP graph;
+        :meth:`replay` then decides the whole batch with a single graph launch.  The
+        graph holds kernel nodes only (the resets are fill kernels, not memsets)."""
+        import torch
+
+        torch.cuda.synchronize()
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            self.launch()  # warm the allocator and the code objects outside capture
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self.launch()
+        torch.cuda.synchronize()
+
+    def replay(self, events=None) -> None:
+        """One graph launch of the captured passes (see :meth:`capture`)."""
+        if events is not None:
+            events[0].record()
+        self.graph.replay()
+        if events is not None:
+            events[1].record()
+
     def verdicts(self) -> np.ndarray:
         out = np.zeros(self.n, dtype=np.uint8)
         for ex, sub in zip(self.execs, self.subsets):
