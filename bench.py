@@ -88,6 +88,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workgroups", type=int, default=0)
     ap.add_argument("--threads", type=int, default=0, help="host planning threads")
+    ap.add_argument("--kernel", default="auto", choices=["auto", "lds", "global"],
+                    help="batched kernel: LDS arenas, device-memory arenas, or auto (LDS when the arenas fit)")
     ap.add_argument("--workload", default="batched", choices=["batched", "dense", "learned", "copy"])
     ap.add_argument("--learned-instances", type=int, default=8)
     ap.add_argument("--learned-width", type=int, default=22, help="learned workload: cube width W")
@@ -121,7 +123,8 @@ def main() -> int:
     cnfs = batched_3sat(args.instances)
     log(f"generated {len(cnfs)} instances in {time.perf_counter() - t0:.1f}s")
     solver = BatchSolver(width=args.width, first_pass=args.first_pass, mode=args.mode, threads=args.threads,
-                         num_workgroups=args.workgroups or None)
+                         num_workgroups=args.workgroups or None,
+                         lds={"auto": None, "lds": True, "global": False}[args.kernel])
     t0 = time.perf_counter()
     solver.prepare(cnfs)
     prep_s = time.perf_counter() - t0
@@ -208,6 +211,7 @@ def main() -> int:
             "pass2_instances": st.pass2_instances,
             "mode": args.mode,
             "workgroups": solver.execs[0].num_workgroups,
+            "kernel": "k_batch_lds" if solver.execs[0].lds else "k_batch",
             "plan_s": st.plan_s,
             "end_to_end_s": st.plan_s + ms_per_step / 1000.0,
             "sat": int(verdicts.sum()),
@@ -222,7 +226,7 @@ def main() -> int:
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
-            "kernel": "k_batch",
+            "kernel": "k_batch_lds" if solver.execs[0].lds else "k_batch",
             "bytes_per_launch": bytes_per_launch,
         },
         "cpu_baseline": cpu,

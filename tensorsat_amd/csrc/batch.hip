@@ -15,12 +15,11 @@
 // (tensorsat/contractions/cotengra.py:120-126 -> _autoray.einsum -> FinRel._contract2,
 // fin_rel.py:255-288) and the final bool() (Box.__bool__, diagrams.py:949-958).
 //
-// Small jobs (arena up to 64 KiB: every job of cube widths up to 12 on the batched
-// workload) run on k_batch_lds instead: one wave per workgroup and per job, the job's
-// arena in LDS (leaves copied in, every intermediate written and read there), so a step
-// costs no L2 round trip and its barrier is a single-wave one.  Operand offsets come from
-// the step record's bit positions: a per-lane part for the 6 low out bits and a
-// wave-uniform part for the rest, with no lookup tables.
+// Jobs whose arenas fit in 64 KiB (every job of cube widths up to 12 on the batched
+// workload) can keep the arena in LDS instead (k_batch<true>): leaves are copied in and
+// every intermediate is written and read there.  Steps with at most 64 out entries (most
+// steps) then run on one wave with offsets scattered directly from the record, and a
+// workgroup barrier is taken only where a larger step (4-wave table engine) follows.
 //
 // Every loop bound comes from the job/step records or the queue length; the dispatch loop
 // ends once the counter passes the queue length, which every workgroup reaches.
@@ -41,12 +40,63 @@ __device__ __forceinline__ bool decided(const uint32_t* verdicts, int instance) 
   return __hip_atomic_load(&verdicts[instance], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
 }
 
+// Offset of the bits of v (LSB-first, nbits <= NB of them) placed at positions pos[i];
+// TS_NOPOS entries contribute nothing.
+template <int NB>
+__device__ __forceinline__ uint32_t scatter_bits(uint32_t v, int nbits, const uint8_t* pos) {
+  uint32_t off = 0;
+#pragma unroll
+  for (int i = 0; i < NB; ++i) {
+    if (i < nbits) {
+      const uint32_t p = pos[i];
+      if (((v >> i) & 1u) && p != TS_NOPOS) off |= 1u << p;
+    }
+  }
+  return off;
+}
+
+// Steps with at most 64 out entries run on one wave: lane = out index, offsets scattered
+// directly from the record's bit positions (no tables, no workgroup barrier).
+__device__ __forceinline__ bool small_step(const ts_step& st) { return st.out_rank <= 6; }
+
+__device__ inline void step_small(const ts_step& st, uint8_t* arena, int mode, int lane) {
+  const int ro = st.out_rank, nc = st.sum_rank, flags = st.flags;
+  if (lane >= (1 << ro)) return;
+  const bool norhs = flags & TS_STEP_NORHS, kinner = flags & TS_STEP_KINNER;
+  const uint8_t* lhs = arena + st.lhs_off;
+  const uint8_t* rhs = arena + (norhs ? st.lhs_off : st.rhs_off);
+  const uint32_t offl = scatter_bits<6>((uint32_t)lane, ro, st.o_lpos);
+  const uint32_t offr = scatter_bits<6>((uint32_t)lane, ro, st.o_rpos);
+  uint32_t acc;
+  if (nc == 0) {
+    acc = lhs[offl];
+    if (!norhs) acc &= rhs[offr];
+  } else if (kinner) {
+    acc = popc_and(lhs + offl, rhs + offr, nc, norhs);
+  } else {
+    acc = 0;
+    const uint32_t nterm = 1u << nc;
+    for (uint32_t c = 0; c < nterm; ++c) {
+      const uint8_t a = lhs[offl | scatter_bits<16>(c, nc, st.c_lpos)];
+      const uint8_t bb = norhs ? (uint8_t)1 : rhs[offr | scatter_bits<16>(c, nc, st.c_rpos)];
+      acc += a & bb;
+    }
+  }
+  arena[st.out_off + (uint32_t)lane] = (uint8_t)(mode == TS_MODE_MOD256 ? ((acc & 255u) != 0) : (acc != 0));
+}
+
+// LDSA: the job arena lives in LDS (dynamic shared memory of arena_stride bytes; every
+// tensor then has at most 16 bits, so two table chunks suffice) instead of device memory.
+template <bool LDSA>
 __global__ __launch_bounds__(WG) void k_batch(ts_batch b) {
-  __shared__ Lut L;
-  __shared__ ts_step s_steps[SREC];
+  constexpr int NQ = LDSA ? 2 : 4;
+  constexpr int NREC = LDSA ? 16 : SREC;
+  extern __shared__ __attribute__((aligned(16))) uint8_t dyn_arena[];
+  __shared__ LutT<NQ> L;
+  __shared__ ts_step s_steps[NREC];
   __shared__ int s_job;
   const int tid = threadIdx.x;
-  uint8_t* __restrict__ arena = b.arenas + (int64_t)blockIdx.x * b.arena_stride;
+  uint8_t* __restrict__ arena = LDSA ? dyn_arena : b.arenas + (int64_t)blockIdx.x * b.arena_stride;
   const int num_jobs = b.num_jobs_dev ? *b.num_jobs_dev : b.num_jobs;
   for (;;) {
     if (tid == 0) {
@@ -73,14 +123,25 @@ __global__ __launch_bounds__(WG) void k_batch(ts_batch b) {
         reinterpret_cast<uint4*>(arena)[i] = reinterpret_cast<const uint4*>(src)[i];
       for (int i = (n16 << 4) + tid; i < nbytes; i += WG) arena[i] = src[i];
     }
-    for (int s0 = 0; s0 < job.num_steps; s0 += SREC) {
-      const int ns = min(SREC, job.num_steps - s0);
-      // Stage up to 64 step records (128 bytes = 8 x uint4 each).
+    for (int s0 = 0; s0 < job.num_steps; s0 += NREC) {
+      const int ns = min(NREC, job.num_steps - s0);
+      // Stage up to NREC step records (128 bytes = 8 x uint4 each); every wave is past
+      // the previous chunk (its last step ends with a barrier).
       const uint4* src = reinterpret_cast<const uint4*>(b.steps + job.step_begin + s0);
       for (int i = tid; i < ns * 8; i += WG) reinterpret_cast<uint4*>(s_steps)[i] = src[i];
       __syncthreads();
       for (int s = 0; s < ns; ++s) {
         const ts_step& st = s_steps[s];
+        if (LDSA && small_step(st)) {
+          // Wave 0 alone; its LDS writes are ordered before its own later reads.  A
+          // workgroup barrier is needed only before a large step and at the chunk end.
+          if (tid < 64) {
+            step_small(st, arena, b.mode, tid);
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+          }
+          if (s + 1 == ns || !small_step(s_steps[s + 1])) __syncthreads();
+          continue;
+        }
         const bool need_c = !(st.flags & TS_STEP_KINNER) && st.sum_rank > 0;
         build_lut(st, L, need_c, tid);
         __syncthreads();
@@ -90,111 +151,19 @@ __global__ __launch_bounds__(WG) void k_batch(ts_batch b) {
       }
     }
     __syncthreads();  // leaves of step-less jobs were written by all threads
-    if (tid == 0) {
+    if (tid < 64) {
       // Decision jobs end in a scalar; a non-scalar result counts as satisfiable when
       // any entry is set.
-      uint32_t v = 0;
       const int64_t nr = int64_t(1) << job.result_rank;
-      for (int64_t i = 0; i < nr && !v; ++i) v = arena[job.result_off + i] != 0;
-      if (v) atomicOr(&b.verdicts[job.instance], 1u);
-      if (b.job_done) b.job_done[j] = 1;
-    }
-    __syncthreads();
-  }
-}
-
-constexpr int LW = 64;  // k_batch_lds: one wave per workgroup
-
-// Offset of the bits of v (LSB-first, nbits of them) placed at positions pos[i];
-// TS_NOPOS entries contribute nothing.
-__device__ __forceinline__ uint32_t scatter_bits(uint32_t v, int nbits, const uint8_t* pos) {
-  uint32_t off = 0;
-  for (int i = 0; i < nbits; ++i) {
-    const uint32_t p = pos[i];
-    if (((v >> i) & 1u) && p != TS_NOPOS) off |= 1u << p;
-  }
-  return off;
-}
-
-// One Boolean step by one wave on an LDS arena: out index o = o0 + lane.
-__device__ inline void step_lds(const ts_step* __restrict__ sp, uint8_t* arena, int mode, int lane) {
-  const int ro = sp->out_rank, nc = sp->sum_rank, flags = sp->flags;
-  const bool norhs = flags & TS_STEP_NORHS, kinner = flags & TS_STEP_KINNER;
-  const uint8_t* lhs = arena + sp->lhs_off;
-  const uint8_t* rhs = arena + (norhs ? sp->lhs_off : sp->rhs_off);
-  uint8_t* out = arena + sp->out_off;
-  const int lowbits = ro < 6 ? ro : 6;
-  const uint32_t lo_l = scatter_bits((uint32_t)lane, lowbits, sp->o_lpos);
-  const uint32_t lo_r = scatter_bits((uint32_t)lane, lowbits, sp->o_rpos);
-  const uint32_t n = 1u << ro;
-  for (uint32_t o0 = 0; o0 < n; o0 += LW) {
-    const uint32_t hi = o0 >> 6;
-    const uint32_t hl = ro > 6 ? scatter_bits(hi, ro - 6, sp->o_lpos + 6) : 0u;
-    const uint32_t hr = ro > 6 ? scatter_bits(hi, ro - 6, sp->o_rpos + 6) : 0u;
-    const uint32_t o = o0 + (uint32_t)lane;
-    if (o >= n) continue;
-    const uint32_t offl = lo_l | hl, offr = lo_r | hr;
-    uint32_t acc;
-    if (nc == 0) {
-      acc = lhs[offl];
-      if (!norhs) acc &= rhs[offr];
-    } else if (kinner) {
-      acc = popc_and(lhs + offl, rhs + offr, nc, norhs);
-    } else {
-      acc = 0;
-      const uint32_t nterm = 1u << nc;
-      for (uint32_t c = 0; c < nterm; ++c) {
-        const uint8_t a = lhs[offl | scatter_bits(c, nc, sp->c_lpos)];
-        const uint8_t bb = norhs ? (uint8_t)1 : rhs[offr | scatter_bits(c, nc, sp->c_rpos)];
-        acc += a & bb;
+      uint32_t v = 0;
+      for (int64_t i = tid; i < nr; i += 64) v |= arena[job.result_off + i];
+      const bool any = __ballot(v != 0) != 0;
+      if (tid == 0) {
+        if (any) atomicOr(&b.verdicts[job.instance], 1u);
+        if (b.job_done) b.job_done[j] = 1;
       }
     }
-    out[o] = (uint8_t)(mode == TS_MODE_MOD256 ? ((acc & 255u) != 0) : (acc != 0));
-  }
-}
-
-__global__ __launch_bounds__(LW) void k_batch_lds(ts_batch b) {
-  extern __shared__ __attribute__((aligned(16))) uint8_t larena[];
-  const int lane = threadIdx.x;
-  const int num_jobs = b.num_jobs_dev ? *b.num_jobs_dev : b.num_jobs;
-  for (;;) {
-    int j = -1;
-    if (lane == 0) {
-      const int k = atomicAdd(b.counter, 1);
-      j = k < num_jobs ? b.order[k] : -1;
-      if (j >= 0 && b.early_exit && decided(b.verdicts, b.jobs[j].instance)) {
-        if (b.job_done) b.job_done[j] = 2;
-        j = -2;
-      }
-    }
-    j = __builtin_amdgcn_readfirstlane(__shfl(j, 0));
-    if (j == -1) break;
-    if (j < 0) continue;
-    const ts_job* __restrict__ jp = b.jobs + j;
-    const int nbytes = jp->num_leaf_bytes;
-    const uint8_t* src = b.leaves + jp->leaf_begin;
-    const int n16 = nbytes >> 4;
-    for (int i = lane; i < n16; i += LW)
-      reinterpret_cast<uint4*>(larena)[i] = reinterpret_cast<const uint4*>(src)[i];
-    for (int i = (n16 << 4) + lane; i < nbytes; i += LW) larena[i] = src[i];
     __syncthreads();
-    const ts_step* __restrict__ sp = b.steps + jp->step_begin;
-    const int ns = jp->num_steps;
-    for (int s = 0; s < ns; ++s) {
-      step_lds(sp + s, larena, b.mode, lane);
-      __syncthreads();
-    }
-    // Decision jobs end in a scalar; a non-scalar result counts as satisfiable when any
-    // entry is set.
-    const int64_t nr = int64_t(1) << jp->result_rank;
-    uint32_t v = 0;
-    for (int64_t i = lane; i < nr; i += LW) v |= larena[jp->result_off + i];
-    const bool any = __ballot(v != 0) != 0;
-    if (lane == 0) {
-      if (any) atomicOr(&b.verdicts[jp->instance], 1u);
-      if (b.job_done) b.job_done[j] = 1;
-    }
-    __syncthreads();  // the next job's leaves overwrite the arena
   }
 }
 
@@ -296,7 +265,7 @@ int ts_batch_max_workgroups(int32_t* out) {
   int dev = 0, ncu = 0, per = 0;
   if (hipGetDevice(&dev) != hipSuccess) return TS_EHIP;
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return TS_EHIP;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void*>(k_batch), WG, 0) !=
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void*>(k_batch<false>), WG, 0) !=
       hipSuccess)
     return TS_EHIP;
   *out = ncu * (per > 0 ? per : 1);
@@ -308,7 +277,7 @@ int ts_batch_max_workgroups_lds(int64_t arena_bytes, int32_t* out) {
   int dev = 0, ncu = 0, per = 0;
   if (hipGetDevice(&dev) != hipSuccess) return TS_EHIP;
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return TS_EHIP;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void*>(k_batch_lds), LW,
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void*>(k_batch<true>), WG,
                                                    (size_t)arena_bytes) != hipSuccess)
     return TS_EHIP;
   *out = ncu * (per > 0 ? per : 1);
@@ -324,10 +293,11 @@ int ts_batch_run(const ts_batch* b, ts_stream stream) {
   if (b->mode != TS_MODE_GT0 && b->mode != TS_MODE_MOD256) return TS_EINVAL;
   if (b->num_jobs == 0) return TS_OK;
   if (b->lds_arena)
-    hipLaunchKernelGGL(k_batch_lds, dim3(b->num_workgroups), dim3(LW), (size_t)b->arena_stride,
+    hipLaunchKernelGGL((k_batch<true>), dim3(b->num_workgroups), dim3(WG), (size_t)b->arena_stride,
                        reinterpret_cast<hipStream_t>(stream), *b);
   else
-    hipLaunchKernelGGL(k_batch, dim3(b->num_workgroups), dim3(WG), 0, reinterpret_cast<hipStream_t>(stream), *b);
+    hipLaunchKernelGGL((k_batch<false>), dim3(b->num_workgroups), dim3(WG), 0, reinterpret_cast<hipStream_t>(stream),
+                       *b);
   return hipGetLastError() == hipSuccess ? TS_OK : TS_EHIP;
 }
 

@@ -30,22 +30,28 @@ namespace tsd {
 
 constexpr int WG = 256;
 
-struct Lut {
-  uint32_t ol[4][256];
-  uint32_t orr[4][256];
-  uint32_t cl[4][256];
-  uint32_t cr[4][256];
+// NQ byte chunks per table: 4 cover every rank up to 27 bits (TS_MAX_BITS); 2 suffice
+// when every tensor has at most 16 bits (LDS-resident arenas).
+template <int NQ>
+struct LutT {
+  uint32_t ol[NQ][256];
+  uint32_t orr[NQ][256];
+  uint32_t cl[NQ][256];
+  uint32_t cr[NQ][256];
 };
+using Lut = LutT<4>;
 
 __device__ __forceinline__ int nchunks(int bits) { return (bits + 7) >> 3; }
 
 // Build the chunk tables of one step (all threads of the workgroup participate; the
 // caller synchronises afterwards).
-__device__ inline void build_lut(const ts_step& st, Lut& L, bool need_c, int tid) {
+template <int NQ>
+__device__ inline void build_lut(const ts_step& st, LutT<NQ>& L, bool need_c, int tid) {
   const int ro = st.out_rank;
   // Chunk 0 is always built: a scalar step (out_rank 0) reads entry [0][0], which must
   // be 0 and not a value left in LDS by an earlier step.
-  const int nqo = nchunks(ro) > 0 ? nchunks(ro) : 1;
+  int nqo = nchunks(ro) > 0 ? nchunks(ro) : 1;
+  nqo = nqo < NQ ? nqo : NQ;  // callers keep ranks within the table (16 bits for NQ = 2)
   for (int e = tid; e < nqo * 256; e += WG) {
     const int q = e >> 8, v = e & 255;
     uint32_t al = 0, ar = 0;
@@ -63,7 +69,8 @@ __device__ inline void build_lut(const ts_step& st, Lut& L, bool need_c, int tid
   }
   if (need_c) {
     const int nc = st.sum_rank;
-    const int nqc = nchunks(nc) > 0 ? nchunks(nc) : 1;
+    int nqc = nchunks(nc) > 0 ? nchunks(nc) : 1;
+    nqc = nqc < NQ ? nqc : NQ;
     for (int e = tid; e < nqc * 256; e += WG) {
       const int q = e >> 8, v = e & 255;
       uint32_t al = 0, ar = 0;
@@ -83,19 +90,21 @@ __device__ inline void build_lut(const ts_step& st, Lut& L, bool need_c, int tid
 }
 
 // High-chunk part of an out offset (o >> 8), workgroup-uniform in the block loops.
+template <int NQ>
 __device__ __forceinline__ uint32_t hi_off(const uint32_t (*T)[256], uint32_t hi, int nq) {
   uint32_t r = 0;
   if (nq > 1) r += T[1][hi & 255];
-  if (nq > 2) r += T[2][(hi >> 8) & 255];
-  if (nq > 3) r += T[3][hi >> 16];
+  if (NQ > 2 && nq > 2) r += T[NQ > 2 ? 2 : 0][(hi >> 8) & 255];
+  if (NQ > 3 && nq > 3) r += T[NQ > 3 ? 3 : 0][hi >> 16];
   return r;
 }
 
+template <int NQ>
 __device__ __forceinline__ uint32_t c_off(const uint32_t (*T)[256], uint32_t c, int nq) {
   uint32_t r = T[0][c & 255];
   if (nq > 1) r += T[1][(c >> 8) & 255];
-  if (nq > 2) r += T[2][(c >> 16) & 255];
-  if (nq > 3) r += T[3][c >> 24];
+  if (NQ > 2 && nq > 2) r += T[NQ > 2 ? 2 : 0][(c >> 16) & 255];
+  if (NQ > 3 && nq > 3) r += T[NQ > 3 ? 3 : 0][c >> 24];
   return r;
 }
 
@@ -139,7 +148,8 @@ __device__ __forceinline__ uint32_t popc_and(const uint8_t* a, const uint8_t* b,
 
 // Boolean step over out indices o = first + k * stride (stride a multiple of 256 and
 // first % 256 == tid % 256 whenever out_rank >= 8).
-__device__ inline void step_u8(const ts_step& st, const Lut& L, const uint8_t* __restrict__ lhs,
+template <int NQ>
+__device__ inline void step_u8(const ts_step& st, const LutT<NQ>& L, const uint8_t* __restrict__ lhs,
                                const uint8_t* __restrict__ rhs, uint8_t* __restrict__ out,
                                uint64_t first, uint64_t stride, int mode) {
   const int ro = st.out_rank, nc = st.sum_rank;
@@ -152,8 +162,8 @@ __device__ inline void step_u8(const ts_step& st, const Lut& L, const uint8_t* _
   const uint32_t lo_r = L.orr[0][first & 255];
   for (uint64_t o = first; o < n; o += stride) {
     const uint32_t hi = (uint32_t)(o >> 8);
-    const uint32_t offl = lo_l + hi_off(L.ol, hi, nqo);
-    const uint32_t offr = lo_r + hi_off(L.orr, hi, nqo);
+    const uint32_t offl = lo_l + hi_off<NQ>(L.ol, hi, nqo);
+    const uint32_t offr = lo_r + hi_off<NQ>(L.orr, hi, nqo);
     uint32_t acc;
     if (nc == 0) {
       acc = lhs[offl];
@@ -164,8 +174,8 @@ __device__ inline void step_u8(const ts_step& st, const Lut& L, const uint8_t* _
       acc = 0;
       const uint32_t nterm = 1u << nc;
       for (uint32_t c = 0; c < nterm; ++c) {
-        const uint8_t a = lhs[offl + c_off(L.cl, c, nqc)];
-        const uint8_t b = norhs ? (uint8_t)1 : rhs[offr + c_off(L.cr, c, nqc)];
+        const uint8_t a = lhs[offl + c_off<NQ>(L.cl, c, nqc)];
+        const uint8_t b = norhs ? (uint8_t)1 : rhs[offr + c_off<NQ>(L.cr, c, nqc)];
         acc += a & b;
       }
     }
@@ -188,13 +198,13 @@ __device__ inline bool step_u64(const ts_step& st, const Lut& L, const uint64_t*
   const uint32_t lo_r = L.orr[0][first & 255];
   for (uint64_t o = first; o < n; o += stride) {
     const uint32_t hi = (uint32_t)(o >> 8);
-    const uint32_t offl = lo_l + hi_off(L.ol, hi, nqo);
-    const uint32_t offr = lo_r + hi_off(L.orr, hi, nqo);
+    const uint32_t offl = lo_l + hi_off<4>(L.ol, hi, nqo);
+    const uint32_t offr = lo_r + hi_off<4>(L.orr, hi, nqo);
     uint64_t acc = 0;
     const uint32_t nterm = 1u << nc;
     for (uint32_t c = 0; c < nterm; ++c) {
-      const uint32_t cl = kinner ? c : c_off(L.cl, c, nqc);
-      const uint32_t cr = kinner ? c : c_off(L.cr, c, nqc);
+      const uint32_t cl = kinner ? c : c_off<4>(L.cl, c, nqc);
+      const uint32_t cr = kinner ? c : c_off<4>(L.cr, c, nqc);
       const uint64_t a = lhs[offl + cl];
       const uint64_t b = norhs ? 1ull : rhs[offr + cr];
       uint64_t p;

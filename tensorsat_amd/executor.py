@@ -118,16 +118,17 @@ def count_program(dp: DeviceProgram) -> list[int]:
 def dispatch_order(prog: Program) -> np.ndarray:
     """
     Job order for the persistent queue: by cube index within the instance (so early
-    exit can skip later cubes of decided instances), then largest job first.
+    exit can skip later cubes of decided instances), then longest job first.
     """
     jobs = prog.jobs
     if len(jobs) == 0:
         return np.zeros(0, dtype=np.int32)
     steps = prog.steps
-    # Work estimate: sum over steps of 2^(out_rank + sum_rank).
+    # Latency estimate: steps run one after the other with a barrier each, and a step
+    # takes one pass of the workgroup per 256 out x summed entries.
     work = np.zeros(len(jobs), dtype=np.float64)
     if len(steps):
-        per = np.ldexp(1.0, steps["out_rank"].astype(np.int64) + steps["sum_rank"].astype(np.int64))
+        per = 1.0 + np.ldexp(1.0, steps["out_rank"].astype(np.int64) + steps["sum_rank"].astype(np.int64)) / 256.0
         csum = np.concatenate([[0.0], np.cumsum(per)])
         b = jobs["step_begin"].astype(np.int64)
         e = b + jobs["num_steps"].astype(np.int64)
@@ -149,7 +150,7 @@ class BatchExecutor:
     def __init__(self, prog: Program, *, num_workgroups: int | None = None, device: str = "cuda",
                  order: np.ndarray | None = None, track_jobs: bool = False,
                  round_bounds=ROUND_BOUNDS, lds: bool | None = None) -> None:
-        """``lds``: run every job on one wave with its arena in LDS (``k_batch_lds``);
+        """``lds``: keep each job's arena in LDS (the LDS variant of the batched kernel);
         default: whenever the largest arena fits (``BATCH_LDS_MAX`` bytes)."""
         torch = _torch()
         L = N.lib()

@@ -38,7 +38,8 @@ class SolveStats:
 
 class BatchSolver:
     def __init__(self, *, width: int = 10, first_pass: int = 4, mode: str = "GT0", threads: int = 0,
-                 plan_trials: int = 1, num_workgroups: int | None = None, track_jobs: bool = True) -> None:
+                 plan_trials: int = 1, num_workgroups: int | None = None, track_jobs: bool = True,
+                 lds: bool | None = None) -> None:
         self.width = width
         self.first_pass = first_pass
         self.mode = mode
@@ -46,6 +47,7 @@ class BatchSolver:
         self.plan_trials = plan_trials
         self.num_workgroups = num_workgroups
         self.track_jobs = track_jobs
+        self.lds = lds
         self.stats = SolveStats()
         self.progs: list[PG.Program] = []
         self.execs: list[BatchExecutor] = []
@@ -60,7 +62,7 @@ class BatchSolver:
         self.stats.plan_s = time.perf_counter() - t0
         self.stats.pass1_jobs = len(p1.jobs)
         self.stats.host_unsat = int((p1.stats[:, 5] == 0).sum())
-        ex1 = BatchExecutor(p1, num_workgroups=self.num_workgroups, track_jobs=self.track_jobs)
+        ex1 = BatchExecutor(p1, num_workgroups=self.num_workgroups, track_jobs=self.track_jobs, lds=self.lds)
         v1 = ex1.run(self.mode)
         self.progs = [p1]
         self.execs = [ex1]
@@ -76,7 +78,7 @@ class BatchSolver:
             if len(p2.jobs):
                 self.progs.append(p2)
                 self.execs.append(BatchExecutor(p2, num_workgroups=self.num_workgroups,
-                                                track_jobs=self.track_jobs))
+                                                track_jobs=self.track_jobs, lds=self.lds))
                 self.subsets.append(need)
 
     def launch(self, events=None) -> None:

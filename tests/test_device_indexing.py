@@ -139,76 +139,24 @@ def test_whole_network_indexing(enc):
         np.testing.assert_array_equal(got, ref)
 
 
-def _scatter(v, nbits, pos):
-    """scatter_bits of k_batch_lds: bits of v (LSB-first) to positions pos (NOPOS: none)."""
-    off = np.zeros_like(v)
-    for i in range(nbits):
-        p = int(pos[i])
-        if p != NOPOS:
-            off |= ((v >> i) & 1) << p
-    return off
-
-
-def emulate_job_lds(prog, j, mode, lds_bytes):
-    """The addressing of k_batch_lds: out index o = o0 + lane, offset = per-lane part of
-    the 6 low out bits | wave-uniform part of the rest; the arena is the LDS block."""
-    job = prog.jobs[j]
-    assert int(job["arena_elems"]) <= lds_bytes
-    arena = np.full(lds_bytes, 0xAB, np.uint8)  # LDS is not cleared between jobs
-    lv = prog.job_leaves(j)
-    arena[: lv.size] = lv
-    lane = np.arange(64, dtype=np.int64)
-    for st in prog.job_steps(j):
-        ro, nc = int(st["out_rank"]), int(st["sum_rank"])
-        rl, rr = int(st["lhs_rank"]), int(st["rhs_rank"])
-        norhs = bool(int(st["flags"]) & PG.STEP_NORHS)
-        kinner = bool(int(st["flags"]) & PG.STEP_KINNER)
-        lo_l = _scatter(lane, min(ro, 6), st["o_lpos"])
-        lo_r = _scatter(lane, min(ro, 6), st["o_rpos"])
-        n = 1 << ro
-        lb, rb, ob = int(st["lhs_off"]), int(st["lhs_off"] if norhs else st["rhs_off"]), int(st["out_off"])
-        out = np.zeros(n, np.uint8)
-        for o0 in range(0, n, 64):
-            hi = np.int64(o0 >> 6)
-            hl = _scatter(np.array([hi]), ro - 6, st["o_lpos"][6:])[0] if ro > 6 else 0
-            hr = _scatter(np.array([hi]), ro - 6, st["o_rpos"][6:])[0] if ro > 6 else 0
-            o = o0 + lane
-            live = o < n
-            offl, offr = (lo_l | hl)[live], (lo_r | hr)[live]
-            if nc == 0:
-                terms_l, terms_r = offl[:, None], offr[:, None]
-            elif kinner:
-                c = np.arange(1 << nc)
-                assert np.all((lb + offl) % min(16, 1 << nc) == 0), "unaligned vector load"
-                terms_l, terms_r = offl[:, None] + c, offr[:, None] + c
-            else:
-                c = np.arange(1 << nc)
-                terms_l = offl[:, None] | _scatter(c, nc, st["c_lpos"])[None, :]
-                terms_r = offr[:, None] | _scatter(c, nc, st["c_rpos"])[None, :]
-            assert terms_l.max() < (1 << rl) and lb + terms_l.max() < lds_bytes, "lhs outside operand"
-            a = arena[lb + terms_l].astype(np.int64)
-            if not norhs:
-                assert terms_r.max() < (1 << rr) and rb + terms_r.max() < lds_bytes, "rhs outside operand"
-                a = a & arena[rb + terms_r].astype(np.int64)
-            acc = a.sum(axis=1)
-            out[o[live]] = ((acc % 256) != 0) if mode == "MOD256" else (acc != 0)
-        assert ob + n <= int(job["arena_elems"])
-        arena[ob : ob + n] = out
-    r0, rr_ = int(job["result_off"]), int(job["result_rank"])
-    return arena[r0 : r0 + (1 << rr_)].copy()
-
-
 @pytest.mark.parametrize("mode", ["GT0", "MOD256"])
-def test_lds_kernel_indexing(mode):
+def test_lds_arena_programs(mode):
+    """Programs that the batched kernel runs with LDS arenas (largest arena <= 64 KiB):
+    every tensor then has at most 16 bits, which the two-chunk tables of the LDS variant
+    cover, and the table walk (as emulated above) equals the oracle on every job."""
     from oracle import executor as OX
+    from tensorsat_amd import _native as N
     from tensorsat_amd.workloads import batched_3sat
 
     cnfs = batched_3sat(24)
     for width in (8, 12):
         prog = PG.sat_batch(cnfs, width=width, max_cubes=2)
         PG.validate(prog)
-        lds = (int(prog.jobs["arena_elems"].max()) + 255) // 256 * 256
+        stride = (int(prog.jobs["arena_elems"].max()) + 255) // 256 * 256
+        assert stride <= N.BATCH_LDS_MAX
+        st = prog.steps
+        assert int(max(st["out_rank"].max(), st["sum_rank"].max(), st["lhs_rank"].max(), st["rhs_rank"].max())) <= 16
         for j in range(len(prog.jobs)):
-            got = emulate_job_lds(prog, j, mode, lds)
+            got = emulate_job(prog, j, mode)
             ref, _ = OX.run_job(prog, j, mode)
             np.testing.assert_array_equal(got, ref)
