@@ -88,6 +88,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workgroups", type=int, default=0)
     ap.add_argument("--threads", type=int, default=0, help="host planning threads")
+    ap.add_argument("--rounds", default="", help="dispatch round boundaries by cube index, e.g. 0,4,64 "
+                    "(default: the executor's)")
     ap.add_argument("--kernel", default="auto", choices=["auto", "lds", "global"],
                     help="batched kernel: LDS arenas, device-memory arenas, or auto (LDS when the arenas fit)")
     ap.add_argument("--workload", default="batched", choices=["batched", "dense", "learned", "copy"])
@@ -124,7 +126,8 @@ def main() -> int:
     log(f"generated {len(cnfs)} instances in {time.perf_counter() - t0:.1f}s")
     solver = BatchSolver(width=args.width, first_pass=args.first_pass, mode=args.mode, threads=args.threads,
                          num_workgroups=args.workgroups or None,
-                         lds={"auto": None, "lds": True, "global": False}[args.kernel])
+                         lds={"auto": None, "lds": True, "global": False}[args.kernel],
+                         round_bounds=[int(x) for x in args.rounds.split(",")] + [1 << 30] if args.rounds else None)
     t0 = time.perf_counter()
     solver.prepare(cnfs)
     prep_s = time.perf_counter() - t0
@@ -212,6 +215,7 @@ def main() -> int:
             "mode": args.mode,
             "workgroups": solver.execs[0].num_workgroups,
             "kernel": "k_batch_lds" if solver.execs[0].lds else "k_batch",
+            "rounds_per_pass": [len(ex.rounds) for ex in solver.execs],
             "plan_s": st.plan_s,
             "end_to_end_s": st.plan_s + ms_per_step / 1000.0,
             "sat": int(verdicts.sum()),

@@ -136,8 +136,12 @@ def dispatch_order(prog: Program) -> np.ndarray:
     return np.lexsort((-work, jobs["cube"])).astype(np.int32)
 
 
-ROUND_BOUNDS = (0, 1, 2, 4, 8, 16, 64, 1 << 30)
-"""Cube-index boundaries of the dispatch rounds: [0,1), [1,2), [2,4), ... ."""
+ROUND_BOUNDS = (0, 1 << 30)
+"""Cube-index boundaries of the dispatch rounds.  Default: one round per program; jobs
+of decided instances are skipped by the per-job check inside the persistent kernel.
+Finer boundaries such as (0, 1, 2, 4, 8, 16, 64, 1 << 30) add a compaction pass between
+rounds but each round then waits for its longest job (measured slower on the batched
+workload: 0.61 vs 0.41 ms per 4096 instances at W=10)."""
 
 
 class BatchExecutor:

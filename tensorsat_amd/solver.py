@@ -39,7 +39,7 @@ class SolveStats:
 class BatchSolver:
     def __init__(self, *, width: int = 10, first_pass: int = 4, mode: str = "GT0", threads: int = 0,
                  plan_trials: int = 1, num_workgroups: int | None = None, track_jobs: bool = True,
-                 lds: bool | None = None) -> None:
+                 lds: bool | None = None, round_bounds=None) -> None:
         self.width = width
         self.first_pass = first_pass
         self.mode = mode
@@ -48,6 +48,7 @@ class BatchSolver:
         self.num_workgroups = num_workgroups
         self.track_jobs = track_jobs
         self.lds = lds
+        self.round_bounds = round_bounds
         self.stats = SolveStats()
         self.progs: list[PG.Program] = []
         self.execs: list[BatchExecutor] = []
@@ -62,7 +63,8 @@ class BatchSolver:
         self.stats.plan_s = time.perf_counter() - t0
         self.stats.pass1_jobs = len(p1.jobs)
         self.stats.host_unsat = int((p1.stats[:, 5] == 0).sum())
-        ex1 = BatchExecutor(p1, num_workgroups=self.num_workgroups, track_jobs=self.track_jobs, lds=self.lds)
+        rb = {} if self.round_bounds is None else {"round_bounds": tuple(self.round_bounds)}
+        ex1 = BatchExecutor(p1, num_workgroups=self.num_workgroups, track_jobs=self.track_jobs, lds=self.lds, **rb)
         v1 = ex1.run(self.mode)
         self.progs = [p1]
         self.execs = [ex1]
@@ -78,7 +80,7 @@ class BatchSolver:
             if len(p2.jobs):
                 self.progs.append(p2)
                 self.execs.append(BatchExecutor(p2, num_workgroups=self.num_workgroups,
-                                                track_jobs=self.track_jobs, lds=self.lds))
+                                                track_jobs=self.track_jobs, lds=self.lds, **rb))
                 self.subsets.append(need)
 
     def launch(self, events=None) -> None:
