@@ -81,7 +81,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--instances", type=int, default=4096)
-    ap.add_argument("--width", type=int, default=10, help="cube width W (max residual width)")
+    ap.add_argument("--width", default="auto",
+                    help="cube width W (max residual width), or 'auto': autotuned on a 512-instance sample")
     ap.add_argument("--first-pass", type=int, default=4, help="cubes per instance in the first pass")
     ap.add_argument("--mode", default="GT0", choices=["GT0", "MOD256"])
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
@@ -124,7 +125,19 @@ def main() -> int:
     t0 = time.perf_counter()
     cnfs = batched_3sat(args.instances)
     log(f"generated {len(cnfs)} instances in {time.perf_counter() - t0:.1f}s")
-    solver = BatchSolver(width=args.width, first_pass=args.first_pass, mode=args.mode, threads=args.threads,
+    tune = None
+    if args.width == "auto":
+        from tensorsat_amd.solver import autotune_width
+
+        t0 = time.perf_counter()
+        width, table = autotune_width(cnfs, first_pass=args.first_pass, mode=args.mode, threads=args.threads)
+        tune = {"chosen": width, "seconds": time.perf_counter() - t0, "sample": 512,
+                "candidates": {str(k): v for k, v in table.items()}}
+        log(f"autotuned cube width W={width} in {tune['seconds']:.1f}s: " +
+            ", ".join(f"W={k}: plan {v['plan_s']:.3f}s + gpu {v['gpu_ms']:.3f}ms" for k, v in table.items()))
+    else:
+        width = int(args.width)
+    solver = BatchSolver(width=width, first_pass=args.first_pass, mode=args.mode, threads=args.threads,
                          num_workgroups=args.workgroups or None,
                          lds={"auto": None, "lds": True, "global": False}[args.kernel],
                          round_bounds=[int(x) for x in args.rounds.split(",")] + [1 << 30] if args.rounds else None)
@@ -207,7 +220,9 @@ def main() -> int:
             "workload": "batched random 3-SAT decisions: 4096 instances, n~U{20..80}, ratio 4.26/3.5 "
                         "(default_rng(10000+i)), conditioned slicing to cube width W, device-resident jobs",
             "instances": len(cnfs),
-            "cube_width": args.width,
+            "cube_width": width,
+            "autotune": tune,
+            "host_decided_unsat": st.host_unsat,
             "first_pass_cubes": args.first_pass,
             "jobs": st.pass1_jobs + st.pass2_jobs,
             "jobs_executed": jobs_executed,

@@ -136,3 +136,32 @@ class BatchSolver:
     def executed_jobs(self) -> list[np.ndarray]:
         """Per pass: boolean mask of jobs that ran in the last launch (needs track_jobs)."""
         return [ex.job_done.cpu().numpy()[: len(p.jobs)] == 1 for ex, p in zip(self.execs, self.progs)]
+
+
+def autotune_width(cnfs: list[CNFInstance], *, widths=(6, 8, 10, 12), sample: int = 512, first_pass: int = 4,
+                   mode: str = "GT0", reps: int = 3, threads: int = 0) -> tuple[int, dict]:
+    """
+    Pick the cube width W for a batch: on the first ``sample`` instances, each candidate
+    is planned (host time) and decided ``reps`` times on the GPU (device time); the width
+    with the smallest plan + decide time wins.  Returns (W, {W: measurements}).
+    Smaller W moves work from the contraction to the host search (more branches refuted
+    by propagation, smaller residuals); larger W means fewer, wider cubes.
+    """
+    import torch
+
+    sub = cnfs[:sample]
+    table = {}
+    for w in widths:
+        s = BatchSolver(width=w, first_pass=first_pass, mode=mode, threads=threads, track_jobs=False)
+        s.prepare(sub)
+        s.run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            s.launch()
+        torch.cuda.synchronize()
+        gpu_s = (time.perf_counter() - t0) / reps
+        table[w] = {"plan_s": s.stats.plan_s, "gpu_ms": gpu_s * 1000.0, "cost_s": s.stats.plan_s + gpu_s,
+                    "jobs": s.stats.pass1_jobs + s.stats.pass2_jobs, "host_unsat": s.stats.host_unsat}
+    best = min(table, key=lambda w: table[w]["cost_s"])
+    return best, table
