@@ -93,7 +93,8 @@ def main() -> int:
                     "(default: the executor's)")
     ap.add_argument("--kernel", default="auto", choices=["auto", "lds", "global"],
                     help="batched kernel: LDS arenas, device-memory arenas, or auto (LDS when the arenas fit)")
-    ap.add_argument("--workload", default="batched", choices=["batched", "dense", "learned", "copy"])
+    ap.add_argument("--workload", default="batched", choices=["batched", "dense", "learned", "copy", "single", "count"])
+    ap.add_argument("--seeds", type=int, default=3, help="single / count workloads: instances (seeds 0..)")
     ap.add_argument("--learned-instances", type=int, default=8)
     ap.add_argument("--learned-width", type=int, default=22, help="learned workload: cube width W")
     ap.add_argument("--learned-k", type=int, default=0, help="learned workload: keep only clauses of this k (0: all)")
@@ -113,6 +114,10 @@ def main() -> int:
         return bench_learned(args)
     if args.workload == "copy":
         return bench_copy(args)
+    if args.workload == "single":
+        return bench_single(args)
+    if args.workload == "count":
+        return bench_count(args)
 
     import torch
 
@@ -253,6 +258,144 @@ def main() -> int:
     }
     print(json.dumps(out), flush=True)
     return 0 if parity["mismatches"] == 0 and parity.get("dpll_mismatches", 0) == 0 else 1
+
+
+def _step_bytes(steps, elem: int = 1) -> float:
+    """Algorithmic bytes of a step list: 2^rank(lhs) + 2^rank(rhs) + 2^rank(out) entries."""
+    norhs = (steps["flags"] & 2) != 0
+    b = np.ldexp(1.0, steps["lhs_rank"].astype(np.int64)) + np.ldexp(1.0, steps["out_rank"].astype(np.int64))
+    b += np.where(norhs, 0.0, np.ldexp(1.0, steps["rhs_rank"].astype(np.int64)))
+    return float(b.sum()) * elem
+
+
+def bench_single(args) -> int:
+    """
+    One large random 3-SAT instance on the whole GPU (n=120, m=510, seeds 0..): conditioned
+    slicing into cubes of width up to 26 (tensors up to 2^26 bytes), cubes decided one
+    after the other with every step a grid-wide kernel (``single.decide``'s loop), until
+    the first satisfiable cube.  Reported: plan and GPU seconds per instance, and the step
+    kernels' algorithmic bytes over the GPU time (HBM bound).  Verdicts are checked against
+    the independent DPLL.
+    """
+    import torch
+
+    from tensorsat_amd import program as PG
+    from tensorsat_amd.executor import DeviceProgram, run_job
+    from tensorsat_amd.workloads import single_3sat
+
+    def log(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    rows = []
+    tot_bytes = tot_gpu = tot_plan = 0.0
+    mism = 0
+    for seed in range(args.seeds):
+        cnf = single_3sat(seed)
+        t0 = time.perf_counter()
+        prog = PG.sat_batch([cnf], width=26, plan_trials=2)
+        plan_s = time.perf_counter() - t0
+        dp = DeviceProgram(prog)
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        verdict, ran, nbytes = 0, 0, 0.0
+        a.record()
+        for j in range(len(prog.jobs)):
+            res, _ = run_job(dp, j, args.mode)
+            ran += 1
+            nbytes += _step_bytes(prog.job_steps(j))
+            if bool(res.any().item()):
+                verdict = 1
+                break
+        b.record()
+        torch.cuda.synchronize()
+        gpu_s = a.elapsed_time(b) / 1000.0
+        want = None
+        if not args.no_cpu:  # CPU checking leg: the independent DPLL
+            from oracle import dpll
+
+            want = int(dpll.solve(cnf.num_vars, cnf.clauses))
+            mism += int(verdict != want)
+        rows.append({"seed": seed, "verdict": verdict, "dpll": want, "plan_s": plan_s, "gpu_s": gpu_s,
+                     "cubes": len(prog.jobs), "cubes_run": ran, "max_rank": int(prog.stats[0, 4]), "bytes": nbytes})
+        tot_bytes += nbytes
+        tot_gpu += gpu_s
+        tot_plan += plan_s
+        log(f"seed {seed}: verdict {verdict} (dpll {want}), plan {plan_s:.2f}s, gpu {gpu_s:.3f}s, "
+            f"{ran}/{len(prog.jobs)} cubes, {nbytes / gpu_s / 1e9:.0f} GB/s")
+    gbs = tot_bytes / tot_gpu / 1e9
+    print(json.dumps({
+        "metric": "plan+contract s per instance, single n=120 3-SAT decision (bit-exact)",
+        "value": (tot_plan + tot_gpu) / len(rows), "unit": "s", "n_gpus": 1, "steps": len(rows), "warmup": 0,
+        "ms_per_step": tot_gpu * 1000.0 / len(rows), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+        "config": {"workload": "single random 3-SAT n=120 m=510 (seeds 0..), cubes of width <= 26, grid-wide steps",
+                   "plan_s_mean": tot_plan / len(rows), "gpu_s_mean": tot_gpu / len(rows), "rows": rows},
+        "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                     "traffic": None, "kernel": "k_step_u8 (all steps of the executed cubes, launch gaps included)"},
+        "cpu_baseline": None,
+        "parity": {"checked": len(rows), "dpll_mismatches": mism},
+    }), flush=True)
+    return 0 if mism == 0 else 1
+
+
+def bench_count(args) -> int:
+    """
+    Exact model counts (#SAT) of random 3-SAT n=100, m=300 (seeds 0..): cubes of width 18
+    counted in uint64 by the persistent counting kernel, each cube's result times
+    2^free added per instance with overflow detection.  Parity: the same count from a
+    second decomposition (width 16) - a dropped, doubled or wrapped term would change it.
+    """
+    import torch
+
+    from tensorsat_amd import program as PG
+    from tensorsat_amd.executor import BatchCounter
+    from tensorsat_amd.workloads import count_3sat
+
+    def log(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    rows = []
+    mism = 0
+    tot_gpu = tot_plan = tot_bytes = 0.0
+    for seed in range(args.seeds):
+        cnf = count_3sat(seed)
+        t0 = time.perf_counter()
+        prog = PG.sat_batch([cnf], width=18, plan_trials=1)
+        plan_s = time.perf_counter() - t0
+        bc = BatchCounter(prog)
+        bc.run()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        a.record()
+        bc.launch()
+        b.record()
+        torch.cuda.synchronize()
+        gpu_s = a.elapsed_time(b) / 1000.0
+        (c18,) = bc.run()
+        (c16,) = BatchCounter(PG.sat_batch([cnf], width=16, plan_trials=1)).run()
+        mism += int(c18 != c16)
+        nbytes = _step_bytes(prog.steps, 8)
+        rows.append({"seed": seed, "count": str(c18), "count_w16": str(c16), "plan_s": plan_s, "gpu_s": gpu_s,
+                     "cubes": len(prog.jobs), "bytes": nbytes})
+        tot_gpu += gpu_s
+        tot_plan += plan_s
+        tot_bytes += nbytes
+        log(f"seed {seed}: count {c18} (width 16: {c16}), plan {plan_s:.2f}s, gpu {gpu_s:.3f}s, {len(prog.jobs)} cubes")
+    gbs = tot_bytes / tot_gpu / 1e9
+    print(json.dumps({
+        "metric": "plan+contract s per instance, exact #SAT n=100 m=300 (uint64, checked)",
+        "value": (tot_plan + tot_gpu) / len(rows), "unit": "s", "n_gpus": 1, "steps": len(rows), "warmup": 1,
+        "ms_per_step": tot_gpu * 1000.0 / len(rows), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u64", "data": "synthetic",
+        "config": {"workload": "#SAT of random 3-SAT n=100 m=300 (seeds 0..), cubes of width <= 18, "
+                               "persistent uint64 counting kernel",
+                   "plan_s_mean": tot_plan / len(rows), "gpu_s_mean": tot_gpu / len(rows), "rows": rows},
+        "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                     "traffic": None, "kernel": "k_batch_count"},
+        "cpu_baseline": None,
+        "parity": {"checked": len(rows), "decomposition_mismatches": mism},
+    }), flush=True)
+    return 0 if mism == 0 else 1
 
 
 def bench_copy(args) -> int:

@@ -11,6 +11,7 @@
 // and compiled into one job.  The enumeration does not depend on contraction results.
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <cstring>
@@ -81,6 +82,14 @@ struct Enumerator {
   std::vector<Job>* jobs_out;
   std::vector<int32_t>* free_out;
   const float* prio = nullptr;  // per-variable priorities of this instance (learned planner)
+  // Split mode (parallel enumeration): nodes that would branch at depth split_depth are
+  // recorded as subtrees (their decision literals) instead of being searched; items
+  // keeps the depth-first order of emitted jobs (kind 0) and subtrees (kind 1).
+  int split_depth = -1;
+  int depth = 0;
+  std::vector<int> decisions;
+  std::vector<std::vector<int>>* frontier = nullptr;
+  std::vector<std::pair<int, int>>* items = nullptr;
   bool budget_hit = false, sat_found = false;
   int64_t learned_trees = 0, fallback_trees = 0;
 
@@ -225,6 +234,7 @@ struct Enumerator {
     max_rank = std::max(max_rank, job.max_rank);
     jobs_out->push_back(std::move(job));
     free_out->push_back(free_vars);
+    if (items) items->push_back({0, (int)jobs_out->size() - 1});
     ++emitted;
     if (rclauses_idx.empty()) {
       ++empty_residual;
@@ -285,6 +295,11 @@ struct Enumerator {
       truncated = true;
       return;
     }
+    if (split_depth >= 0 && depth >= split_depth) {
+      items->push_back({1, (int)frontier->size()});
+      frontier->push_back(decisions);
+      return;
+    }
     ++nodes;
     int bv = rv[0];
     if (prio && o.prior_branch) {
@@ -297,8 +312,15 @@ struct Enumerator {
     int first = occ_pos[bv] >= occ_neg[bv] ? bv + 1 : -(bv + 1);
     for (int l : {first, -first}) {
       size_t mark = trail.size();
-      if (assign(l) && propagate()) search();
-      else ++pruned;
+      if (assign(l) && propagate()) {
+        decisions.push_back(l);
+        ++depth;
+        search();
+        --depth;
+        decisions.pop_back();
+      } else {
+        ++pruned;
+      }
       queue.clear();
       undo(mark);
       if (truncated) return;
@@ -540,49 +562,126 @@ int tsc_sat_batch(int32_t num_inst, const int32_t* nvars, const int64_t* clause_
   }
   std::atomic<int> next{0};
   std::atomic<int> bad{0};
-  auto worker = [&]() {
-    for (;;) {
-      int i = next.fetch_add(1);
-      if (i >= num_inst) break;
-      auto t0 = std::chrono::steady_clock::now();
-      PerInst& r = res[i];
-      try {
-        Formula f = normalise(nvars[i], lits, lit_offsets, clause_offsets[i], clause_offsets[i + 1]);
-        for (auto& c : f.clauses)
-          for (int l : c)
-            if (l == 0 || std::abs(l) > f.n) throw std::invalid_argument("literal");
-        const float* pr = opts->priority ? opts->priority + var_offset[i] : nullptr;
-        Enumerator e(f, *opts, opts->seed + (uint64_t)i * 1000003u, &r.jobs, &r.free_vars, pr);
-        bool alive = !f.has_empty && e.initial_units();
-        if (alive) e.search();
-        r.stats[0] = (int32_t)r.jobs.size();
-        r.stats[1] = (int32_t)e.nodes;
-        r.stats[2] = (int32_t)e.pruned + (alive ? 0 : 1);
-        r.stats[3] = e.budget_hit ? 2 : (e.truncated ? 1 : 0);
-        r.stats[4] = e.max_rank;
-        // host verdict: 0 every branch pruned (UNSAT), 1 an all-satisfied cube was reached
-        // with stop_on_sat, -1 undecided on the host
-        r.stats[5] = e.sat_found ? 1 : (e.seen == 0 && !e.budget_hit ? 0 : -1);
-        r.stats[7] = (int32_t)e.empty_residual;
-        r.stats[8] = (int32_t)e.learned_trees;
-        r.stats[9] = (int32_t)e.fallback_trees;
-      } catch (const std::invalid_argument&) {
-        r.ok = false;
-        bad++;
-      } catch (...) {
-        r.ok = false;
-        bad++;
-      }
-      auto t1 = std::chrono::steady_clock::now();
-      r.stats[6] = (int32_t)std::chrono::duration_cast<std::chrono::microseconds>(t1 - t0).count();
-    }
-  };
   int threads = opts->threads > 0 ? opts->threads : (int)std::thread::hardware_concurrency();
-  threads = std::max(1, std::min(threads, std::max(1, num_inst)));
-  std::vector<std::thread> pool;
-  for (int t = 1; t < threads; ++t) pool.emplace_back(worker);
-  worker();
-  for (auto& th : pool) th.join();
+  threads = std::max(1, threads);
+  // Few instances and a full enumeration: split each instance's search tree at a fixed
+  // branch depth and enumerate the subtrees in parallel.  Concatenating the subtrees'
+  // jobs in left-to-right order gives exactly the sequential depth-first order.
+  const bool full = opts->max_cubes == 0 && opts->cube_begin == 0 && !opts->stop_on_sat && opts->max_nodes == 0;
+  const bool intra = full && threads > 1 && num_inst * 2 <= threads;
+  auto solve_one = [&](int i, int inner) {
+    auto t0 = std::chrono::steady_clock::now();
+    PerInst& r = res[i];
+    try {
+      Formula f = normalise(nvars[i], lits, lit_offsets, clause_offsets[i], clause_offsets[i + 1]);
+      for (auto& c : f.clauses)
+        for (int l : c)
+          if (l == 0 || std::abs(l) > f.n) throw std::invalid_argument("literal");
+      const float* pr = opts->priority ? opts->priority + var_offset[i] : nullptr;
+      const uint64_t sd = opts->seed + (uint64_t)i * 1000003u;
+      std::vector<Job> jobsA;
+      std::vector<int32_t> freeA;
+      std::vector<std::vector<int>> frontier;
+      std::vector<std::pair<int, int>> items;
+      Enumerator e(f, *opts, sd, inner > 1 ? &jobsA : &r.jobs, inner > 1 ? &freeA : &r.free_vars, pr);
+      if (inner > 1) {
+        int d = 0;
+        while ((1 << d) < 8 * inner) ++d;
+        e.split_depth = d;
+        e.frontier = &frontier;
+        e.items = &items;
+      }
+      bool alive = !f.has_empty && e.initial_units();
+      if (alive) e.search();
+      int64_t nodes = e.nodes, pruned = e.pruned, empty = e.empty_residual, lt = e.learned_trees,
+              ft = e.fallback_trees;
+      int max_rank = e.max_rank;
+      int64_t seen = e.seen;
+      if (inner > 1) {
+        const int nf = (int)frontier.size();
+        std::vector<std::vector<Job>> jobsB(nf);
+        std::vector<std::vector<int32_t>> freeB(nf);
+        std::vector<std::array<int64_t, 7>> st(nf);
+        std::atomic<int> nx{0};
+        auto sub = [&]() {
+          for (;;) {
+            const int k = nx.fetch_add(1);
+            if (k >= nf) break;
+            Enumerator eb(f, *opts, sd + 7919u * (uint64_t)(k + 1), &jobsB[k], &freeB[k], pr);
+            bool ok = eb.initial_units();
+            for (int l : frontier[k]) {
+              if (!ok) break;
+              ok = eb.assign(l) && eb.propagate();
+            }
+            if (ok) eb.search();
+            st[k] = {eb.nodes, eb.pruned + (ok ? 0 : 1), eb.empty_residual, eb.learned_trees, eb.fallback_trees,
+                     eb.max_rank, eb.seen};
+          }
+        };
+        std::vector<std::thread> workers;
+        for (int t = 1; t < std::min(inner, std::max(1, nf)); ++t) workers.emplace_back(sub);
+        sub();
+        for (auto& th : workers) th.join();
+        for (const auto& it : items) {
+          if (it.first == 0) {
+            r.jobs.push_back(std::move(jobsA[it.second]));
+            r.free_vars.push_back(freeA[it.second]);
+          } else {
+            auto& jb = jobsB[it.second];
+            for (size_t c = 0; c < jb.size(); ++c) {
+              r.jobs.push_back(std::move(jb[c]));
+              r.free_vars.push_back(freeB[it.second][c]);
+            }
+            jb.clear();
+          }
+        }
+        for (const auto& x : st) {
+          nodes += x[0];
+          pruned += x[1];
+          empty += x[2];
+          lt += x[3];
+          ft += x[4];
+          max_rank = std::max<int>(max_rank, (int)x[5]);
+          seen += x[6];
+        }
+      }
+      r.stats[0] = (int32_t)r.jobs.size();
+      r.stats[1] = (int32_t)std::min<int64_t>(nodes, INT32_MAX);
+      r.stats[2] = (int32_t)std::min<int64_t>(pruned + (alive ? 0 : 1), INT32_MAX);
+      r.stats[3] = e.budget_hit ? 2 : (e.truncated ? 1 : 0);
+      r.stats[4] = max_rank;
+      // host verdict: 0 every branch pruned (UNSAT), 1 an all-satisfied cube was reached
+      // with stop_on_sat, -1 undecided on the host
+      r.stats[5] = e.sat_found ? 1 : (seen == 0 && !e.budget_hit ? 0 : -1);
+      r.stats[7] = (int32_t)empty;
+      r.stats[8] = (int32_t)lt;
+      r.stats[9] = (int32_t)ft;
+    } catch (const std::invalid_argument&) {
+      r.ok = false;
+      bad++;
+    } catch (...) {
+      r.ok = false;
+      bad++;
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    r.stats[6] = (int32_t)std::chrono::duration_cast<std::chrono::microseconds>(t1 - t0).count();
+  };
+  if (intra) {
+    for (int i = 0; i < num_inst; ++i) solve_one(i, threads);
+  } else {
+    auto worker = [&]() {
+      for (;;) {
+        int i = next.fetch_add(1);
+        if (i >= num_inst) break;
+        solve_one(i, 1);
+      }
+    };
+    const int nt = std::max(1, std::min(threads, std::max(1, num_inst)));
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; ++t) pool.emplace_back(worker);
+    worker();
+    for (auto& th : pool) th.join();
+  }
   if (bad) return TSP_EINVAL;
   auto* prog = new (std::nothrow) ts_program();
   if (!prog) return TSP_ENOMEM;

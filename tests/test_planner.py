@@ -115,3 +115,21 @@ def test_cube_passes_partition_the_cubes():
     c_split = [a + b for a, b in zip(OX.count(p1), OX.count(p2))]
     assert c_full == c_split
     assert len(p1.jobs) + len(p2.jobs) == len(full.jobs)
+
+
+@pytest.mark.parametrize("width", [6, 10])
+def test_parallel_enumeration_reproduces_sequential_program(width):
+    """One instance, full enumeration: the search tree is split at a fixed depth and the
+    subtrees are enumerated by 8 threads; concatenated in order they give exactly the
+    sequential program (deterministic trees with one planning trial)."""
+    from tensorsat_amd.workloads import count_3sat
+
+    cnf = count_3sat(1)
+    seq = PG.sat_batch([cnf], width=width, plan_trials=1, threads=1)
+    par = PG.sat_batch([cnf], width=width, plan_trials=1, threads=8)
+    assert len(seq.jobs) == len(par.jobs) > 64
+    for f in ("num_steps", "num_leaf_bytes", "result_rank", "free_vars", "arena_elems", "cube"):
+        np.testing.assert_array_equal(seq.jobs[f], par.jobs[f])
+    np.testing.assert_array_equal(seq.steps.view(np.uint8), par.steps.view(np.uint8))
+    np.testing.assert_array_equal(seq.leaves, par.leaves)
+    assert seq.stats[0, 1] == par.stats[0, 1]  # branch nodes
