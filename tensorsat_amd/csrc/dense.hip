@@ -62,15 +62,23 @@ __global__ __launch_bounds__(256) void k_step_mfma(const uint8_t* __restrict__ l
   __shared__ __attribute__((aligned(16))) uint8_t sB[2][BN][LDSW];
   __shared__ uint32_t rowL[BM], rowO[BM], colR[BN], colO[BN];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  // Tile order: n fastest, then m, then batch.  Consecutive workgroups land on different
-  // XCDs, so regroup them: workgroup w runs on XCD w % 8; give each XCD a contiguous run
-  // of tiles (shared rows of lhs stay in that XCD's L2).
+  // Tile order.  Consecutive workgroups land on different XCDs (w % 8), so first give
+  // each XCD a contiguous run of tile ids; within a batch index, ids then walk groups of
+  // 8 tile rows column by column, so the workgroups resident on one XCD at a time cover
+  // a compact block of rows and columns and share their K chunks through that XCD's L2
+  // (row-major order would stream the whole rhs once per tile row).
   const uint64_t nwg = gridDim.x;
   uint64_t w = blockIdx.x;
   if ((nwg & 7) == 0) w = (w & 7) * (nwg >> 3) + (w >> 3);
-  const uint64_t nt = w % a.ntiles;
-  const uint64_t mt = (w / a.ntiles) % a.mtiles;
-  const uint64_t b = w / (a.ntiles * a.mtiles);
+  const uint64_t tiles = (uint64_t)a.mtiles * (uint64_t)a.ntiles;
+  const uint64_t b = w / tiles;
+  const uint64_t t = w % tiles;
+  constexpr uint64_t G = 8;
+  const uint64_t gspan = G * (uint64_t)a.ntiles;
+  const uint64_t first_m = (t / gspan) * G;
+  const uint64_t gsize = ((uint64_t)a.mtiles - first_m) < G ? ((uint64_t)a.mtiles - first_m) : G;
+  const uint64_t mt = first_m + (t % gspan) % gsize;
+  const uint64_t nt = (t % gspan) / gsize;
   const uint64_t m0 = mt * BM, n0 = nt * BN;
   const uint64_t mcount = 1ull << a.mbits, ncount = 1ull << a.nbits;
   for (int i = tid; i < BM; i += 256) {
