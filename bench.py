@@ -33,6 +33,14 @@ import numpy as np
 METRIC = "instances/s batched 3-SAT (bit-exact) ; plan+contract s, learned vs plain"
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per timed step measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (scaled
+# by the copy calibration: FETCH x 2.0, WRITE x 1.0) for the configurations profiled;
+# other configurations report null.
+MEASURED_TRAFFIC = {
+    ("batched", 6, 4096, 4): (15.35e6, "profiles/k_batch_lds_batched3sat_w6.md"),
+    ("dense", 13, 13, 13): (1.468e9, "profiles/k_step_mfma_dense_composition_13_13_13.md"),
+}
+
 
 def job_bytes(prog) -> np.ndarray:
     """Algorithmic bytes of each job: leaves once, then every step reads its operands
@@ -249,7 +257,8 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": MEASURED_TRAFFIC.get(("batched", width, len(cnfs), args.first_pass), (None, None))[0],
+            "traffic_source": MEASURED_TRAFFIC.get(("batched", width, len(cnfs), args.first_pass), (None, None))[1],
             "kernel": "k_batch_lds" if solver.execs[0].lds else "k_batch",
             "bytes_per_launch": bytes_per_launch,
         },
@@ -607,7 +616,9 @@ def bench_dense(args) -> int:
                    "kernel_ms": kernel_ms, "valu_ms": valu_ms, "bytes": nbytes,
                    "hbm_gbs": nbytes / (kernel_ms / 1000.0) / 1e9},
         "roofline": {"bound": "mfma", "achieved": tops, "peak": peak, "unit": "TFLOP/s", "frac": tops / peak,
-                     "traffic": None, "kernel": "k_step_mfma"},
+                     "traffic": MEASURED_TRAFFIC.get(("dense", mb, nb, kb), (None, None))[0],
+                     "traffic_source": MEASURED_TRAFFIC.get(("dense", mb, nb, kb), (None, None))[1],
+                     "kernel": "k_step_mfma"},
         "cpu_baseline": cpu,
         "parity": {"sampled_rows": len(rows), "mismatches": mism, "valu_mismatches": valu_mism},
     }
