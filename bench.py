@@ -101,7 +101,7 @@ def main() -> int:
                     "(default: the executor's)")
     ap.add_argument("--kernel", default="auto", choices=["auto", "lds", "global"],
                     help="batched kernel: LDS arenas, device-memory arenas, or auto (LDS when the arenas fit)")
-    ap.add_argument("--workload", default="batched", choices=["batched", "dense", "learned", "copy", "single", "count"])
+    ap.add_argument("--workload", default="batched", choices=["batched", "dense", "learned", "copy", "single", "count", "rewire"])
     ap.add_argument("--seeds", type=int, default=3, help="single / count workloads: instances (seeds 0..)")
     ap.add_argument("--learned-instances", type=int, default=8)
     ap.add_argument("--learned-width", type=int, default=22, help="learned workload: cube width W")
@@ -124,6 +124,8 @@ def main() -> int:
         return bench_copy(args)
     if args.workload == "single":
         return bench_single(args)
+    if args.workload == "rewire":
+        return bench_rewire(args)
     if args.workload == "count":
         return bench_count(args)
 
@@ -403,6 +405,94 @@ def bench_count(args) -> int:
                      "traffic": None, "kernel": "k_batch_count"},
         "cpu_baseline": None,
         "parity": {"checked": len(rows), "decomposition_mismatches": mism},
+    }), flush=True)
+    return 0 if mism == 0 else 1
+
+
+def bench_rewire(args) -> int:
+    """
+    Kernels (a) and (b) of ts_rewire on a 2^27-byte relation (the largest device rank): a
+    random permutation of its 27 bit axes (LDS-staged tiles, 2 x 2^27 bytes moved) and an
+    existential reduction over 11 of them (wave per out entry, ballot OR; 2^27 + 2^16
+    bytes).  GB/s of algorithmic bytes
+    against the 8 TB/s HBM peak; a plain device copy of the same bytes is timed beside the
+    permutation.  Parity: sampled entries against numpy on the host.
+    """
+    import torch
+
+    from tensorsat_amd import _native as N
+
+    def log(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    r = 27
+    g = torch.Generator(device="cuda").manual_seed(3)
+    x = (torch.rand(1 << r, device="cuda", generator=g) < 0.5).to(torch.uint8)
+    rng = np.random.default_rng(5)
+    perm = [int(p) for p in rng.permutation(r)]
+    keep = sorted(int(p) for p in rng.choice(r, size=16, replace=False))
+    out_p = torch.empty(1 << r, dtype=torch.uint8, device="cuda")
+    out_r = torch.empty(1 << 16, dtype=torch.uint8, device="cuda")
+    L = N.lib()
+
+    def rw(out, ports):
+        N.check(L.ts_rewire(N.dptr(x), r, N.dptr(out), len(ports), N.i32arr(ports), N.MODES[args.mode],
+                            N.stream_ptr()), "ts_rewire")
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(args.steps):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / args.steps
+
+    ms_p = timed(lambda: rw(out_p, perm))
+    ms_r = timed(lambda: rw(out_r, keep))
+    scratch = torch.empty_like(x)
+    ms_t = timed(lambda: scratch.copy_(x))  # a plain copy of the same bytes: the ideal
+    gbs_p = 2.0 * (1 << r) / (ms_p / 1000.0) / 1e9
+    gbs_r = ((1 << r) + (1 << 16)) / (ms_r / 1000.0) / 1e9
+    gbs_t = 2.0 * (1 << r) / (ms_t / 1000.0) / 1e9
+    # parity on sampled entries
+    xh = x.cpu().numpy().reshape((2,) * r)
+    ph = out_p.cpu().numpy()
+    idx = rng.integers(0, 1 << r, size=4096)
+    bits = ((idx[:, None] >> np.arange(r - 1, -1, -1)) & 1)  # out axis j = bit r-1-j
+    src = np.zeros((len(idx), r), np.int64)
+    for j, p in enumerate(perm):
+        src[:, p] = bits[:, j]
+    mism = int((ph[idx] != xh[tuple(src.T)]).sum())
+    red = np.asarray(xh).reshape(-1)
+    rh = out_r.cpu().numpy()
+    ridx = rng.integers(0, 1 << 16, size=64)
+    drop = [p for p in range(r) if p not in keep]
+    for o in ridx:
+        sl = [slice(None)] * r
+        for j, p in enumerate(keep):
+            sl[p] = (int(o) >> (15 - j)) & 1
+        cnt = int(xh[tuple(sl)].sum())
+        want = (cnt > 0) if args.mode == "GT0" else (cnt % 256 != 0)
+        mism += int(rh[o] != want)
+    log(f"permute {ms_p:.3f} ms ({gbs_p:.0f} GB/s; plain copy {ms_t:.3f} ms, {gbs_t:.0f} GB/s), "
+        f"reduce {ms_r:.3f} ms ({gbs_r:.0f} GB/s); sampled mismatches {mism}")
+    print(json.dumps({
+        "metric": "GB/s rewire kernels on a 2^27-byte relation (permutation; reduction over 11 axes)",
+        "value": gbs_p, "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_p, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8",
+        "data": "synthetic",
+        "config": {"workload": "ts_rewire on a rank-27 0/1 tensor: random axis permutation, and keep 16 of 27 axes",
+                   "permute_ms": ms_p, "permute_gbs": gbs_p, "copy_ms": ms_t, "copy_gbs": gbs_t,
+                   "reduce_ms": ms_r, "reduce_gbs": gbs_r, "mode": args.mode},
+        "roofline": {"bound": "hbm", "achieved": gbs_p, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": gbs_p / HBM_PEAK_GBS, "traffic": None, "kernel": "k_permute_lds",
+                     "reduce": {"achieved": gbs_r, "frac": gbs_r / HBM_PEAK_GBS, "kernel": "k_reduce"}},
+        "cpu_baseline": None,
+        "parity": {"sampled": 4096 + 64, "mismatches": mism},
     }), flush=True)
     return 0 if mism == 0 else 1
 

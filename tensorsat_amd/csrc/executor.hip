@@ -93,9 +93,8 @@ __global__ __launch_bounds__(WG) void k_rewire(const uint8_t* __restrict__ in, u
   }
 }
 
-// Permutation path of ts_rewire (kernel (a)): out[o] = in[pi(o)], with the input tile of
-// each workgroup staged through LDS so both the gather and the store are coalesced when
-// the low bits of in and out overlap.  Here: plain coalesced-store gather with byte LUTs.
+// Gather path of ts_select: out[o] = in[L(o)] over the kept axes, byte lookup tables,
+// coalesced stores.
 __global__ __launch_bounds__(WG) void k_permute(const uint8_t* __restrict__ in, uint8_t* __restrict__ out,
                                                 ts_step st) {
   __shared__ Lut L;
@@ -103,6 +102,184 @@ __global__ __launch_bounds__(WG) void k_permute(const uint8_t* __restrict__ in, 
   __syncthreads();
   step_u8(st, L, in, nullptr, out, (uint64_t)blockIdx.x * WG + threadIdx.x, (uint64_t)gridDim.x * WG,
           TS_MODE_GT0);
+}
+
+// Kernel (a): LDS-staged bit permutation, out[o] = in[pi(o)].  A tile is the set S of out
+// bits made of the B lowest out bits and the out bits that pi sends to the B lowest in
+// bits (|S| <= 2B).  A workgroup reads its tile in input order (the B lowest in bits
+// vary fastest: 2^B consecutive bytes per wave), stores it to LDS by tile-local out
+// index, then writes it in output order (the B lowest out bits fastest): both HBM sides
+// are coalesced whatever the permutation.
+constexpr int PB = 6;
+struct PermArgs {
+  int32_t s, nr;                         // tile bits, remaining (tile-index) bits
+  uint8_t load_in[2 * PB], load_lds[2 * PB];  // per tile bit in input order: in pos, LDS bit
+  uint8_t store_out[2 * PB];                  // per tile bit in output order: out pos
+  uint8_t rest_out[TS_MAX_BITS], rest_in[TS_MAX_BITS];
+};
+
+// Bits of v (LSB-first, the first n <= N of them) placed at positions pos[lo + i]; fully
+// unrolled so the positions stay in registers.
+template <int N>
+__device__ __forceinline__ uint32_t spread(uint32_t v, int lo, int n, const uint8_t* pos) {
+  uint32_t r = 0;
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+    if (i < n) r |= ((v >> i) & 1u) << pos[lo + i];
+  return r;
+}
+
+__global__ __launch_bounds__(WG) void k_permute_lds(const uint8_t* __restrict__ in, uint8_t* __restrict__ out,
+                                                    PermArgs a) {
+  __shared__ uint8_t tile[1 << (2 * PB)];
+  // Tile element u = tid + 256 k: the low 8 tile bits come from the thread, the rest from
+  // k; both parts are fixed for the whole launch.
+  const int tid = threadIdx.x;
+  const uint32_t ts = 1u << a.s;
+  const int nlo = a.s < 8 ? a.s : 8, nhi = a.s - nlo;
+  const uint32_t lo_in = spread<8>(tid, 0, nlo, a.load_in), lo_lds = spread<8>(tid, 0, nlo, a.load_lds);
+  const uint32_t lo_out = spread<8>(tid, 0, nlo, a.store_out);
+  const bool live = (uint32_t)tid < ts;
+  const uint32_t nk = ts > WG ? ts / WG : 1;
+  const uint64_t ntiles = 1ull << a.nr;
+  for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const uint32_t ib = spread<TS_MAX_BITS>((uint32_t)t, 0, a.nr, a.rest_in);
+    const uint32_t ob = spread<TS_MAX_BITS>((uint32_t)t, 0, a.nr, a.rest_out);
+    if (live)
+      for (uint32_t k = 0; k < nk; ++k)
+        tile[lo_lds | spread<2 * PB - 8>(k, 8, nhi, a.load_lds)] = in[ib | lo_in | spread<2 * PB - 8>(k, 8, nhi, a.load_in)];
+    __syncthreads();
+    if (live)
+      for (uint32_t k = 0; k < nk; ++k)
+        out[ob | lo_out | spread<2 * PB - 8>(k, 8, nhi, a.store_out)] = tile[(uint32_t)tid + WG * k];
+    __syncthreads();
+  }
+}
+
+// 16-byte variant (tile >= 16 bytes, 16-byte aligned tensors): the 4 lowest tile bits are
+// the 4 lowest input bits in load order and the 4 lowest output bits in store order, so
+// thread g moves bytes 16g..16g+15 of the tile with one uint4 load and one uint4 store;
+// only the LDS writes scatter.
+__global__ __launch_bounds__(WG) void k_permute_lds16(const uint8_t* __restrict__ in, uint8_t* __restrict__ out,
+                                                      PermArgs a) {
+  __shared__ __attribute__((aligned(16))) uint8_t tile[1 << (2 * PB)];
+  const int g = threadIdx.x;
+  const uint32_t ngroups = (1u << a.s) >> 4;  // <= 256
+  const bool live = (uint32_t)g < ngroups;
+  const int ng = a.s - 4;
+  const uint32_t in_g = spread<2 * PB - 4>(g, 4, ng, a.load_in);
+  const uint32_t lds_g = spread<2 * PB - 4>(g, 4, ng, a.load_lds);
+  const uint32_t out_g = spread<2 * PB - 4>(g, 4, ng, a.store_out);
+  uint32_t lds_j[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) lds_j[j] = lds_g | spread<4>(j, 0, 4, a.load_lds);
+  const uint64_t ntiles = 1ull << a.nr;
+  for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const uint32_t ib = spread<TS_MAX_BITS>((uint32_t)t, 0, a.nr, a.rest_in);
+    const uint32_t ob = spread<TS_MAX_BITS>((uint32_t)t, 0, a.nr, a.rest_out);
+    if (live) {
+      const uint4 v = *reinterpret_cast<const uint4*>(in + (ib | in_g));
+      const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int j = 0; j < 16; ++j) tile[lds_j[j]] = (uint8_t)(w[j >> 2] >> (8 * (j & 3)));
+    }
+    __syncthreads();
+    if (live) *reinterpret_cast<uint4*>(out + (ob | out_g)) = *reinterpret_cast<const uint4*>(&tile[16 * g]);
+    __syncthreads();
+  }
+}
+
+// Existential reduction (GT0) with the lanes on the 6 lowest input bits: each load is 64
+// contiguous bytes whichever axes are discarded.  A wave handles one value of the
+// remaining kept bits: it ORs the ballot masks of all values of the remaining discarded
+// bits, and the out entry of each value of the low kept bits is set when its lanes' mask
+// bits are.
+struct ReduceLowArgs {
+  int32_t nkl, ndl;                // kept / discarded bits among the 6 lowest input bits
+  int32_t nkh, ndh;                // kept / discarded bits above them
+  uint8_t kl_out[6];               // out bit position of each low kept bit (by input position)
+  uint8_t kl_lane[6];              // lane bit of each low kept bit
+  uint8_t kh_in[TS_MAX_BITS], kh_out[TS_MAX_BITS];  // high kept bits: in / out positions
+  uint8_t dh_in[TS_MAX_BITS];      // high discarded bits: in positions
+};
+
+__global__ __launch_bounds__(WG) void k_reduce_low(const uint8_t* __restrict__ in, uint8_t* __restrict__ out,
+                                                   ReduceLowArgs a) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t ngroups = 1ull << a.nkh;
+  const uint64_t waves = (uint64_t)gridDim.x * (WG / 64);
+  const uint32_t nterm = 1u << a.ndh;
+  // Lane mask of each low kept value kappa (lanes whose low kept bits equal kappa), and
+  // the out offset of kappa.
+  uint64_t my_mask = 0;
+  for (int l = 0; l < 64; ++l) {
+    uint32_t kl = 0;
+    for (int i = 0; i < a.nkl; ++i) kl |= ((uint32_t)(l >> a.kl_lane[i]) & 1u) << i;
+    if (kl == (uint32_t)lane) my_mask |= 1ull << l;  // lane kappa collects the lanes of kappa
+  }
+  uint32_t kout = 0;
+  for (int i = 0; i < a.nkl; ++i) kout |= ((uint32_t)(lane >> i) & 1u) << a.kl_out[i];
+  // Discarded high bits: the 3 lowest from an 8-entry table, the rest once per group of 8
+  // terms; 8 loads in flight per group.
+  const int n3 = a.ndh < 3 ? a.ndh : 3;
+  uint32_t low3[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) low3[j] = spread<3>(j, 0, n3, a.dh_in);
+  for (uint64_t gidx = (uint64_t)blockIdx.x * (WG / 64) + (threadIdx.x >> 6); gidx < ngroups; gidx += waves) {
+    const uint32_t ibase = spread<TS_MAX_BITS>((uint32_t)gidx, 0, a.nkh, a.kh_in) | (uint32_t)lane;
+    uint64_t acc = 0;
+    if (nterm >= 8) {
+      for (uint32_t kg = 0; kg < (nterm >> 3); ++kg) {
+        const uint32_t hb = ibase | spread<TS_MAX_BITS - 3>(kg, 3, a.ndh - 3, a.dh_in);
+        uint8_t v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = in[hb | low3[j]];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc |= __ballot(v[j] != 0);
+      }
+    } else {
+      for (uint32_t k = 0; k < nterm; ++k) acc |= __ballot(in[ibase | low3[k]] != 0);
+    }
+    if ((uint32_t)lane < (1u << a.nkl)) {
+      const uint32_t obase = spread<TS_MAX_BITS>((uint32_t)gidx, 0, a.nkh, a.kh_out);
+      out[obase | kout] = (uint8_t)((acc & my_mask) != 0);
+    }
+  }
+}
+
+// Kernel (b): reduction over discarded axes with many terms per output: one wave per out
+// entry, lanes sweep the 2^nd terms (lowest discarded bits on the lanes), then a
+// wave-level OR through ballot (GT0) or a wave sum (MOD256, count mod 256).
+struct ReduceArgs {
+  int32_t out_rank, nd;
+  uint8_t keep_in[TS_MAX_BITS];  // in bit position of out bit q (LSB-first)
+  uint8_t disc_in[TS_MAX_BITS];  // in bit positions of the discarded bits, ascending
+};
+
+__global__ __launch_bounds__(WG) void k_reduce(const uint8_t* __restrict__ in, uint8_t* __restrict__ out,
+                                               ReduceArgs a, int mode) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t nout = 1ull << a.out_rank;
+  const uint64_t waves = (uint64_t)gridDim.x * (WG / 64);
+  // term c = lane + 64 k (nd >= 6): the lane part is fixed for the launch.
+  const uint32_t lane_off = spread<6>(lane, 0, 6, a.disc_in);
+  const uint32_t nk = 1u << (a.nd - 6);
+  for (uint64_t o = (uint64_t)blockIdx.x * (WG / 64) + (threadIdx.x >> 6); o < nout; o += waves) {
+    const uint32_t base = spread<TS_MAX_BITS>((uint32_t)o, 0, a.out_rank, a.keep_in) | lane_off;
+    uint32_t acc = 0;
+    for (uint32_t k = 0; k < nk; ++k) {
+      const uint32_t v = in[base | spread<TS_MAX_BITS - 6>(k, 6, a.nd - 6, a.disc_in)];
+      acc = mode == TS_MODE_MOD256 ? acc + v : (acc | v);
+    }
+    uint32_t bit;
+    if (mode == TS_MODE_MOD256) {
+      for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+      bit = (acc & 255u) != 0;
+    } else {
+      bit = __ballot(acc != 0) != 0;
+    }
+    if (lane == 0) out[o] = (uint8_t)bit;
+  }
 }
 
 __global__ __launch_bounds__(WG) void k_diagonal(const uint8_t* __restrict__ in, uint8_t* __restrict__ out,
@@ -325,7 +502,80 @@ int ts_rewire(const uint8_t* in, int32_t in_rank, uint8_t* out, int32_t out_rank
     int rc = make_step(in_rank, iw.data(), 0, nullptr, out_rank, ow.data(), true, st);
     if (rc) return rc;
     if (a.num_disc == 0) {
-      hipLaunchKernelGGL(k_permute, dim3(grid_for(1ull << out_rank)), dim3(WG), 0, s, in, out, st);
+      // Bit positions: out bit q is out axis r-1-q; its in bit is r-1-out_port[r-1-q].
+      const int r = out_rank;
+      std::vector<int> pi(r);
+      for (int q = 0; q < r; ++q) pi[q] = r - 1 - out_ports[r - 1 - q];
+      const int b = std::min(PB, r);
+      std::vector<char> inS(r, 0);
+      for (int q = 0; q < r; ++q)
+        if (q < b || pi[q] < b) inS[q] = 1;
+      std::vector<int> Sout, Sin, rest;
+      for (int q = 0; q < r; ++q) (inS[q] ? Sout : rest).push_back(q);
+      Sin = Sout;
+      std::sort(Sin.begin(), Sin.end(), [&](int x, int y) { return pi[x] < pi[y]; });
+      PermArgs pa;
+      std::memset(&pa, 0, sizeof(pa));
+      pa.s = (int)Sout.size();
+      pa.nr = (int)rest.size();
+      for (int k = 0; k < pa.s; ++k) {
+        pa.load_in[k] = (uint8_t)pi[Sin[k]];
+        pa.load_lds[k] = (uint8_t)(std::find(Sout.begin(), Sout.end(), Sin[k]) - Sout.begin());
+        pa.store_out[k] = (uint8_t)Sout[k];
+      }
+      for (int k = 0; k < pa.nr; ++k) {
+        pa.rest_out[k] = (uint8_t)rest[k];
+        pa.rest_in[k] = (uint8_t)pi[rest[k]];
+      }
+      const uint64_t ntiles = 1ull << pa.nr;
+      const int grid = (int)std::min<uint64_t>(ntiles, 65536);
+      const bool aligned = ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15) == 0;
+      if (pa.s >= 4 && aligned)
+        hipLaunchKernelGGL(k_permute_lds16, dim3(grid), dim3(WG), 0, s, in, out, pa);
+      else
+        hipLaunchKernelGGL(k_permute_lds, dim3(grid), dim3(WG), 0, s, in, out, pa);
+      return hip_rc(hipGetLastError());
+    }
+    if (a.num_disc >= 6 && mode == TS_MODE_GT0 && in_rank >= 6) {
+      // Lanes on the 6 lowest input bits (in bit position q of in axis in_rank-1-q).
+      ReduceLowArgs rl;
+      std::memset(&rl, 0, sizeof(rl));
+      std::vector<int> out_of_in(in_rank, -1);  // in bit -> out bit (kept)
+      for (int q = 0; q < out_rank; ++q) out_of_in[in_rank - 1 - out_ports[out_rank - 1 - q]] = q;
+      for (int p = 0; p < in_rank; ++p) {
+        const int o = out_of_in[p];
+        if (p < 6) {
+          if (o >= 0) {
+            rl.kl_out[rl.nkl] = (uint8_t)o;
+            rl.kl_lane[rl.nkl++] = (uint8_t)p;
+          } else {
+            ++rl.ndl;
+          }
+        } else if (o >= 0) {
+          rl.kh_in[rl.nkh] = (uint8_t)p;
+          rl.kh_out[rl.nkh++] = (uint8_t)o;
+        } else {
+          rl.dh_in[rl.ndh++] = (uint8_t)p;
+        }
+      }
+      const uint64_t groups = 1ull << rl.nkh;
+      const int grid = (int)std::min<uint64_t>((groups + 3) / 4, 65536);
+      hipLaunchKernelGGL(k_reduce_low, dim3(grid), dim3(WG), 0, s, in, out, rl);
+      return hip_rc(hipGetLastError());
+    }
+    if (a.num_disc >= 6) {
+      ReduceArgs ra;
+      std::memset(&ra, 0, sizeof(ra));
+      ra.out_rank = out_rank;
+      ra.nd = a.num_disc;
+      for (int q = 0; q < out_rank; ++q) ra.keep_in[q] = (uint8_t)(in_rank - 1 - out_ports[out_rank - 1 - q]);
+      std::vector<int> dpos;
+      for (int i = 0; i < a.num_disc; ++i) dpos.push_back(in_rank - 1 - a.disc_port[i]);
+      std::sort(dpos.begin(), dpos.end());
+      for (int i = 0; i < a.num_disc; ++i) ra.disc_in[i] = (uint8_t)dpos[i];
+      const uint64_t waves_needed = 1ull << out_rank;
+      const int grid = (int)std::min<uint64_t>((waves_needed + 3) / 4, 65536);
+      hipLaunchKernelGGL(k_reduce, dim3(grid), dim3(WG), 0, s, in, out, ra, mode);
       return hip_rc(hipGetLastError());
     }
     return launch_step_u8(in, in, out, st, mode, s);  // unary: rhs is never read
