@@ -105,12 +105,12 @@ __global__ __launch_bounds__(WG) void k_permute(const uint8_t* __restrict__ in, 
 }
 
 // Kernel (a): LDS-staged bit permutation, out[o] = in[pi(o)].  A tile is the set S of out
-// bits made of the B lowest out bits and the out bits that pi sends to the B lowest in
-// bits (|S| <= 2B).  A workgroup reads its tile in input order (the B lowest in bits
-// vary fastest: 2^B consecutive bytes per wave), stores it to LDS by tile-local out
-// index, then writes it in output order (the B lowest out bits fastest): both HBM sides
-// are coalesced whatever the permutation.
-constexpr int PB = 6;
+// bits made of the PB lowest out bits and the out bits that pi sends to the PB lowest in
+// bits (|S| <= 2 PB).  A workgroup reads its tile in input order (runs of 2^PB = 128
+// consecutive bytes, whole L2 lines), stores it to LDS by tile-local out index, then
+// writes it in output order (128-byte runs): both HBM sides are coalesced whatever the
+// permutation.
+constexpr int PB = 7;  // 2^7 = 128 contiguous bytes: one L2 line per run
 struct PermArgs {
   int32_t s, nr;                         // tile bits, remaining (tile-index) bits
   uint8_t load_in[2 * PB], load_lds[2 * PB];  // per tile bit in input order: in pos, LDS bit
@@ -158,46 +158,46 @@ __global__ __launch_bounds__(WG) void k_permute_lds(const uint8_t* __restrict__ 
 
 // 16-byte variant (tile >= 16 bytes, 16-byte aligned tensors): the 4 lowest tile bits are
 // the 4 lowest input bits in load order and the 4 lowest output bits in store order, so
-// thread g moves bytes 16g..16g+15 of the tile with one uint4 load and one uint4 store;
+// group g moves bytes 16g..16g+15 of the tile with one uint4 load and one uint4 store;
 // only the LDS writes scatter.
 __global__ __launch_bounds__(WG) void k_permute_lds16(const uint8_t* __restrict__ in, uint8_t* __restrict__ out,
                                                       PermArgs a) {
   __shared__ __attribute__((aligned(16))) uint8_t tile[1 << (2 * PB)];
-  const int g = threadIdx.x;
-  const uint32_t ngroups = (1u << a.s) >> 4;  // <= 256
-  const bool live = (uint32_t)g < ngroups;
+  const uint32_t ngroups = (1u << a.s) >> 4;  // <= 1024
   const int ng = a.s - 4;
-  const uint32_t in_g = spread<2 * PB - 4>(g, 4, ng, a.load_in);
-  const uint32_t lds_g = spread<2 * PB - 4>(g, 4, ng, a.load_lds);
-  const uint32_t out_g = spread<2 * PB - 4>(g, 4, ng, a.store_out);
   uint32_t lds_j[16];
 #pragma unroll
-  for (int j = 0; j < 16; ++j) lds_j[j] = lds_g | spread<4>(j, 0, 4, a.load_lds);
+  for (int j = 0; j < 16; ++j) lds_j[j] = spread<4>(j, 0, 4, a.load_lds);
   const uint64_t ntiles = 1ull << a.nr;
   for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const uint32_t ib = spread<TS_MAX_BITS>((uint32_t)t, 0, a.nr, a.rest_in);
     const uint32_t ob = spread<TS_MAX_BITS>((uint32_t)t, 0, a.nr, a.rest_out);
-    if (live) {
-      const uint4 v = *reinterpret_cast<const uint4*>(in + (ib | in_g));
+    for (uint32_t g = threadIdx.x; g < ngroups; g += WG) {
+      const uint4 v = *reinterpret_cast<const uint4*>(in + (ib | spread<2 * PB - 4>(g, 4, ng, a.load_in)));
+      const uint32_t lg = spread<2 * PB - 4>(g, 4, ng, a.load_lds);
       const uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-      for (int j = 0; j < 16; ++j) tile[lds_j[j]] = (uint8_t)(w[j >> 2] >> (8 * (j & 3)));
+      for (int j = 0; j < 16; ++j) tile[lg | lds_j[j]] = (uint8_t)(w[j >> 2] >> (8 * (j & 3)));
     }
     __syncthreads();
-    if (live) *reinterpret_cast<uint4*>(out + (ob | out_g)) = *reinterpret_cast<const uint4*>(&tile[16 * g]);
+    for (uint32_t g = threadIdx.x; g < ngroups; g += WG)
+      *reinterpret_cast<uint4*>(out + (ob | spread<2 * PB - 4>(g, 4, ng, a.store_out))) =
+          *reinterpret_cast<const uint4*>(&tile[16 * g]);
     __syncthreads();
   }
 }
 
-// Existential reduction (GT0) with the lanes on the 6 lowest input bits: each load is 64
+// Existential reduction (GT0) over the 7 lowest input bits: lane l reads the two bytes at
+// 2l and 2l+1 (in bit 0 inside the pair, bits 1..6 on the lanes), so each load covers 128
 // contiguous bytes whichever axes are discarded.  A wave handles one value of the
-// remaining kept bits: it ORs the ballot masks of all values of the remaining discarded
-// bits, and the out entry of each value of the low kept bits is set when its lanes' mask
-// bits are.
+// remaining kept bits: it ORs the ballot masks of both bytes over all values of the
+// remaining discarded bits, and the out entry of each value of the low kept bits is set
+// when its lanes' mask bits are.
 struct ReduceLowArgs {
-  int32_t nkl, ndl;                // kept / discarded bits among the 6 lowest input bits
-  int32_t nkh, ndh;                // kept / discarded bits above them
-  uint8_t kl_out[6];               // out bit position of each low kept bit (by input position)
+  int32_t nkl;                     // kept bits among in bits 1..6
+  int32_t nkh, ndh;                // kept / discarded bits above bit 6
+  int32_t bit0_out;                // out bit of in bit 0, or -1 when it is discarded
+  uint8_t kl_out[6];               // out bit position of each low kept bit
   uint8_t kl_lane[6];              // lane bit of each low kept bit
   uint8_t kh_in[TS_MAX_BITS], kh_out[TS_MAX_BITS];  // high kept bits: in / out positions
   uint8_t dh_in[TS_MAX_BITS];      // high discarded bits: in positions
@@ -209,13 +209,12 @@ __global__ __launch_bounds__(WG) void k_reduce_low(const uint8_t* __restrict__ i
   const uint64_t ngroups = 1ull << a.nkh;
   const uint64_t waves = (uint64_t)gridDim.x * (WG / 64);
   const uint32_t nterm = 1u << a.ndh;
-  // Lane mask of each low kept value kappa (lanes whose low kept bits equal kappa), and
-  // the out offset of kappa.
+  // Lanes of low kept value kappa = lane (for lane < 2^nkl), and kappa's out offset.
   uint64_t my_mask = 0;
   for (int l = 0; l < 64; ++l) {
     uint32_t kl = 0;
     for (int i = 0; i < a.nkl; ++i) kl |= ((uint32_t)(l >> a.kl_lane[i]) & 1u) << i;
-    if (kl == (uint32_t)lane) my_mask |= 1ull << l;  // lane kappa collects the lanes of kappa
+    if (kl == (uint32_t)lane) my_mask |= 1ull << l;
   }
   uint32_t kout = 0;
   for (int i = 0; i < a.nkl; ++i) kout |= ((uint32_t)(lane >> i) & 1u) << a.kl_out[i];
@@ -225,24 +224,37 @@ __global__ __launch_bounds__(WG) void k_reduce_low(const uint8_t* __restrict__ i
   uint32_t low3[8];
 #pragma unroll
   for (int j = 0; j < 8; ++j) low3[j] = spread<3>(j, 0, n3, a.dh_in);
+  const uint16_t* in2 = reinterpret_cast<const uint16_t*>(in);
   for (uint64_t gidx = (uint64_t)blockIdx.x * (WG / 64) + (threadIdx.x >> 6); gidx < ngroups; gidx += waves) {
-    const uint32_t ibase = spread<TS_MAX_BITS>((uint32_t)gidx, 0, a.nkh, a.kh_in) | (uint32_t)lane;
-    uint64_t acc = 0;
+    const uint32_t ibase = spread<TS_MAX_BITS>((uint32_t)gidx, 0, a.nkh, a.kh_in) | ((uint32_t)lane << 1);
+    uint64_t acc0 = 0, acc1 = 0;
     if (nterm >= 8) {
       for (uint32_t kg = 0; kg < (nterm >> 3); ++kg) {
         const uint32_t hb = ibase | spread<TS_MAX_BITS - 3>(kg, 3, a.ndh - 3, a.dh_in);
-        uint8_t v[8];
+        uint32_t v[8];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) v[j] = in[hb | low3[j]];
+        for (int j = 0; j < 8; ++j) v[j] = in2[(hb | low3[j]) >> 1];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) acc |= __ballot(v[j] != 0);
+        for (int j = 0; j < 8; ++j) {
+          acc0 |= __ballot((v[j] & 0xffu) != 0);
+          acc1 |= __ballot((v[j] >> 8) != 0);
+        }
       }
     } else {
-      for (uint32_t k = 0; k < nterm; ++k) acc |= __ballot(in[ibase | low3[k]] != 0);
+      for (uint32_t k = 0; k < nterm; ++k) {
+        const uint32_t v = in2[(ibase | low3[k]) >> 1];
+        acc0 |= __ballot((v & 0xffu) != 0);
+        acc1 |= __ballot((v >> 8) != 0);
+      }
     }
     if ((uint32_t)lane < (1u << a.nkl)) {
-      const uint32_t obase = spread<TS_MAX_BITS>((uint32_t)gidx, 0, a.nkh, a.kh_out);
-      out[obase | kout] = (uint8_t)((acc & my_mask) != 0);
+      const uint32_t obase = spread<TS_MAX_BITS>((uint32_t)gidx, 0, a.nkh, a.kh_out) | kout;
+      if (a.bit0_out >= 0) {
+        out[obase] = (uint8_t)((acc0 & my_mask) != 0);
+        out[obase | (1u << a.bit0_out)] = (uint8_t)((acc1 & my_mask) != 0);
+      } else {
+        out[obase] = (uint8_t)(((acc0 | acc1) & my_mask) != 0);
+      }
     }
   }
 }
@@ -536,20 +548,21 @@ int ts_rewire(const uint8_t* in, int32_t in_rank, uint8_t* out, int32_t out_rank
         hipLaunchKernelGGL(k_permute_lds, dim3(grid), dim3(WG), 0, s, in, out, pa);
       return hip_rc(hipGetLastError());
     }
-    if (a.num_disc >= 6 && mode == TS_MODE_GT0 && in_rank >= 6) {
-      // Lanes on the 6 lowest input bits (in bit position q of in axis in_rank-1-q).
+    if (a.num_disc >= 6 && mode == TS_MODE_GT0 && in_rank >= 7 && (reinterpret_cast<uintptr_t>(in) & 1) == 0) {
+      // In bit 0 inside each lane's byte pair, in bits 1..6 on the lanes.
       ReduceLowArgs rl;
       std::memset(&rl, 0, sizeof(rl));
+      rl.bit0_out = -1;
       std::vector<int> out_of_in(in_rank, -1);  // in bit -> out bit (kept)
       for (int q = 0; q < out_rank; ++q) out_of_in[in_rank - 1 - out_ports[out_rank - 1 - q]] = q;
       for (int p = 0; p < in_rank; ++p) {
         const int o = out_of_in[p];
-        if (p < 6) {
+        if (p == 0) {
+          rl.bit0_out = o;
+        } else if (p < 7) {
           if (o >= 0) {
             rl.kl_out[rl.nkl] = (uint8_t)o;
-            rl.kl_lane[rl.nkl++] = (uint8_t)p;
-          } else {
-            ++rl.ndl;
+            rl.kl_lane[rl.nkl++] = (uint8_t)(p - 1);
           }
         } else if (o >= 0) {
           rl.kh_in[rl.nkh] = (uint8_t)p;
