@@ -135,16 +135,23 @@ typedef struct {
   int32_t num_workgroups;
   int32_t mode;             /* TS_MODE_GT0 / TS_MODE_MOD256 */
   uint32_t* verdicts;       /* device, per instance, zeroed by the caller: OR of job results */
-  int32_t* counter;         /* device scratch int32, zeroed by the caller */
+  int32_t* counter;         /* device scratch: TS_BATCH_SEGMENTS * TS_BATCH_COUNTER_STRIDE
+                               int32, zeroed by the caller; the queue past each workgroup's
+                               first job is taken in interleaved segments, the counter of
+                               segment x at counter[x * TS_BATCH_COUNTER_STRIDE] */
   int32_t early_exit;       /* 1: skip jobs of instances already decided SAT */
   uint32_t* job_done;       /* device, per job (optional): 1 ran, 2 skipped */
   const int32_t* num_jobs_dev; /* optional device count overriding num_jobs (compacted lists) */
-  int32_t lds_arena;        /* 1: each job runs on one wave with its arena in LDS (arena_stride
-                               bytes of LDS per workgroup, <= TS_BATCH_LDS_MAX; `arenas` unused;
-                               num_workgroups from ts_batch_max_workgroups_lds) */
+  int32_t lds_arena;        /* 0: arenas in device memory (`arenas`); 1: each job's arena in LDS
+                               (arena_stride bytes of LDS per workgroup, <= TS_BATCH_LDS_MAX;
+                               `arenas` unused); 2: as 1 with one-wave workgroups, for programs
+                               whose tensors all have at most 6 bits (every step <= 64 out
+                               entries).  num_workgroups from ts_batch_max_workgroups_lds. */
 } ts_batch;
 
 #define TS_BATCH_LDS_MAX 65536
+#define TS_BATCH_SEGMENTS 8
+#define TS_BATCH_COUNTER_STRIDE 64 /* int32s between queue counters: one 256-byte line each */
 
 /* One persistent launch: each workgroup takes the next job in `order`, copies its leaves
  * into its arena (device memory, or LDS with lds_arena) and runs every step with the
@@ -169,8 +176,9 @@ int ts_batch_compact(const int32_t* order, int32_t n, const ts_job* jobs, const 
 int ts_batch_max_workgroups(int32_t* out);
 
 /* Maximum resident workgroups of the LDS-arena batched kernel for arenas of arena_bytes
- * (a multiple of 256, at most TS_BATCH_LDS_MAX). */
-int ts_batch_max_workgroups_lds(int64_t arena_bytes, int32_t* out);
+ * (a multiple of 256, at most TS_BATCH_LDS_MAX); one_wave selects the lds_arena = 2
+ * variant. */
+int ts_batch_max_workgroups_lds(int64_t arena_bytes, int32_t one_wave, int32_t* out);
 
 /* ---- Learned planner: message-passing network kernels (bf16 in, fp32 accumulate) --
  * No counterpart in the reference: the learned planner is an addition of this build.

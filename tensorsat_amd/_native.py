@@ -22,6 +22,9 @@ TS_EUNSUPPORTED = -3
 TS_EOVERFLOW = -4
 TS_EHIP = -5
 BATCH_LDS_MAX = 65536
+BATCH_SEGMENTS = 8  # queue counters per dispatch (TS_BATCH_SEGMENTS)
+BATCH_COUNTER_STRIDE = 64  # int32s between queue counters (TS_BATCH_COUNTER_STRIDE)
+BATCH_COUNTER_INTS = BATCH_SEGMENTS * BATCH_COUNTER_STRIDE
 
 MODE_GT0 = 0
 MODE_MOD256 = 1
@@ -89,7 +92,7 @@ def load() -> ctypes.CDLL:
         L.ts_gnn_adam.argtypes = [vp, vp, vp, vp, vp, i8, f32, f32, f32, f32, i32, f32, vp]
         L.ts_gnn_cast_bf16.argtypes = [vp, vp, i8, vp]
         L.ts_batch_max_workgroups.argtypes = [P(i32)]
-        L.ts_batch_max_workgroups_lds.argtypes = [i64, P(i32)]
+        L.ts_batch_max_workgroups_lds.argtypes = [i64, i32, P(i32)]
         L.ts_dense_config.argtypes = [i32, i32]
         _lib = L
     return _lib

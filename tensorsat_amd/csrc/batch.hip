@@ -27,6 +27,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstddef>
 
 #include "device.h"
 
@@ -38,6 +39,31 @@ constexpr int SREC = 64;  // step records staged in LDS at a time
 
 __device__ __forceinline__ bool decided(const uint32_t* verdicts, int instance) {
   return __hip_atomic_load(&verdicts[instance], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+}
+
+// Queue position w is workgroup w's first job, taken without an atomic (every workgroup
+// starts at once, and thousands of atomics on one address at launch serialise).  The
+// positions from num_workgroups on are split into QSEG interleaved segments (position
+// num_workgroups + k belongs to segment k % QSEG, so each keeps the queue's order) with
+// one counter each, TS_BATCH_COUNTER_STRIDE int32s (one 256-byte line) apart; workgroup w
+// takes them from segment w % QSEG, i.e. from the segment of its own XCD.  A plain load
+// of the counter first keeps workgroups that find their segment exhausted off the atomic.
+// Returns the next job, or -1 once the workgroup's segment is exhausted.
+constexpr int QSEG = TS_BATCH_SEGMENTS;
+constexpr int QSTRIDE = TS_BATCH_COUNTER_STRIDE;
+
+__device__ __forceinline__ int take_job(const ts_batch& b, int num_jobs, bool& first) {
+  const int w = (int)blockIdx.x, g = b.num_workgroups;
+  if (first) {
+    first = false;
+    if (w < num_jobs) return b.order[w];
+  }
+  const int rest = num_jobs - g, x = w % QSEG;
+  const int seg_len = rest > x ? (rest - x + QSEG - 1) / QSEG : 0;
+  int32_t* c = b.counter + x * QSTRIDE;
+  if (seg_len == 0 || __hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= seg_len) return -1;
+  const int kl = atomicAdd(c, 1);
+  return kl < seg_len ? b.order[g + x + QSEG * kl] : -1;
 }
 
 // Offset of the bits of v (LSB-first, nbits <= NB of them) placed at positions pos[i];
@@ -56,33 +82,77 @@ __device__ __forceinline__ uint32_t scatter_bits(uint32_t v, int nbits, const ui
 }
 
 // Steps with at most 64 out entries run on one wave: lane = out index, offsets scattered
-// directly from the record's bit positions (no tables, no workgroup barrier).
+// directly from the record's bit positions (no tables, no workgroup barrier).  The record
+// (in LDS) is read as a few aligned dwords issued together, and the scatter is
+// branch-free on the position bytes held in registers.
 __device__ __forceinline__ bool small_step(const ts_step& st) { return st.out_rank <= 6; }
 
+// 8 bytes of the record starting at byte `off` (compile-time), from aligned dword reads.
+__device__ __forceinline__ uint64_t rec_bytes8(const uint32_t* w, int off) {
+  const int q = off >> 2, r = off & 3;
+  const uint64_t lo = (uint64_t)w[q] | ((uint64_t)w[q + 1] << 32);
+  if (r == 0) return lo;
+  const uint64_t hi = w[q + 2];
+  return (lo >> (8 * r)) | (hi << (64 - 8 * r));
+}
+
+// Bits of v (the first n <= 6 of them) placed at the positions packed in pw (one byte per
+// bit, TS_NOPOS = none).
+__device__ __forceinline__ uint32_t scatter6(uint32_t v, int n, uint64_t pw) {
+  uint32_t off = 0;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    const uint32_t p = (uint32_t)(pw >> (8 * i)) & 0xffu;
+    const bool on = i < n && ((v >> i) & 1u) && p != TS_NOPOS;
+    off |= on ? (1u << (p & 31u)) : 0u;
+  }
+  return off;
+}
+
 __device__ inline void step_small(const ts_step& st, uint8_t* arena, int mode, int lane) {
-  const int ro = st.out_rank, nc = st.sum_rank, flags = st.flags;
-  if (lane >= (1 << ro)) return;
-  const bool norhs = flags & TS_STEP_NORHS, kinner = flags & TS_STEP_KINNER;
-  const uint8_t* lhs = arena + st.lhs_off;
-  const uint8_t* rhs = arena + (norhs ? st.lhs_off : st.rhs_off);
-  const uint32_t offl = scatter_bits<6>((uint32_t)lane, ro, st.o_lpos);
-  const uint32_t offr = scatter_bits<6>((uint32_t)lane, ro, st.o_rpos);
+  const uint32_t* w = reinterpret_cast<const uint32_t*>(&st);
+  // Header: lhs_off, rhs_off, out_off, ranks (bytes 12..15), flags (byte 16).
+  const uint32_t lhs_off = w[0], rhs_off = w[1], out_off = w[2], ranks = w[3], fl = w[4];
+  const uint64_t lpw = rec_bytes8(w, (int)offsetof(ts_step, o_lpos));
+  const uint64_t rpw = rec_bytes8(w, (int)offsetof(ts_step, o_rpos));
+  const int ro = (int)((ranks >> 16) & 0xffu), nc = (int)(ranks >> 24);
+  const bool norhs = fl & TS_STEP_NORHS, kinner = fl & TS_STEP_KINNER;
+  const uint8_t* lhs = arena + lhs_off;
+  const uint8_t* rhs = arena + (norhs ? lhs_off : rhs_off);
+  // Every lane stays active (cross-lane reads below); lanes >= 2^ro compute in-bounds
+  // aliases (only the low ro bits of the lane index are scattered) and do not store.
+  const uint32_t offl = scatter6((uint32_t)lane, ro, lpw);
+  const uint32_t offr = scatter6((uint32_t)lane, ro, rpw);
   uint32_t acc;
   if (nc == 0) {
     acc = lhs[offl];
     if (!norhs) acc &= rhs[offr];
   } else if (kinner) {
     acc = popc_and(lhs + offl, rhs + offr, nc, norhs);
+  } else if (nc <= 6) {
+    // Lane t holds the summed-bit offsets of term c = t; the term loop reads them across
+    // lanes.
+    const uint64_t cl = rec_bytes8(w, (int)offsetof(ts_step, c_lpos));
+    const uint64_t cr = rec_bytes8(w, (int)offsetof(ts_step, c_rpos));
+    const int tl = (int)scatter6((uint32_t)lane, nc, cl), tr = (int)scatter6((uint32_t)lane, nc, cr);
+    acc = 0;
+    const int nterm = 1 << nc;
+    for (int c = 0; c < nterm; ++c) {
+      const uint32_t a = lhs[offl | (uint32_t)__shfl(tl, c)];
+      const uint32_t bb = norhs ? 1u : rhs[offr | (uint32_t)__shfl(tr, c)];
+      acc += a & bb;
+    }
   } else {
     acc = 0;
     const uint32_t nterm = 1u << nc;
     for (uint32_t c = 0; c < nterm; ++c) {
-      const uint8_t a = lhs[offl | scatter_bits<16>(c, nc, st.c_lpos)];
-      const uint8_t bb = norhs ? (uint8_t)1 : rhs[offr | scatter_bits<16>(c, nc, st.c_rpos)];
+      const uint8_t a = lhs[offl | scatter_bits<TS_MAX_BITS>(c, nc, st.c_lpos)];
+      const uint8_t bb = norhs ? (uint8_t)1 : rhs[offr | scatter_bits<TS_MAX_BITS>(c, nc, st.c_rpos)];
       acc += a & bb;
     }
   }
-  arena[st.out_off + (uint32_t)lane] = (uint8_t)(mode == TS_MODE_MOD256 ? ((acc & 255u) != 0) : (acc != 0));
+  if (lane < (1 << ro))
+    arena[out_off + (uint32_t)lane] = (uint8_t)(mode == TS_MODE_MOD256 ? ((acc & 255u) != 0) : (acc != 0));
 }
 
 // LDSA: the job arena lives in LDS (dynamic shared memory of arena_stride bytes; every
@@ -98,10 +168,10 @@ __global__ __launch_bounds__(WG) void k_batch(ts_batch b) {
   const int tid = threadIdx.x;
   uint8_t* __restrict__ arena = LDSA ? dyn_arena : b.arenas + (int64_t)blockIdx.x * b.arena_stride;
   const int num_jobs = b.num_jobs_dev ? *b.num_jobs_dev : b.num_jobs;
+  bool first = true;  // queue state of thread 0
   for (;;) {
     if (tid == 0) {
-      const int k = atomicAdd(b.counter, 1);
-      int j = k < num_jobs ? b.order[k] : -1;
+      int j = take_job(b, num_jobs, first);
       if (j >= 0 && b.early_exit && decided(b.verdicts, b.jobs[j].instance)) {
         if (b.job_done) b.job_done[j] = 2;
         j = -2;  // skipped: take the next entry
@@ -167,6 +237,61 @@ __global__ __launch_bounds__(WG) void k_batch(ts_batch b) {
   }
 }
 
+// Programs whose tensors all have at most 6 bits (cube width <= 6): every step has at
+// most 64 out entries, so a job needs one wave.  One-wave workgroups with the arena in
+// LDS: 4x the concurrent jobs of the 4-wave kernel per CU; steps are ordered by the
+// wave's own LDS order plus a workgroup-scope fence (no barrier between steps).
+__global__ __launch_bounds__(64) void k_batch_wave(ts_batch b) {
+  extern __shared__ __attribute__((aligned(16))) uint8_t arena[];
+  constexpr int NR = 16;
+  __shared__ ts_step s_steps[NR];
+  const int lane = threadIdx.x;
+  const int num_jobs = b.num_jobs_dev ? *b.num_jobs_dev : b.num_jobs;
+  bool first = true;  // queue state of lane 0
+  for (;;) {
+    int j = -1;
+    if (lane == 0) {
+      for (;;) {
+        j = take_job(b, num_jobs, first);
+        if (j >= 0 && b.early_exit && decided(b.verdicts, b.jobs[j].instance)) {
+          if (b.job_done) b.job_done[j] = 2;
+          continue;  // skipped: take the next entry
+        }
+        break;
+      }
+    }
+    j = __shfl(j, 0);
+    if (j < 0) break;
+    const ts_job job = b.jobs[j];
+    {
+      const uint8_t* src = b.leaves + job.leaf_begin;
+      const int nbytes = job.num_leaf_bytes;
+      const int n16 = nbytes >> 4;
+      for (int i = lane; i < n16; i += 64) reinterpret_cast<uint4*>(arena)[i] = reinterpret_cast<const uint4*>(src)[i];
+      for (int i = (n16 << 4) + lane; i < nbytes; i += 64) arena[i] = src[i];
+    }
+    for (int s0 = 0; s0 < job.num_steps; s0 += NR) {
+      const int ns = min(NR, job.num_steps - s0);
+      const uint4* src = reinterpret_cast<const uint4*>(b.steps + job.step_begin + s0);
+      for (int i = lane; i < ns * 8; i += 64) reinterpret_cast<uint4*>(s_steps)[i] = src[i];
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+      for (int s = 0; s < ns; ++s) {
+        step_small(s_steps[s], arena, b.mode, lane);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+      }
+    }
+    const int64_t nr = int64_t(1) << job.result_rank;
+    uint32_t v = 0;
+    for (int64_t i = lane; i < nr; i += 64) v |= arena[job.result_off + i];
+    const bool any = __ballot(v != 0) != 0;
+    if (lane == 0) {
+      if (any) atomicOr(&b.verdicts[job.instance], 1u);
+      if (b.job_done) b.job_done[j] = 1;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");  // the next job's leaves overwrite the arena
+  }
+}
+
 // Counting semiring: the same walk with uint64 arenas (arena_stride is in bytes) and
 // checked arithmetic; each job's scalar result times 2^free_vars is added to its
 // instance's count; any overflow sets the instance's flag.
@@ -180,10 +305,10 @@ __global__ __launch_bounds__(WG) void k_batch_count(ts_batch b, unsigned long lo
   uint64_t* __restrict__ arena =
       reinterpret_cast<uint64_t*>(b.arenas + (int64_t)blockIdx.x * b.arena_stride);
   const int num_jobs = b.num_jobs_dev ? *b.num_jobs_dev : b.num_jobs;
+  bool first = true;  // queue state of thread 0
   for (;;) {
     if (tid == 0) {
-      const int k = atomicAdd(b.counter, 1);
-      s_job = k < num_jobs ? b.order[k] : -1;
+      s_job = take_job(b, num_jobs, first);
       s_ovf = 0;
     }
     __syncthreads();
@@ -272,13 +397,13 @@ int ts_batch_max_workgroups(int32_t* out) {
   return TS_OK;
 }
 
-int ts_batch_max_workgroups_lds(int64_t arena_bytes, int32_t* out) {
+int ts_batch_max_workgroups_lds(int64_t arena_bytes, int32_t one_wave, int32_t* out) {
   if (!out || arena_bytes <= 0 || arena_bytes > TS_BATCH_LDS_MAX || (arena_bytes & 255)) return TS_EINVAL;
   int dev = 0, ncu = 0, per = 0;
   if (hipGetDevice(&dev) != hipSuccess) return TS_EHIP;
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return TS_EHIP;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void*>(k_batch<true>), WG,
-                                                   (size_t)arena_bytes) != hipSuccess)
+  const void* fn = one_wave ? reinterpret_cast<const void*>(k_batch_wave) : reinterpret_cast<const void*>(k_batch<true>);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, one_wave ? 64 : WG, (size_t)arena_bytes) != hipSuccess)
     return TS_EHIP;
   *out = ncu * (per > 0 ? per : 1);
   return TS_OK;
@@ -290,9 +415,15 @@ int ts_batch_run(const ts_batch* b, ts_stream stream) {
   if (b->num_jobs > 0 && !b->lds_arena && !b->arenas) return TS_EINVAL;
   if (b->arena_stride <= 0 || (b->arena_stride & 255)) return TS_EINVAL;
   if (b->lds_arena && b->arena_stride > TS_BATCH_LDS_MAX) return TS_EINVAL;
+  // Segment x of the queue past the first num_workgroups positions is served by
+  // workgroups w with w % QSEG == x: every non-empty segment needs one.
+  if (b->num_workgroups < QSEG && b->num_jobs > b->num_workgroups) return TS_EINVAL;
   if (b->mode != TS_MODE_GT0 && b->mode != TS_MODE_MOD256) return TS_EINVAL;
   if (b->num_jobs == 0) return TS_OK;
-  if (b->lds_arena)
+  if (b->lds_arena == 2)
+    hipLaunchKernelGGL(k_batch_wave, dim3(b->num_workgroups), dim3(64), (size_t)b->arena_stride,
+                       reinterpret_cast<hipStream_t>(stream), *b);
+  else if (b->lds_arena)
     hipLaunchKernelGGL((k_batch<true>), dim3(b->num_workgroups), dim3(WG), (size_t)b->arena_stride,
                        reinterpret_cast<hipStream_t>(stream), *b);
   else
@@ -303,6 +434,7 @@ int ts_batch_run(const ts_batch* b, ts_stream stream) {
 
 int ts_batch_count(const ts_batch* b, uint64_t* counts, int32_t* overflow, ts_stream stream) {
   if (!b || !counts || !overflow || b->num_jobs < 0 || b->num_workgroups <= 0 || !b->counter) return TS_EINVAL;
+  if (b->num_workgroups < QSEG && b->num_jobs > b->num_workgroups) return TS_EINVAL;
   if (b->num_jobs > 0 && (!b->jobs || !b->order || !b->steps || !b->leaves || !b->arenas)) return TS_EINVAL;
   if (b->arena_stride <= 0 || (b->arena_stride & 255)) return TS_EINVAL;
   if (b->num_jobs == 0) return TS_OK;

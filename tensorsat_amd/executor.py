@@ -167,15 +167,20 @@ class BatchExecutor:
         self.lds = fits if lds is None else bool(lds)
         if self.lds and not fits:
             raise ValueError(f"job arenas of {self.arena_stride} bytes exceed the LDS limit")
+        # Every tensor of at most 6 bits: one-wave workgroups (every step <= 64 entries).
+        st = prog.steps
+        small = bool(nj) and self.lds and (len(st) == 0 or int(st["out_rank"].max()) <= 6)
+        self.one_wave = small
         if num_workgroups is None:
             mx = N.i32(0)
             if self.lds:
-                N.check(L.ts_batch_max_workgroups_lds(self.arena_stride, ctypes.byref(mx)),
+                N.check(L.ts_batch_max_workgroups_lds(self.arena_stride, int(self.one_wave), ctypes.byref(mx)),
                         "ts_batch_max_workgroups_lds")
             else:
                 N.check(L.ts_batch_max_workgroups(ctypes.byref(mx)), "ts_batch_max_workgroups")
             num_workgroups = int(mx.value)
-        self.num_workgroups = max(1, min(int(num_workgroups), max(1, nj)))
+        # at least one workgroup per non-empty queue segment
+        self.num_workgroups = max(1, min(max(int(num_workgroups), N.BATCH_SEGMENTS), max(1, nj)))
         n_arena = 16 if self.lds else self.num_workgroups * self.arena_stride
         self.arenas = torch.empty(n_arena, dtype=torch.uint8, device=device)
         full = dispatch_order(prog) if order is None else np.asarray(order, dtype=np.int32)
@@ -193,8 +198,10 @@ class BatchExecutor:
         longest = max((b - a for a, b in self.rounds), default=1)
         self.worklist = torch.empty(max(1, longest), dtype=torch.int32, device=device)
         self.verdicts = torch.zeros(max(1, prog.num_instances), dtype=torch.int32, device=device)
-        # counters[2r] dispatch counter of round r, counters[2r+1] its compacted length
-        self.counters = torch.zeros(2 * max(1, len(self.rounds)), dtype=torch.int32, device=device)
+        # per round r (CS int32s): the queue counters (BATCH_COUNTER_INTS), then the compacted
+        # length on a line of its own
+        self.cs = N.BATCH_COUNTER_INTS + N.BATCH_COUNTER_STRIDE
+        self.counters = torch.zeros(self.cs * max(1, len(self.rounds)), dtype=torch.int32, device=device)
         self.job_done = torch.zeros(max(1, nj), dtype=torch.int32, device=device) if track_jobs else None
 
     def launch(self, mode: str = "GT0", early_exit: bool = True, events=None) -> None:
@@ -218,16 +225,17 @@ class BatchExecutor:
             if r > 0 and early_exit:
                 N.check(L.ts_batch_compact(ctypes.c_void_p(order_ptr), n, N.dptr(self.dp.jobs),
                                            N.dptr(self.verdicts), N.dptr(self.worklist),
-                                           ctypes.c_void_p(cbase + 4 * (2 * r + 1)), stream), "ts_batch_compact")
+                                           ctypes.c_void_p(cbase + 4 * (self.cs * r + N.BATCH_COUNTER_INTS)), stream),
+                        "ts_batch_compact")
                 order_ptr = self.worklist.data_ptr()
-                num_dev = cbase + 4 * (2 * r + 1)
+                num_dev = cbase + 4 * (self.cs * r + N.BATCH_COUNTER_INTS)
             bt = N.Batch(
                 jobs=self.dp.jobs.data_ptr(), order=order_ptr, num_jobs=n,
                 steps=self.dp.steps.data_ptr(), leaves=self.dp.leaves.data_ptr(), arenas=self.arenas.data_ptr(),
                 arena_stride=self.arena_stride, num_workgroups=self.num_workgroups, mode=N.MODES[mode],
-                verdicts=self.verdicts.data_ptr(), counter=cbase + 4 * (2 * r), early_exit=int(early_exit),
+                verdicts=self.verdicts.data_ptr(), counter=cbase + 4 * (self.cs * r), early_exit=int(early_exit),
                 job_done=self.job_done.data_ptr() if self.job_done is not None else None,
-                num_jobs_dev=num_dev, lds_arena=int(self.lds),
+                num_jobs_dev=num_dev, lds_arena=(2 if self.one_wave else 1) if self.lds else 0,
             )
             N.check(L.ts_batch_run(ctypes.byref(bt), stream), "ts_batch_run")
         if events is not None:
@@ -257,7 +265,8 @@ class BatchCounter:
             mx = N.i32(0)
             N.check(L.ts_batch_max_workgroups(ctypes.byref(mx)), "ts_batch_max_workgroups")
             num_workgroups = int(mx.value)
-        self.num_workgroups = max(1, min(int(num_workgroups), max(1, nj)))
+        # at least one workgroup per non-empty queue segment
+        self.num_workgroups = max(1, min(max(int(num_workgroups), N.BATCH_SEGMENTS), max(1, nj)))
         max_arena = int(prog.jobs["arena_elems"].max()) if nj else 16
         self.arena_stride = (8 * max_arena + 255) // 256 * 256
         self.arenas = torch.empty(self.num_workgroups * self.arena_stride, dtype=torch.uint8, device=device)
@@ -265,7 +274,7 @@ class BatchCounter:
         n = max(1, prog.num_instances)
         self.counts = torch.zeros(n, dtype=torch.int64, device=device)
         self.overflow = torch.zeros(n, dtype=torch.int32, device=device)
-        self.counter = torch.zeros(1, dtype=torch.int32, device=device)
+        self.counter = torch.zeros(N.BATCH_COUNTER_INTS, dtype=torch.int32, device=device)
         self.dummy = torch.zeros(n, dtype=torch.int32, device=device)
 
     def launch(self, events=None) -> None:

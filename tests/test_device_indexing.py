@@ -160,3 +160,44 @@ def test_lds_arena_programs(mode):
             got = emulate_job(prog, j, mode)
             ref, _ = OX.run_job(prog, j, mode)
             np.testing.assert_array_equal(got, ref)
+
+
+def _take_positions(num_jobs: int, g: int, seg: int, rng) -> list[int]:
+    """Queue positions handed out by the batched kernels' take_job (first job static at
+    position w, then interleaved segments of the rest, one counter each), with the
+    workgroups' takes interleaved in a random order."""
+    counters = [0] * seg
+    first = [True] * g
+    live = list(range(g))
+    out = []
+    while live:
+        w = live[rng.integers(len(live))]
+        if first[w]:
+            first[w] = False
+            if w < num_jobs:
+                out.append(w)
+                continue
+        rest, x = num_jobs - g, w % seg
+        seg_len = (rest - x + seg - 1) // seg if rest > x else 0
+        if seg_len == 0 or counters[x] >= seg_len:
+            live.remove(w)
+            continue
+        kl = counters[x]
+        counters[x] += 1
+        if kl < seg_len:
+            out.append(g + x + seg * kl)
+        else:
+            live.remove(w)
+    return out
+
+
+@pytest.mark.parametrize("num_jobs,g", [(0, 8), (5, 8), (8, 8), (9, 8), (100, 8), (100, 13), (6979, 6144),
+                                        (12046, 6144), (50, 64), (1000, 1000), (1001, 1000), (3, 3)])
+def test_job_queue_takes_every_position_once(num_jobs, g):
+    from tensorsat_amd import _native as N
+
+    if g < N.BATCH_SEGMENTS and num_jobs > g:
+        pytest.skip("rejected by ts_batch_run")
+    rng = np.random.default_rng(num_jobs * 7 + g)
+    pos = _take_positions(num_jobs, g, N.BATCH_SEGMENTS, rng)
+    assert sorted(pos) == list(range(num_jobs))
