@@ -41,29 +41,34 @@ __device__ __forceinline__ bool decided(const uint32_t* verdicts, int instance) 
   return __hip_atomic_load(&verdicts[instance], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
 }
 
-// Queue position w is workgroup w's first job, taken without an atomic (every workgroup
-// starts at once, and thousands of atomics on one address at launch serialise).  The
-// positions from num_workgroups on are split into QSEG interleaved segments (position
-// num_workgroups + k belongs to segment k % QSEG, so each keeps the queue's order) with
-// one counter each, TS_BATCH_COUNTER_STRIDE int32s (one 256-byte line) apart; workgroup w
-// takes them from segment w % QSEG, i.e. from the segment of its own XCD.  A plain load
-// of the counter first keeps workgroups that find their segment exhausted off the atomic.
-// Returns the next job, or -1 once the workgroup's segment is exhausted.
+// The first two rounds of the queue are assigned statically: workgroup w takes position
+// w, then position 2G-1-w (G = num_workgroups), so the longest jobs at the front of the
+// queue pair with the shortest of the second round and no atomic is needed (every
+// workgroup starts at once, and thousands of atomics on one address at launch
+// serialise).  Positions from 2G on are split into QSEG interleaved segments (position
+// 2G + k belongs to segment k % QSEG, so each keeps the queue's order) with one counter
+// each, TS_BATCH_COUNTER_STRIDE int32s (one 256-byte line) apart; workgroup w takes them
+// from segment w % QSEG, i.e. from the segment of its own XCD.  A plain load of the
+// counter first keeps workgroups that find their segment exhausted off the atomic.
+// `taken` counts the workgroup's takes; returns the next job, or -1 once the
+// workgroup's share is exhausted.
 constexpr int QSEG = TS_BATCH_SEGMENTS;
 constexpr int QSTRIDE = TS_BATCH_COUNTER_STRIDE;
 
-__device__ __forceinline__ int take_job(const ts_batch& b, int num_jobs, bool& first) {
+__device__ __forceinline__ int take_job(const ts_batch& b, int num_jobs, int& taken) {
   const int w = (int)blockIdx.x, g = b.num_workgroups;
-  if (first) {
-    first = false;
-    if (w < num_jobs) return b.order[w];
+  while (taken < 2) {
+    const int64_t pos = taken == 0 ? (int64_t)w : 2 * (int64_t)g - 1 - w;
+    ++taken;
+    if (pos < num_jobs) return b.order[pos];
   }
-  const int rest = num_jobs - g, x = w % QSEG;
-  const int seg_len = rest > x ? (rest - x + QSEG - 1) / QSEG : 0;
+  const int64_t rest = (int64_t)num_jobs - 2 * (int64_t)g;
+  const int x = w % QSEG;
+  const int seg_len = rest > x ? (int)((rest - x + QSEG - 1) / QSEG) : 0;
   int32_t* c = b.counter + x * QSTRIDE;
   if (seg_len == 0 || __hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= seg_len) return -1;
   const int kl = atomicAdd(c, 1);
-  return kl < seg_len ? b.order[g + x + QSEG * kl] : -1;
+  return kl < seg_len ? b.order[2 * (int64_t)g + x + (int64_t)QSEG * kl] : -1;
 }
 
 // Offset of the bits of v (LSB-first, nbits <= NB of them) placed at positions pos[i];
@@ -129,26 +134,32 @@ __device__ inline void step_small(const ts_step& st, uint8_t* arena, int mode, i
     if (!norhs) acc &= rhs[offr];
   } else if (kinner) {
     acc = popc_and(lhs + offl, rhs + offr, nc, norhs);
-  } else if (nc <= 6) {
-    // Lane t holds the summed-bit offsets of term c = t; the term loop reads them across
-    // lanes.
+  } else {
+    // Lane t holds the offsets of the low (<= 6) summed bits of term t; the term loop
+    // reads them across lanes.  Summed bits past the sixth (leaves of more than 6 bits)
+    // are walked in blocks of 64 terms, their offsets built by a short loop per block.
     const uint64_t cl = rec_bytes8(w, (int)offsetof(ts_step, c_lpos));
     const uint64_t cr = rec_bytes8(w, (int)offsetof(ts_step, c_rpos));
-    const int tl = (int)scatter6((uint32_t)lane, nc, cl), tr = (int)scatter6((uint32_t)lane, nc, cr);
+    const int nlo = nc < 6 ? nc : 6, nhi = nc - nlo;
+    const int tl = (int)scatter6((uint32_t)lane, nlo, cl), tr = (int)scatter6((uint32_t)lane, nlo, cr);
     acc = 0;
-    const int nterm = 1 << nc;
-    for (int c = 0; c < nterm; ++c) {
-      const uint32_t a = lhs[offl | (uint32_t)__shfl(tl, c)];
-      const uint32_t bb = norhs ? 1u : rhs[offr | (uint32_t)__shfl(tr, c)];
-      acc += a & bb;
-    }
-  } else {
-    acc = 0;
-    const uint32_t nterm = 1u << nc;
-    for (uint32_t c = 0; c < nterm; ++c) {
-      const uint8_t a = lhs[offl | scatter_bits<TS_MAX_BITS>(c, nc, st.c_lpos)];
-      const uint8_t bb = norhs ? (uint8_t)1 : rhs[offr | scatter_bits<TS_MAX_BITS>(c, nc, st.c_rpos)];
-      acc += a & bb;
+    const int nterm = 1 << nlo;
+    for (int chi = 0; chi < (1 << nhi); ++chi) {
+      uint32_t hl = 0, hr = 0;
+#pragma unroll 1
+      for (int i = 0; i < nhi; ++i) {
+        if ((chi >> i) & 1) {
+          const uint32_t pl = st.c_lpos[6 + i], pr = st.c_rpos[6 + i];
+          hl |= pl != TS_NOPOS ? 1u << pl : 0u;
+          hr |= pr != TS_NOPOS ? 1u << pr : 0u;
+        }
+      }
+      const uint32_t bl = offl | hl, br = offr | hr;
+      for (int c = 0; c < nterm; ++c) {
+        const uint32_t a = lhs[bl | (uint32_t)__shfl(tl, c)];
+        const uint32_t bb = norhs ? 1u : rhs[br | (uint32_t)__shfl(tr, c)];
+        acc += a & bb;
+      }
     }
   }
   if (lane < (1 << ro))
@@ -168,10 +179,10 @@ __global__ __launch_bounds__(WG) void k_batch(ts_batch b) {
   const int tid = threadIdx.x;
   uint8_t* __restrict__ arena = LDSA ? dyn_arena : b.arenas + (int64_t)blockIdx.x * b.arena_stride;
   const int num_jobs = b.num_jobs_dev ? *b.num_jobs_dev : b.num_jobs;
-  bool first = true;  // queue state of thread 0
+  int taken = 0;  // queue state of thread 0
   for (;;) {
     if (tid == 0) {
-      int j = take_job(b, num_jobs, first);
+      int j = take_job(b, num_jobs, taken);
       if (j >= 0 && b.early_exit && decided(b.verdicts, b.jobs[j].instance)) {
         if (b.job_done) b.job_done[j] = 2;
         j = -2;  // skipped: take the next entry
@@ -239,20 +250,24 @@ __global__ __launch_bounds__(WG) void k_batch(ts_batch b) {
 
 // Programs whose tensors all have at most 6 bits (cube width <= 6): every step has at
 // most 64 out entries, so a job needs one wave.  One-wave workgroups with the arena in
-// LDS: 4x the concurrent jobs of the 4-wave kernel per CU; steps are ordered by the
-// wave's own LDS order plus a workgroup-scope fence (no barrier between steps).
+// LDS: 4x the concurrent jobs of the 4-wave kernel per CU.  The workgroup is one
+// wavefront, whose LDS accesses execute in order, so steps are separated by
+// wavefront-scope fences (ordering for the compiler, no wait).  Step records are staged
+// NR at a time, the next chunk loaded into registers while the current one runs.
 __global__ __launch_bounds__(64) void k_batch_wave(ts_batch b) {
   extern __shared__ __attribute__((aligned(16))) uint8_t arena[];
   constexpr int NR = 16;
+  static_assert(NR * 8 == 128, "two uint4 of records per lane");
   __shared__ ts_step s_steps[NR];
+  uint4* const s16 = reinterpret_cast<uint4*>(s_steps);
   const int lane = threadIdx.x;
   const int num_jobs = b.num_jobs_dev ? *b.num_jobs_dev : b.num_jobs;
-  bool first = true;  // queue state of lane 0
+  int taken = 0;  // queue state of lane 0
   for (;;) {
     int j = -1;
     if (lane == 0) {
       for (;;) {
-        j = take_job(b, num_jobs, first);
+        j = take_job(b, num_jobs, taken);
         if (j >= 0 && b.early_exit && decided(b.verdicts, b.jobs[j].instance)) {
           if (b.job_done) b.job_done[j] = 2;
           continue;  // skipped: take the next entry
@@ -263,21 +278,33 @@ __global__ __launch_bounds__(64) void k_batch_wave(ts_batch b) {
     j = __shfl(j, 0);
     if (j < 0) break;
     const ts_job job = b.jobs[j];
+    // The first chunk of step records and the leaves are loaded together: one memory
+    // round trip before the first step.
+    const uint4* rsrc = reinterpret_cast<const uint4*>(b.steps + job.step_begin);
+    int n = min(NR, job.num_steps) * 8;
+    uint4 r0 = {0, 0, 0, 0}, r1 = {0, 0, 0, 0};
+    if (lane < n) r0 = rsrc[lane];
+    if (lane + 64 < n) r1 = rsrc[lane + 64];
     {
       const uint8_t* src = b.leaves + job.leaf_begin;
       const int nbytes = job.num_leaf_bytes;
       const int n16 = nbytes >> 4;
+#pragma unroll 1
       for (int i = lane; i < n16; i += 64) reinterpret_cast<uint4*>(arena)[i] = reinterpret_cast<const uint4*>(src)[i];
-      for (int i = (n16 << 4) + lane; i < nbytes; i += 64) arena[i] = src[i];
+      if (lane < nbytes - (n16 << 4)) arena[(n16 << 4) + lane] = src[(n16 << 4) + lane];  // < 16 tail bytes
     }
     for (int s0 = 0; s0 < job.num_steps; s0 += NR) {
+      // r0/r1 hold records s0.. (n uint4): stage them, then fetch the next chunk.
+      if (lane < n) s16[lane] = r0;
+      if (lane + 64 < n) s16[lane + 64] = r1;
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       const int ns = min(NR, job.num_steps - s0);
-      const uint4* src = reinterpret_cast<const uint4*>(b.steps + job.step_begin + s0);
-      for (int i = lane; i < ns * 8; i += 64) reinterpret_cast<uint4*>(s_steps)[i] = src[i];
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+      n = max(0, min(NR, job.num_steps - s0 - NR)) * 8;
+      if (lane < n) r0 = rsrc[(s0 + NR) * 8 + lane];
+      if (lane + 64 < n) r1 = rsrc[(s0 + NR) * 8 + lane + 64];
       for (int s = 0; s < ns; ++s) {
         step_small(s_steps[s], arena, b.mode, lane);
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       }
     }
     const int64_t nr = int64_t(1) << job.result_rank;
@@ -288,7 +315,7 @@ __global__ __launch_bounds__(64) void k_batch_wave(ts_batch b) {
       if (any) atomicOr(&b.verdicts[job.instance], 1u);
       if (b.job_done) b.job_done[j] = 1;
     }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");  // the next job's leaves overwrite the arena
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // the next job's leaves overwrite the arena
   }
 }
 
@@ -305,10 +332,10 @@ __global__ __launch_bounds__(WG) void k_batch_count(ts_batch b, unsigned long lo
   uint64_t* __restrict__ arena =
       reinterpret_cast<uint64_t*>(b.arenas + (int64_t)blockIdx.x * b.arena_stride);
   const int num_jobs = b.num_jobs_dev ? *b.num_jobs_dev : b.num_jobs;
-  bool first = true;  // queue state of thread 0
+  int taken = 0;  // queue state of thread 0
   for (;;) {
     if (tid == 0) {
-      s_job = take_job(b, num_jobs, first);
+      s_job = take_job(b, num_jobs, taken);
       s_ovf = 0;
     }
     __syncthreads();
@@ -415,9 +442,9 @@ int ts_batch_run(const ts_batch* b, ts_stream stream) {
   if (b->num_jobs > 0 && !b->lds_arena && !b->arenas) return TS_EINVAL;
   if (b->arena_stride <= 0 || (b->arena_stride & 255)) return TS_EINVAL;
   if (b->lds_arena && b->arena_stride > TS_BATCH_LDS_MAX) return TS_EINVAL;
-  // Segment x of the queue past the first num_workgroups positions is served by
+  // Segment x of the queue past the first 2 * num_workgroups positions is served by
   // workgroups w with w % QSEG == x: every non-empty segment needs one.
-  if (b->num_workgroups < QSEG && b->num_jobs > b->num_workgroups) return TS_EINVAL;
+  if (b->num_workgroups < QSEG && b->num_jobs > 2 * (int64_t)b->num_workgroups) return TS_EINVAL;
   if (b->mode != TS_MODE_GT0 && b->mode != TS_MODE_MOD256) return TS_EINVAL;
   if (b->num_jobs == 0) return TS_OK;
   if (b->lds_arena == 2)
@@ -434,7 +461,7 @@ int ts_batch_run(const ts_batch* b, ts_stream stream) {
 
 int ts_batch_count(const ts_batch* b, uint64_t* counts, int32_t* overflow, ts_stream stream) {
   if (!b || !counts || !overflow || b->num_jobs < 0 || b->num_workgroups <= 0 || !b->counter) return TS_EINVAL;
-  if (b->num_workgroups < QSEG && b->num_jobs > b->num_workgroups) return TS_EINVAL;
+  if (b->num_workgroups < QSEG && b->num_jobs > 2 * (int64_t)b->num_workgroups) return TS_EINVAL;
   if (b->num_jobs > 0 && (!b->jobs || !b->order || !b->steps || !b->leaves || !b->arenas)) return TS_EINVAL;
   if (b->arena_stride <= 0 || (b->arena_stride & 255)) return TS_EINVAL;
   if (b->num_jobs == 0) return TS_OK;

@@ -163,21 +163,26 @@ def test_lds_arena_programs(mode):
 
 
 def _take_positions(num_jobs: int, g: int, seg: int, rng) -> list[int]:
-    """Queue positions handed out by the batched kernels' take_job (first job static at
-    position w, then interleaved segments of the rest, one counter each), with the
+    """Queue positions handed out by the batched kernels' take_job (positions w and
+    2g-1-w static, then interleaved segments of the rest, one counter each), with the
     workgroups' takes interleaved in a random order."""
     counters = [0] * seg
-    first = [True] * g
+    taken = [0] * g
     live = list(range(g))
     out = []
     while live:
         w = live[rng.integers(len(live))]
-        if first[w]:
-            first[w] = False
-            if w < num_jobs:
-                out.append(w)
-                continue
-        rest, x = num_jobs - g, w % seg
+        got = None
+        while taken[w] < 2:
+            pos = w if taken[w] == 0 else 2 * g - 1 - w
+            taken[w] += 1
+            if pos < num_jobs:
+                got = pos
+                break
+        if got is not None:
+            out.append(got)
+            continue
+        rest, x = num_jobs - 2 * g, w % seg
         seg_len = (rest - x + seg - 1) // seg if rest > x else 0
         if seg_len == 0 or counters[x] >= seg_len:
             live.remove(w)
@@ -185,19 +190,48 @@ def _take_positions(num_jobs: int, g: int, seg: int, rng) -> list[int]:
         kl = counters[x]
         counters[x] += 1
         if kl < seg_len:
-            out.append(g + x + seg * kl)
+            out.append(2 * g + x + seg * kl)
         else:
             live.remove(w)
     return out
 
 
-@pytest.mark.parametrize("num_jobs,g", [(0, 8), (5, 8), (8, 8), (9, 8), (100, 8), (100, 13), (6979, 6144),
-                                        (12046, 6144), (50, 64), (1000, 1000), (1001, 1000), (3, 3)])
+@pytest.mark.parametrize("num_jobs,g", [(0, 8), (5, 8), (8, 8), (9, 8), (16, 8), (17, 8), (100, 8), (100, 13),
+                                        (6979, 6144), (12046, 6144), (12046, 8192), (20000, 8192), (50, 64),
+                                        (1000, 1000), (2001, 1000), (3, 3), (6, 3)])
 def test_job_queue_takes_every_position_once(num_jobs, g):
     from tensorsat_amd import _native as N
 
-    if g < N.BATCH_SEGMENTS and num_jobs > g:
+    if g < N.BATCH_SEGMENTS and num_jobs > 2 * g:
         pytest.skip("rejected by ts_batch_run")
     rng = np.random.default_rng(num_jobs * 7 + g)
     pos = _take_positions(num_jobs, g, N.BATCH_SEGMENTS, rng)
     assert sorted(pos) == list(range(num_jobs))
+
+
+@pytest.mark.parametrize("out_wires", [(0, 1), (), (4,), (2, 7, 9)])
+def test_permuted_rhs_step_keeps_its_value(out_wires):
+    """The general (not K-innermost) one-step programs of the GPU tests: permuting the rhs
+    leaf keeps the oracle's value, which equals a direct einsum over the wires."""
+    from oracle import executor as OX
+    from wide_steps import wide_sum_program
+
+    for seed in range(6):
+        for mode, dens in (("GT0", None), ("MOD256", 0.7)):
+            plain, leaves = wide_sum_program(seed, out_wires, density=dens, permute=False)
+            perm, _ = wide_sum_program(seed, out_wires, density=dens, permute=True)
+            assert int(perm.steps["flags"][0]) & PG.STEP_KINNER == 0
+            assert int(perm.steps["sum_rank"][0]) == 10 - len(out_wires)
+            a = OX.run_job(plain, 0, mode)[0]
+            b = OX.run_job(perm, 0, mode)[0]
+            np.testing.assert_array_equal(a, b)
+            # direct: leaves are in slot wire order, wire i of a slot's list = axis i
+            rng = np.random.default_rng(seed)
+            wb = [int(x) for x in rng.permutation(10)]
+            A = leaves[0].reshape((2,) * 10).astype(np.int64)
+            B = leaves[1].reshape((2,) * 10).astype(np.int64)
+            letters = "abcdefghij"
+            spec = f"{letters},{''.join(letters[x] for x in wb)}->{''.join(letters[x] for x in out_wires)}"
+            acc = np.einsum(spec, A, B).reshape(-1)
+            want = ((acc % 256) != 0) if mode == "MOD256" else (acc != 0)
+            np.testing.assert_array_equal(np.asarray(a).reshape(-1), want.astype(np.uint8))

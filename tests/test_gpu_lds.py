@@ -34,3 +34,26 @@ def test_lds_kernel_matches_global_kernel_and_oracle(width, mode):
     # without early exit every job ran exactly once
     lds.run(mode, early_exit=False)
     assert (lds.job_done.cpu().numpy()[: len(prog.jobs)] == 1).all()
+
+
+@pytest.mark.parametrize("out_wires", [(0, 1), (), (4,), (2, 7, 9)])
+@pytest.mark.parametrize("mode", ["GT0", "MOD256"])
+def test_one_wave_kernel_general_steps_with_many_summed_bits(out_wires, mode):
+    """A general step (summed bits not innermost) summing 7 to 10 bits: the one-wave
+    kernel's block loop over summed bits past the sixth, against the oracle."""
+    from tensorsat_amd.executor import BatchExecutor
+    from wide_steps import wide_sum_program
+
+    seen = set()
+    for seed in range(12):
+        prog, _ = wide_sum_program(seed, out_wires, density=0.7 if mode == "MOD256" else None)
+        want = np.uint8(np.asarray(OX.run_job(prog, 0, mode)[0]).any())
+        seen.add(int(want))
+        ex = BatchExecutor(prog, lds=True)
+        assert ex.one_wave
+        glb = BatchExecutor(prog, lds=False)
+        print(f"{mode} out {out_wires} seed {seed}", flush=True)
+        assert ex.run(mode, early_exit=False)[0] == want
+        assert glb.run(mode, early_exit=False)[0] == want
+    if mode == "GT0":
+        assert seen == {0, 1}
