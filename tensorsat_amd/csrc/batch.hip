@@ -101,33 +101,46 @@ __device__ __forceinline__ uint64_t rec_bytes8(const uint32_t* w, int off) {
   return (lo >> (8 * r)) | (hi << (64 - 8 * r));
 }
 
-// Bits of v (the first n <= 6 of them) placed at the positions packed in pw (one byte per
-// bit, TS_NOPOS = none).
-__device__ __forceinline__ uint32_t scatter6(uint32_t v, int n, uint64_t pw) {
+// Bits of v placed at the positions packed in pw (one byte per bit, TS_NOPOS = none);
+// v must have no bits at or above the list's length (callers mask it).  Evaluated per
+// lane in vector registers: the 64-bit shift takes its amount mod 64, so a TS_NOPOS byte
+// (255 -> 63) moves the bit out of the low word and contributes nothing, while real
+// positions are below 32.
+__device__ __forceinline__ uint32_t scatter6(uint32_t v, uint64_t pw) {
   uint32_t off = 0;
 #pragma unroll
   for (int i = 0; i < 6; ++i) {
-    const uint32_t p = (uint32_t)(pw >> (8 * i)) & 0xffu;
-    const bool on = i < n && ((v >> i) & 1u) && p != TS_NOPOS;
-    off |= on ? (1u << (p & 31u)) : 0u;
+    const uint32_t p = (uint32_t)(pw >> (8 * i)) & 63u;
+    off |= (uint32_t)((uint64_t)((v >> i) & 1u) << p);
   }
   return off;
+}
+
+// A zero the compiler cannot see through: record words read through an address offset
+// by it are held per lane, so their decoding runs on the vector ALUs (four per CU)
+// rather than the one scalar ALU that every wave of the CU shares.
+__device__ __forceinline__ int opaque_zero() {
+  int z;
+  asm volatile("v_mov_b32 %0, 0" : "=v"(z));
+  return z;
 }
 
 __device__ inline void step_small(const ts_step& st, uint8_t* arena, int mode, int lane) {
   const uint32_t* w = reinterpret_cast<const uint32_t*>(&st);
   // Header: lhs_off, rhs_off, out_off, ranks (bytes 12..15), flags (byte 16).
   const uint32_t lhs_off = w[0], rhs_off = w[1], out_off = w[2], ranks = w[3], fl = w[4];
-  const uint64_t lpw = rec_bytes8(w, (int)offsetof(ts_step, o_lpos));
-  const uint64_t rpw = rec_bytes8(w, (int)offsetof(ts_step, o_rpos));
+  const uint32_t* wv = w + opaque_zero();  // position words, decoded per lane
+  const uint64_t lpw = rec_bytes8(wv, (int)offsetof(ts_step, o_lpos));
+  const uint64_t rpw = rec_bytes8(wv, (int)offsetof(ts_step, o_rpos));
   const int ro = (int)((ranks >> 16) & 0xffu), nc = (int)(ranks >> 24);
   const bool norhs = fl & TS_STEP_NORHS, kinner = fl & TS_STEP_KINNER;
   const uint8_t* lhs = arena + lhs_off;
   const uint8_t* rhs = arena + (norhs ? lhs_off : rhs_off);
   // Every lane stays active (cross-lane reads below); lanes >= 2^ro compute in-bounds
   // aliases (only the low ro bits of the lane index are scattered) and do not store.
-  const uint32_t offl = scatter6((uint32_t)lane, ro, lpw);
-  const uint32_t offr = scatter6((uint32_t)lane, ro, rpw);
+  const uint32_t le = (uint32_t)lane & ((1u << ro) - 1u);
+  const uint32_t offl = scatter6(le, lpw);
+  const uint32_t offr = scatter6(le, rpw);
   uint32_t acc;
   if (nc == 0) {
     acc = lhs[offl];
@@ -138,10 +151,11 @@ __device__ inline void step_small(const ts_step& st, uint8_t* arena, int mode, i
     // Lane t holds the offsets of the low (<= 6) summed bits of term t; the term loop
     // reads them across lanes.  Summed bits past the sixth (leaves of more than 6 bits)
     // are walked in blocks of 64 terms, their offsets built by a short loop per block.
-    const uint64_t cl = rec_bytes8(w, (int)offsetof(ts_step, c_lpos));
-    const uint64_t cr = rec_bytes8(w, (int)offsetof(ts_step, c_rpos));
+    const uint64_t cl = rec_bytes8(wv, (int)offsetof(ts_step, c_lpos));
+    const uint64_t cr = rec_bytes8(wv, (int)offsetof(ts_step, c_rpos));
     const int nlo = nc < 6 ? nc : 6, nhi = nc - nlo;
-    const int tl = (int)scatter6((uint32_t)lane, nlo, cl), tr = (int)scatter6((uint32_t)lane, nlo, cr);
+    const uint32_t lc = (uint32_t)lane & ((1u << nlo) - 1u);
+    const int tl = (int)scatter6(lc, cl), tr = (int)scatter6(lc, cr);
     acc = 0;
     const int nterm = 1 << nlo;
     for (int chi = 0; chi < (1 << nhi); ++chi) {
