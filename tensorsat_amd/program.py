@@ -242,6 +242,44 @@ def sat_batch(
     return prog
 
 
+def concat(progs: list[Program], instances: list[np.ndarray], num_instances: int) -> Program:
+    """
+    One program holding the jobs of ``progs``, program k's instance i renamed
+    ``instances[k][i]`` (in ``range(num_instances)``), so that one persistent launch runs
+    them all and early exit sees every cube of an instance.  Steps and leaves are
+    appended (leaf blocks stay 16-byte aligned); ``stats`` rows are taken from the first
+    program that holds each instance.
+    """
+    jobs, steps, leaves = [], [], []
+    nsteps = nleaves = 0
+    stats = np.zeros((num_instances, NSTAT), dtype=np.int32)
+    have = np.zeros(num_instances, dtype=bool)
+    for p, m in zip(progs, instances):
+        m = np.asarray(m, dtype=np.int64)
+        if m.shape != (p.num_instances,) or (m.size and (m.min() < 0 or m.max() >= num_instances)):
+            raise ValueError("instance map must rename every instance into range(num_instances)")
+        j = p.jobs.copy()
+        j["step_begin"] += nsteps
+        j["leaf_begin"] += nleaves
+        j["instance"] = m[j["instance"]] if len(j) else j["instance"]
+        jobs.append(j)
+        steps.append(p.steps)
+        pad = (-p.leaves.size) % 16
+        leaves.append(np.concatenate([p.leaves, np.zeros(pad, np.uint8)]) if pad else p.leaves)
+        nsteps += len(p.steps)
+        nleaves += p.leaves.size + pad
+        new = ~have[m]
+        stats[m[new]] = p.stats[new]
+        have[m] = True
+    if nsteps >= 2**31:
+        raise ValueError("too many steps for one program")
+    out = Program(np.concatenate(jobs) if jobs else np.zeros(0, JOB_DTYPE),
+                  np.concatenate(steps) if steps else np.zeros(0, STEP_DTYPE),
+                  np.concatenate(leaves) if leaves else np.zeros(0, np.uint8), stats)
+    out.meta["parts"] = [len(p.jobs) for p in progs]
+    return out
+
+
 def step_labels(st) -> tuple[list[int], list[int], list[int]]:
     """
     Einsum labels of one step record: out bit j is label j, summed bit j is label

@@ -5,7 +5,8 @@ Batched SAT decisions: conditioned slicing on the host, contraction on the devic
 (``program.sat_batch``) slices every instance into cubes of width <= W and compiles the
 first ``first_pass`` cubes of each into contraction jobs; the device runs them in
 persistent launches with early exit.  Instances still undecided whose cube list was cut
-off get a second pass over the rest of their cubes.  Both passes are prepared once
+off get a second pass over the rest of their cubes, whose jobs are appended to the first
+pass's program (one launch decides the batch).  Both passes are prepared once
 (:meth:`prepare`), so :meth:`run` only launches kernels on device-resident jobs.
 
 The verdict of an instance is the OR over all its cubes; every pass keeps that OR, so
@@ -39,8 +40,11 @@ class SolveStats:
 class BatchSolver:
     def __init__(self, *, width: int = 10, first_pass: int = 4, mode: str = "GT0", threads: int = 0,
                  plan_trials: int = 1, num_workgroups: int | None = None, track_jobs: bool = True,
-                 lds: bool | None = None, round_bounds=None) -> None:
+                 lds: bool | None = None, round_bounds=None, merge_passes: bool = True) -> None:
+        """``merge_passes``: run the second pass's jobs in the first pass's launch (one
+        program; default) rather than in a launch of their own after it."""
         self.width = width
+        self.merge_passes = merge_passes
         self.first_pass = first_pass
         self.mode = mode
         self.threads = threads
@@ -77,7 +81,14 @@ class BatchSolver:
             self.stats.plan_s += time.perf_counter() - t0
             self.stats.pass2_jobs = len(p2.jobs)
             self.stats.pass2_instances = int(need.size)
-            if len(p2.jobs):
+            if len(p2.jobs) and self.merge_passes:
+                # One program, one launch: the second pass's cubes queue behind the first
+                # pass's (dispatch order is by cube index) and share its early exit.
+                merged = PG.concat([p1, p2], [np.arange(self.n), need], self.n)
+                self.progs = [merged]
+                self.execs = [BatchExecutor(merged, num_workgroups=self.num_workgroups,
+                                            track_jobs=self.track_jobs, lds=self.lds, **rb)]
+            elif len(p2.jobs):
                 self.progs.append(p2)
                 self.execs.append(BatchExecutor(p2, num_workgroups=self.num_workgroups,
                                                 track_jobs=self.track_jobs, lds=self.lds, **rb))

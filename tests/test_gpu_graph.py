@@ -13,16 +13,20 @@ from tensorsat_amd.workloads import batched_3sat
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("width", [8, 12])
-def test_graph_replays_match_eager_and_dpll(width):
+@pytest.mark.parametrize("merge", [True, False])
+@pytest.mark.parametrize("width", [6, 8, 12])
+def test_graph_replays_match_eager_and_dpll(width, merge):
     import torch
 
     from tensorsat_amd.solver import BatchSolver
 
     cnfs = batched_3sat(384)
     want = np.array([dpll.solve(c.num_vars, c.clauses) for c in cnfs], dtype=np.uint8)
-    solver = BatchSolver(width=width, first_pass=2)
+    solver = BatchSolver(width=width, first_pass=2, merge_passes=merge)
     solver.prepare(cnfs)
+    print(f"width {width} merge {merge}: {solver.stats.pass2_instances} second-pass instances, "
+          f"{len(solver.execs)} launches", flush=True)
+    assert len(solver.execs) == (1 if merge or not solver.stats.pass2_jobs else 2)
     eager = solver.run()
     np.testing.assert_array_equal(eager, want)
     solver.capture()

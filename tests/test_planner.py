@@ -133,3 +133,25 @@ def test_parallel_enumeration_reproduces_sequential_program(width):
     np.testing.assert_array_equal(seq.steps.view(np.uint8), par.steps.view(np.uint8))
     np.testing.assert_array_equal(seq.leaves, par.leaves)
     assert seq.stats[0, 1] == par.stats[0, 1]  # branch nodes
+
+
+def test_concat_programs_keeps_every_verdict():
+    """Two programs over overlapping instance sets, concatenated with renamed instances:
+    valid (aligned leaf blocks, in-range steps) and decided like the OR of the parts."""
+    from oracle import executor as OX
+    from tensorsat_amd.workloads import batched_3sat
+
+    cnfs = batched_3sat(40, start=300)
+    p1 = PG.sat_batch(cnfs, width=6, max_cubes=2)
+    sub = np.arange(0, 40, 3)
+    p2 = PG.sat_batch([cnfs[i] for i in sub], width=6, cube_begin=2)
+    p1.leaves = np.concatenate([p1.leaves, np.ones(5, np.uint8)])  # an unaligned tail to pad
+    m = PG.concat([p1, p2], [np.arange(40), sub], 40)
+    PG.validate(m)
+    assert len(m.jobs) == len(p1.jobs) + len(p2.jobs)
+    want = OX.decide(p1, "GT0")
+    want[sub] |= OX.decide(p2, "GT0")
+    np.testing.assert_array_equal(OX.decide(m, "GT0"), want)
+    np.testing.assert_array_equal(m.stats, p1.stats)
+    with pytest.raises(ValueError):
+        PG.concat([p2], [sub + 40], 40)
