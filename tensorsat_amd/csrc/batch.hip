@@ -101,19 +101,18 @@ __device__ __forceinline__ uint64_t rec_bytes8(const uint32_t* w, int off) {
   return (lo >> (8 * r)) | (hi << (64 - 8 * r));
 }
 
-// Bits of v placed at the positions packed in pw (one byte per bit, TS_NOPOS = none);
-// v must have no bits at or above the list's length (callers mask it).  Evaluated per
-// lane in vector registers: the 64-bit shift takes its amount mod 64, so a TS_NOPOS byte
-// (255 -> 63) moves the bit out of the low word and contributes nothing, while real
-// positions are below 32.
-__device__ __forceinline__ uint32_t scatter6(uint32_t v, uint64_t pw) {
+// Offset of an index whose bits 0..5 are lb[0..5] (0/1) placed at the positions packed
+// in pw (one byte per bit, TS_NOPOS = none), only the first n of them taken.  Evaluated
+// per lane in vector registers, one bit-field extract and one shift-or per bit: bytes
+// from the n-th on are forced to TS_NOPOS first, every TS_NOPOS byte shifts its bit to
+// position 31 (255 mod 32), and bit 31 is cleared at the end (real positions are below
+// TS_MAX_BITS <= 31).
+__device__ __forceinline__ uint32_t scatter6(const uint32_t (&lb)[6], int n, uint64_t pw) {
+  pw |= ~0ull << (8 * n);  // n <= 6
   uint32_t off = 0;
 #pragma unroll
-  for (int i = 0; i < 6; ++i) {
-    const uint32_t p = (uint32_t)(pw >> (8 * i)) & 63u;
-    off |= (uint32_t)((uint64_t)((v >> i) & 1u) << p);
-  }
-  return off;
+  for (int i = 0; i < 6; ++i) off |= lb[i] << ((uint32_t)(pw >> (8 * i)) & 31u);
+  return off & 0x7fffffffu;
 }
 
 // A zero the compiler cannot see through: record words read through an address offset
@@ -138,9 +137,11 @@ __device__ inline void step_small(const ts_step& st, uint8_t* arena, int mode, i
   const uint8_t* rhs = arena + (norhs ? lhs_off : rhs_off);
   // Every lane stays active (cross-lane reads below); lanes >= 2^ro compute in-bounds
   // aliases (only the low ro bits of the lane index are scattered) and do not store.
-  const uint32_t le = (uint32_t)lane & ((1u << ro) - 1u);
-  const uint32_t offl = scatter6(le, lpw);
-  const uint32_t offr = scatter6(le, rpw);
+  uint32_t lb[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) lb[i] = ((uint32_t)lane >> i) & 1u;
+  const uint32_t offl = scatter6(lb, ro, lpw);
+  const uint32_t offr = scatter6(lb, ro, rpw);
   uint32_t acc;
   if (nc == 0) {
     acc = lhs[offl];
@@ -154,8 +155,7 @@ __device__ inline void step_small(const ts_step& st, uint8_t* arena, int mode, i
     const uint64_t cl = rec_bytes8(wv, (int)offsetof(ts_step, c_lpos));
     const uint64_t cr = rec_bytes8(wv, (int)offsetof(ts_step, c_rpos));
     const int nlo = nc < 6 ? nc : 6, nhi = nc - nlo;
-    const uint32_t lc = (uint32_t)lane & ((1u << nlo) - 1u);
-    const int tl = (int)scatter6(lc, cl), tr = (int)scatter6(lc, cr);
+    const int tl = (int)scatter6(lb, nlo, cl), tr = (int)scatter6(lb, nlo, cr);
     acc = 0;
     const int nterm = 1 << nlo;
     for (int chi = 0; chi < (1 << nhi); ++chi) {
