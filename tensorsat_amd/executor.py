@@ -197,12 +197,17 @@ class BatchExecutor:
         self.order = torch.from_numpy(np.ascontiguousarray(full, dtype=np.int32)).to(device)
         longest = max((b - a for a, b in self.rounds), default=1)
         self.worklist = torch.empty(max(1, longest), dtype=torch.int32, device=device)
-        self.verdicts = torch.zeros(max(1, prog.num_instances), dtype=torch.int32, device=device)
-        # per round r (CS int32s): the queue counters (BATCH_COUNTER_INTS), then the compacted
-        # length on a line of its own
+        # Everything a launch resets lives in one buffer, cleared by one fill kernel: per
+        # round r (CS int32s) the queue counters (BATCH_COUNTER_INTS) and the compacted
+        # length on a line of its own, then the verdicts, then the per-job flags.
         self.cs = N.BATCH_COUNTER_INTS + N.BATCH_COUNTER_STRIDE
-        self.counters = torch.zeros(self.cs * max(1, len(self.rounds)), dtype=torch.int32, device=device)
-        self.job_done = torch.zeros(max(1, nj), dtype=torch.int32, device=device) if track_jobs else None
+        n_cnt = self.cs * max(1, len(self.rounds))
+        n_ver = (max(1, prog.num_instances) + 63) // 64 * 64
+        n_done = max(1, nj) if track_jobs else 0
+        self.resets = torch.zeros(n_cnt + n_ver + n_done, dtype=torch.int32, device=device)
+        self.counters = self.resets[:n_cnt]
+        self.verdicts = self.resets[n_cnt : n_cnt + max(1, prog.num_instances)]
+        self.job_done = self.resets[n_cnt + n_ver :] if track_jobs else None
 
     def launch(self, mode: str = "GT0", early_exit: bool = True, events=None) -> None:
         """
@@ -210,10 +215,7 @@ class BatchExecutor:
         ``events`` (start, end) are recorded right around the round kernels.
         """
         L = N.lib()
-        self.verdicts.zero_()
-        self.counters.zero_()
-        if self.job_done is not None:
-            self.job_done.zero_()
+        self.resets.zero_()  # counters, verdicts and job flags: one fill kernel
         if events is not None:
             events[0].record()
         stream = N.stream_ptr()
