@@ -115,25 +115,41 @@ def count_program(dp: DeviceProgram) -> list[int]:
     return [int(v) for v in vals]
 
 
-def dispatch_order(prog: Program) -> np.ndarray:
-    """
-    Job order for the persistent queue: by cube index within the instance (so early
-    exit can skip later cubes of decided instances), then longest job first.
-    """
+def job_work(prog: Program) -> np.ndarray:
+    """Latency estimate per job: steps run one after the other, and a step takes one pass
+    of the workgroup per 256 out x summed entries."""
     jobs = prog.jobs
-    if len(jobs) == 0:
-        return np.zeros(0, dtype=np.int32)
     steps = prog.steps
-    # Latency estimate: steps run one after the other with a barrier each, and a step
-    # takes one pass of the workgroup per 256 out x summed entries.
     work = np.zeros(len(jobs), dtype=np.float64)
-    if len(steps):
+    if len(jobs) and len(steps):
         per = 1.0 + np.ldexp(1.0, steps["out_rank"].astype(np.int64) + steps["sum_rank"].astype(np.int64)) / 256.0
         csum = np.concatenate([[0.0], np.cumsum(per)])
         b = jobs["step_begin"].astype(np.int64)
         e = b + jobs["num_steps"].astype(np.int64)
         work = csum[e] - csum[b]
-    return np.lexsort((-work, jobs["cube"])).astype(np.int32)
+    return work
+
+
+def dispatch_order(prog: Program, num_workgroups: int | None = None) -> np.ndarray:
+    """
+    Job order for the persistent queue: by cube index within the instance (so early
+    exit can skip later cubes of decided instances), then longest job first.
+
+    With ``num_workgroups`` = G, the two statically assigned rounds (workgroup w takes
+    positions w and 2G-1-w) keep their members but are each sorted longest first, so the
+    longest second-round jobs go to the workgroups whose first job is shortest.
+    """
+    jobs = prog.jobs
+    if len(jobs) == 0:
+        return np.zeros(0, dtype=np.int32)
+    work = job_work(prog)
+    order = np.lexsort((-work, jobs["cube"])).astype(np.int32)
+    if num_workgroups:
+        g = int(num_workgroups)
+        r1, r2 = order[:g], order[g : 2 * g]
+        order = np.concatenate([r1[np.argsort(-work[r1], kind="stable")],
+                                r2[np.argsort(-work[r2], kind="stable")], order[2 * g :]]).astype(np.int32)
+    return order
 
 
 ROUND_BOUNDS = (0, 1 << 30)
@@ -183,11 +199,17 @@ class BatchExecutor:
         self.num_workgroups = max(1, min(max(int(num_workgroups), N.BATCH_SEGMENTS), max(1, nj)))
         n_arena = 16 if self.lds else self.num_workgroups * self.arena_stride
         self.arenas = torch.empty(n_arena, dtype=torch.uint8, device=device)
-        full = dispatch_order(prog) if order is None else np.asarray(order, dtype=np.int32)
+        one_round = order is not None or (len(round_bounds) == 2 and round_bounds[0] <= 0
+                                          and (nj == 0 or int(prog.jobs["cube"].max()) < round_bounds[1]))
+        if order is not None:
+            full = np.asarray(order, dtype=np.int32)
+        else:
+            # one round: the static rounds are paired (dispatch_order with G)
+            full = dispatch_order(prog, self.num_workgroups if one_round else None)
         cubes = prog.jobs["cube"][full] if nj else np.zeros(0, np.int32)
         self.rounds: list[tuple[int, int]] = []
-        if order is not None:
-            self.rounds = [(0, len(full))]
+        if one_round:
+            self.rounds = [(0, len(full))] if len(full) else []
         else:
             for lo, hi in zip(round_bounds[:-1], round_bounds[1:]):
                 a = int(np.searchsorted(cubes, lo, side="left"))
