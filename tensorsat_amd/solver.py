@@ -95,8 +95,12 @@ class BatchSolver:
                 self.subsets.append(need)
 
     def launch(self, events=None) -> None:
-        """Enqueue every pass on the current stream; ``events`` (start, end) bracket all
-        of them (the verdict/counter resets of each pass included)."""
+        """Enqueue every pass on the current stream.  ``events`` (start, end): with one
+        launch (passes merged) they bracket its persistent kernel alone, after the reset;
+        with separate passes they bracket all of them, resets included."""
+        if len(self.execs) == 1:
+            self.execs[0].launch(self.mode, events=events)
+            return
         if events is not None:
             events[0].record()
         for ex in self.execs:
