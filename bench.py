@@ -47,7 +47,9 @@ def batch_kernel_name(ex) -> str:
     """The persistent kernel a BatchExecutor launches."""
     if not ex.lds:
         return "k_batch"
-    return "k_batch_wave" if ex.one_wave else "k_batch<true>"
+    if ex.one_wave:
+        return "k_batch_wave<true> (packet form)" if ex.packets else "k_batch_wave<false>"
+    return "k_batch<true>"
 
 
 def job_bytes(prog) -> np.ndarray:

@@ -146,7 +146,9 @@ typedef struct {
                                (arena_stride bytes of LDS per workgroup, <= TS_BATCH_LDS_MAX;
                                `arenas` unused); 2: as 1 with one-wave workgroups, for programs
                                whose tensors all have at most 6 bits (every step <= 64 out
-                               entries).  num_workgroups from ts_batch_max_workgroups_lds. */
+                               entries); 3: as 2 for packet-form programs (tsc_packetize):
+                               one packet of independent steps per wave round.
+                               num_workgroups from ts_batch_max_workgroups_lds. */
 } ts_batch;
 
 #define TS_BATCH_LDS_MAX 65536
@@ -176,8 +178,8 @@ int ts_batch_compact(const int32_t* order, int32_t n, const ts_job* jobs, const 
 int ts_batch_max_workgroups(int32_t* out);
 
 /* Maximum resident workgroups of the LDS-arena batched kernel for arenas of arena_bytes
- * (a multiple of 256, at most TS_BATCH_LDS_MAX); one_wave selects the lds_arena = 2
- * variant. */
+ * (a multiple of 256, at most TS_BATCH_LDS_MAX); one_wave = 1 selects the lds_arena = 2
+ * variant, 2 the lds_arena = 3 (packet form) one. */
 int ts_batch_max_workgroups_lds(int64_t arena_bytes, int32_t one_wave, int32_t* out);
 
 /* ---- Learned planner: message-passing network kernels (bf16 in, fp32 accumulate) --

@@ -43,6 +43,10 @@ typedef struct {
 
 #define TS_STEP_KINNER 1  /* summed bits are the lowest sum_rank bits of both operands, in order */
 #define TS_STEP_NORHS 2   /* unary step (rewire/reduce): rhs is the constant 1 */
+#define TS_STEP_PACKET 4  /* packet form (tsc_packetize): pad[0] = first lane of this step */
+#define TS_STEP_PEND 8    /* packet form: last record of its packet */
+#define TS_PACKET_LANES 64 /* lanes of one packet (one wavefront) */
+#define TS_PACKET_CHUNK 16 /* packets never straddle a boundary of this many records of a job */
 
 /* One job: a contraction of its leaves down to one tensor. */
 typedef struct {
@@ -130,6 +134,21 @@ int tsp_order_cost(const tsp_network* net, const int32_t* order, int32_t* path_o
 int tsc_sat_batch(int32_t num_inst, const int32_t* nvars, const int64_t* clause_offsets,
                   const int64_t* lit_offsets, const int32_t* lits, const tsc_sat_opts* opts,
                   ts_program** out);
+
+/* Packet form of a compiled program for one-wave workgroups (no counterpart in the
+ * reference: an execution layout of this build).  In place, per job: the steps are
+ * regrouped into packets of mutually independent steps sharing TS_PACKET_LANES lanes
+ * (step k owns lanes pad[0] .. pad[0] + lanes_k - 1, lanes_k = 2^out_rank, or
+ * 2^min(sum_rank, 6) if larger for a step that is neither K-innermost nor sum-free),
+ * list-scheduled on true dependencies, packets in order and never straddling a
+ * TS_PACKET_CHUNK-record boundary; intermediates are reallocated above the leaves by
+ * first fit over their lifetimes in packets (result_off, arena_elems updated), so no
+ * step of a packet touches memory another step of it writes.  Every record gets
+ * TS_STEP_PACKET, the last of each packet TS_STEP_PEND, and the first of each packet
+ * the 64-bit mask of first lanes in o_lpos[16..23] (little endian).  Requires every
+ * step to have at most 6 out bits and no packet flags yet (TSP_EINVAL otherwise, jobs
+ * already done stay packetized).  threads <= 0: hardware concurrency. */
+int tsc_packetize(ts_job* jobs, int64_t num_jobs, ts_step* steps, int64_t num_steps, int32_t threads);
 
 int tsc_program_sizes(const ts_program* p, int64_t* num_jobs, int64_t* num_steps,
                       int64_t* num_leaf_bytes, int64_t* num_instances);

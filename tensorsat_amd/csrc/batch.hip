@@ -180,6 +180,69 @@ __device__ inline void step_small(const ts_step& st, uint8_t* arena, int mode, i
     arena[out_off + (uint32_t)lane] = (uint8_t)(mode == TS_MODE_MOD256 ? ((acc & 255u) != 0) : (acc != 0));
 }
 
+// One packet of a packet-form job (tsc_packetize) on one wave: lane l works for the
+// packet's record that owns it (the one whose first lane is the highest <= l, found by
+// counting the set bits of the packet's lane mask below l) on out index le = l - first
+// lane.  Everything is per lane: the record's header and positions, the K-innermost
+// popcount of 2^sum_rank bytes, and the summed-term loop of general steps, whose term
+// offsets lane first+t holds for term t; the loops that read across lanes run with the
+// whole wave (trip counts from ballots), the lanes that have no such term masked.
+// Lanes past the packet's last step alias its index range (bits >= out_rank are not
+// scattered) and store nothing.
+__device__ inline void step_packet(const ts_step* recs, int my, uint8_t* arena, int mode, int lane) {
+  const uint32_t* w = reinterpret_cast<const uint32_t*>(recs + my);
+  const uint32_t lhs_off = w[0], rhs_off = w[1], out_off = w[2], ranks = w[3], fl4 = w[4];
+  const int ro = (int)((ranks >> 16) & 0xffu), nc = (int)(ranks >> 24);
+  const bool norhs = fl4 & TS_STEP_NORHS, kinner = fl4 & TS_STEP_KINNER;
+  const int first = (int)((fl4 >> 8) & 0xffu);  // pad[0]
+  const uint32_t le = (uint32_t)(lane - first);
+  uint32_t lb[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) lb[i] = (le >> i) & 1u;
+  const uint32_t offl = scatter6(lb, ro, rec_bytes8(w, (int)offsetof(ts_step, o_lpos)));
+  const uint32_t offr = scatter6(lb, ro, rec_bytes8(w, (int)offsetof(ts_step, o_rpos)));
+  const uint8_t* lhs = arena + lhs_off;
+  const uint8_t* rhs = arena + (norhs ? lhs_off : rhs_off);
+  const bool general = !kinner && nc > 0;
+  uint32_t acc = 0;
+  if (!general) {
+    if (nc == 0) {
+      acc = lhs[offl];
+      if (!norhs) acc &= rhs[offr];
+    } else {
+      acc = popc_and(lhs + offl, rhs + offr, nc, norhs);
+    }
+  }
+  if (__ballot(general) != 0) {
+    const int nlo = general ? (nc < 6 ? nc : 6) : 0, nhi = general ? nc - nlo : 0;
+    const int tl = (int)scatter6(lb, nlo, rec_bytes8(w, (int)offsetof(ts_step, c_lpos)));
+    const int tr = (int)scatter6(lb, nlo, rec_bytes8(w, (int)offsetof(ts_step, c_rpos)));
+    const int nterm = 1 << nlo, nblk = general ? 1 << nhi : 0;
+    const uint8_t* cpl = reinterpret_cast<const uint8_t*>(w) + offsetof(ts_step, c_lpos);
+    const uint8_t* cpr = reinterpret_cast<const uint8_t*>(w) + offsetof(ts_step, c_rpos);
+    uint32_t g = 0;
+    for (int chi = 0; __ballot(chi < nblk) != 0; ++chi) {
+      uint32_t hl = 0, hr = 0;
+      for (int i = 0; i < nhi; ++i) {
+        if ((chi >> i) & 1) {
+          const uint32_t pl = cpl[6 + i], pr = cpr[6 + i];
+          hl |= pl != TS_NOPOS ? 1u << pl : 0u;
+          hr |= pr != TS_NOPOS ? 1u << pr : 0u;
+        }
+      }
+      const uint32_t bl = offl | hl, br = offr | hr;
+      for (int c = 0; __ballot(chi < nblk && c < nterm) != 0; ++c) {
+        const uint32_t tlc = (uint32_t)__shfl(tl, (first + c) & 63);
+        const uint32_t trc = (uint32_t)__shfl(tr, (first + c) & 63);
+        if (chi < nblk && c < nterm) g += (uint32_t)lhs[bl | tlc] & (norhs ? 1u : (uint32_t)rhs[br | trc]);
+      }
+    }
+    if (general) acc = g;
+  }
+  if (le < (1u << ro))
+    arena[out_off + le] = (uint8_t)(mode == TS_MODE_MOD256 ? ((acc & 255u) != 0) : (acc != 0));
+}
+
 // LDSA: the job arena lives in LDS (dynamic shared memory of arena_stride bytes; every
 // tensor then has at most 16 bits, so two table chunks suffice) instead of device memory.
 template <bool LDSA>
@@ -268,6 +331,7 @@ __global__ __launch_bounds__(WG) void k_batch(ts_batch b) {
 // wavefront, whose LDS accesses execute in order, so steps are separated by
 // wavefront-scope fences (ordering for the compiler, no wait).  Step records are staged
 // NR at a time, the next chunk loaded into registers while the current one runs.
+template <bool PK>
 __global__ __launch_bounds__(64) void k_batch_wave(ts_batch b) {
   extern __shared__ __attribute__((aligned(16))) uint8_t arena[];
   constexpr int NR = 16;
@@ -316,9 +380,25 @@ __global__ __launch_bounds__(64) void k_batch_wave(ts_batch b) {
       n = max(0, min(NR, job.num_steps - s0 - NR)) * 8;
       if (lane < n) r0 = rsrc[(s0 + NR) * 8 + lane];
       if (lane + 64 < n) r1 = rsrc[(s0 + NR) * 8 + lane + 64];
-      for (int s = 0; s < ns; ++s) {
-        step_small(s_steps[s], arena, b.mode, lane);
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      if constexpr (PK) {
+        // Packets never straddle a chunk; the first record of each holds its lane mask
+        // (o_lpos[16..23], bytes 36..43).  Counts are clamped to the chunk so a malformed
+        // mask cannot stall the loop or index past the staged records.
+        for (int s = 0; s < ns;) {
+          const uint32_t* w0 = reinterpret_cast<const uint32_t*>(&s_steps[s]);
+          const uint64_t mask = (uint64_t)w0[9] | ((uint64_t)w0[10] << 32);
+          const int k = min(max(__popcll(mask), 1), ns - s);
+          const uint64_t m1 = mask >> 1;
+          const int below = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m1 >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m1, 0u));
+          step_packet(s_steps, s + min(below, k - 1), arena, b.mode, lane);
+          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+          s += k;
+        }
+      } else {
+        for (int s = 0; s < ns; ++s) {
+          step_small(s_steps[s], arena, b.mode, lane);
+          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        }
       }
     }
     const int64_t nr = int64_t(1) << job.result_rank;
@@ -439,11 +519,15 @@ int ts_batch_max_workgroups(int32_t* out) {
 }
 
 int ts_batch_max_workgroups_lds(int64_t arena_bytes, int32_t one_wave, int32_t* out) {
-  if (!out || arena_bytes <= 0 || arena_bytes > TS_BATCH_LDS_MAX || (arena_bytes & 255)) return TS_EINVAL;
+  if (!out || arena_bytes <= 0 || arena_bytes > TS_BATCH_LDS_MAX || (arena_bytes & 255) || one_wave < 0 ||
+      one_wave > 2)
+    return TS_EINVAL;
   int dev = 0, ncu = 0, per = 0;
   if (hipGetDevice(&dev) != hipSuccess) return TS_EHIP;
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return TS_EHIP;
-  const void* fn = one_wave ? reinterpret_cast<const void*>(k_batch_wave) : reinterpret_cast<const void*>(k_batch<true>);
+  const void* fn = one_wave == 2   ? reinterpret_cast<const void*>(k_batch_wave<true>)
+                   : one_wave == 1 ? reinterpret_cast<const void*>(k_batch_wave<false>)
+                                   : reinterpret_cast<const void*>(k_batch<true>);
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, one_wave ? 64 : WG, (size_t)arena_bytes) != hipSuccess)
     return TS_EHIP;
   *out = ncu * (per > 0 ? per : 1);
@@ -456,13 +540,17 @@ int ts_batch_run(const ts_batch* b, ts_stream stream) {
   if (b->num_jobs > 0 && !b->lds_arena && !b->arenas) return TS_EINVAL;
   if (b->arena_stride <= 0 || (b->arena_stride & 255)) return TS_EINVAL;
   if (b->lds_arena && b->arena_stride > TS_BATCH_LDS_MAX) return TS_EINVAL;
+  if (b->lds_arena < 0 || b->lds_arena > 3) return TS_EINVAL;
   // Segment x of the queue past the first 2 * num_workgroups positions is served by
   // workgroups w with w % QSEG == x: every non-empty segment needs one.
   if (b->num_workgroups < QSEG && b->num_jobs > 2 * (int64_t)b->num_workgroups) return TS_EINVAL;
   if (b->mode != TS_MODE_GT0 && b->mode != TS_MODE_MOD256) return TS_EINVAL;
   if (b->num_jobs == 0) return TS_OK;
-  if (b->lds_arena == 2)
-    hipLaunchKernelGGL(k_batch_wave, dim3(b->num_workgroups), dim3(64), (size_t)b->arena_stride,
+  if (b->lds_arena == 3)
+    hipLaunchKernelGGL(k_batch_wave<true>, dim3(b->num_workgroups), dim3(64), (size_t)b->arena_stride,
+                       reinterpret_cast<hipStream_t>(stream), *b);
+  else if (b->lds_arena == 2)
+    hipLaunchKernelGGL(k_batch_wave<false>, dim3(b->num_workgroups), dim3(64), (size_t)b->arena_stride,
                        reinterpret_cast<hipStream_t>(stream), *b);
   else if (b->lds_arena)
     hipLaunchKernelGGL((k_batch<true>), dim3(b->num_workgroups), dim3(WG), (size_t)b->arena_stride,

@@ -17,6 +17,7 @@ import ctypes
 import numpy as np
 
 from . import _native as N
+from . import program as PG
 from .program import JOB_DTYPE, Program, validate
 
 
@@ -169,28 +170,40 @@ class BatchExecutor:
 
     def __init__(self, prog: Program, *, num_workgroups: int | None = None, device: str = "cuda",
                  order: np.ndarray | None = None, track_jobs: bool = False,
-                 round_bounds=ROUND_BOUNDS, lds: bool | None = None) -> None:
+                 round_bounds=ROUND_BOUNDS, lds: bool | None = None, packets: bool | None = None) -> None:
         """``lds``: keep each job's arena in LDS (the LDS variant of the batched kernel);
-        default: whenever the largest arena fits (``BATCH_LDS_MAX`` bytes)."""
+        default: whenever the largest arena fits (``BATCH_LDS_MAX`` bytes).  ``packets``:
+        run one-wave programs in packet form (``program.packetize``: independent steps of
+        a job share the wave's lanes); default: whenever the program runs on one-wave
+        workgroups."""
         torch = _torch()
         L = N.lib()
-        self.dp = DeviceProgram(prog, device)
         self.prog = prog
         nj = len(prog.jobs)
-        max_arena = int(prog.jobs["arena_elems"].max()) if nj else 16
-        self.arena_stride = (max_arena + 255) // 256 * 256
-        fits = self.arena_stride <= N.BATCH_LDS_MAX
+
+        def stride(p: Program) -> int:
+            return (int(p.jobs["arena_elems"].max()) if nj else 16) + 255 & ~255
+
+        fits = stride(prog) <= N.BATCH_LDS_MAX
         self.lds = fits if lds is None else bool(lds)
         if self.lds and not fits:
-            raise ValueError(f"job arenas of {self.arena_stride} bytes exceed the LDS limit")
+            raise ValueError(f"job arenas of {stride(prog)} bytes exceed the LDS limit")
         # Every tensor of at most 6 bits: one-wave workgroups (every step <= 64 entries).
         st = prog.steps
         small = bool(nj) and self.lds and (len(st) == 0 or int(st["out_rank"].max()) <= 6)
         self.one_wave = small
+        self.packets = small if packets is None else bool(packets) and small
+        run = PG.packetize(prog) if self.packets and len(st) else prog
+        if self.packets and stride(run) > N.BATCH_LDS_MAX:
+            self.packets, run = False, prog  # packet lifetimes need more arena than LDS holds
+        self.run_prog = run
+        self.dp = DeviceProgram(run, device)
+        self.arena_stride = stride(run)
         if num_workgroups is None:
             mx = N.i32(0)
             if self.lds:
-                N.check(L.ts_batch_max_workgroups_lds(self.arena_stride, int(self.one_wave), ctypes.byref(mx)),
+                variant = 2 if self.packets else 1 if self.one_wave else 0
+                N.check(L.ts_batch_max_workgroups_lds(self.arena_stride, variant, ctypes.byref(mx)),
                         "ts_batch_max_workgroups_lds")
             else:
                 N.check(L.ts_batch_max_workgroups(ctypes.byref(mx)), "ts_batch_max_workgroups")
@@ -205,7 +218,7 @@ class BatchExecutor:
             full = np.asarray(order, dtype=np.int32)
         else:
             # one round: the static rounds are paired (dispatch_order with G)
-            full = dispatch_order(prog, self.num_workgroups if one_round else None)
+            full = dispatch_order(run, self.num_workgroups if one_round else None)
         cubes = prog.jobs["cube"][full] if nj else np.zeros(0, np.int32)
         self.rounds: list[tuple[int, int]] = []
         if one_round:
@@ -259,7 +272,7 @@ class BatchExecutor:
                 arena_stride=self.arena_stride, num_workgroups=self.num_workgroups, mode=N.MODES[mode],
                 verdicts=self.verdicts.data_ptr(), counter=cbase + 4 * (self.cs * r), early_exit=int(early_exit),
                 job_done=self.job_done.data_ptr() if self.job_done is not None else None,
-                num_jobs_dev=num_dev, lds_arena=(2 if self.one_wave else 1) if self.lds else 0,
+                num_jobs_dev=num_dev, lds_arena=(3 if self.packets else 2 if self.one_wave else 1) if self.lds else 0,
             )
             N.check(L.ts_batch_run(ctypes.byref(bt), stream), "ts_batch_run")
         if events is not None:

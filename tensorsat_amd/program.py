@@ -23,6 +23,10 @@ MAX_BITS = 27
 NOPOS = 0xFF
 STEP_KINNER = 1
 STEP_NORHS = 2
+STEP_PACKET = 4  # packet form (packetize): pad[0] = first lane of the step
+STEP_PEND = 8  # packet form: last record of its packet
+PACKET_LANES = 64
+PACKET_CHUNK = 16
 NSTAT = 10
 
 STEP_DTYPE = np.dtype(
@@ -81,8 +85,9 @@ def _lib():
         L.tsp_order_cost.argtypes = [P(_plan._Net), P(ctypes.c_int32), P(ctypes.c_int32), P(_plan._Cost)]
         L.tsp_elim_order_prior.argtypes = [P(_plan._Net), P(ctypes.c_float), ctypes.c_float, P(ctypes.c_int32),
                                            P(_plan._Cost)]
+        L.tsc_packetize.argtypes = [vp, ctypes.c_int64, vp, ctypes.c_int64, ctypes.c_int32]
         for f in (L.tsc_compile, L.tsc_sat_batch, L.tsc_program_sizes, L.tsc_program_copy, L.tsp_elim,
-                  L.tsp_elim_order, L.tsp_order_cost, L.tsp_elim_order_prior):
+                  L.tsp_elim_order, L.tsp_order_cost, L.tsp_elim_order_prior, L.tsc_packetize):
             f.restype = ctypes.c_int
         _bound = True
     return L
@@ -280,6 +285,32 @@ def concat(progs: list[Program], instances: list[np.ndarray], num_instances: int
     return out
 
 
+def packetize(prog: Program, threads: int = 0) -> Program:
+    """
+    The packet form of ``prog`` for one-wave workgroups (``tsc_packetize``): each job's
+    steps regrouped into packets of mutually independent steps sharing a wave's 64 lanes,
+    intermediates reallocated over packet lifetimes.  Every step must have at most 6 out
+    bits.  Values are unchanged (run in record order, the steps compute what the original
+    program computes); returns a new Program.
+    """
+    jobs = prog.jobs.copy()
+    steps = prog.steps.copy()
+    if len(steps) and int(steps["out_rank"].max()) > 6:
+        raise ValueError("packet form needs steps of at most 6 out bits")
+    _plan._check(_lib().tsc_packetize(jobs.ctypes.data, len(jobs), steps.ctypes.data, len(steps), int(threads)),
+                 "tsc_packetize")
+    out = Program(jobs, steps, prog.leaves, prog.stats, dict(prog.meta))
+    out.meta["packetized"] = True
+    return out
+
+
+def packet_spans(steps: np.ndarray) -> list[tuple[int, int]]:
+    """[begin, end) record ranges of the packets of one packetized job's steps."""
+    ends = np.flatnonzero(steps["flags"] & STEP_PEND) + 1
+    begins = np.concatenate([[0], ends[:-1]]) if len(ends) else np.zeros(0, np.int64)
+    return [(int(b), int(e)) for b, e in zip(begins, ends)]
+
+
 def step_labels(st) -> tuple[list[int], list[int], list[int]]:
     """
     Einsum labels of one step record: out bit j is label j, summed bit j is label
@@ -346,10 +377,15 @@ def validate(prog: Program) -> None:
         raise ValueError("rhs outside the job arena")
     if np.any(st["out_off"].astype(np.int64) + (one << st["out_rank"].astype(np.int64)) > arena):
         raise ValueError("out outside the job arena")
+    packet = (st["flags"] & STEP_PACKET) != 0
+    if np.any(packet & (st["out_rank"] > 6)):
+        raise ValueError("packet-form steps have at most 6 out bits")
     for pos, rank in (("o_lpos", "lhs_rank"), ("c_lpos", "lhs_rank"), ("o_rpos", "rhs_rank"), ("c_rpos", "rhs_rank")):
         p = st[pos].astype(np.int64)
         lim = st[rank].astype(np.int64)[:, None]
         valid = (p == NOPOS) | (p < lim)
+        if pos == "o_lpos":  # packet form: bytes 16..23 carry the packet's lane mask
+            valid[:, 16:24] |= packet[:, None]
         if not np.all(valid):
             raise ValueError(f"{pos} outside its operand")
     if np.any(st["lhs_off"] % 16) or np.any(~norhs & (st["rhs_off"] % 16 != 0)) or np.any(st["out_off"] % 16):

@@ -36,9 +36,10 @@ def test_lds_kernel_matches_global_kernel_and_oracle(width, mode):
     assert (lds.job_done.cpu().numpy()[: len(prog.jobs)] == 1).all()
 
 
+@pytest.mark.parametrize("packets", [False, True])
 @pytest.mark.parametrize("out_wires", [(0, 1), (), (4,), (2, 7, 9)])
 @pytest.mark.parametrize("mode", ["GT0", "MOD256"])
-def test_one_wave_kernel_general_steps_with_many_summed_bits(out_wires, mode):
+def test_one_wave_kernel_general_steps_with_many_summed_bits(out_wires, mode, packets):
     """A general step (summed bits not innermost) summing 7 to 10 bits: the one-wave
     kernel's block loop over summed bits past the sixth, against the oracle."""
     from tensorsat_amd.executor import BatchExecutor
@@ -49,11 +50,33 @@ def test_one_wave_kernel_general_steps_with_many_summed_bits(out_wires, mode):
         prog, _ = wide_sum_program(seed, out_wires, density=0.7 if mode == "MOD256" else None)
         want = np.uint8(np.asarray(OX.run_job(prog, 0, mode)[0]).any())
         seen.add(int(want))
-        ex = BatchExecutor(prog, lds=True)
-        assert ex.one_wave
+        ex = BatchExecutor(prog, lds=True, packets=packets)
+        assert ex.one_wave and ex.packets == packets
         glb = BatchExecutor(prog, lds=False)
-        print(f"{mode} out {out_wires} seed {seed}", flush=True)
+        print(f"{mode} out {out_wires} seed {seed} packets {packets}", flush=True)
         assert ex.run(mode, early_exit=False)[0] == want
         assert glb.run(mode, early_exit=False)[0] == want
     if mode == "GT0":
         assert seen == {0, 1}
+
+
+@pytest.mark.parametrize("width", [4, 5, 6])
+@pytest.mark.parametrize("mode", ["GT0", "MOD256"])
+def test_packet_kernel_matches_step_kernel_and_oracle(width, mode):
+    """One-wave kernel in packet form (independent steps of a job sharing the wave's
+    lanes) against the same kernel step by step and the oracle, with and without early
+    exit; without it every job runs once."""
+    from tensorsat_amd.executor import BatchExecutor
+
+    cnfs = batched_3sat(512, start=2000)
+    prog = PG.sat_batch(cnfs, width=width, max_cubes=4)
+    ref = np.asarray(OX.decide(prog, mode), dtype=np.uint8)
+    pk = BatchExecutor(prog, track_jobs=True)
+    st = BatchExecutor(prog, packets=False)
+    assert pk.packets and pk.one_wave and not st.packets
+    print(f"width {width} {mode}: {len(prog.jobs)} jobs, {len(prog.steps)} steps", flush=True)
+    for early in (False, True):
+        np.testing.assert_array_equal(pk.run(mode, early_exit=early), ref)
+        np.testing.assert_array_equal(st.run(mode, early_exit=early), ref)
+    pk.run(mode, early_exit=False)
+    assert (pk.job_done.cpu().numpy()[: len(prog.jobs)] == 1).all()
