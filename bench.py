@@ -48,7 +48,9 @@ def batch_kernel_name(ex) -> str:
     if not ex.lds:
         return "k_batch"
     if ex.one_wave:
-        return "k_batch_wave<true> (packet form)" if ex.packets else "k_batch_wave<false>"
+        if ex.lane_form:
+            return "k_batch_wave<2> (lane form)"
+        return "k_batch_wave<1> (packet form)" if ex.packets else "k_batch_wave<0>"
     return "k_batch<true>"
 
 

@@ -147,7 +147,9 @@ typedef struct {
                                `arenas` unused); 2: as 1 with one-wave workgroups, for programs
                                whose tensors all have at most 6 bits (every step <= 64 out
                                entries); 3: as 2 for packet-form programs (tsc_packetize):
-                               one packet of independent steps per wave round.
+                               one packet of independent steps per wave round; 4: as 3 with
+                               `steps` holding the program's lane records (tsc_lanes; the
+                               jobs' step_begin / num_steps count lane records).
                                num_workgroups from ts_batch_max_workgroups_lds. */
 } ts_batch;
 
@@ -179,7 +181,7 @@ int ts_batch_max_workgroups(int32_t* out);
 
 /* Maximum resident workgroups of the LDS-arena batched kernel for arenas of arena_bytes
  * (a multiple of 256, at most TS_BATCH_LDS_MAX); one_wave = 1 selects the lds_arena = 2
- * variant, 2 the lds_arena = 3 (packet form) one. */
+ * variant, 2 the lds_arena = 3 (packet form) one, 3 the lds_arena = 4 (lane form) one. */
 int ts_batch_max_workgroups_lds(int64_t arena_bytes, int32_t one_wave, int32_t* out);
 
 /* ---- Learned planner: message-passing network kernels (bf16 in, fp32 accumulate) --

@@ -150,6 +150,35 @@ int tsc_sat_batch(int32_t num_inst, const int32_t* nvars, const int64_t* clause_
  * already done stay packetized).  threads <= 0: hardware concurrency. */
 int tsc_packetize(ts_job* jobs, int64_t num_jobs, ts_step* steps, int64_t num_steps, int32_t threads);
 
+/* Lane form of a packet-form program (no counterpart in the reference: an execution
+ * layout of this build): one 16-byte record per lane of every packet, all index math
+ * done here, so a wave only loads, counts and stores.  Lane l of a packet runs out index
+ * le = l - pad[0] of the step that owns it; its record holds absolute arena offsets. */
+typedef struct {
+  uint16_t lhs;    /* lhs byte of term 0 for this lane's out index */
+  uint16_t rhs;    /* rhs byte of term 0 (lhs again for unary steps) */
+  uint16_t out;    /* output byte, TS_LANE_NONE: this lane stores nothing */
+  uint16_t tl, tr; /* general steps: offsets of summed term le within lhs / rhs (table entry) */
+  uint8_t op;      /* bits 0-2: summed bits (0..6); TS_LANE_* flags */
+  uint8_t first;   /* general steps: lane holding term 0 of the step's table */
+  uint8_t count;   /* first record of a packet: its lanes (with TS_LANE_SKIP: records to skip) */
+  uint8_t pad[3];
+} ts_lane;
+
+#define TS_LANE_NONE 0xFFFF
+#define TS_LANE_GENERAL 8  /* summed terms through the table (not K-innermost) */
+#define TS_LANE_NORHS 16   /* unary step: rhs is the constant 1 */
+#define TS_LANE_SKIP 32    /* filler up to the next TS_LANE_CHUNK boundary */
+#define TS_LANE_CHUNK 128  /* packets never straddle a boundary of this many lane records */
+
+/* Lane form of packetized jobs (tsc_packetize output; every step at most 6 out and 6
+ * summed bits, arenas below 64 KiB).  lane_begin[j] .. lane_begin[j+1] are job j's lane
+ * records (lane_begin has num_jobs + 1 entries, lane_begin[0] = 0).  With lanes == NULL
+ * only lane_begin is computed; otherwise lanes (cap records) is filled.  TSP_EINVAL for
+ * programs outside these limits. */
+int tsc_lanes(const ts_job* jobs, int64_t num_jobs, const ts_step* steps, int64_t num_steps, ts_lane* lanes,
+              int64_t cap, int64_t* lane_begin, int32_t threads);
+
 int tsc_program_sizes(const ts_program* p, int64_t* num_jobs, int64_t* num_steps,
                       int64_t* num_leaf_bytes, int64_t* num_instances);
 int tsc_program_copy(const ts_program* p, ts_job* jobs, ts_step* steps, uint8_t* leaf_bytes,

@@ -243,6 +243,38 @@ __device__ inline void step_packet(const ts_step* recs, int my, uint8_t* arena, 
     arena[out_off + le] = (uint8_t)(mode == TS_MODE_MOD256 ? ((acc & 255u) != 0) : (acc != 0));
 }
 
+// One packet of a lane-form job (tsc_lanes): lane l < count runs its own record, which
+// holds absolute arena offsets (all index math was done on the host).  Sum-free and
+// K-innermost lanes AND and count 2^nc contiguous bytes; general lanes walk the step's
+// summed terms, whose offsets the table lanes first .. first + 2^nc - 1 hold, with the
+// whole wave taking part in the cross-lane reads (trip counts from ballots).
+__device__ inline void lane_packet(uint4 r, bool active, uint8_t* arena, int mode, int lane) {
+  const uint32_t lhs = r.x & 0xffffu, rhs = r.x >> 16, out = r.y & 0xffffu;
+  const int tl = (int)(r.y >> 16), tr = (int)(r.z & 0xffffu);
+  const uint32_t op = (r.z >> 16) & 0xffu;
+  const int first = (int)(r.z >> 24), nc = (int)(op & 7u);
+  const bool general = active && (op & TS_LANE_GENERAL), norhs = op & TS_LANE_NORHS;
+  uint32_t acc = 0;
+  if (active && !general) {
+    if (nc == 0)
+      acc = (uint32_t)arena[lhs] & (norhs ? 1u : (uint32_t)arena[rhs]);
+    else
+      acc = popc_and(arena + lhs, arena + rhs, nc, norhs);
+  }
+  if (__ballot(general) != 0) {
+    const int nterm = general ? 1 << nc : 0;
+    uint32_t g = 0;
+    for (int c = 0; __ballot(c < nterm) != 0; ++c) {
+      const uint32_t tlc = (uint32_t)__shfl(tl, (first + c) & 63);
+      const uint32_t trc = (uint32_t)__shfl(tr, (first + c) & 63);
+      if (c < nterm) g += (uint32_t)arena[lhs + tlc] & (norhs ? 1u : (uint32_t)arena[rhs + trc]);
+    }
+    if (general) acc = g;
+  }
+  if (active && out != TS_LANE_NONE)
+    arena[out] = (uint8_t)(mode == TS_MODE_MOD256 ? ((acc & 255u) != 0) : (acc != 0));
+}
+
 // LDSA: the job arena lives in LDS (dynamic shared memory of arena_stride bytes; every
 // tensor then has at most 16 bits, so two table chunks suffice) instead of device memory.
 template <bool LDSA>
@@ -331,7 +363,7 @@ __global__ __launch_bounds__(WG) void k_batch(ts_batch b) {
 // wavefront, whose LDS accesses execute in order, so steps are separated by
 // wavefront-scope fences (ordering for the compiler, no wait).  Step records are staged
 // NR at a time, the next chunk loaded into registers while the current one runs.
-template <bool PK>
+template <int FORM>  // 0: step records, 1: packet form, 2: lane form
 __global__ __launch_bounds__(64) void k_batch_wave(ts_batch b) {
   extern __shared__ __attribute__((aligned(16))) uint8_t arena[];
   constexpr int NR = 16;
@@ -358,8 +390,12 @@ __global__ __launch_bounds__(64) void k_batch_wave(ts_batch b) {
     const ts_job job = b.jobs[j];
     // The first chunk of step records and the leaves are loaded together: one memory
     // round trip before the first step.
-    const uint4* rsrc = reinterpret_cast<const uint4*>(b.steps + job.step_begin);
-    int n = min(NR, job.num_steps) * 8;
+    // Records in uint4 units: 8 per step record, 1 per lane record (lane form: step_begin
+    // and num_steps count lane records); staged 128 units (2 KiB) at a time.
+    const uint4* rsrc = FORM == 2 ? reinterpret_cast<const uint4*>(b.steps) + job.step_begin
+                                  : reinterpret_cast<const uint4*>(b.steps + job.step_begin);
+    const int units = FORM == 2 ? job.num_steps : job.num_steps * 8;
+    int n = min(NR * 8, units);
     uint4 r0 = {0, 0, 0, 0}, r1 = {0, 0, 0, 0};
     if (lane < n) r0 = rsrc[lane];
     if (lane + 64 < n) r1 = rsrc[lane + 64];
@@ -371,16 +407,34 @@ __global__ __launch_bounds__(64) void k_batch_wave(ts_batch b) {
       for (int i = lane; i < n16; i += 64) reinterpret_cast<uint4*>(arena)[i] = reinterpret_cast<const uint4*>(src)[i];
       if (lane < nbytes - (n16 << 4)) arena[(n16 << 4) + lane] = src[(n16 << 4) + lane];  // < 16 tail bytes
     }
-    for (int s0 = 0; s0 < job.num_steps; s0 += NR) {
-      // r0/r1 hold records s0.. (n uint4): stage them, then fetch the next chunk.
+    for (int u0 = 0; u0 < units; u0 += NR * 8) {
+      // r0/r1 hold units u0.. (n uint4): stage them, then fetch the next chunk.
       if (lane < n) s16[lane] = r0;
       if (lane + 64 < n) s16[lane + 64] = r1;
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      const int ns = min(NR, job.num_steps - s0);
-      n = max(0, min(NR, job.num_steps - s0 - NR)) * 8;
-      if (lane < n) r0 = rsrc[(s0 + NR) * 8 + lane];
-      if (lane + 64 < n) r1 = rsrc[(s0 + NR) * 8 + lane + 64];
-      if constexpr (PK) {
+      const int nu = min(NR * 8, units - u0), ns = nu >> 3, s0 = u0 >> 3;
+      n = max(0, min(NR * 8, units - u0 - NR * 8));
+      if (lane < n) r0 = rsrc[u0 + NR * 8 + lane];
+      if (lane + 64 < n) r1 = rsrc[u0 + NR * 8 + lane + 64];
+      (void)s0;
+      if constexpr (FORM == 2) {
+        // Packets never straddle a chunk; the first record of each holds its lane count
+        // (or, on a filler, the records to skip).  Counts are clamped to the chunk.
+        for (int q = 0; q < nu;) {
+          const bool in = lane < nu - q;
+          const uint4 r = in ? s16[q + lane] : make_uint4(0, 0, 0, 0);
+          const uint32_t head = __builtin_amdgcn_readfirstlane(r.w), hop = __builtin_amdgcn_readfirstlane(r.z);
+          const int cnt = max((int)(head & 0xffu), 1);
+          if ((hop >> 16) & TS_LANE_SKIP) {
+            q += cnt;
+            continue;
+          }
+          const int k = min(min(cnt, 64), nu - q);
+          lane_packet(r, lane < k, arena, b.mode, lane);
+          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+          q += k;
+        }
+      } else if constexpr (FORM == 1) {
         // Packets never straddle a chunk; the first record of each holds its lane mask
         // (o_lpos[16..23], bytes 36..43).  Counts are clamped to the chunk so a malformed
         // mask cannot stall the loop or index past the staged records.
@@ -520,13 +574,14 @@ int ts_batch_max_workgroups(int32_t* out) {
 
 int ts_batch_max_workgroups_lds(int64_t arena_bytes, int32_t one_wave, int32_t* out) {
   if (!out || arena_bytes <= 0 || arena_bytes > TS_BATCH_LDS_MAX || (arena_bytes & 255) || one_wave < 0 ||
-      one_wave > 2)
+      one_wave > 3)
     return TS_EINVAL;
   int dev = 0, ncu = 0, per = 0;
   if (hipGetDevice(&dev) != hipSuccess) return TS_EHIP;
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return TS_EHIP;
-  const void* fn = one_wave == 2   ? reinterpret_cast<const void*>(k_batch_wave<true>)
-                   : one_wave == 1 ? reinterpret_cast<const void*>(k_batch_wave<false>)
+  const void* fn = one_wave == 3   ? reinterpret_cast<const void*>(k_batch_wave<2>)
+                   : one_wave == 2 ? reinterpret_cast<const void*>(k_batch_wave<1>)
+                   : one_wave == 1 ? reinterpret_cast<const void*>(k_batch_wave<0>)
                                    : reinterpret_cast<const void*>(k_batch<true>);
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, one_wave ? 64 : WG, (size_t)arena_bytes) != hipSuccess)
     return TS_EHIP;
@@ -540,17 +595,20 @@ int ts_batch_run(const ts_batch* b, ts_stream stream) {
   if (b->num_jobs > 0 && !b->lds_arena && !b->arenas) return TS_EINVAL;
   if (b->arena_stride <= 0 || (b->arena_stride & 255)) return TS_EINVAL;
   if (b->lds_arena && b->arena_stride > TS_BATCH_LDS_MAX) return TS_EINVAL;
-  if (b->lds_arena < 0 || b->lds_arena > 3) return TS_EINVAL;
+  if (b->lds_arena < 0 || b->lds_arena > 4) return TS_EINVAL;
   // Segment x of the queue past the first 2 * num_workgroups positions is served by
   // workgroups w with w % QSEG == x: every non-empty segment needs one.
   if (b->num_workgroups < QSEG && b->num_jobs > 2 * (int64_t)b->num_workgroups) return TS_EINVAL;
   if (b->mode != TS_MODE_GT0 && b->mode != TS_MODE_MOD256) return TS_EINVAL;
   if (b->num_jobs == 0) return TS_OK;
-  if (b->lds_arena == 3)
-    hipLaunchKernelGGL(k_batch_wave<true>, dim3(b->num_workgroups), dim3(64), (size_t)b->arena_stride,
+  if (b->lds_arena == 4)
+    hipLaunchKernelGGL(k_batch_wave<2>, dim3(b->num_workgroups), dim3(64), (size_t)b->arena_stride,
+                       reinterpret_cast<hipStream_t>(stream), *b);
+  else if (b->lds_arena == 3)
+    hipLaunchKernelGGL(k_batch_wave<1>, dim3(b->num_workgroups), dim3(64), (size_t)b->arena_stride,
                        reinterpret_cast<hipStream_t>(stream), *b);
   else if (b->lds_arena == 2)
-    hipLaunchKernelGGL(k_batch_wave<false>, dim3(b->num_workgroups), dim3(64), (size_t)b->arena_stride,
+    hipLaunchKernelGGL(k_batch_wave<0>, dim3(b->num_workgroups), dim3(64), (size_t)b->arena_stride,
                        reinterpret_cast<hipStream_t>(stream), *b);
   else if (b->lds_arena)
     hipLaunchKernelGGL((k_batch<true>), dim3(b->num_workgroups), dim3(WG), (size_t)b->arena_stride,

@@ -186,3 +186,137 @@ extern "C" int tsc_packetize(ts_job* jobs, int64_t num_jobs, ts_step* steps, int
     if (r != TSP_OK) return r;
   return TSP_OK;
 }
+
+// ---- lane form ------------------------------------------------------------------
+
+static_assert(sizeof(ts_lane) == 16, "one uint4 per lane record");
+
+namespace {
+
+uint32_t scatter(uint32_t v, int n, const uint8_t* pos) {
+  uint32_t off = 0;
+  for (int i = 0; i < n; ++i)
+    if (((v >> i) & 1u) && pos[i] != TS_NOPOS) off |= 1u << pos[i];
+  return off;
+}
+
+// Lane records of one packetized job (out == nullptr: count only).  Returns the count or
+// a negative TSP_ code.
+int64_t lanes_job(const ts_job& job, const ts_step* st, ts_lane* out) {
+  const int n = job.num_steps;
+  if (job.arena_elems > 0xFFFF) return TSP_EINVAL;
+  int64_t pos = 0;
+  for (int b = 0; b < n;) {
+    int e = b;
+    while (e < n && !(st[e].flags & TS_STEP_PEND)) ++e;
+    if (e == n) return TSP_EINVAL;  // unterminated packet
+    ++e;
+    // lanes of the packet: past the last step's first lane + its lane count
+    int nl = 0;
+    for (int k = b; k < e; ++k) {
+      const ts_step& s = st[k];
+      if (!(s.flags & TS_STEP_PACKET) || s.out_rank > 6 || s.sum_rank > 6) return TSP_EINVAL;
+      const bool general = !(s.flags & TS_STEP_KINNER) && s.sum_rank > 0;
+      const int lb = std::max<int>(s.out_rank, general ? s.sum_rank : 0);
+      nl = std::max(nl, (int)s.pad[0] + (1 << lb));
+    }
+    if (nl > TS_PACKET_LANES) return TSP_EINVAL;
+    if (pos % TS_LANE_CHUNK + nl > TS_LANE_CHUNK) {  // filler to the chunk boundary
+      const int skip = (int)(TS_LANE_CHUNK - pos % TS_LANE_CHUNK);
+      if (out) {
+        for (int i = 0; i < skip; ++i) {
+          ts_lane& r = out[pos + i];
+          std::memset(&r, 0, sizeof r);
+          r.out = TS_LANE_NONE;
+          r.op = TS_LANE_SKIP;
+        }
+        out[pos].count = (uint8_t)skip;
+      }
+      pos += skip;
+    }
+    if (out) {
+      for (int l = 0; l < nl; ++l) {
+        ts_lane& r = out[pos + l];
+        std::memset(&r, 0, sizeof r);
+        r.out = TS_LANE_NONE;
+      }
+      for (int k = b; k < e; ++k) {
+        const ts_step& s = st[k];
+        const bool norhs = s.flags & TS_STEP_NORHS;
+        const bool general = !(s.flags & TS_STEP_KINNER) && s.sum_rank > 0;
+        const int ro = s.out_rank, nc = s.sum_rank;
+        const int lb = std::max<int>(ro, general ? nc : 0);
+        const uint32_t rbase = norhs ? s.lhs_off : s.rhs_off;
+        for (int le = 0; le < (1 << lb); ++le) {
+          ts_lane& r = out[pos + s.pad[0] + le];
+          const uint32_t oi = (uint32_t)le & ((1u << ro) - 1u);  // aliases past 2^ro
+          r.lhs = (uint16_t)(s.lhs_off + scatter(oi, ro, s.o_lpos));
+          r.rhs = (uint16_t)(norhs ? r.lhs : rbase + scatter(oi, ro, s.o_rpos));
+          r.out = le < (1 << ro) ? (uint16_t)(s.out_off + (uint32_t)le) : (uint16_t)TS_LANE_NONE;
+          r.op = (uint8_t)(nc | (general ? TS_LANE_GENERAL : 0) | (norhs ? TS_LANE_NORHS : 0));
+          r.first = s.pad[0];
+          if (general && le < (1 << nc)) {
+            r.tl = (uint16_t)scatter((uint32_t)le, nc, s.c_lpos);
+            r.tr = (uint16_t)(norhs ? 0 : scatter((uint32_t)le, nc, s.c_rpos));
+          }
+        }
+      }
+      out[pos].count = (uint8_t)nl;
+    }
+    pos += nl;
+    b = e;
+  }
+  return pos;
+}
+
+}  // namespace
+
+extern "C" int tsc_lanes(const ts_job* jobs, int64_t num_jobs, const ts_step* steps, int64_t num_steps, ts_lane* lanes,
+                         int64_t cap, int64_t* lane_begin, int32_t threads) {
+  if (num_jobs < 0 || !lane_begin || (num_jobs > 0 && (!jobs || !steps))) return TSP_EINVAL;
+  for (int64_t j = 0; j < num_jobs; ++j)
+    if (jobs[j].step_begin < 0 || jobs[j].num_steps < 0 || (int64_t)jobs[j].step_begin + jobs[j].num_steps > num_steps)
+      return TSP_EINVAL;
+  int nt = threads > 0 ? threads : (int)std::max(1u, std::thread::hardware_concurrency());
+  nt = (int)std::max<int64_t>(1, std::min<int64_t>(nt, num_jobs / 64 + 1));
+  std::vector<int> rc(nt, TSP_OK);
+  if (!lanes) {
+    std::vector<int64_t> cnt(num_jobs);
+    auto work = [&](int t) {
+      for (int64_t j = t; j < num_jobs; j += nt) {
+        const int64_t c = lanes_job(jobs[j], steps + jobs[j].step_begin, nullptr);
+        if (c < 0) {
+          rc[t] = (int)c;
+          return;
+        }
+        cnt[j] = c;
+      }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; ++t) pool.emplace_back(work, t);
+    work(0);
+    for (auto& th : pool) th.join();
+    for (int r : rc)
+      if (r != TSP_OK) return r;
+    lane_begin[0] = 0;
+    for (int64_t j = 0; j < num_jobs; ++j) lane_begin[j + 1] = lane_begin[j] + cnt[j];
+    return TSP_OK;
+  }
+  if (lane_begin[num_jobs] > cap) return TSP_EINVAL;
+  auto work = [&](int t) {
+    for (int64_t j = t; j < num_jobs; j += nt) {
+      const int64_t c = lanes_job(jobs[j], steps + jobs[j].step_begin, lanes + lane_begin[j]);
+      if (c < 0 || c != lane_begin[j + 1] - lane_begin[j]) {
+        rc[t] = TSP_EINVAL;
+        return;
+      }
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nt; ++t) pool.emplace_back(work, t);
+  work(0);
+  for (auto& th : pool) th.join();
+  for (int r : rc)
+    if (r != TSP_OK) return r;
+  return TSP_OK;
+}

@@ -30,13 +30,19 @@ def _torch():
 class DeviceProgram:
     """Device copies of a program's step, job and leaf arrays."""
 
-    def __init__(self, prog: Program, device: str = "cuda") -> None:
+    def __init__(self, prog, device: str = "cuda") -> None:
+        """``prog``: a Program, or a LaneProgram (its lane records then take the place of
+        the step records)."""
         torch = _torch()
         N.lib()
-        validate(prog)
+        lane_form = isinstance(prog, PG.LaneProgram)
+        if lane_form:
+            PG.validate_lanes(prog)
+        else:
+            validate(prog)
         self.prog = prog
         self.device = device
-        self.steps_host = np.ascontiguousarray(prog.steps)
+        self.steps_host = np.ascontiguousarray(prog.lanes if lane_form else prog.steps)
         self.jobs_host = np.ascontiguousarray(prog.jobs)
         leaves = prog.leaves if prog.leaves.size else np.zeros(16, np.uint8)
         self.leaves = torch.from_numpy(np.ascontiguousarray(leaves)).to(device)
@@ -170,12 +176,15 @@ class BatchExecutor:
 
     def __init__(self, prog: Program, *, num_workgroups: int | None = None, device: str = "cuda",
                  order: np.ndarray | None = None, track_jobs: bool = False,
-                 round_bounds=ROUND_BOUNDS, lds: bool | None = None, packets: bool | None = None) -> None:
+                 round_bounds=ROUND_BOUNDS, lds: bool | None = None, packets: bool | None = None,
+                 lanes: bool | None = None) -> None:
         """``lds``: keep each job's arena in LDS (the LDS variant of the batched kernel);
         default: whenever the largest arena fits (``BATCH_LDS_MAX`` bytes).  ``packets``:
         run one-wave programs in packet form (``program.packetize``: independent steps of
         a job share the wave's lanes); default: whenever the program runs on one-wave
-        workgroups."""
+        workgroups.  ``lanes``: run the packet form as lane records (``program.lanes``:
+        all index math on the host); default: whenever the packet form runs and every step
+        sums at most 6 bits."""
         torch = _torch()
         L = N.lib()
         self.prog = prog
@@ -197,12 +206,14 @@ class BatchExecutor:
         if self.packets and stride(run) > N.BATCH_LDS_MAX:
             self.packets, run = False, prog  # packet lifetimes need more arena than LDS holds
         self.run_prog = run
-        self.dp = DeviceProgram(run, device)
+        can_lanes = self.packets and len(st) > 0 and int(st["sum_rank"].max()) <= 6 and stride(run) <= 0xFFFF
+        self.lane_form = can_lanes if lanes is None else bool(lanes) and can_lanes
+        self.dp = DeviceProgram(PG.lanes(run) if self.lane_form else run, device)
         self.arena_stride = stride(run)
         if num_workgroups is None:
             mx = N.i32(0)
             if self.lds:
-                variant = 2 if self.packets else 1 if self.one_wave else 0
+                variant = 3 if self.lane_form else 2 if self.packets else 1 if self.one_wave else 0
                 N.check(L.ts_batch_max_workgroups_lds(self.arena_stride, variant, ctypes.byref(mx)),
                         "ts_batch_max_workgroups_lds")
             else:
@@ -272,7 +283,8 @@ class BatchExecutor:
                 arena_stride=self.arena_stride, num_workgroups=self.num_workgroups, mode=N.MODES[mode],
                 verdicts=self.verdicts.data_ptr(), counter=cbase + 4 * (self.cs * r), early_exit=int(early_exit),
                 job_done=self.job_done.data_ptr() if self.job_done is not None else None,
-                num_jobs_dev=num_dev, lds_arena=(3 if self.packets else 2 if self.one_wave else 1) if self.lds else 0,
+                num_jobs_dev=num_dev,
+                lds_arena=(4 if self.lane_form else 3 if self.packets else 2 if self.one_wave else 1) if self.lds else 0,
             )
             N.check(L.ts_batch_run(ctypes.byref(bt), stream), "ts_batch_run")
         if events is not None:

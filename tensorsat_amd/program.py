@@ -86,8 +86,9 @@ def _lib():
         L.tsp_elim_order_prior.argtypes = [P(_plan._Net), P(ctypes.c_float), ctypes.c_float, P(ctypes.c_int32),
                                            P(_plan._Cost)]
         L.tsc_packetize.argtypes = [vp, ctypes.c_int64, vp, ctypes.c_int64, ctypes.c_int32]
+        L.tsc_lanes.argtypes = [vp, ctypes.c_int64, vp, ctypes.c_int64, vp, ctypes.c_int64, vp, ctypes.c_int32]
         for f in (L.tsc_compile, L.tsc_sat_batch, L.tsc_program_sizes, L.tsc_program_copy, L.tsp_elim,
-                  L.tsp_elim_order, L.tsp_order_cost, L.tsp_elim_order_prior, L.tsc_packetize):
+                  L.tsp_elim_order, L.tsp_order_cost, L.tsp_elim_order_prior, L.tsc_packetize, L.tsc_lanes):
             f.restype = ctypes.c_int
         _bound = True
     return L
@@ -302,6 +303,97 @@ def packetize(prog: Program, threads: int = 0) -> Program:
     out = Program(jobs, steps, prog.leaves, prog.stats, dict(prog.meta))
     out.meta["packetized"] = True
     return out
+
+
+LANE_DTYPE = np.dtype(
+    [("lhs", "<u2"), ("rhs", "<u2"), ("out", "<u2"), ("tl", "<u2"), ("tr", "<u2"), ("op", "u1"), ("first", "u1"),
+     ("count", "u1"), ("pad", "u1", (3,))]
+)
+assert LANE_DTYPE.itemsize == 16
+LANE_NONE = 0xFFFF
+LANE_GENERAL = 8
+LANE_NORHS = 16
+LANE_SKIP = 32
+LANE_CHUNK = 128
+
+
+@dataclass
+class LaneProgram:
+    """Lane form (``tsc_lanes``) of a packet-form program: ``jobs`` are the program's jobs
+    with step_begin / num_steps counting lane records, ``lanes`` the records."""
+
+    jobs: np.ndarray
+    lanes: np.ndarray
+    leaves: np.ndarray
+    stats: np.ndarray
+    source: Program
+
+    @property
+    def num_instances(self) -> int:
+        return int(self.stats.shape[0])
+
+    def job_lanes(self, j: int) -> np.ndarray:
+        b = int(self.jobs["step_begin"][j])
+        return self.lanes[b : b + int(self.jobs["num_steps"][j])]
+
+
+def lanes(pk: Program, threads: int = 0) -> LaneProgram:
+    """
+    Lane form of a packet-form program: one 16-byte record per lane of every packet with
+    absolute arena offsets (all index math on the host), packets padded so that none
+    straddles a ``LANE_CHUNK``-record boundary.  Needs steps of at most 6 out and 6
+    summed bits and arenas below 64 KiB.
+    """
+    L = _lib()
+    nj = len(pk.jobs)
+    begin = np.zeros(nj + 1, np.int64)
+    jobs = np.ascontiguousarray(pk.jobs)
+    steps = np.ascontiguousarray(pk.steps)
+    _plan._check(L.tsc_lanes(jobs.ctypes.data, nj, steps.ctypes.data, len(steps), None, 0, begin.ctypes.data,
+                             int(threads)), "tsc_lanes")
+    total = int(begin[-1])
+    if total >= 2**31:
+        raise ValueError("too many lane records for one program")
+    out = np.zeros(max(1, total), LANE_DTYPE)
+    _plan._check(L.tsc_lanes(jobs.ctypes.data, nj, steps.ctypes.data, len(steps), out.ctypes.data, total,
+                             begin.ctypes.data, int(threads)), "tsc_lanes")
+    lj = jobs.copy()
+    lj["step_begin"] = begin[:-1]
+    lj["num_steps"] = np.diff(begin)
+    return LaneProgram(lj, out[:total], pk.leaves, pk.stats, pk)
+
+
+def validate_lanes(lp: LaneProgram) -> None:
+    """Host-side bounds check of a lane-form program before any device launch."""
+    jobs = lp.jobs
+    if len(jobs) == 0:
+        return
+    if np.any(jobs["arena_elems"] > 0xFFFF):
+        raise ValueError("lane form needs arenas below 64 KiB")
+    if np.any(jobs["step_begin"].astype(np.int64) + jobs["num_steps"] > len(lp.lanes)) or np.any(jobs["step_begin"] < 0):
+        raise ValueError("lane range outside the lane array")
+    if np.any(jobs["leaf_begin"] + jobs["num_leaf_bytes"] > lp.leaves.size) or np.any(jobs["leaf_begin"] % 16):
+        raise ValueError("leaf range outside the leaf buffer or unaligned")
+    if np.any(jobs["num_leaf_bytes"].astype(np.int64) > jobs["arena_elems"]):
+        raise ValueError("leaves exceed the job arena")
+    if np.any(jobs["result_off"].astype(np.int64) + (1 << jobs["result_rank"].astype(np.int64)) > jobs["arena_elems"]):
+        raise ValueError("result outside the job arena")
+    if len(lp.lanes) == 0:
+        return
+    owner = np.repeat(np.arange(len(jobs)), jobs["num_steps"])
+    ln = lp.lanes[np.concatenate([np.arange(b, b + n) for b, n in zip(jobs["step_begin"], jobs["num_steps"])])]
+    arena = jobs["arena_elems"][owner].astype(np.int64)
+    nc = (ln["op"] & 7).astype(np.int64)
+    skip = (ln["op"] & LANE_SKIP) != 0
+    general = (ln["op"] & LANE_GENERAL) != 0
+    # K-innermost / sum-free lanes read 2^nc contiguous bytes; a general lane's terms add
+    # table offsets whose bits are disjoint from its own (in bounds by construction).
+    span = np.where(general, 1, 1 << nc)
+    ok = skip | ((ln["lhs"].astype(np.int64) + span <= arena) & (ln["rhs"].astype(np.int64) + span <= arena))
+    ok &= skip | ~general | ((ln["tl"].astype(np.int64) < arena) & (ln["tr"].astype(np.int64) < arena))
+    ok &= skip | (ln["out"] == LANE_NONE) | (ln["out"].astype(np.int64) < arena)
+    if not np.all(ok):
+        raise ValueError("lane record outside its job arena")
 
 
 def packet_spans(steps: np.ndarray) -> list[tuple[int, int]]:

@@ -98,3 +98,77 @@ def test_packetize_rejects_wide_outputs_and_repeats():
     pk = PG.packetize(PG.sat_batch(cnfs, width=6, max_cubes=2))
     with pytest.raises(ValueError):
         PG.packetize(pk)
+
+
+def _run_lanes(lp: PG.LaneProgram, j: int, mode: str) -> np.ndarray:
+    """CPU interpreter of one lane-form job, following the kernel: packets in order, each
+    lane from its own record, stores after the whole packet is read."""
+    job = lp.jobs[j]
+    arena = np.zeros(int(job["arena_elems"]), np.int64)
+    lv = lp.source.job_leaves(j)
+    arena[: lv.size] = lv
+    recs = lp.job_lanes(j)
+    q = 0
+    while q < len(recs):
+        head = recs[q]
+        cnt = max(int(head["count"]), 1)
+        if int(head["op"]) & PG.LANE_SKIP:
+            assert (q + cnt) % PG.LANE_CHUNK == 0
+            q += cnt
+            continue
+        assert q // PG.LANE_CHUNK == (q + cnt - 1) // PG.LANE_CHUNK and cnt <= 64
+        pk = recs[q : q + cnt]
+        stores = []
+        for r in pk:
+            op, nc = int(r["op"]), int(r["op"]) & 7
+            norhs = bool(op & PG.LANE_NORHS)
+            lhs, rhs = int(r["lhs"]), int(r["rhs"])
+            if op & PG.LANE_GENERAL:
+                first = int(r["first"])
+                acc = 0
+                for c in range(1 << nc):
+                    t = pk[first + c]
+                    a = arena[lhs + int(t["tl"])]
+                    acc += a & (1 if norhs else arena[rhs + int(t["tr"])])
+            else:
+                a = arena[lhs : lhs + (1 << nc)]
+                acc = int((a & (1 if norhs else arena[rhs : rhs + (1 << nc)])).sum())
+            if int(r["out"]) != PG.LANE_NONE:
+                stores.append((int(r["out"]), int((acc % 256) != 0) if mode == "MOD256" else int(acc != 0)))
+        for o, v in stores:
+            arena[o] = v
+        q += cnt
+    r0, rr = int(job["result_off"]), int(job["result_rank"])
+    return arena[r0 : r0 + (1 << rr)].astype(np.uint8)
+
+
+@pytest.mark.parametrize("width,mode", [(6, "GT0"), (6, "MOD256"), (4, "GT0")])
+def test_lane_form_matches_oracle(width, mode):
+    cnfs = batched_3sat(64, start=700)
+    prog = PG.sat_batch(cnfs, width=width, max_cubes=3)
+    lp = PG.lanes(PG.packetize(prog))
+    PG.validate_lanes(lp)
+    assert np.all(lp.jobs["result_off"] == lp.source.jobs["result_off"])
+    for j in range(len(prog.jobs)):
+        np.testing.assert_array_equal(_run_lanes(lp, j, mode), OX.run_job(prog, j, mode)[0])
+
+
+def test_lane_form_general_steps():
+    from wide_steps import wide_sum_program
+
+    for seed in range(4):
+        for ow in ((0, 1), (4, 5), (1, 3, 5), (0, 2, 4, 6)):
+            prog, _ = wide_sum_program(seed, ow, n=8)
+            lp = PG.lanes(PG.packetize(prog))
+            PG.validate_lanes(lp)
+            assert np.any(lp.lanes["op"] & PG.LANE_GENERAL)
+            for mode in ("GT0", "MOD256"):
+                np.testing.assert_array_equal(_run_lanes(lp, 0, mode), OX.run_job(prog, 0, mode)[0])
+
+
+def test_lane_form_rejects_wide_sums():
+    from wide_steps import wide_sum_program
+
+    prog, _ = wide_sum_program(0, (4,), n=8)  # 7 summed bits
+    with pytest.raises(ValueError):
+        PG.lanes(PG.packetize(prog))
