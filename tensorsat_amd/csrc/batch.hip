@@ -42,10 +42,10 @@ __device__ __forceinline__ bool decided(const uint32_t* verdicts, int instance) 
 }
 
 // The first two rounds of the queue are assigned statically: workgroup w takes position
-// w, then position 2G-1-w (G = num_workgroups), so the longest jobs at the front of the
-// queue pair with the shortest of the second round and no atomic is needed (every
-// workgroup starts at once, and thousands of atomics on one address at launch
-// serialise).  Positions from 2G on are split into QSEG interleaved segments (position
+// w, then position G + w (G = num_workgroups), so no atomic is needed (every workgroup
+// starts at once, and thousands of atomics on one address at launch serialise); the
+// host orders both rounds so that the workgroups dispatched first, which alone get a
+// second job, pair short first jobs with long second ones.  Positions from 2G on are split into QSEG interleaved segments (position
 // 2G + k belongs to segment k % QSEG, so each keeps the queue's order) with one counter
 // each, TS_BATCH_COUNTER_STRIDE int32s (one 256-byte line) apart; workgroup w takes them
 // from segment w % QSEG, i.e. from the segment of its own XCD.  A plain load of the
@@ -58,7 +58,7 @@ constexpr int QSTRIDE = TS_BATCH_COUNTER_STRIDE;
 __device__ __forceinline__ int take_job(const ts_batch& b, int num_jobs, int& taken) {
   const int w = (int)blockIdx.x, g = b.num_workgroups;
   while (taken < 2) {
-    const int64_t pos = taken == 0 ? (int64_t)w : 2 * (int64_t)g - 1 - w;
+    const int64_t pos = taken == 0 ? (int64_t)w : (int64_t)g + w;
     ++taken;
     if (pos < num_jobs) return b.order[pos];
   }

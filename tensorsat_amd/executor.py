@@ -143,8 +143,11 @@ def dispatch_order(prog: Program, num_workgroups: int | None = None) -> np.ndarr
     exit can skip later cubes of decided instances), then longest job first.
 
     With ``num_workgroups`` = G, the two statically assigned rounds (workgroup w takes
-    positions w and 2G-1-w) keep their members but are each sorted longest first, so the
-    longest second-round jobs go to the workgroups whose first job is shortest.
+    positions w and G + w) keep their members but are reordered: the m second-round jobs
+    go, longest first, to workgroups 0 .. m-1 (dispatched first), whose first jobs are the
+    m shortest of the first round, shortest first (long second jobs pair with short first
+    ones); the other first-round jobs follow longest first, so the workgroups dispatched
+    last get the shortest of them.
     """
     jobs = prog.jobs
     if len(jobs) == 0:
@@ -154,7 +157,10 @@ def dispatch_order(prog: Program, num_workgroups: int | None = None) -> np.ndarr
     if num_workgroups:
         g = int(num_workgroups)
         r1, r2 = order[:g], order[g : 2 * g]
-        order = np.concatenate([r1[np.argsort(-work[r1], kind="stable")],
+        m = len(r2)
+        r1 = r1[np.argsort(work[r1], kind="stable")]  # shortest first
+        rest = r1[m:]
+        order = np.concatenate([r1[:m], rest[np.argsort(-work[rest], kind="stable")],
                                 r2[np.argsort(-work[r2], kind="stable")], order[2 * g :]]).astype(np.int32)
     return order
 

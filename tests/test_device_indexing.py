@@ -164,7 +164,7 @@ def test_lds_arena_programs(mode):
 
 def _take_positions(num_jobs: int, g: int, seg: int, rng) -> list[int]:
     """Queue positions handed out by the batched kernels' take_job (positions w and
-    2g-1-w static, then interleaved segments of the rest, one counter each), with the
+    g + w static, then interleaved segments of the rest, one counter each), with the
     workgroups' takes interleaved in a random order."""
     counters = [0] * seg
     taken = [0] * g
@@ -174,7 +174,7 @@ def _take_positions(num_jobs: int, g: int, seg: int, rng) -> list[int]:
         w = live[rng.integers(len(live))]
         got = None
         while taken[w] < 2:
-            pos = w if taken[w] == 0 else 2 * g - 1 - w
+            pos = w if taken[w] == 0 else g + w
             taken[w] += 1
             if pos < num_jobs:
                 got = pos
