@@ -38,7 +38,7 @@ HBM_PEAK_GBS = 8000.0
 # by the copy calibration: FETCH x 2.0, WRITE x 1.0) for the configurations profiled;
 # other configurations report null.
 MEASURED_TRAFFIC = {
-    ("batched", 6, 4096, 4): (30.85e6, "profiles/k_batch_wave_batched3sat_w6.md"),
+    ("batched", 6, 4096, 4): (44.4e6, "profiles/k_batch_wave_batched3sat_w6.md"),
     ("dense", 13, 13, 13): (1.468e9, "profiles/k_step_mfma_dense_composition_13_13_13.md"),
 }
 
