@@ -126,7 +126,8 @@ int ts_program_count(const ts_job* job_host, const ts_step* steps_host, const ui
 
 typedef struct {
   const ts_job* jobs;       /* device, num_jobs records */
-  const int32_t* order;     /* device, dispatch order (num_jobs job indices) */
+  const int32_t* order;     /* device, dispatch order (num_jobs job indices); NULL: the jobs
+                               array is already in dispatch order (position = job index) */
   int32_t num_jobs;
   const ts_step* steps;     /* device */
   const uint8_t* leaves;    /* device */

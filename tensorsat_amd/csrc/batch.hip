@@ -60,7 +60,7 @@ __device__ __forceinline__ int take_job(const ts_batch& b, int num_jobs, int& ta
   while (taken < 2) {
     const int64_t pos = taken == 0 ? (int64_t)w : (int64_t)g + w;
     ++taken;
-    if (pos < num_jobs) return b.order[pos];
+    if (pos < num_jobs) return b.order ? b.order[pos] : (int)pos;
   }
   const int64_t rest = (int64_t)num_jobs - 2 * (int64_t)g;
   const int x = w % QSEG;
@@ -68,7 +68,8 @@ __device__ __forceinline__ int take_job(const ts_batch& b, int num_jobs, int& ta
   int32_t* c = b.counter + x * QSTRIDE;
   if (seg_len == 0 || __hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= seg_len) return -1;
   const int kl = atomicAdd(c, 1);
-  return kl < seg_len ? b.order[2 * (int64_t)g + x + (int64_t)QSEG * kl] : -1;
+  const int64_t pos = 2 * (int64_t)g + x + (int64_t)QSEG * kl;
+  return kl < seg_len ? (b.order ? b.order[pos] : (int)pos) : -1;
 }
 
 // Offset of the bits of v (LSB-first, nbits <= NB of them) placed at positions pos[i];
@@ -378,7 +379,8 @@ __global__ __launch_bounds__(64) void k_batch_wave(ts_batch b) {
     if (lane == 0) {
       for (;;) {
         j = take_job(b, num_jobs, taken);
-        if (j >= 0 && b.early_exit && decided(b.verdicts, b.jobs[j].instance)) {
+        // The first job starts with the dispatch, before any verdict can be set: no check.
+        if (j >= 0 && taken > 1 && b.early_exit && decided(b.verdicts, b.jobs[j].instance)) {
           if (b.job_done) b.job_done[j] = 2;
           continue;  // skipped: take the next entry
         }
@@ -591,7 +593,8 @@ int ts_batch_max_workgroups_lds(int64_t arena_bytes, int32_t one_wave, int32_t* 
 
 int ts_batch_run(const ts_batch* b, ts_stream stream) {
   if (!b || b->num_jobs < 0 || b->num_workgroups <= 0 || !b->counter || !b->verdicts) return TS_EINVAL;
-  if (b->num_jobs > 0 && (!b->jobs || !b->order || !b->steps || !b->leaves)) return TS_EINVAL;
+  if (b->num_jobs > 0 && (!b->jobs || !b->steps || !b->leaves)) return TS_EINVAL;
+  if (b->num_jobs_dev && !b->order) return TS_EINVAL;  // compacted lists are explicit orders
   if (b->num_jobs > 0 && !b->lds_arena && !b->arenas) return TS_EINVAL;
   if (b->arena_stride <= 0 || (b->arena_stride & 255)) return TS_EINVAL;
   if (b->lds_arena && b->arena_stride > TS_BATCH_LDS_MAX) return TS_EINVAL;
@@ -622,7 +625,7 @@ int ts_batch_run(const ts_batch* b, ts_stream stream) {
 int ts_batch_count(const ts_batch* b, uint64_t* counts, int32_t* overflow, ts_stream stream) {
   if (!b || !counts || !overflow || b->num_jobs < 0 || b->num_workgroups <= 0 || !b->counter) return TS_EINVAL;
   if (b->num_workgroups < QSEG && b->num_jobs > 2 * (int64_t)b->num_workgroups) return TS_EINVAL;
-  if (b->num_jobs > 0 && (!b->jobs || !b->order || !b->steps || !b->leaves || !b->arenas)) return TS_EINVAL;
+  if (b->num_jobs > 0 && (!b->jobs || !b->steps || !b->leaves || !b->arenas)) return TS_EINVAL;
   if (b->arena_stride <= 0 || (b->arena_stride & 255)) return TS_EINVAL;
   if (b->num_jobs == 0) return TS_OK;
   hipLaunchKernelGGL(k_batch_count, dim3(b->num_workgroups), dim3(WG), 0, reinterpret_cast<hipStream_t>(stream),

@@ -247,6 +247,14 @@ class BatchExecutor:
                 if b > a:
                     self.rounds.append((a, b))
         self.order = torch.from_numpy(np.ascontiguousarray(full, dtype=np.int32)).to(device)
+        # One round: the jobs go to the device already in dispatch order and the kernel
+        # takes position p as job p (no order lookup before a job starts); job flags are
+        # then per position (job_flags() maps them back).
+        self.perm = np.ascontiguousarray(full, dtype=np.int64) if one_round and nj else None
+        if self.perm is not None:
+            permuted = np.ascontiguousarray(self.dp.jobs_host[self.perm])
+            self.dp.jobs_host = permuted
+            self.dp.jobs = torch.from_numpy(permuted.view(np.uint8).copy()).to(device)
         longest = max((b - a for a, b in self.rounds), default=1)
         self.worklist = torch.empty(max(1, longest), dtype=torch.int32, device=device)
         # Everything a launch resets lives in one buffer, cleared by one fill kernel: per
@@ -274,7 +282,7 @@ class BatchExecutor:
         cbase = self.counters.data_ptr()
         for r, (a, b) in enumerate(self.rounds):
             n = b - a
-            order_ptr = self.order.data_ptr() + 4 * a
+            order_ptr = None if self.perm is not None else self.order.data_ptr() + 4 * a
             num_dev = None
             if r > 0 and early_exit:
                 N.check(L.ts_batch_compact(ctypes.c_void_p(order_ptr), n, N.dptr(self.dp.jobs),
@@ -295,6 +303,19 @@ class BatchExecutor:
             N.check(L.ts_batch_run(ctypes.byref(bt), stream), "ts_batch_run")
         if events is not None:
             events[1].record()
+
+    def job_flags(self) -> np.ndarray:
+        """Per job of the program (job order): 1 ran, 2 skipped, 0 not taken, from the last
+        launch (needs track_jobs)."""
+        if self.job_done is None:
+            raise ValueError("job flags need track_jobs=True")
+        nj = len(self.prog.jobs)
+        f = self.job_done.cpu().numpy()[:nj]
+        if self.perm is None:
+            return f
+        out = np.zeros(nj, dtype=f.dtype)
+        out[self.perm] = f
+        return out
 
     def run(self, mode: str = "GT0", early_exit: bool = True) -> np.ndarray:
         torch = _torch()

@@ -150,7 +150,7 @@ class BatchSolver:
 
     def executed_jobs(self) -> list[np.ndarray]:
         """Per pass: boolean mask of jobs that ran in the last launch (needs track_jobs)."""
-        return [ex.job_done.cpu().numpy()[: len(p.jobs)] == 1 for ex, p in zip(self.execs, self.progs)]
+        return [ex.job_flags() == 1 for ex in self.execs]
 
 
 def autotune_width(cnfs: list[CNFInstance], *, widths=(6, 8, 10, 12), sample: int = 512, first_pass: int = 4,
