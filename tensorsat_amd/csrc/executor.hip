@@ -259,6 +259,125 @@ __global__ __launch_bounds__(WG) void k_reduce_low(const uint8_t* __restrict__ i
   }
 }
 
+// Kernel (b), 16-byte lanes: existential reduction (GT0) of a tensor of >= 10 bits.  In
+// bits 0..3 are the 16 bytes of a lane's load, bits 4..9 the lanes, higher bits index
+// groups (kept) and terms (discarded).  Each load becomes a 16-bit presence mask (byte
+// != 0), the discarded in-register bits are folded into the kept positions by
+// shift-OR, and every kept in-register combination k collects the lanes holding a set
+// byte in a ballot mask; an out entry is then the OR of its lanes' bits (the lanes whose
+// kept lane bits spell it).  8 loads of 1 KiB per wave in flight.  A work item is a group
+// and one of 2^cb chunks of its terms, so that small groups counts still fill the GPU; the
+// output is cleared first and items store only the entries they find set (every writer
+// writes 1).
+struct ReduceWideArgs {
+  int32_t nkl;               // kept lane bits (in bits 4..9)
+  int32_t nkh, ndh;          // kept / discarded bits above bit 9
+  int32_t ndb;               // discarded in-register bits (in bits 0..3)
+  uint8_t kb_in[4], kb_out[4];  // kept in-register bits: in position (0..3), out position
+  uint8_t db_in[4];          // discarded in-register bits (0..3)
+  uint8_t kl_out[6], kl_lane[6];  // kept lane bits: out position, lane bit
+  uint8_t kh_in[TS_MAX_BITS], kh_out[TS_MAX_BITS];
+  uint8_t dh_in[TS_MAX_BITS];
+  int32_t cb;                // log2 of the term chunks per group
+  uint64_t lanes0;           // lanes whose kept lane bits are all 0
+  uint32_t fold;             // bit d set: in-register bit d is discarded
+};
+
+__global__ __launch_bounds__(WG) void k_zero16(uint4* __restrict__ p, uint64_t n16) {
+  for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < n16; i += (uint64_t)gridDim.x * WG)
+    p[i] = make_uint4(0, 0, 0, 0);
+}
+
+// Bit 7 of each byte set iff that byte of x is nonzero (the other bits clear).
+__device__ __forceinline__ uint32_t nonzero_bytes(uint32_t x) {
+  return (((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x) & 0x80808080u;
+}
+
+template <int NKB>
+__global__ __launch_bounds__(WG) void k_reduce_w16(const uint8_t* __restrict__ in, uint8_t* __restrict__ out,
+                                                   ReduceWideArgs a) {
+  constexpr int NK = 1 << NKB;
+  const int lane = threadIdx.x & 63;
+  const uint64_t nitems = 1ull << (a.nkh + a.cb);
+  const uint64_t waves = (uint64_t)gridDim.x * (WG / 64);
+  const uint32_t nterm = 1u << a.ndh;
+  const uint32_t ngk = (nterm + 7) >> 3;              // groups of 8 terms
+  const uint32_t per = ngk >> a.cb;                   // per chunk (cb <= log2 ngk)
+  // Lanes whose kept lane bits spell this lane's index: the lanes with those bits all 0,
+  // shifted by the index's kept bits (disjoint from the discarded ones: no carries).
+  uint32_t kin = 0, kout = 0;
+  for (int i = 0; i < a.nkl; ++i) {
+    kin |= ((uint32_t)(lane >> i) & 1u) << a.kl_lane[i];
+    kout |= ((uint32_t)(lane >> i) & 1u) << a.kl_out[i];
+  }
+  const uint64_t my_mask = a.lanes0 << kin;
+  uint32_t kpos[NK], kofs[NK];  // in-register position and out offset of kept combination k
+#pragma unroll
+  for (int k = 0; k < NK; ++k) {
+    kpos[k] = 0;
+    kofs[k] = 0;
+#pragma unroll
+    for (int i = 0; i < NKB; ++i) {
+      kpos[k] |= ((uint32_t)(k >> i) & 1u) << a.kb_in[i];
+      kofs[k] |= ((uint32_t)(k >> i) & 1u) << a.kb_out[i];
+    }
+  }
+  const int n3 = a.ndh < 3 ? a.ndh : 3;
+  uint32_t low3[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) low3[j] = spread<3>(j, 0, n3, a.dh_in);
+  for (uint64_t item = (uint64_t)blockIdx.x * (WG / 64) + (threadIdx.x >> 6); item < nitems; item += waves) {
+    const uint32_t gidx = (uint32_t)(item >> a.cb), chunk = (uint32_t)(item & ((1u << a.cb) - 1u));
+    const uint32_t ibase = spread<TS_MAX_BITS>(gidx, 0, a.nkh, a.kh_in) | ((uint32_t)lane << 4);
+    uint64_t acc[NK];
+#pragma unroll
+    for (int k = 0; k < NK; ++k) acc[k] = 0;
+    const uint32_t nj = nterm < 8 ? nterm : 8;
+    for (uint32_t kg = chunk * per; kg < (chunk + 1) * per; ++kg) {
+      const uint32_t hb = ibase | (a.ndh > 3 ? spread<TS_MAX_BITS - 3>(kg, 3, a.ndh - 3, a.dh_in) : 0u);
+      uint4 v[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if ((uint32_t)j < nj) v[j] = *reinterpret_cast<const uint4*>(in + (hb | low3[j]));
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        if ((uint32_t)j >= nj) break;
+        // Flags of the 16 bytes (byte i = in bits 0..3 = i), discarded in-register bits
+        // folded into the positions where they are 0.
+        uint32_t f[4] = {nonzero_bytes(v[j].x), nonzero_bytes(v[j].y), nonzero_bytes(v[j].z), nonzero_bytes(v[j].w)};
+        if (a.fold & 1u) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) f[q] |= f[q] >> 8;
+        }
+        if (a.fold & 2u) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) f[q] |= f[q] >> 16;
+        }
+        if (a.fold & 4u) {
+          f[0] |= f[1];
+          f[2] |= f[3];
+        }
+        if (a.fold & 8u) {
+          f[0] |= f[2];
+          f[1] |= f[3];
+        }
+#pragma unroll
+        for (int k = 0; k < NK; ++k) {
+          const uint32_t q = kpos[k] >> 2, sh = ((kpos[k] & 3u) << 3) + 7u;
+          const uint32_t word = q == 0 ? f[0] : q == 1 ? f[1] : q == 2 ? f[2] : f[3];
+          acc[k] |= __ballot((word >> sh) & 1u);
+        }
+      }
+    }
+    if ((uint32_t)lane < (1u << a.nkl)) {
+      const uint32_t obase = spread<TS_MAX_BITS>(gidx, 0, a.nkh, a.kh_out) | kout;
+#pragma unroll
+      for (int k = 0; k < NK; ++k)
+        if (acc[k] & my_mask) out[obase | kofs[k]] = 1;
+    }
+  }
+}
+
 // Kernel (b): reduction over discarded axes with many terms per output: one wave per out
 // entry, lanes sweep the 2^nd terms (lowest discarded bits on the lanes), then a
 // wave-level OR through ballot (GT0) or a wave sum (MOD256, count mod 256).
@@ -546,6 +665,64 @@ int ts_rewire(const uint8_t* in, int32_t in_rank, uint8_t* out, int32_t out_rank
         hipLaunchKernelGGL(k_permute_lds16, dim3(grid), dim3(WG), 0, s, in, out, pa);
       else
         hipLaunchKernelGGL(k_permute_lds, dim3(grid), dim3(WG), 0, s, in, out, pa);
+      return hip_rc(hipGetLastError());
+    }
+    if (a.num_disc >= 6 && mode == TS_MODE_GT0 && in_rank >= 10 && (reinterpret_cast<uintptr_t>(in) & 15) == 0) {
+      // Bits 0..3 in each lane's 16 bytes, bits 4..9 on the lanes.
+      ReduceWideArgs rw;
+      std::memset(&rw, 0, sizeof(rw));
+      std::vector<int> out_of_in(in_rank, -1);  // in bit -> out bit (kept)
+      for (int q = 0; q < out_rank; ++q) out_of_in[in_rank - 1 - out_ports[out_rank - 1 - q]] = q;
+      int nkb = 0;
+      for (int p = 0; p < in_rank; ++p) {
+        const int o = out_of_in[p];
+        if (p < 4) {
+          if (o >= 0) {
+            rw.kb_in[nkb] = (uint8_t)p;
+            rw.kb_out[nkb++] = (uint8_t)o;
+          } else {
+            rw.db_in[rw.ndb++] = (uint8_t)p;
+          }
+        } else if (p < 10) {
+          if (o >= 0) {
+            rw.kl_out[rw.nkl] = (uint8_t)o;
+            rw.kl_lane[rw.nkl++] = (uint8_t)(p - 4);
+          }
+        } else if (o >= 0) {
+          rw.kh_in[rw.nkh] = (uint8_t)p;
+          rw.kh_out[rw.nkh++] = (uint8_t)o;
+        } else {
+          rw.dh_in[rw.ndh++] = (uint8_t)p;
+        }
+      }
+      // Term chunks: up to 32k work items (waves), a chunk being at least two groups of 8
+      // terms (8 loads of 1 KiB in flight each).
+      const int ngk_bits = rw.ndh > 3 ? rw.ndh - 3 : 0;
+      rw.cb = 0;
+      while (rw.cb < ngk_bits - 1 && rw.nkh + rw.cb < 15) ++rw.cb;
+      uint32_t kmask = 0;
+      for (int i = 0; i < rw.nkl; ++i) kmask |= 1u << rw.kl_lane[i];
+      rw.fold = 0;
+      for (int i = 0; i < rw.ndb; ++i) rw.fold |= 1u << rw.db_in[i];
+      rw.lanes0 = 0;
+      for (int l = 0; l < 64; ++l)
+        if ((l & kmask) == 0) rw.lanes0 |= 1ull << l;
+      const uint64_t items = 1ull << (rw.nkh + rw.cb);
+      const int grid = (int)std::min<uint64_t>((items + 3) / 4, 8192);
+      const uint64_t nout = 1ull << out_rank;
+      if (nout % 16 == 0 && (reinterpret_cast<uintptr_t>(out) & 15) == 0) {
+        hipLaunchKernelGGL(k_zero16, dim3((int)std::min<uint64_t>((nout / 16 + WG - 1) / WG, 1024)), dim3(WG), 0, s,
+                           reinterpret_cast<uint4*>(out), nout / 16);
+      } else {
+        hipLaunchKernelGGL(k_fill, dim3(grid_for(nout)), dim3(WG), 0, s, out, (uint8_t)0, (int64_t)nout);
+      }
+      switch (nkb) {
+        case 0: hipLaunchKernelGGL(k_reduce_w16<0>, dim3(grid), dim3(WG), 0, s, in, out, rw); break;
+        case 1: hipLaunchKernelGGL(k_reduce_w16<1>, dim3(grid), dim3(WG), 0, s, in, out, rw); break;
+        case 2: hipLaunchKernelGGL(k_reduce_w16<2>, dim3(grid), dim3(WG), 0, s, in, out, rw); break;
+        case 3: hipLaunchKernelGGL(k_reduce_w16<3>, dim3(grid), dim3(WG), 0, s, in, out, rw); break;
+        default: hipLaunchKernelGGL(k_reduce_w16<4>, dim3(grid), dim3(WG), 0, s, in, out, rw); break;
+      }
       return hip_rc(hipGetLastError());
     }
     if (a.num_disc >= 6 && mode == TS_MODE_GT0 && in_rank >= 7 && (reinterpret_cast<uintptr_t>(in) & 1) == 0) {

@@ -53,3 +53,25 @@ def test_reduction_matches_oracle(rank, keep, mode):
         np.testing.assert_array_equal(got, want.astype(np.uint8))
         if order == kept:
             np.testing.assert_array_equal(got, F.rewire(x, order, mode))
+
+
+@pytest.mark.parametrize("rank,kept", [
+    (10, [4, 5, 6, 7]),                      # no in-register bit kept, all lane bits
+    (12, [0, 1, 2, 3, 10, 11]),              # all four in-register bits kept
+    (13, [1, 3, 6, 12]),                     # two in-register bits kept
+    (20, [0, 2, 3, 5, 9, 15, 19]),           # three kept, discarded high bits
+    (22, [1, 4, 8, 13, 17, 21]),             # 16 discarded: term chunks across waves
+    (24, [2, 7, 11, 12, 16, 20, 23]),        # 17 discarded
+])
+def test_wide_reduction_matches_oracle(rank, kept):
+    """The 16-byte-lane existential reduction (tensors of >= 10 bits, GT0): every split
+    of the four in-register bits into kept and discarded, term chunks across waves, and
+    bytes other than 0/1 (any nonzero byte is true)."""
+    rng = np.random.default_rng(rank)
+    for trial, dens in enumerate((0.3, 0.002, 0.0)):
+        order = [int(p) for p in rng.permutation(kept)] if trial == 1 else list(kept)
+        x = ((rng.random((2,) * rank) < dens) * rng.integers(1, 256, size=(2,) * rank)).astype(np.uint8)
+        print(f"rank {rank} out_ports {order} density {dens}", flush=True)
+        got = _rewire(x, order, "GT0")
+        want = F.rewire_exact(x != 0, order) > 0
+        np.testing.assert_array_equal(got, want.astype(np.uint8))
