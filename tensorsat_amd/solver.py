@@ -108,12 +108,16 @@ class BatchSolver:
         if events is not None:
             events[1].record()
 
-    def capture(self) -> None:
-        """Record every pass (resets, compactions, persistent launches) into one HIP graph;
-        :meth:`replay` then decides the whole batch with a single graph launch.  The
-        graph holds kernel nodes only (the resets are fill kernels, not memsets)."""
+    def capture(self, steps: int = 1) -> None:
+        """Record every pass (resets, compactions, persistent launches) into one HIP graph,
+        ``steps`` complete decisions of the batch back to back; :meth:`replay` then runs
+        them with a single graph launch (several decisions per graph amortise the host
+        cost of a graph launch, which exceeds one decision's device time).  The graph
+        holds kernel nodes only (the resets are fill kernels, not memsets)."""
         import torch
 
+        if steps < 1:
+            raise ValueError("steps must be >= 1")
         torch.cuda.synchronize()
         side = torch.cuda.Stream()
         side.wait_stream(torch.cuda.current_stream())
@@ -123,11 +127,13 @@ class BatchSolver:
         torch.cuda.synchronize()
         self.graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self.graph):
-            self.launch()
+            for _ in range(steps):
+                self.launch()
         torch.cuda.synchronize()
+        self.graph_steps = steps
 
     def replay(self, events=None) -> None:
-        """One graph launch of the captured passes (see :meth:`capture`)."""
+        """One graph launch of the captured decisions (see :meth:`capture`)."""
         if events is not None:
             events[0].record()
         self.graph.replay()
