@@ -681,6 +681,12 @@ def bench_dense(args) -> int:
         torch.cuda.synchronize()
         return a.elapsed_time(b) / steps, (time.perf_counter() - t0) * 1000.0 / steps
 
+    # Tile autotuner: every MFMA workgroup tile timed on this step shape, the fastest
+    # registered for it (counts are exact with any tile).
+    from tensorsat_amd import dense_tune as DT
+
+    tune = DT.autotune_tiles([(mb, nb, 0, kb)], reps=3).get((mb, nb, 0, kb))
+    log(f"tile autotune: {tune}")
     kernel_ms, ms_per_step = timed(args.steps)
     res = out.cpu().numpy().reshape(1 << mb, 1 << nb)
     N.check(L.ts_dense_config(0, 6), "ts_dense_config")
@@ -731,7 +737,10 @@ def bench_dense(args) -> int:
         "config": {"workload": f"dense composition 2^{mb} x 2^{kb} by 2^{kb} x 2^{nb}, K innermost, "
                                f"density 2^-{kb / 2:g}, {args.mode}",
                    "kernel_ms": kernel_ms, "valu_ms": valu_ms, "bytes": nbytes,
-                   "hbm_gbs": nbytes / (kernel_ms / 1000.0) / 1e9},
+                   "hbm_gbs": nbytes / (kernel_ms / 1000.0) / 1e9,
+                   "tile_autotune": None if tune is None else {
+                       "chosen": "x".join(map(str, DT.TILES[tune["tile"]])),
+                       "ms": {"x".join(map(str, DT.TILES[t])): v for t, v in tune["ms"].items()}}},
         "roofline": {"bound": "mfma", "achieved": tops, "peak": peak, "unit": "TFLOP/s", "frac": tops / peak,
                      "traffic": MEASURED_TRAFFIC.get(("dense", mb, nb, kb), (None, None))[0],
                      "traffic_source": MEASURED_TRAFFIC.get(("dense", mb, nb, kb), (None, None))[1],

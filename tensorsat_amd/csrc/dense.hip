@@ -21,6 +21,8 @@
 #include <stdint.h>
 
 #include <cstring>
+#include <map>
+#include <mutex>
 
 #include "../../include/tensorsat_amd.h"
 #include "dense.h"
@@ -157,6 +159,23 @@ __global__ __launch_bounds__(256) void k_step_mfma(const uint8_t* __restrict__ l
 bool g_enabled = true;
 int g_min_kbits = 6;
 
+// Tile per step shape (mbits, nbits, bbits, kbits): 0 the built-in rule, 1 64x64,
+// 2 128x64, 3 64x128, 4 128x128 (rows x columns); set by the tile autotuner.
+std::mutex g_tiles_mu;
+std::map<uint32_t, int> g_tiles;
+
+uint32_t shape_key(int m, int n, int b, int k) {
+  return (uint32_t)m | ((uint32_t)n << 5) | ((uint32_t)b << 10) | ((uint32_t)k << 15);
+}
+
+int tile_rule(int mbits, int nbits) { return mbits >= 7 && nbits >= 7 ? 4 : 1; }
+
+int tile_for(int m, int n, int b, int k) {
+  std::lock_guard<std::mutex> g(g_tiles_mu);
+  const auto it = g_tiles.find(shape_key(m, n, b, k));
+  return it != g_tiles.end() && it->second != 0 ? it->second : tile_rule(m, n);
+}
+
 }  // namespace
 
 bool dense_eligible(const ts_step& st, const void* lhs, const void* rhs) {
@@ -193,21 +212,45 @@ int launch_dense(const uint8_t* lhs, const uint8_t* rhs, uint8_t* out, const ts_
       a.n_opos[a.nbits++] = (uint8_t)j;
     }
   }
-  // 128 x 128 tiles when both sides have at least 128 entries, else 64 x 64.
-  const bool big = a.mbits >= 7 && a.nbits >= 7;
-  const int64_t BM = big ? 128 : 64, BN = big ? 128 : 64;
+  // Tile from the autotuned table, else 128 x 128 when both sides have at least 128
+  // entries and 64 x 64 otherwise.
+  const int tile = tile_for(a.mbits, a.nbits, a.bbits, a.kbits);
+  const int64_t BM = (tile == 2 || tile == 4) ? 128 : 64, BN = (tile == 3 || tile == 4) ? 128 : 64;
   a.mtiles = ((1ll << a.mbits) + BM - 1) / BM;
   a.ntiles = ((1ll << a.nbits) + BN - 1) / BN;
   const int64_t blocks = a.mtiles * a.ntiles * (1ll << a.bbits);
   if (blocks > (1ll << 24)) return TS_EUNSUPPORTED;  // grid.x * 256 threads < 2^32
-  if (big)
-    hipLaunchKernelGGL((k_step_mfma<4, 4>), dim3((unsigned)blocks), dim3(256), 0, s, lhs, rhs, out, a);
-  else
-    hipLaunchKernelGGL((k_step_mfma<2, 2>), dim3((unsigned)blocks), dim3(256), 0, s, lhs, rhs, out, a);
+  const dim3 grid((unsigned)blocks), block(256);
+  switch (tile) {
+    case 4: hipLaunchKernelGGL((k_step_mfma<4, 4>), grid, block, 0, s, lhs, rhs, out, a); break;
+    case 3: hipLaunchKernelGGL((k_step_mfma<2, 4>), grid, block, 0, s, lhs, rhs, out, a); break;
+    case 2: hipLaunchKernelGGL((k_step_mfma<4, 2>), grid, block, 0, s, lhs, rhs, out, a); break;
+    default: hipLaunchKernelGGL((k_step_mfma<2, 2>), grid, block, 0, s, lhs, rhs, out, a); break;
+  }
   return hipGetLastError() == hipSuccess ? TS_OK : TS_EHIP;
 }
 
 }  // namespace tsd
+
+extern "C" int ts_dense_tiles(int32_t mbits, int32_t nbits, int32_t bbits, int32_t kbits, int32_t tile) {
+  if (mbits < 0 || nbits < 0 || bbits < 0 || kbits < 0 || mbits > TS_MAX_BITS || nbits > TS_MAX_BITS ||
+      bbits > TS_MAX_BITS || kbits > TS_MAX_BITS || tile < 0 || tile > 4)
+    return TS_EINVAL;
+  std::lock_guard<std::mutex> g(tsd::g_tiles_mu);
+  if (tile == 0)
+    tsd::g_tiles.erase(tsd::shape_key(mbits, nbits, bbits, kbits));
+  else
+    tsd::g_tiles[tsd::shape_key(mbits, nbits, bbits, kbits)] = tile;
+  return TS_OK;
+}
+
+extern "C" int ts_dense_tile_of(int32_t mbits, int32_t nbits, int32_t bbits, int32_t kbits, int32_t* tile) {
+  if (!tile || mbits < 0 || nbits < 0 || bbits < 0 || kbits < 0 || mbits > TS_MAX_BITS || nbits > TS_MAX_BITS ||
+      bbits > TS_MAX_BITS || kbits > TS_MAX_BITS)
+    return TS_EINVAL;
+  *tile = tsd::tile_for(mbits, nbits, bbits, kbits);
+  return TS_OK;
+}
 
 extern "C" int ts_dense_config(int32_t enable, int32_t min_sum_bits) {
   if (min_sum_bits < 6 || min_sum_bits > TS_MAX_BITS) return TS_EINVAL;

@@ -41,3 +41,27 @@ def test_hip_library_exports_everything():
         assert hasattr(lib, name), name
     assert set(N.EXPORTS) == declared
     assert lib.ts_version().startswith(b"tensorsat_amd")
+
+
+def test_dense_tile_table_host_side():
+    """The dense tile table is host state (no GPU needed): set, read back, reset to the
+    built-in rule, bad arguments."""
+    import ctypes
+
+    from tensorsat_amd import _native as N
+
+    L = N.load()
+
+    def tile_of(*shape):
+        t = ctypes.c_int32(0)
+        assert L.ts_dense_tile_of(*shape, ctypes.byref(t)) == 0
+        return t.value
+
+    assert tile_of(8, 8, 0, 8) == 4 and tile_of(5, 9, 0, 8) == 1  # built-in rule
+    assert L.ts_dense_tiles(8, 8, 0, 8, 2) == 0
+    assert tile_of(8, 8, 0, 8) == 2
+    assert L.ts_dense_tiles(8, 8, 0, 8, 0) == 0
+    assert tile_of(8, 8, 0, 8) == 4
+    assert L.ts_dense_tiles(8, 8, 0, 8, 5) == N.TS_EINVAL
+    assert L.ts_dense_tiles(-1, 8, 0, 8, 1) == N.TS_EINVAL
+    assert L.ts_dense_tile_of(8, 8, 0, 28, None) == N.TS_EINVAL

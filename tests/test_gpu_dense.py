@@ -91,3 +91,38 @@ def test_dense_route_matches_valu_route(shape):
             N.check(N.lib().ts_dense_config(1, 6), "ts_dense_config")
         np.testing.assert_array_equal(mfma, valu)
         assert 0 < int(mfma.sum()) < mfma.size
+
+
+@pytest.mark.parametrize("tile", [1, 2, 3, 4])
+@pytest.mark.parametrize("case", [1, 4, 6, 8])
+def test_every_tile_matches_oracle(case, tile):
+    """Each workgroup tile of the MFMA kernel (64x64, 128x64, 64x128, 128x128), forced for
+    the case's shape, gives the oracle's result (tiles larger than a side included)."""
+    from tensorsat_amd import dense_tune as DT
+
+    rng = np.random.default_rng(100 + case)
+    kb, mb, nb, bb, dens = CASES[case]
+    lhs, lw, rhs, rw, ow = _case(rng, kb, mb, nb, bb, dens)
+    DT.set_tile(mb, nb, bb, kb, tile)
+    try:
+        assert DT.tile_of(mb, nb, bb, kb) == tile
+        print(f"case {CASES[case]} tile {tile}", flush=True)
+        for mode in ("GT0", "MOD256"):
+            np.testing.assert_array_equal(_run(lhs, lw, rhs, rw, ow, mode), F.contract2(lhs, lw, rhs, rw, ow, mode))
+    finally:
+        DT.set_tile(mb, nb, bb, kb, 0)
+
+
+def test_tile_autotuner_registers_the_fastest():
+    from tensorsat_amd import dense_tune as DT
+
+    shape = (8, 8, 1, 10)
+    try:
+        table = DT.autotune_tiles([shape, (3, 8, 0, 8)], reps=3)
+        assert list(table) == [shape]  # the second shape is not on the MFMA route
+        row = table[shape]
+        assert set(row["ms"]) == {1, 2, 3, 4} and row["tile"] == min(row["ms"], key=row["ms"].get)
+        assert DT.tile_of(*shape) == row["tile"]
+        print(f"tile table {row}", flush=True)
+    finally:
+        DT.set_tile(*shape, 0)
