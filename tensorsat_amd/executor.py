@@ -165,6 +165,10 @@ def dispatch_order(prog: Program, num_workgroups: int | None = None) -> np.ndarr
     return order
 
 
+ARENA_BUDGET = 16 << 30
+"""Device memory for the per-workgroup arenas of the device-memory batched kernels (bytes):
+programs with large cubes run on fewer workgroups rather than allocating beyond it."""
+
 ROUND_BOUNDS = (0, 1 << 30)
 """Cube-index boundaries of the dispatch rounds.  Default: one round per program; jobs
 of decided instances are skipped by the per-job check inside the persistent kernel.
@@ -225,6 +229,8 @@ class BatchExecutor:
             else:
                 N.check(L.ts_batch_max_workgroups(ctypes.byref(mx)), "ts_batch_max_workgroups")
             num_workgroups = int(mx.value)
+        if not self.lds:  # device arenas within budget
+            num_workgroups = min(int(num_workgroups), max(N.BATCH_SEGMENTS, ARENA_BUDGET // self.arena_stride))
         # at least one workgroup per non-empty queue segment
         self.num_workgroups = max(1, min(max(int(num_workgroups), N.BATCH_SEGMENTS), max(1, nj)))
         n_arena = 16 if self.lds else self.num_workgroups * self.arena_stride
@@ -341,10 +347,11 @@ class BatchCounter:
             mx = N.i32(0)
             N.check(L.ts_batch_max_workgroups(ctypes.byref(mx)), "ts_batch_max_workgroups")
             num_workgroups = int(mx.value)
-        # at least one workgroup per non-empty queue segment
-        self.num_workgroups = max(1, min(max(int(num_workgroups), N.BATCH_SEGMENTS), max(1, nj)))
         max_arena = int(prog.jobs["arena_elems"].max()) if nj else 16
         self.arena_stride = (8 * max_arena + 255) // 256 * 256
+        # at least one workgroup per non-empty queue segment; device arenas within budget
+        num_workgroups = min(int(num_workgroups), max(N.BATCH_SEGMENTS, ARENA_BUDGET // self.arena_stride))
+        self.num_workgroups = max(1, min(max(num_workgroups, N.BATCH_SEGMENTS), max(1, nj)))
         self.arenas = torch.empty(self.num_workgroups * self.arena_stride, dtype=torch.uint8, device=device)
         self.order = torch.from_numpy(dispatch_order(prog)).to(device)
         n = max(1, prog.num_instances)
