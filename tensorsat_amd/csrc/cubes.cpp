@@ -91,6 +91,9 @@ struct Enumerator {
   std::vector<std::vector<int>>* frontier = nullptr;
   std::vector<std::pair<int, int>>* items = nullptr;
   bool budget_hit = false, sat_found = false;
+  // search() scratch (see search()).
+  std::vector<int> s_rc, s_rv, s_score, s_pos, s_neg, s_vmap;
+  std::vector<std::vector<int>> s_cl;
   int64_t learned_trees = 0, fallback_trees = 0;
 
   Enumerator(const Formula& f_, const tsc_sat_opts& o_, uint64_t sd, std::vector<Job>* jobs,
@@ -102,6 +105,10 @@ struct Enumerator {
       for (int l : f.clauses[c]) occ[lit_index(l)].push_back((int)c);
     nsat.assign(f.clauses.size(), 0);
     nfalse.assign(f.clauses.size(), 0);
+    s_score.assign(f.n, 0);
+    s_pos.assign(f.n, 0);
+    s_neg.assign(f.n, 0);
+    s_vmap.assign(f.n, -1);
   }
   static int lit_index(int l) { return 2 * (std::abs(l) - 1) + (l < 0 ? 1 : 0); }
   bool lit_true(int l) const {
@@ -249,16 +256,23 @@ struct Enumerator {
 
   void search() {
     if (truncated) return;
-    std::vector<int> rc;
+    // Per-node scratch lives in members: everything below is consumed before the node
+    // branches (only the first literal is kept across the recursion), so one set of
+    // buffers serves every depth.
+    std::vector<int>& rc = s_rc;
+    std::vector<int>& rv = s_rv;
+    rc.clear();
+    rv.clear();
     for (size_t c = 0; c < f.clauses.size(); ++c)
       if (nsat[c] == 0) rc.push_back((int)c);
     if (rc.empty()) {
       emit(rc);
       return;
     }
-    // Residual variables and degrees.
-    std::vector<int> score(f.n, 0), occ_pos(f.n, 0), occ_neg(f.n, 0);
-    std::vector<int> rv;
+    // Residual variables and degrees (score/occ_pos/occ_neg are zero outside rv).
+    std::vector<int>& score = s_score;
+    std::vector<int>& occ_pos = s_pos;
+    std::vector<int>& occ_neg = s_neg;
     for (int c : rc) {
       int free_in = 0;
       for (int l : f.clauses[c])
@@ -274,22 +288,44 @@ struct Enumerator {
     }
     bool leaf = (int)rv.size() <= o.width;
     if (!leaf && (int)rv.size() <= 4 * o.width + 8) {
-      std::vector<int> vmap(f.n, -1);
+      std::vector<int>& vmap = s_vmap;
       for (size_t i = 0; i < rv.size(); ++i) vmap[rv[i]] = (int)i;
-      std::vector<std::vector<int>> cl;
-      cl.reserve(rc.size());
-      for (int c : rc) {
-        std::vector<int> ws;
-        for (int l : f.clauses[c]) {
+      std::vector<std::vector<int>>& cl = s_cl;
+      cl.resize(rc.size());
+      for (size_t i = 0; i < rc.size(); ++i) {
+        std::vector<int>& ws = cl[i];
+        ws.clear();
+        for (int l : f.clauses[rc[i]]) {
           int v = std::abs(l) - 1;
           if (val[v] == 0) ws.push_back(vmap[v]);
         }
-        cl.push_back(std::move(ws));
       }
+      for (int v : rv) vmap[v] = -1;
       // The estimate only filters; the planned tree decides (emit).
       leaf = mindeg_width((int)rv.size(), cl) <= o.width + 1;
     }
-    if (leaf && emit(rc)) return;
+    int bv = -1, first = 0;
+    auto pick = [&]() {
+      if (rv.empty()) return;
+      bv = rv[0];
+      if (prio && o.prior_branch) {
+        for (int v : rv)
+          if (prio[v] > prio[bv] || (prio[v] == prio[bv] && v < bv)) bv = v;
+      } else {
+        for (int v : rv)
+          if (score[v] > score[bv] || (score[v] == score[bv] && v < bv)) bv = v;
+      }
+      first = occ_pos[bv] >= occ_neg[bv] ? bv + 1 : -(bv + 1);
+      for (int v : rv) score[v] = occ_pos[v] = occ_neg[v] = 0;
+    };
+    if (leaf) {
+      // emit() reuses none of this scratch, but it may return false (branch further):
+      // take the branch literal first so the scratch is clean either way.
+      pick();
+      if (emit(rc)) return;
+    } else {
+      pick();
+    }
     if (o.max_nodes > 0 && nodes >= o.max_nodes) {
       budget_hit = true;
       truncated = true;
@@ -301,15 +337,6 @@ struct Enumerator {
       return;
     }
     ++nodes;
-    int bv = rv[0];
-    if (prio && o.prior_branch) {
-      for (int v : rv)
-        if (prio[v] > prio[bv] || (prio[v] == prio[bv] && v < bv)) bv = v;
-    } else {
-      for (int v : rv)
-        if (score[v] > score[bv] || (score[v] == score[bv] && v < bv)) bv = v;
-    }
-    int first = occ_pos[bv] >= occ_neg[bv] ? bv + 1 : -(bv + 1);
     for (int l : {first, -first}) {
       size_t mark = trail.size();
       if (assign(l) && propagate()) {
