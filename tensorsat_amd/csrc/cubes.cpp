@@ -69,7 +69,17 @@ struct Enumerator {
   const Formula& f;
   const tsc_sat_opts& o;
   std::vector<int8_t> val;                 // per var: 0 free, +1 true, -1 false
-  std::vector<std::vector<int>> occ;       // literal index 2v+(neg) -> clauses
+  // Flat (CSR) copies of the occurrence lists (literal index 2v+(neg) -> clauses) and of
+  // the clause literals, for the propagation and scoring loops.
+  std::vector<int> occ_off, occ_cl, lit_off, lits;
+  struct Span {
+    const int* b;
+    const int* e;
+    const int* begin() const { return b; }
+    const int* end() const { return e; }
+  };
+  Span occ(int li) const { return {occ_cl.data() + occ_off[li], occ_cl.data() + occ_off[li + 1]}; }
+  Span clause(int c) const { return {lits.data() + lit_off[c], lits.data() + lit_off[c + 1]}; }
   std::vector<int> nsat, nfalse;
   std::vector<int> trail;
   std::vector<int> queue;
@@ -100,9 +110,20 @@ struct Enumerator {
              std::vector<int32_t>* fr, const float* pr)
       : f(f_), o(o_), seed(sd), jobs_out(jobs), free_out(fr), prio(pr) {
     val.assign(f.n, 0);
-    occ.assign(2 * f.n, {});
+    occ_off.assign(2 * f.n + 1, 0);
+    lit_off.assign(1, 0);
+    for (size_t c = 0; c < f.clauses.size(); ++c) {
+      for (int l : f.clauses[c]) {
+        occ_off[lit_index(l) + 1]++;
+        lits.push_back(l);
+      }
+      lit_off.push_back((int)lits.size());
+    }
+    for (int i = 0; i < 2 * f.n; ++i) occ_off[i + 1] += occ_off[i];
+    occ_cl.resize(occ_off[2 * f.n]);
+    std::vector<int> fill(occ_off.begin(), occ_off.end() - 1);
     for (size_t c = 0; c < f.clauses.size(); ++c)
-      for (int l : f.clauses[c]) occ[lit_index(l)].push_back((int)c);
+      for (int l : f.clauses[c]) occ_cl[fill[lit_index(l)]++] = (int)c;
     nsat.assign(f.clauses.size(), 0);
     nfalse.assign(f.clauses.size(), 0);
     s_score.assign(f.n, 0);
@@ -122,14 +143,14 @@ struct Enumerator {
     val[v] = l > 0 ? 1 : -1;
     trail.push_back(l);
     bool ok = true;
-    for (int c : occ[lit_index(l)]) nsat[c]++;
-    for (int c : occ[lit_index(-l)]) {
+    for (int c : occ(lit_index(l))) nsat[c]++;
+    for (int c : occ(lit_index(-l))) {
       nfalse[c]++;
       if (nsat[c] == 0) {
-        int len = (int)f.clauses[c].size();
+        int len = lit_off[c + 1] - lit_off[c];
         if (nfalse[c] == len) ok = false;
         else if (nfalse[c] == len - 1) {
-          for (int x : f.clauses[c])
+          for (int x : clause(c))
             if (val[std::abs(x) - 1] == 0) {
               queue.push_back(x);
               break;
@@ -162,8 +183,8 @@ struct Enumerator {
     while (trail.size() > mark) {
       int l = trail.back();
       trail.pop_back();
-      for (int c : occ[lit_index(l)]) nsat[c]--;
-      for (int c : occ[lit_index(-l)]) nfalse[c]--;
+      for (int c : occ(lit_index(l))) nsat[c]--;
+      for (int c : occ(lit_index(-l))) nfalse[c]--;
       val[std::abs(l) - 1] = 0;
     }
   }
@@ -263,21 +284,18 @@ struct Enumerator {
     std::vector<int>& rv = s_rv;
     rc.clear();
     rv.clear();
-    for (size_t c = 0; c < f.clauses.size(); ++c)
-      if (nsat[c] == 0) rc.push_back((int)c);
-    if (rc.empty()) {
-      emit(rc);
-      return;
-    }
-    // Residual variables and degrees (score/occ_pos/occ_neg are zero outside rv).
+    // Residual clauses, variables and degrees in one pass (score/occ_pos/occ_neg are zero
+    // outside rv).  A residual clause has no true literal, so its assigned literals are
+    // all false and it has len - nfalse free literals.
     std::vector<int>& score = s_score;
     std::vector<int>& occ_pos = s_pos;
     std::vector<int>& occ_neg = s_neg;
-    for (int c : rc) {
-      int free_in = 0;
-      for (int l : f.clauses[c])
-        if (val[std::abs(l) - 1] == 0) ++free_in;
-      for (int l : f.clauses[c]) {
+    const int nclauses = (int)f.clauses.size();
+    for (int c = 0; c < nclauses; ++c) {
+      if (nsat[c] != 0) continue;
+      rc.push_back(c);
+      const int free_in = lit_off[c + 1] - lit_off[c] - nfalse[c];
+      for (int l : clause(c)) {
         int v = std::abs(l) - 1;
         if (val[v] != 0) continue;
         if (score[v] == 0 && occ_pos[v] == 0 && occ_neg[v] == 0) rv.push_back(v);
@@ -285,6 +303,10 @@ struct Enumerator {
         if (l > 0) occ_pos[v]++;
         else occ_neg[v]++;
       }
+    }
+    if (rc.empty()) {
+      emit(rc);
+      return;
     }
     bool leaf = (int)rv.size() <= o.width;
     if (!leaf && (int)rv.size() <= 4 * o.width + 8) {
@@ -295,7 +317,7 @@ struct Enumerator {
       for (size_t i = 0; i < rc.size(); ++i) {
         std::vector<int>& ws = cl[i];
         ws.clear();
-        for (int l : f.clauses[rc[i]]) {
+        for (int l : clause(rc[i])) {
           int v = std::abs(l) - 1;
           if (val[v] == 0) ws.push_back(vmap[v]);
         }
