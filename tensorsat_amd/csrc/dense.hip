@@ -32,8 +32,17 @@ namespace {
 
 typedef int i4 __attribute__((ext_vector_type(4)));
 
-constexpr int KC = 64;         // bytes of K per LDS stage (one MFMA K step)
-constexpr int LDSW = KC + 16;  // padded LDS row (bytes)
+constexpr int KC = 64;    // bytes of K per LDS stage (one MFMA K step)
+constexpr int LDSW = KC;  // LDS row (bytes): unpadded, 16-byte segments XOR-swizzled
+
+// LDS segment swizzle: K segment s (16 bytes) of tile row r sits at segment s ^ seg_swz(r),
+// which depends on r mod 16 only.
+// The MFMA operand reads take 16 consecutive rows at one K segment; with this swizzle any
+// 8 consecutive rows of equal parity (32 banks x 4 bytes per pass) and any 16 consecutive
+// rows of equal r mod 4 (64 banks) land on distinct 16-byte bank groups, and the loader's
+// writes (4 lanes per row, 16 consecutive rows per wave) cover whole 64-byte rows, so
+// neither side has bank conflicts.
+__device__ __forceinline__ int seg_swz(int r) { return ((r >> 1) ^ ((r & 15) >> 3)) & 3; }
 
 struct DenseArgs {
   int32_t mbits, nbits, bbits, kbits;
@@ -96,8 +105,10 @@ __global__ __launch_bounds__(256) void k_step_mfma(const uint8_t* __restrict__ l
   const uint32_t bL = scatter(b, a.bbits, a.b_lpos), bR = scatter(b, a.bbits, a.b_rpos);
   const uint32_t bO = scatter(b, a.bbits, a.b_opos);
   __syncthreads();
-  // Loader: load j of thread t moves 16 bytes (segment t % 4) of row t / 4 + 64 j.
+  // Loader: load j of thread t moves 16 bytes (segment t % 4) of row t / 4 + 64 j; rows
+  // 64 j apart share the swizzle, so its LDS segment is the same for every j.
   const int ls = (tid & 3) * 16;
+  const int lsw = ((tid & 3) ^ seg_swz(tid >> 2)) * 16;
   const uint8_t* pa[LA];
   const uint8_t* pb[LB];
   bool oka[LA], okb[LB];
@@ -117,8 +128,8 @@ __global__ __launch_bounds__(256) void k_step_mfma(const uint8_t* __restrict__ l
     for (int j = 0; j < LB; ++j) rb[j] = okb[j] ? *reinterpret_cast<const i4*>(pb[j] + k0) : (i4){0, 0, 0, 0};
   };
   auto lstore = [&](int buf) {
-    for (int j = 0; j < LA; ++j) *reinterpret_cast<i4*>(&sA[buf][(tid >> 2) + 64 * j][ls]) = ra[j];
-    for (int j = 0; j < LB; ++j) *reinterpret_cast<i4*>(&sB[buf][(tid >> 2) + 64 * j][ls]) = rb[j];
+    for (int j = 0; j < LA; ++j) *reinterpret_cast<i4*>(&sA[buf][(tid >> 2) + 64 * j][lsw]) = ra[j];
+    for (int j = 0; j < LB; ++j) *reinterpret_cast<i4*>(&sB[buf][(tid >> 2) + 64 * j][lsw]) = rb[j];
   };
   const int wm = (wave >> 1) * 16 * TM, wn = (wave & 1) * 16 * TN;
   i4 acc[TM][TN];
@@ -129,8 +140,9 @@ __global__ __launch_bounds__(256) void k_step_mfma(const uint8_t* __restrict__ l
   lstore(0);
   __syncthreads();
   // Lane l feeds row / column l % 16 with 16 bytes of K at 16 * (l / 16); A and B use the
-  // same K assignment, so the sum over K does not depend on the hardware's order.
-  const int rr = lane & 15, kk = (lane >> 4) * 16;
+  // same K assignment, so the sum over K does not depend on the hardware's order.  Operand
+  // rows are 16-aligned plus l % 16, so the swizzled segment is the same for every tile.
+  const int rr = lane & 15, kk = ((lane >> 4) ^ seg_swz(rr)) * 16;
   for (uint64_t c = 0; c < nstages; ++c) {
     const int buf = (int)(c & 1);
     const bool more = c + 1 < nstages;
