@@ -324,7 +324,7 @@ struct Enumerator {
       }
       for (int v : rv) vmap[v] = -1;
       // The estimate only filters; the planned tree decides (emit).
-      leaf = mindeg_width((int)rv.size(), cl) <= o.width + 1;
+      leaf = mindeg_width((int)rv.size(), cl, o.width + 1) <= o.width + 1;
     }
     int bv = -1, first = 0;
     auto pick = [&]() {

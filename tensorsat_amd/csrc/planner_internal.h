@@ -3,6 +3,7 @@
 
 #include <cstdint>
 #include <random>
+#include <climits>
 #include <vector>
 
 #include "../../include/tensorsat_plan.h"
@@ -45,8 +46,9 @@ void prior_plan(const BitNet& net, const float* prior, float lam, std::vector<in
                 std::vector<int>* order_out = nullptr);
 
 // Width estimate (min-degree heuristic) of the primal graph of a clause list over
-// variables 0..nv-1.
-int mindeg_width(int nv, const std::vector<std::vector<int>>& clauses_vars);
+// variables 0..nv-1.  The pass stops once the width exceeds stop_above: results above it
+// are only known to be above it.
+int mindeg_width(int nv, const std::vector<std::vector<int>>& clauses_vars, int stop_above = INT_MAX);
 
 // Compiled job.
 struct Job {

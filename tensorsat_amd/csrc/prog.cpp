@@ -8,6 +8,7 @@
 // tensors and leaves follow tensorsat/lib/sat.py:208-262 and lib/bincirc.py:28-72.
 
 #include <algorithm>
+#include <climits>
 #include <array>
 #include <atomic>
 #include <chrono>
@@ -133,10 +134,12 @@ struct Bits {
 // Elimination order over `nv` vertices (cliques given), skipping `keep` vertices.
 // Returns the order and its width (max degree at elimination).
 // With `prior` (one value per vertex), the score is fill + lam * prior[v] and ties go to
-// the smaller prior: a learned proposal steering the adaptive heuristic.
+// the smaller prior: a learned proposal steering the adaptive heuristic.  With
+// `stop_above`, the pass stops as soon as the width exceeds it (the order is then partial
+// and the returned width is some value > stop_above).
 int elim_order(int nv, const std::vector<std::vector<int>>& cliques, const std::vector<char>& keep,
                bool min_fill, std::mt19937_64* rng, std::vector<int>& order, const float* prior = nullptr,
-               float lam = 0.f) {
+               float lam = 0.f, int stop_above = INT_MAX) {
   Bits adj;
   adj.init(nv, nv);
   for (const auto& c : cliques)
@@ -155,8 +158,7 @@ int elim_order(int nv, const std::vector<std::vector<int>>& cliques, const std::
   int remaining = 0;
   for (int v = 0; v < nv; ++v)
     if (!keep[v]) ++remaining;
-  std::vector<int> cand;
-  std::vector<uint64_t> tmp(adj.nw);
+  std::vector<int> cand, nb;
   while (remaining > 0) {
     long best = -1;
     double bestp = 0.0;
@@ -206,12 +208,13 @@ int elim_order(int nv, const std::vector<std::vector<int>>& cliques, const std::
     int v = cand[0];
     if (rng && cand.size() > 1) v = cand[(*rng)() % cand.size()];
     width = std::max(width, deg[v]);
+    if (width > stop_above) return width;
     order.push_back(v);
     gone[v] = 1;
     --remaining;
     // Connect neighbours, remove v.
     uint64_t* rv = adj.row(v);
-    std::vector<int> nb;
+    nb.clear();
     for (int i = 0; i < adj.nw; ++i) {
       uint64_t m = rv[i];
       while (m) {
@@ -372,10 +375,10 @@ void prior_plan(const BitNet& net, const float* prior, float lam, std::vector<in
   if (order_out) *order_out = order;
 }
 
-int mindeg_width(int nv, const std::vector<std::vector<int>>& clauses_vars) {
+int mindeg_width(int nv, const std::vector<std::vector<int>>& clauses_vars, int stop_above) {
   std::vector<char> keep(nv, 0);
   std::vector<int> order;
-  return elim_order(nv, clauses_vars, keep, false, nullptr, order);
+  return elim_order(nv, clauses_vars, keep, false, nullptr, order, nullptr, 0.f, stop_above);
 }
 
 namespace {
