@@ -264,10 +264,12 @@ __device__ inline void lane_packet(uint4 r, bool active, uint8_t* arena, int mod
   }
   if (__ballot(general) != 0) {
     const int nterm = general ? 1 << nc : 0;
+    // Both 16-bit term offsets travel in one cross-lane read.
+    const uint32_t tlr = (uint32_t)tl | ((uint32_t)tr << 16);
     uint32_t g = 0;
     for (int c = 0; __ballot(c < nterm) != 0; ++c) {
-      const uint32_t tlc = (uint32_t)__shfl(tl, (first + c) & 63);
-      const uint32_t trc = (uint32_t)__shfl(tr, (first + c) & 63);
+      const uint32_t t2 = (uint32_t)__shfl((int)tlr, (first + c) & 63);
+      const uint32_t tlc = t2 & 0xffffu, trc = t2 >> 16;
       if (c < nterm) g += (uint32_t)arena[lhs + tlc] & (norhs ? 1u : (uint32_t)arena[rhs + trc]);
     }
     if (general) acc = g;
