@@ -83,11 +83,12 @@ int ts_as_scalar(const uint8_t* x, int64_t n, int32_t* result, ts_stream stream)
 
 /* Tile of the dense int8 MFMA route for steps of one shape (bits of rows held by lhs
  * only, of columns held by rhs only, of shared batch, of the contraction): 1 64x64, 2
- * 128x64, 3 64x128, 4 128x128 (rows x columns), 0 the built-in rule (128x128 when both
- * sides have >= 7 bits, else 64x64).  Set by the tile autotuner
- * (tensorsat_amd.dense_tune); no counterpart in the reference. */
+ * 128x64, 3 64x128, 4 128x128, 5 256x128, 6 128x256 (rows x columns), 0 the built-in
+ * rule (128x256 when both sides have >= 10 bits, 128x128 when both have >= 7, else
+ * 64x64).  Set by the tile autotuner (tensorsat_amd.dense_tune); no counterpart in the
+ * reference. */
 int ts_dense_tiles(int32_t mbits, int32_t nbits, int32_t bbits, int32_t kbits, int32_t tile);
-/* The tile steps of that shape use now (1..4). */
+/* The tile steps of that shape use now (1..6). */
 int ts_dense_tile_of(int32_t mbits, int32_t nbits, int32_t bbits, int32_t kbits, int32_t* tile);
 
 /* Routing of dense steps (ts_contract2 and ts_program_run) to the int8 MFMA kernel

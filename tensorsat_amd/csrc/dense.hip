@@ -9,8 +9,9 @@
 //   batch   b : the out bits held by both        (one offset triple per batch index)
 //   count[b, m, n] = sum_k lhs[b, m, k] * rhs[b, n, k]      (0/1 int8, exact int32)
 //
-// Each workgroup computes a 128 x 128 (or, for narrow steps, 64 x 64) tile of (m, n) for
-// one batch index with 4 waves of 4 x 4 (2 x 2) v_mfma_i32_16x16x64_i8; K is staged
+// Each workgroup computes a (32 TM) x (32 TN) tile of (m, n) for one batch index with 4
+// waves of TM x TN v_mfma_i32_16x16x64_i8 (TM, TN in {2, 4, 8}: 64 x 64 up to 256 x 128
+// and 128 x 256; larger wave tiles read fewer LDS bytes per MFMA); K is staged
 // through double-buffered LDS in 64-byte chunks (16-byte vector loads, four lanes per
 // 64-byte row segment, the next chunk's loads in flight during the current MFMAs).  Row/column offsets are scattered by bit
 // position (any interleaving of m, n and b bits in the operands and the output), so they
@@ -64,8 +65,10 @@ __device__ __forceinline__ uint32_t scatter(uint64_t idx, int nb, const uint8_t*
 // 16 x 16.  K advances 64 bytes per stage through two LDS buffers: the next stage's
 // global loads are issued before the current stage's MFMAs, and one barrier per stage
 // separates the LDS write of stage c+1 from the reads of stage c.
+// At least two waves per SIMD: the 256-entry tiles hold 128 accumulator registers per lane
+// and would otherwise be register-allocated for one.
 template <int TM, int TN>
-__global__ __launch_bounds__(256) void k_step_mfma(const uint8_t* __restrict__ lhs, const uint8_t* __restrict__ rhs,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_step_mfma(const uint8_t* __restrict__ lhs, const uint8_t* __restrict__ rhs,
                                                    uint8_t* __restrict__ out, DenseArgs a) {
   constexpr int BM = 32 * TM, BN = 32 * TN;
   constexpr int LA = BM * KC / 16 / 256, LB = BN * KC / 16 / 256;  // 16-byte loads per thread
@@ -172,7 +175,8 @@ bool g_enabled = true;
 int g_min_kbits = 6;
 
 // Tile per step shape (mbits, nbits, bbits, kbits): 0 the built-in rule, 1 64x64,
-// 2 128x64, 3 64x128, 4 128x128 (rows x columns); set by the tile autotuner.
+// 2 128x64, 3 64x128, 4 128x128, 5 256x128, 6 128x256 (rows x columns); set by the tile
+// autotuner.
 std::mutex g_tiles_mu;
 std::map<uint32_t, int> g_tiles;
 
@@ -180,7 +184,10 @@ uint32_t shape_key(int m, int n, int b, int k) {
   return (uint32_t)m | ((uint32_t)n << 5) | ((uint32_t)b << 10) | ((uint32_t)k << 15);
 }
 
-int tile_rule(int mbits, int nbits) { return mbits >= 7 && nbits >= 7 ? 4 : 1; }
+int tile_rule(int mbits, int nbits) {
+  if (mbits >= 10 && nbits >= 10) return 6;
+  return mbits >= 7 && nbits >= 7 ? 4 : 1;
+}
 
 int tile_for(int m, int n, int b, int k) {
   std::lock_guard<std::mutex> g(g_tiles_mu);
@@ -224,16 +231,19 @@ int launch_dense(const uint8_t* lhs, const uint8_t* rhs, uint8_t* out, const ts_
       a.n_opos[a.nbits++] = (uint8_t)j;
     }
   }
-  // Tile from the autotuned table, else 128 x 128 when both sides have at least 128
+  // Tile from the autotuned table, else the built-in rule (tile_rule)
   // entries and 64 x 64 otherwise.
   const int tile = tile_for(a.mbits, a.nbits, a.bbits, a.kbits);
-  const int64_t BM = (tile == 2 || tile == 4) ? 128 : 64, BN = (tile == 3 || tile == 4) ? 128 : 64;
+  static constexpr int kTM[7] = {2, 2, 4, 2, 4, 8, 4}, kTN[7] = {2, 2, 2, 4, 4, 4, 8};
+  const int64_t BM = 32 * kTM[tile], BN = 32 * kTN[tile];
   a.mtiles = ((1ll << a.mbits) + BM - 1) / BM;
   a.ntiles = ((1ll << a.nbits) + BN - 1) / BN;
   const int64_t blocks = a.mtiles * a.ntiles * (1ll << a.bbits);
   if (blocks > (1ll << 24)) return TS_EUNSUPPORTED;  // grid.x * 256 threads < 2^32
   const dim3 grid((unsigned)blocks), block(256);
   switch (tile) {
+    case 6: hipLaunchKernelGGL((k_step_mfma<4, 8>), grid, block, 0, s, lhs, rhs, out, a); break;
+    case 5: hipLaunchKernelGGL((k_step_mfma<8, 4>), grid, block, 0, s, lhs, rhs, out, a); break;
     case 4: hipLaunchKernelGGL((k_step_mfma<4, 4>), grid, block, 0, s, lhs, rhs, out, a); break;
     case 3: hipLaunchKernelGGL((k_step_mfma<2, 4>), grid, block, 0, s, lhs, rhs, out, a); break;
     case 2: hipLaunchKernelGGL((k_step_mfma<4, 2>), grid, block, 0, s, lhs, rhs, out, a); break;
@@ -246,7 +256,7 @@ int launch_dense(const uint8_t* lhs, const uint8_t* rhs, uint8_t* out, const ts_
 
 extern "C" int ts_dense_tiles(int32_t mbits, int32_t nbits, int32_t bbits, int32_t kbits, int32_t tile) {
   if (mbits < 0 || nbits < 0 || bbits < 0 || kbits < 0 || mbits > TS_MAX_BITS || nbits > TS_MAX_BITS ||
-      bbits > TS_MAX_BITS || kbits > TS_MAX_BITS || tile < 0 || tile > 4)
+      bbits > TS_MAX_BITS || kbits > TS_MAX_BITS || tile < 0 || tile > 6)
     return TS_EINVAL;
   std::lock_guard<std::mutex> g(tsd::g_tiles_mu);
   if (tile == 0)

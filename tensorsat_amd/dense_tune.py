@@ -3,8 +3,8 @@ Tile autotuner of the dense int8 MFMA route (``csrc/dense.hip``).
 
 A dense step is a batched product of 0/1 int8 matrices: 2^m rows held by lhs only, 2^n
 columns held by rhs only, 2^b shared batch entries and 2^k contracted entries (K innermost
-in both operands).  For each shape the tuner times the four workgroup tiles of the MFMA
-kernel (64x64, 128x64, 64x128, 128x128 rows x columns) on synthetic operands of that
+in both operands).  For each shape the tuner times the six workgroup tiles of the MFMA
+kernel (64x64, 128x64, 64x128, 128x128, 256x128, 128x256 rows x columns) on synthetic operands of that
 shape, through the ordinary ``ts_contract2`` entry point, and registers the fastest with
 ``ts_dense_tiles``; every later step of that shape (any operands, any caller) uses it.
 Tiles change the schedule only: counts are exact int32 sums either way, so results are
@@ -17,18 +17,18 @@ import ctypes
 
 from . import _native as N
 
-TILES = {1: (64, 64), 2: (128, 64), 3: (64, 128), 4: (128, 128)}
+TILES = {1: (64, 64), 2: (128, 64), 3: (64, 128), 4: (128, 128), 5: (256, 128), 6: (128, 256)}
 
 
 def tile_of(m: int, n: int, b: int, k: int) -> int:
-    """The tile (1..4) steps of this shape use now."""
+    """The tile (1..6) steps of this shape use now."""
     t = N.i32(0)
     N.check(N.lib().ts_dense_tile_of(m, n, b, k, ctypes.byref(t)), "ts_dense_tile_of")
     return int(t.value)
 
 
 def set_tile(m: int, n: int, b: int, k: int, tile: int) -> None:
-    """Register a tile (1..4) for the shape, or 0 for the built-in rule."""
+    """Register a tile (1..6) for the shape, or 0 for the built-in rule."""
     N.check(N.lib().ts_dense_tiles(m, n, b, k, int(tile)), "ts_dense_tiles")
 
 
@@ -59,7 +59,7 @@ def _step(m: int, n: int, b: int, k: int, seed: int = 0):
     return run, (lhs, rhs, out)
 
 
-def autotune_tiles(shapes, *, reps: int = 5, tiles=(1, 2, 3, 4)) -> dict:
+def autotune_tiles(shapes, *, reps: int = 5, tiles=tuple(TILES)) -> dict:
     """
     Time every tile on each (m, n, b, k) shape (median of ``reps`` launches after one warm
     launch, HIP events) and register the fastest.  Returns {shape: {"tile": t,

@@ -57,11 +57,11 @@ def test_dense_tile_table_host_side():
         assert L.ts_dense_tile_of(*shape, ctypes.byref(t)) == 0
         return t.value
 
-    assert tile_of(8, 8, 0, 8) == 4 and tile_of(5, 9, 0, 8) == 1  # built-in rule
+    assert tile_of(8, 8, 0, 8) == 4 and tile_of(5, 9, 0, 8) == 1 and tile_of(10, 12, 0, 8) == 6  # built-in rule
     assert L.ts_dense_tiles(8, 8, 0, 8, 2) == 0
     assert tile_of(8, 8, 0, 8) == 2
     assert L.ts_dense_tiles(8, 8, 0, 8, 0) == 0
     assert tile_of(8, 8, 0, 8) == 4
-    assert L.ts_dense_tiles(8, 8, 0, 8, 5) == N.TS_EINVAL
+    assert L.ts_dense_tiles(8, 8, 0, 8, 7) == N.TS_EINVAL
     assert L.ts_dense_tiles(-1, 8, 0, 8, 1) == N.TS_EINVAL
     assert L.ts_dense_tile_of(8, 8, 0, 28, None) == N.TS_EINVAL

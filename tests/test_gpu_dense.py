@@ -93,11 +93,12 @@ def test_dense_route_matches_valu_route(shape):
         assert 0 < int(mfma.sum()) < mfma.size
 
 
-@pytest.mark.parametrize("tile", [1, 2, 3, 4])
+@pytest.mark.parametrize("tile", [1, 2, 3, 4, 5, 6])
 @pytest.mark.parametrize("case", [1, 4, 6, 8])
 def test_every_tile_matches_oracle(case, tile):
-    """Each workgroup tile of the MFMA kernel (64x64, 128x64, 64x128, 128x128), forced for
-    the case's shape, gives the oracle's result (tiles larger than a side included)."""
+    """Each workgroup tile of the MFMA kernel (64x64, 128x64, 64x128, 128x128, 256x128,
+    128x256), forced for the case's shape, gives the oracle's result (tiles larger than a
+    side included)."""
     from tensorsat_amd import dense_tune as DT
 
     rng = np.random.default_rng(100 + case)
@@ -121,7 +122,7 @@ def test_tile_autotuner_registers_the_fastest():
         table = DT.autotune_tiles([shape, (3, 8, 0, 8)], reps=3)
         assert list(table) == [shape]  # the second shape is not on the MFMA route
         row = table[shape]
-        assert set(row["ms"]) == {1, 2, 3, 4} and row["tile"] == min(row["ms"], key=row["ms"].get)
+        assert set(row["ms"]) == set(DT.TILES) and row["tile"] == min(row["ms"], key=row["ms"].get)
         assert DT.tile_of(*shape) == row["tile"]
         print(f"tile table {row}", flush=True)
     finally:
