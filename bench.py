@@ -39,7 +39,8 @@ HBM_PEAK_GBS = 8000.0
 # other configurations report null.
 MEASURED_TRAFFIC = {
     ("batched", 6, 4096, 4): (44.4e6, "profiles/k_batch_wave_batched3sat_w6.md"),
-    ("dense", 13, 13, 13): (1.468e9, "profiles/k_step_mfma_dense_composition_13_13_13.md"),
+    ("dense", 13, 13, 13, 6): (1.083e9, "profiles/k_step_mfma_dense_composition_13_13_13.md"),
+    ("dense", 13, 13, 13, 4): (1.468e9, "profiles/k_step_mfma_dense_composition_13_13_13.md"),
 }
 
 
@@ -687,6 +688,7 @@ def bench_dense(args) -> int:
 
     tune = DT.autotune_tiles([(mb, nb, 0, kb)], reps=3).get((mb, nb, 0, kb))
     log(f"tile autotune: {tune}")
+    tile = DT.tile_of(mb, nb, 0, kb)
     kernel_ms, ms_per_step = timed(args.steps)
     res = out.cpu().numpy().reshape(1 << mb, 1 << nb)
     N.check(L.ts_dense_config(0, 6), "ts_dense_config")
@@ -742,9 +744,9 @@ def bench_dense(args) -> int:
                        "chosen": "x".join(map(str, DT.TILES[tune["tile"]])),
                        "ms": {"x".join(map(str, DT.TILES[t])): v for t, v in tune["ms"].items()}}},
         "roofline": {"bound": "mfma", "achieved": tops, "peak": peak, "unit": "TFLOP/s", "frac": tops / peak,
-                     "traffic": MEASURED_TRAFFIC.get(("dense", mb, nb, kb), (None, None))[0],
-                     "traffic_source": MEASURED_TRAFFIC.get(("dense", mb, nb, kb), (None, None))[1],
-                     "kernel": "k_step_mfma"},
+                     "traffic": MEASURED_TRAFFIC.get(("dense", mb, nb, kb, tile), (None, None))[0],
+                     "traffic_source": MEASURED_TRAFFIC.get(("dense", mb, nb, kb, tile), (None, None))[1],
+                     "kernel": "k_step_mfma<%d,%d>" % (DT.TILES[tile][0] // 32, DT.TILES[tile][1] // 32)},
         "cpu_baseline": cpu,
         "parity": {"sampled_rows": len(rows), "mismatches": mism, "valu_mismatches": valu_mism},
     }
