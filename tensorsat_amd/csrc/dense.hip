@@ -43,6 +43,12 @@ constexpr int LDSW = KC;  // LDS row (bytes): unpadded, 16-byte segments XOR-swi
 // rows of equal r mod 4 (64 banks) land on distinct 16-byte bank groups, and the loader's
 // writes (4 lanes per row, 16 consecutive rows per wave) cover whole 64-byte rows, so
 // neither side has bank conflicts.
+// One 16-byte direct load per lane from global memory into LDS (global_load_lds_dwordx4):
+// the wave's 64 x 16 bytes land contiguously from the wave-uniform LDS address `l`.
+__device__ __forceinline__ void load_lds16(const uint8_t* g, uint8_t* l) {
+  __builtin_amdgcn_global_load_lds(g, l, 16, 0, 0);
+}
+
 __device__ __forceinline__ int seg_swz(int r) { return ((r >> 1) ^ ((r & 15) >> 3)) & 3; }
 
 struct DenseArgs {
@@ -108,54 +114,47 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   const uint32_t bL = scatter(b, a.bbits, a.b_lpos), bR = scatter(b, a.bbits, a.b_rpos);
   const uint32_t bO = scatter(b, a.bbits, a.b_opos);
   __syncthreads();
-  // Loader: load j of thread t moves 16 bytes (segment t % 4) of row t / 4 + 64 j; rows
-  // 64 j apart share the swizzle, so its LDS segment is the same for every j.
-  const int ls = (tid & 3) * 16;
+  // Loader: direct global -> LDS loads (global_load_lds_dwordx4).  Load j of a wave fills
+  // 1 KiB of LDS, rows 64 j + 16 wave .. + 15: lane l lands at row + l / 4, physical
+  // segment l % 4, so it fetches the logical K segment the swizzle stores there (rows 64 j
+  // apart share the swizzle).  Rows past the operand's end read its first row instead:
+  // they only feed outputs the epilogue discards.
   const int lsw = ((tid & 3) ^ seg_swz(tid >> 2)) * 16;
   const uint8_t* pa[LA];
   const uint8_t* pb[LB];
-  bool oka[LA], okb[LB];
-  for (int j = 0; j < LA; ++j) {
-    const int r = (tid >> 2) + 64 * j;
-    oka[j] = m0 + r < mcount;
-    pa[j] = lhs + bL + rowL[r] + ls;
-  }
-  for (int j = 0; j < LB; ++j) {
-    const int r = (tid >> 2) + 64 * j;
-    okb[j] = n0 + r < ncount;
-    pb[j] = rhs + bR + colR[r] + ls;
-  }
-  i4 ra[LA], rb[LB];
-  auto gload = [&](uint64_t k0) {
-    for (int j = 0; j < LA; ++j) ra[j] = oka[j] ? *reinterpret_cast<const i4*>(pa[j] + k0) : (i4){0, 0, 0, 0};
-    for (int j = 0; j < LB; ++j) rb[j] = okb[j] ? *reinterpret_cast<const i4*>(pb[j] + k0) : (i4){0, 0, 0, 0};
-  };
-  auto lstore = [&](int buf) {
-    for (int j = 0; j < LA; ++j) *reinterpret_cast<i4*>(&sA[buf][(tid >> 2) + 64 * j][lsw]) = ra[j];
-    for (int j = 0; j < LB; ++j) *reinterpret_cast<i4*>(&sB[buf][(tid >> 2) + 64 * j][lsw]) = rb[j];
+  for (int j = 0; j < LA; ++j) pa[j] = lhs + bL + rowL[(tid >> 2) + 64 * j] + lsw;
+  for (int j = 0; j < LB; ++j) pb[j] = rhs + bR + colR[(tid >> 2) + 64 * j] + lsw;
+  auto dload = [&](int buf, uint64_t k0) {
+    for (int j = 0; j < LA; ++j)
+      load_lds16(pa[j] + k0, &sA[buf][64 * j + 16 * wave][0]);
+    for (int j = 0; j < LB; ++j)
+      load_lds16(pb[j] + k0, &sB[buf][64 * j + 16 * wave][0]);
   };
   const int wm = (wave >> 1) * 16 * TM, wn = (wave & 1) * 16 * TN;
   i4 acc[TM][TN];
   for (int i = 0; i < TM; ++i)
     for (int j = 0; j < TN; ++j) acc[i][j] = (i4){0, 0, 0, 0};
   const uint64_t nstages = (1ull << a.kbits) / KC;
-  gload(0);
-  lstore(0);
+  dload(0, 0);
+  __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   // Lane l feeds row / column l % 16 with 16 bytes of K at 16 * (l / 16); A and B use the
   // same K assignment, so the sum over K does not depend on the hardware's order.  Operand
   // rows are 16-aligned plus l % 16, so the swizzled segment is the same for every tile.
+  // Stage c reads its fragments from buffer c % 2, then issues chunk c + 1 into the other
+  // buffer (last read in stage c - 1, before its barrier) so that the loads fly during the
+  // MFMAs; the wait and the barrier close the stage.  The loads are issued after the
+  // fragment reads: LDS reads that follow an in-flight direct load wait for it.
   const int rr = lane & 15, kk = ((lane >> 4) ^ seg_swz(rr)) * 16;
   for (uint64_t c = 0; c < nstages; ++c) {
     const int buf = (int)(c & 1);
-    const bool more = c + 1 < nstages;
-    if (more) gload((c + 1) * KC);
     i4 fa[TM], fb[TN];
     for (int i = 0; i < TM; ++i) fa[i] = *reinterpret_cast<const i4*>(&sA[buf][wm + 16 * i + rr][kk]);
     for (int j = 0; j < TN; ++j) fb[j] = *reinterpret_cast<const i4*>(&sB[buf][wn + 16 * j + rr][kk]);
+    if (c + 1 < nstages) dload(buf ^ 1, (c + 1) * KC);
     for (int i = 0; i < TM; ++i)
       for (int j = 0; j < TN; ++j) acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[i], fb[j], acc[i][j], 0, 0, 0);
-    if (more) lstore(buf ^ 1);
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
   }
   // Accumulator layout: lane l holds rows 4 * (l / 16) + t, column l % 16.
