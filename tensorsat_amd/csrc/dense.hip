@@ -50,6 +50,17 @@ __device__ __forceinline__ void load_lds16(const uint8_t* g, uint8_t* l) {
   __builtin_amdgcn_global_load_lds(g, l, 16, 0, 0);
 }
 
+// Workgroup barrier between K stages without the memory fence of __syncthreads, which
+// would wait for every direct load in flight (including the next chunk's).  What the
+// stage needs is explicit: each wave has waited (s_waitcnt) for the loads of the chunk
+// read next and has consumed its own LDS reads (the MFMAs use them) before it arrives;
+// the empty asm keeps the compiler from moving LDS accesses across the barrier.
+__device__ __forceinline__ void stage_barrier() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
 __device__ __forceinline__ int seg_swz(int r) { return ((r >> 1) ^ ((r & 15) >> 3)) & 3; }
 
 struct DenseArgs {
@@ -79,8 +90,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                                                    uint8_t* __restrict__ out, DenseArgs a) {
   constexpr int BM = 32 * TM, BN = 32 * TN;
   constexpr int LA = BM * KC / 16 / 256, LB = BN * KC / 16 / 256;  // 16-byte loads per thread
-  __shared__ __attribute__((aligned(16))) uint8_t sA[2][BM][LDSW];
-  __shared__ __attribute__((aligned(16))) uint8_t sB[2][BN][LDSW];
+  // Three K buffers per operand, as separate LDS objects: a direct load into one buffer
+  // then does not hold up the reads of another (the compiler tracks them apart).
+  __shared__ __attribute__((aligned(16))) uint8_t sA0[BM][LDSW], sA1[BM][LDSW], sA2[BM][LDSW];
+  __shared__ __attribute__((aligned(16))) uint8_t sB0[BN][LDSW], sB1[BN][LDSW], sB2[BN][LDSW];
   __shared__ uint32_t rowL[BM], rowO[BM], colR[BN], colO[BN];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   // Tile order.  Consecutive workgroups land on different XCDs (w % 8), so first give
@@ -125,39 +138,78 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   const uint8_t* pb[LB];
   for (int j = 0; j < LA; ++j) pa[j] = lhs + bL + rowL[(tid >> 2) + 64 * j] + lsw;
   for (int j = 0; j < LB; ++j) pb[j] = rhs + bR + colR[(tid >> 2) + 64 * j] + lsw;
-  auto dload = [&](int buf, uint64_t k0) {
-    for (int j = 0; j < LA; ++j)
-      load_lds16(pa[j] + k0, &sA[buf][64 * j + 16 * wave][0]);
-    for (int j = 0; j < LB; ++j)
-      load_lds16(pb[j] + k0, &sB[buf][64 * j + 16 * wave][0]);
+  using Q0 = std::integral_constant<int, 0>;
+  using Q1 = std::integral_constant<int, 1>;
+  using Q2 = std::integral_constant<int, 2>;
+  auto bufA = [&](auto Q) -> uint8_t(*)[LDSW] {
+    if constexpr (decltype(Q)::value == 0) return sA0;
+    else if constexpr (decltype(Q)::value == 1) return sA1;
+    else return sA2;
   };
+  auto bufB = [&](auto Q) -> uint8_t(*)[LDSW] {
+    if constexpr (decltype(Q)::value == 0) return sB0;
+    else if constexpr (decltype(Q)::value == 1) return sB1;
+    else return sB2;
+  };
+  auto dload = [&](auto Q, uint64_t k0) {
+    uint8_t(*A)[LDSW] = bufA(Q);
+    uint8_t(*B)[LDSW] = bufB(Q);
+    for (int j = 0; j < LA; ++j) load_lds16(pa[j] + k0, &A[64 * j + 16 * wave][0]);
+    for (int j = 0; j < LB; ++j) load_lds16(pb[j] + k0, &B[64 * j + 16 * wave][0]);
+  };
+  // vmcnt(LA + LB), nothing else: every direct load but the LA + LB issued last has landed.
+  constexpr int kWaitAllButOneChunk = (LA + LB) | 0xF70;
+  static_assert(LA + LB <= 15, "vmcnt field");
   const int wm = (wave >> 1) * 16 * TM, wn = (wave & 1) * 16 * TN;
   i4 acc[TM][TN];
   for (int i = 0; i < TM; ++i)
     for (int j = 0; j < TN; ++j) acc[i][j] = (i4){0, 0, 0, 0};
   const uint64_t nstages = (1ull << a.kbits) / KC;
-  dload(0, 0);
-  __builtin_amdgcn_s_waitcnt(0);
-  __syncthreads();
+  dload(Q0{}, 0);
+  if (nstages > 1) {
+    dload(Q1{}, KC);
+    __builtin_amdgcn_s_waitcnt(kWaitAllButOneChunk);  // chunk 0 landed, chunk 1 in flight
+  } else {
+    __builtin_amdgcn_s_waitcnt(0);
+  }
+  stage_barrier();
   // Lane l feeds row / column l % 16 with 16 bytes of K at 16 * (l / 16); A and B use the
   // same K assignment, so the sum over K does not depend on the hardware's order.  Operand
   // rows are 16-aligned plus l % 16, so the swizzled segment is the same for every tile.
-  // Stage c reads its fragments from buffer c % 2, then issues chunk c + 1 into the other
-  // buffer (last read in stage c - 1, before its barrier) so that the loads fly during the
-  // MFMAs; the wait and the barrier close the stage.  The loads are issued after the
-  // fragment reads: LDS reads that follow an in-flight direct load wait for it.
+  // Stage c (buffer q = c % 3) reads its fragments, then issues chunk c + 2 into buffer
+  // (c + 2) % 3 (last read in stage c - 1, before its barrier), so every chunk has two
+  // stages of MFMAs to land; the stage ends once chunk c + 1 has landed (a wait that leaves
+  // chunk c + 2 in flight) and every wave is past its reads.  The loads are issued after the
+  // fragment reads: LDS reads behind an in-flight direct load into the same buffer wait.
   const int rr = lane & 15, kk = ((lane >> 4) ^ seg_swz(rr)) * 16;
-  for (uint64_t c = 0; c < nstages; ++c) {
-    const int buf = (int)(c & 1);
+  // Steady: the stage is known to load ahead (no branch, so the compiler's own wait
+  // tracking stays exact across the loop); otherwise it checks.
+  auto stage = [&](auto Q, uint64_t c, auto Steady) {
+    constexpr int q = decltype(Q)::value;
+    uint8_t(*A)[LDSW] = bufA(Q);
+    uint8_t(*B)[LDSW] = bufB(Q);
     i4 fa[TM], fb[TN];
-    for (int i = 0; i < TM; ++i) fa[i] = *reinterpret_cast<const i4*>(&sA[buf][wm + 16 * i + rr][kk]);
-    for (int j = 0; j < TN; ++j) fb[j] = *reinterpret_cast<const i4*>(&sB[buf][wn + 16 * j + rr][kk]);
-    if (c + 1 < nstages) dload(buf ^ 1, (c + 1) * KC);
+    for (int i = 0; i < TM; ++i) fa[i] = *reinterpret_cast<const i4*>(&A[wm + 16 * i + rr][kk]);
+    for (int j = 0; j < TN; ++j) fb[j] = *reinterpret_cast<const i4*>(&B[wn + 16 * j + rr][kk]);
+    const bool ahead = decltype(Steady)::value || c + 2 < nstages;
+    if (ahead) dload(std::integral_constant<int, (q + 2) % 3>{}, (c + 2) * KC);
     for (int i = 0; i < TM; ++i)
       for (int j = 0; j < TN; ++j) acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[i], fb[j], acc[i][j], 0, 0, 0);
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
+    if (ahead) __builtin_amdgcn_s_waitcnt(kWaitAllButOneChunk);
+    else __builtin_amdgcn_s_waitcnt(0);
+    stage_barrier();
+  };
+  // Steady state without branches (every stage loads ahead), then at most four tail stages.
+  uint64_t c = 0;
+  for (; c + 4 < nstages; c += 3) {
+    stage(Q0{}, c, std::true_type{});
+    stage(Q1{}, c + 1, std::true_type{});
+    stage(Q2{}, c + 2, std::true_type{});
   }
+  if (c < nstages) stage(Q0{}, c, std::false_type{});
+  if (c + 1 < nstages) stage(Q1{}, c + 1, std::false_type{});
+  if (c + 2 < nstages) stage(Q2{}, c + 2, std::false_type{});
+  if (c + 3 < nstages) stage(Q0{}, c + 3, std::false_type{});
   // Accumulator layout: lane l holds rows 4 * (l / 16) + t, column l % 16.
   for (int i = 0; i < TM; ++i)
     for (int j = 0; j < TN; ++j)
