@@ -12,9 +12,9 @@
 // Each workgroup computes a (32 TM) x (32 TN) tile of (m, n) for one batch index with 4
 // waves of TM x TN v_mfma_i32_16x16x64_i8 (TM, TN in {2, 4, 8}: 64 x 64 up to 256 x 128
 // and 128 x 256; larger wave tiles read fewer LDS bytes per MFMA); K is staged
-// through double-buffered LDS in 64-byte chunks loaded straight into LDS
-// (global_load_lds_dwordx4, four lanes per 64-byte row, XOR-swizzled segments), the next
-// chunk's loads in flight during the current MFMAs.  Row/column offsets are scattered by
+// through three LDS buffers in 64-byte chunks loaded straight into LDS
+// (global_load_lds_dwordx4, four lanes per 64-byte row, XOR-swizzled segments), each
+// chunk's loads issued two stages before its MFMAs.  Row/column offsets are scattered by
 // bit position (any interleaving of m, n and b bits in the operands and the output), so
 // they are computed once per tile into LDS.  The epilogue is the step's: count > 0 (GT0) or
 // (count mod 256) != 0 (MOD256); counts are exact because sum_rank <= 27 < 31.
@@ -80,9 +80,9 @@ __device__ __forceinline__ uint32_t scatter(uint64_t idx, int nb, const uint8_t*
 }
 
 // Workgroup tile BM x BN = (32 TM) x (32 TN): 2 x 2 waves, each TM x TN MFMA tiles of
-// 16 x 16.  K advances 64 bytes per stage through two LDS buffers: the next stage's
-// direct loads are issued before the current stage's MFMAs, and one wait plus barrier
-// per stage separates them from the reads of the next stage.
+// 16 x 16.  K advances 64 bytes per stage through three LDS buffers: stage c's MFMAs run
+// while chunks c + 1 and c + 2 land, and one partial wait plus barrier per stage hands
+// chunk c + 1 to every wave.
 // At least two waves per SIMD: the 256-entry tiles hold 128 accumulator registers per lane
 // and would otherwise be register-allocated for one.
 template <int TM, int TN>
