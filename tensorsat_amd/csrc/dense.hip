@@ -199,17 +199,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     else __builtin_amdgcn_s_waitcnt(0);
     stage_barrier();
   };
-  // Steady state without branches (every stage loads ahead), then at most four tail stages.
+  // Steady state without branches (every stage loads ahead), six stages per trip (the
+  // compiler drains the loads in flight once per trip, at the loop head), then at most
+  // seven tail stages.
   uint64_t c = 0;
-  for (; c + 4 < nstages; c += 3) {
+  for (; c + 7 < nstages; c += 6) {
     stage(Q0{}, c, std::true_type{});
     stage(Q1{}, c + 1, std::true_type{});
     stage(Q2{}, c + 2, std::true_type{});
+    stage(Q0{}, c + 3, std::true_type{});
+    stage(Q1{}, c + 4, std::true_type{});
+    stage(Q2{}, c + 5, std::true_type{});
   }
   if (c < nstages) stage(Q0{}, c, std::false_type{});
   if (c + 1 < nstages) stage(Q1{}, c + 1, std::false_type{});
   if (c + 2 < nstages) stage(Q2{}, c + 2, std::false_type{});
   if (c + 3 < nstages) stage(Q0{}, c + 3, std::false_type{});
+  if (c + 4 < nstages) stage(Q1{}, c + 4, std::false_type{});
+  if (c + 5 < nstages) stage(Q2{}, c + 5, std::false_type{});
+  if (c + 6 < nstages) stage(Q0{}, c + 6, std::false_type{});
   // Accumulator layout: lane l holds rows 4 * (l / 16) + t, column l % 16.
   for (int i = 0; i < TM; ++i)
     for (int j = 0; j < TN; ++j)
