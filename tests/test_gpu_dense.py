@@ -60,6 +60,7 @@ CASES = [
     (6, 7, 8, 1, 0.5),   # 128 x 128 workgroup tiles
     (7, 8, 7, 0, 0.3),
     (8, 7, 7, 0, 1.0),   # 128 x 128 tiles, every count 256
+    (12, 7, 8, 0, 0.05),  # 64 K stages: several unrolled loop trips, then the tail
 ]
 
 
@@ -94,7 +95,7 @@ def test_dense_route_matches_valu_route(shape):
 
 
 @pytest.mark.parametrize("tile", [1, 2, 3, 4, 5, 6])
-@pytest.mark.parametrize("case", [1, 4, 6, 8])
+@pytest.mark.parametrize("case", [1, 4, 6, 8, 11])
 def test_every_tile_matches_oracle(case, tile):
     """Each workgroup tile of the MFMA kernel (64x64, 128x64, 64x128, 128x128, 256x128,
     128x256), forced for the case's shape, gives the oracle's result (tiles larger than a
