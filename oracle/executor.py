@@ -25,18 +25,33 @@ from math import prod
 
 import numpy as np
 
-from tensorsat_amd.program import NOPOS, STEP_NORHS, Program, step_labels
+from . import finrel, records
 
-from . import finrel
+# Decoded records of the programs seen last (keyed by the identity of their buffers).
+_decoded: dict = {}
 
 
-def _step(arena, st, mode: str):
-    lab_l, lab_r, lab_o = step_labels(st)
-    rl, rr, ro = int(st["lhs_rank"]), int(st["rhs_rank"]), int(st["out_rank"])
-    lo, ro_off, oo = int(st["lhs_off"]), int(st["rhs_off"]), int(st["out_off"])
+def _records(prog):
+    """(jobs, steps) of a program, decoded from their raw bytes by :mod:`oracle.records`
+    (the checker does not use the product's decoder)."""
+    key = id(prog)
+    hit = _decoded.get(key)
+    if hit is not None and hit[0] is prog.steps and hit[1] is prog.jobs:
+        return hit[2], hit[3]
+    jobs = records.decode_jobs(prog.jobs)
+    steps = records.decode_steps(prog.steps)
+    if len(_decoded) > 8:
+        _decoded.clear()
+    _decoded[key] = (prog.steps, prog.jobs, jobs, steps)
+    return jobs, steps
+
+
+def _step(arena, st: records.Step, mode: str):
+    lab_l, lab_r, lab_o = list(st.lhs_labels), list(st.rhs_labels), list(st.out_labels)
+    rl, rr, ro = st.lhs_rank, st.rhs_rank, st.out_rank
+    lo, ro_off, oo = st.lhs_off, st.rhs_off, st.out_off
     lhs = arena[lo : lo + (1 << rl)].reshape((2,) * rl)
-    norhs = bool(int(st["flags"]) & STEP_NORHS)
-    if norhs:
+    if st.norhs:
         rhs = np.ones((), dtype=arena.dtype)
         lab_r = []
     else:
@@ -50,7 +65,7 @@ def _step(arena, st, mode: str):
         lt, rt = finrel.safe_contraction_dtype(csize, lhs, rhs)
         res = np.sign(np.einsum(lt, lab_l, rt, lab_r, lab_o), dtype=np.uint8)
     elif mode == "COUNT":
-        nsum = int(st["sum_rank"])
+        nsum = st.sum_rank
         lmax = int(lhs.max()) if lhs.size else 0
         rmax = int(rhs.max()) if rhs.size else 0
         if lmax * rmax * (1 << nsum) < 2**64:
@@ -66,38 +81,41 @@ def _step(arena, st, mode: str):
     return arena[oo : oo + (1 << ro)]
 
 
-def run_job(prog: Program, j: int, mode: str = "GT0", *, keep_intermediates: bool = False):
-    """Run job ``j``; returns (result array, list of step outputs if requested)."""
-    job = prog.jobs[j]
+def run_job(prog, j: int, mode: str = "GT0", *, keep_intermediates: bool = False):
+    """Run job ``j`` of a compiled program (its ``jobs`` / ``steps`` record arrays and
+    ``leaves`` bytes); returns (result array, list of step outputs if requested)."""
+    jobs, steps = _records(prog)
+    job = jobs[j]
     dtype = np.uint64 if mode == "COUNT" else np.uint8
-    arena = np.zeros(int(job["arena_elems"]), dtype=dtype)
-    leaves = prog.job_leaves(j)
+    arena = np.zeros(job.arena_elems, dtype=dtype)
+    leaves = np.asarray(prog.leaves)[job.leaf_begin : job.leaf_begin + job.num_leaf_bytes]
     arena[: leaves.size] = leaves
     inter = []
-    for st in prog.job_steps(j):
+    for st in steps[job.step_begin : job.step_begin + job.num_steps]:
         out = _step(arena, st, mode)
         if keep_intermediates:
             inter.append(out.copy())
-    r0, rr = int(job["result_off"]), int(job["result_rank"])
+    r0, rr = job.result_off, job.result_rank
     return arena[r0 : r0 + (1 << rr)].copy(), inter
 
 
-def decide(prog: Program, mode: str = "GT0") -> np.ndarray:
+def decide(prog, mode: str = "GT0") -> np.ndarray:
     """Per-instance verdicts: OR over each instance's job results (scalars)."""
+    jobs, _ = _records(prog)
     out = np.zeros(prog.num_instances, dtype=np.uint8)
-    for j in range(len(prog.jobs)):
-        inst = int(prog.jobs["instance"][j])
-        if out[inst]:
+    for j, job in enumerate(jobs):
+        if out[job.instance]:
             continue
         res, _ = run_job(prog, j, mode)
-        out[inst] |= np.uint8(res.reshape(-1)[0] != 0)
+        out[job.instance] |= np.uint8(res.reshape(-1)[0] != 0)
     return out
 
 
-def count(prog: Program) -> list[int]:
+def count(prog) -> list[int]:
     """Per-instance exact model counts: sum over jobs of result * 2^free_vars."""
+    jobs, _ = _records(prog)
     out = [0] * prog.num_instances
-    for j in range(len(prog.jobs)):
+    for j, job in enumerate(jobs):
         res, _ = run_job(prog, j, "COUNT")
-        out[int(prog.jobs["instance"][j])] += int(res.reshape(-1)[0]) << int(prog.jobs["free_vars"][j])
+        out[job.instance] += int(res.reshape(-1)[0]) << job.free_vars
     return out

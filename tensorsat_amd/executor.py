@@ -177,6 +177,24 @@ rounds but each round then waits for its longest job (measured slower on the bat
 workload: 0.61 vs 0.41 ms per 4096 instances at W=10)."""
 
 
+def lds_fits(prog: Program) -> tuple[bool, str]:
+    """Whether the LDS-arena variants of the batched kernel can run ``prog``: every job
+    arena within ``BATCH_LDS_MAX`` bytes, and every step wider than one wave (more than
+    6 out bits, which runs on the 4-wave table engine) summing at most 16 bits -- that
+    engine keeps two 8-bit chunk tables per operand in LDS, so higher summed bits would
+    never be visited.  Returns (fits, reason if not)."""
+    nj = len(prog.jobs)
+    need = (int(prog.jobs["arena_elems"].max()) if nj else 16) + 255 & ~255
+    if need > N.BATCH_LDS_MAX:
+        return False, f"job arenas of {need} bytes exceed the LDS limit"
+    st = prog.steps
+    if nj and len(st):
+        wide = st["out_rank"] > 6
+        if np.any(wide) and int(st["sum_rank"][wide].max()) > 16:
+            return False, "a step with more than 6 out bits sums more than 16 bits (LDS table engine limit)"
+    return True, ""
+
+
 class BatchExecutor:
     """
     All jobs of a program in a few persistent launches (``ts_batch_run``): one round per
@@ -203,10 +221,10 @@ class BatchExecutor:
         def stride(p: Program) -> int:
             return (int(p.jobs["arena_elems"].max()) if nj else 16) + 255 & ~255
 
-        fits = stride(prog) <= N.BATCH_LDS_MAX
+        fits, why = lds_fits(prog)
         self.lds = fits if lds is None else bool(lds)
         if self.lds and not fits:
-            raise ValueError(f"job arenas of {stride(prog)} bytes exceed the LDS limit")
+            raise ValueError(why)
         # Every tensor of at most 6 bits: one-wave workgroups (every step <= 64 entries).
         st = prog.steps
         small = bool(nj) and self.lds and (len(st) == 0 or int(st["out_rank"].max()) <= 6)

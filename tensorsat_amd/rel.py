@@ -24,7 +24,10 @@ reference          this class                      kernel
 
 The epilogue mode is ``"GT0"`` (``count > 0``, the relational meaning) by default;
 ``with DeviceRel.mode_scope("MOD256"):`` reproduces the reference's numpy arithmetic,
-which keeps ``(count mod 256) != 0`` (``fin_rel.py:279,287,392``).
+which keeps ``(count mod 256) != 0`` (``fin_rel.py:279,287,392``), including the axis
+order of ``rewire_array`` (``_utils/misc.py:104-123``) when ports are discarded or
+repeated: its slices are added without a transpose, so the kept ports that occur once
+land on the output positions in increasing port order (:meth:`DeviceRel._rewire`).
 """
 
 from __future__ import annotations
@@ -346,6 +349,10 @@ class DeviceRel:
         for w in out_wires:
             if w not in uniq_out:
                 uniq_out.append(w)
+        if len(uniq_out) != len(out_wires) and DeviceRel._mode == "MOD256":
+            # fin_rel.py:281-287: the einsum keeps sorted(set(out_wires)) and rewire_array
+            # places the repeated wires relative to that order.
+            uniq_out = sorted(uniq_out)
         ids: dict = {}
         bl = cls._bitwires(lw, lhs._shape, ids)
         br = cls._bitwires(rw, rhs._shape, ids)
@@ -457,21 +464,52 @@ class DeviceRel:
         return self._rewire(out_ports)
 
     def _rewire(self, out_ports: Sequence[int]) -> "DeviceRel":
-        """``fin_rel.py:379-393``: permute / duplicate / discard (existential sum)."""
+        """``fin_rel.py:379-393``: permute / duplicate / discard (existential sum).
+
+        GT0 computes the intended map.  MOD256 follows the reference literally: outside
+        a pure permutation, ``rewire_array`` (``_utils/misc.py:104-123``) adds slices of
+        the input whose free axes are in increasing port order into output slices whose
+        free axes are in out-port order, so the ports that occur once in ``out_ports``
+        are taken in increasing order (:meth:`_literal_ports`)."""
         out_ports = [int(p) for p in out_ports]
+        out_shape = tuple(self._shape[p] for p in out_ports)
+        ports = out_ports
+        if DeviceRel._mode == "MOD256" and not (len(set(out_ports)) == len(out_ports) == len(self._shape)):
+            ports = self._literal_ports(out_ports)
         start = []
         acc = 0
         for d in self._shape:
             start.append(acc)
             acc += _bits(d)
-        bit_ports = [start[p] + k for p in out_ports for k in range(_bits(self._shape[p]))]
-        out_shape = tuple(self._shape[p] for p in out_ports)
+        bit_ports = [start[p] + k for p in ports for k in range(_bits(self._shape[p]))]
         if len(bit_ports) > MAX_DEVICE_BITS:
             raise NotImplementedError("rewire exceeds 2^27 device entries")
         res = _empty(1 << len(bit_ports))
         N.check(N.lib().ts_rewire(N.dptr(self._data), self.device_bits, N.dptr(res), len(bit_ports),
                                   N.i32arr(bit_ports), N.MODES[DeviceRel._mode], N.stream_ptr()), "ts_rewire")
         return DeviceRel._new(res, out_shape)
+
+    def _literal_ports(self, out_ports: list[int]) -> list[int]:
+        """The ports ``rewire_array`` actually reads for each output position: the
+        ports occurring once in ``out_ports`` are reassigned, position by position, in
+        increasing order; repeated ports stay (diagonals).  numpy adds a slice of
+        extents ``shape[sorted port]`` into one of extents ``shape[out port]``, so
+        unequal extents fail as they do there (extent 1 would broadcast: not
+        supported)."""
+        counts: dict[int, int] = {}
+        for p in out_ports:
+            counts[p] = counts.get(p, 0) + 1
+        single = [q for q, p in enumerate(out_ports) if counts[p] == 1]
+        ports = list(out_ports)
+        for q, s in zip(single, sorted(out_ports[q] for q in single)):
+            want, have = self._shape[out_ports[q]], self._shape[s]
+            if have != want:
+                if have == 1:
+                    raise NotImplementedError("rewire_array would broadcast an extent-1 port")
+                raise ValueError(f"operands could not be broadcast together: port {s} of extent {have} "
+                                 f"lands on a port of extent {want}")
+            ports[q] = s
+        return ports
 
     def __getitem__(self, idxs) -> "DeviceRel":
         """``Box.__getitem__`` (``diagrams.py:972-996``)."""

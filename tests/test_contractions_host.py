@@ -126,3 +126,31 @@ def test_simple_contraction_args_keep_common_wires():
     assert res2 == (3, 0)
     assert c.contract(Diagram(w, (G.bit_unk.reshape(2, 1).repeat(2, 1), np.eye(2, dtype=np.uint8),
                                   np.ones((2, 2), np.uint8)))).tensor.shape == (2, 2)
+
+
+def test_literal_rewire_ports_match_rewire_array():
+    """DeviceRel's MOD256 port rule (kept single ports in increasing order, repeats on
+    the diagonal) reproduces the oracle's literal rewire_array on random unsorted,
+    discarded and repeated ports: rewire_array(t, ports) == intended(t, literal ports)."""
+    from types import SimpleNamespace
+
+    from oracle import finrel as F
+    from tensorsat_amd.rel import DeviceRel
+
+    rng = np.random.default_rng(17)
+    checked = 0
+    for case in range(300):
+        r = int(rng.integers(1, 6))
+        t = (rng.random((2,) * r) < 0.5).astype(np.uint8)
+        ports = [int(p) for p in rng.integers(0, r, size=int(rng.integers(0, r + 3)))]
+        if len(set(ports)) == len(ports) == r:
+            continue  # pure permutation: a transpose in both
+        lit = DeviceRel._literal_ports(SimpleNamespace(_shape=(2,) * r), ports)
+        want = F.rewire(t, ports, "MOD256")
+        got = ((F.rewire_exact(t, lit) % 256) != 0).astype(np.uint8)
+        np.testing.assert_array_equal(got, want, err_msg=f"case {case} ports {ports}")
+        checked += lit != ports
+    assert checked > 20  # the rule moved ports in many cases
+    # unequal extents fail as numpy's broadcast does in the reference
+    with pytest.raises(ValueError):
+        DeviceRel._literal_ports(SimpleNamespace(_shape=(2, 3, 2)), [1, 0])

@@ -89,10 +89,40 @@ def test_rewire_vs_oracle(DR):
         k = int(rng.integers(0, r + 3))
         ports = [int(p) for p in rng.integers(0, r, size=k)]
         np.testing.assert_array_equal(DR(t).rewire(ports).tensor, F.rewire(t, ports, "GT0"), err_msg=str(ports))
-        # increasing kept ports: the reference's rewire_array gives the same tensor
-        inc = sorted(set(ports))
+        # MOD256 follows the reference's rewire_array literally, unsorted and repeated
+        # ports included (its slices are added without a transpose)
         with DR.mode_scope("MOD256"):
-            np.testing.assert_array_equal(DR(t).rewire(inc).tensor, F.rewire(t, inc, "MOD256"))
+            np.testing.assert_array_equal(DR(t).rewire(ports).tensor, F.rewire(t, ports, "MOD256"),
+                                          err_msg=f"MOD256 {ports}")
+
+
+def test_rewire_mod256_unsorted_ports_wide(DR):
+    """Literal MOD256 rewire on tensors wide enough for the device's ∃-reduction kernels
+    (discarding >= 6 axes) with kept ports in decreasing order."""
+    rng = np.random.default_rng(23)
+    for case in range(12):
+        r = int(rng.integers(8, 13))
+        t = (rng.random((2,) * r) < 0.02).astype(np.uint8)
+        kept = [int(p) for p in rng.choice(r, size=int(rng.integers(1, 4)), replace=False)]
+        ports = sorted(kept, reverse=True) + ([kept[0]] if case % 3 == 0 else [])
+        with DR.mode_scope("MOD256"):
+            got = DR(t).rewire(ports).tensor
+        np.testing.assert_array_equal(got, F.rewire(t, ports, "MOD256"), err_msg=f"case {case} {ports}")
+
+
+def test_contract2_repeated_out_wires_mod256(DR):
+    """fin_rel.py:281-287: repeated out wires go through rewire_array relative to the
+    sorted einsum output, in MOD256 reproduced literally."""
+    rng = np.random.default_rng(29)
+    for case in range(40):
+        lhs, lw, rhs, rw, ow = _rand_case(rng, max_wires=7, max_rank=5)
+        if not ow:
+            continue
+        ow = list(ow) + [ow[int(rng.integers(0, len(ow)))]]
+        rng.shuffle(ow)
+        with DR.mode_scope("MOD256"):
+            got = DR._contract2(DR(lhs), lw, DR(rhs), rw, ow).tensor
+        np.testing.assert_array_equal(got, F.contract2(lhs, lw, rhs, rw, ow, "MOD256"), err_msg=f"case {case} {ow}")
 
 
 def test_notebook_box_goldens(DR):

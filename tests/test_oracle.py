@@ -185,3 +185,38 @@ def test_full_adder_relation_by_oracle():
         wires = out
     res = F.rewire(t, [wires.index(w) for w in fa["out_wires"]], "GT0")
     assert sorted(map(tuple, np.argwhere(res).tolist())) == [tuple(x) for x in GOLD["full_adder_relation"]["set"]]
+
+
+def test_oracle_record_decoder_matches_product_decoder():
+    """The oracle decodes step and job records from the documented byte layout with its
+    own code; on real programs (both encodings' cube jobs) it agrees field by field with
+    the product's decoder, so neither can misread a field silently."""
+    from oracle import records
+    from tensorsat_amd import program as PG
+    from tensorsat_amd.workloads import batched_3sat
+
+    prog = PG.sat_batch(batched_3sat(24), width=8, max_cubes=3)
+    steps = records.decode_steps(prog.steps)
+    jobs = records.decode_jobs(prog.jobs)
+    assert len(steps) == len(prog.steps) > 100 and len(jobs) == len(prog.jobs)
+    for st, rec in zip(steps, prog.steps):
+        lab_l, lab_r, lab_o = PG.step_labels(rec)
+        assert (st.lhs_off, st.rhs_off, st.out_off) == (int(rec["lhs_off"]), int(rec["rhs_off"]), int(rec["out_off"]))
+        assert (st.lhs_rank, st.rhs_rank, st.out_rank, st.sum_rank) == tuple(
+            int(rec[k]) for k in ("lhs_rank", "rhs_rank", "out_rank", "sum_rank"))
+        assert list(st.lhs_labels) == lab_l and list(st.out_labels) == lab_o
+        assert st.norhs == bool(int(rec["flags"]) & PG.STEP_NORHS)
+        if not st.norhs:
+            assert list(st.rhs_labels) == lab_r
+    for jb, rec in zip(jobs, prog.jobs):
+        for f in ("instance", "cube", "step_begin", "num_steps", "leaf_begin", "num_leaf_bytes", "result_off",
+                  "result_rank", "free_vars", "arena_elems"):
+            assert getattr(jb, f) == int(rec[f]), f
+    # a corrupted position byte is caught, not decoded into a wrong einsum
+    bad = prog.steps.copy()
+    k = int(np.flatnonzero(bad["lhs_rank"] >= 2)[0])
+    bad["o_lpos"][k, 0] = bad["lhs_rank"][k] + 3 if bad["out_rank"][k] else bad["o_lpos"][k, 0]
+    bad["c_lpos"][k, 0] = bad["lhs_rank"][k] + 3 if bad["sum_rank"][k] and not bad["out_rank"][k] else bad["c_lpos"][k, 0]
+    if bad["out_rank"][k] or bad["sum_rank"][k]:
+        with pytest.raises(ValueError):
+            records.decode_steps(bad[k : k + 1])

@@ -172,3 +172,24 @@ def test_lane_form_rejects_wide_sums():
     prog, _ = wide_sum_program(0, (4,), n=8)  # 7 summed bits
     with pytest.raises(ValueError):
         PG.lanes(PG.packetize(prog))
+
+
+def test_lds_variant_rejects_wide_summed_steps():
+    """The LDS table engine covers 16 summed bits: a step with more than 6 out bits and
+    more than 16 summed bits must not run there (it goes to device arenas)."""
+    from tensorsat_amd.executor import lds_fits
+
+    jobs = np.zeros(1, dtype=PG.JOB_DTYPE)
+    jobs["num_steps"] = 1
+    jobs["arena_elems"] = 4096
+    steps = np.zeros(1, dtype=PG.STEP_DTYPE)
+    steps["out_rank"], steps["sum_rank"] = 7, 16
+    prog = PG.Program(jobs, steps, np.zeros(16, np.uint8), np.zeros((1, PG.NSTAT), np.int32))
+    assert lds_fits(prog) == (True, "")
+    steps["sum_rank"] = 17
+    ok, why = lds_fits(prog)
+    assert not ok and "16 bits" in why
+    steps["out_rank"] = 6  # one-wave steps are not built on the tables
+    assert lds_fits(prog)[0]
+    jobs["arena_elems"] = 70000
+    assert not lds_fits(prog)[0]
