@@ -195,6 +195,78 @@ int ts_batch_max_workgroups(int32_t* out);
  * variant, 2 the lds_arena = 3 (packet form) one, 3 the lds_arena = 4 (lane form) one. */
 int ts_batch_max_workgroups_lds(int64_t arena_bytes, int32_t one_wave, int32_t* out);
 
+/* ---- Device cube search (batched decisions without host planning) ---------------
+ *
+ * The batched counterpart of `bool(CotengraContraction(FinRel, wiring).contract(d))`
+ * (examples/test_sat_solver.py:5-9, contractions/cotengra.py:100-128) for CNF networks,
+ * with the conditioned slicing done on the device: a job is a depth-first branch over
+ * the variable wires of one instance, each branch fixing a wire in every clause tensor
+ * that holds it (Box.__getitem__, diagrams.py:972-996 -> FinRel._getitem,
+ * fin_rel.py:342-343) and the branch results OR-ed (Box.__add__ -> FinRel._union,
+ * fin_rel.py:345-346); all-false clause tensors prune a branch and one-hot ones fix
+ * their variable (unit propagation); a residual network of at most `width` (<= 6) free
+ * variables is contracted into one rank-|R| relation (FinRel._contract2 with every wire
+ * kept, fin_rel.py:255-288) and reduced over all its ports (FinRel._rewire to no ports,
+ * fin_rel.py:379-393; at most 64 terms meet, so GT0 and MOD256 agree).  One 64-lane
+ * wave runs one job.  Search rules (branch variable, polarity, statistics) are those
+ * restated by the oracle (oracle/search.c). */
+
+#define TS_SEARCH_MAX_VARS 128  /* variables per instance */
+#define TS_SEARCH_SLOTS 12      /* clauses per lane: at most 64 * 12 = 768 clauses */
+#define TS_SEARCH_MAX_WIDTH 6   /* residuals of at most 2^6 entries are contracted */
+#define TS_SEARCH_MAX_SPLIT 16  /* branch levels a job's prefix can fix */
+/* Packed clause: bits 8i..8i+7 hold literal i (i < 3) as (variable << 1) | negated,
+ * bits 24..31 the number of literals (0..3). */
+#define TS_SEARCH_CLAUSE(len) ((uint32_t)(len) << 24)
+
+typedef struct {
+  int32_t clause_begin;   /* first packed clause of the instance */
+  int32_t num_clauses;    /* <= 64 * TS_SEARCH_SLOTS */
+  int32_t num_vars;       /* <= TS_SEARCH_MAX_VARS */
+  int32_t reserved;       /* 0 */
+} ts_search_inst;
+
+typedef struct {
+  int32_t instance;
+  int32_t split;          /* branch levels fixed by prefix (0..TS_SEARCH_MAX_SPLIT) */
+  uint32_t prefix;        /* bit k: child taken at depth k < split (0 = first literal) */
+  int32_t reserved;       /* 0 */
+} ts_search_job;
+
+typedef struct {
+  int32_t status;         /* 0 complete, 1 stopped (instance decided by another job),
+                             2 node budget exhausted */
+  int32_t max_depth;
+  int64_t nodes, leaves, conflicts; /* counted where the job owns the node (a node above
+                             its split depth belongs to the job whose remaining prefix bits
+                             are all 0), so the sums over an instance's jobs equal one
+                             unsplit search's in exhaustive mode */
+  uint64_t digest;        /* sum of mix(relation bits, rank) over owned leaves */
+} ts_search_stat;
+
+#define TS_SEARCH_SAT 1u          /* verdict bit: a satisfiable leaf was found */
+#define TS_SEARCH_INCOMPLETE 2u   /* verdict bit: a job ran out of its node budget */
+
+typedef struct {
+  const uint32_t* clauses;         /* device, packed clauses */
+  const ts_search_inst* instances; /* device */
+  int32_t num_instances;
+  const ts_search_job* jobs;       /* device, in launch order (one wave each) */
+  int32_t num_jobs;
+  uint32_t* verdicts;              /* device, per instance, zeroed by the caller */
+  ts_search_stat* stats;           /* device, per job (optional) */
+  int32_t width;                   /* 0..TS_SEARCH_MAX_WIDTH */
+  int32_t exhaustive;              /* 1: visit every branch (statistics) */
+  int32_t early_exit;              /* 1: stop a job once its instance is decided SAT */
+  int32_t reserved;
+  int64_t max_nodes;               /* per job */
+} ts_search;
+
+/* One launch of num_jobs one-wave workgroups.  Rejects (TS_EINVAL) null arrays, widths
+ * or splits out of range; instance sizes are the caller's to check (the host layer
+ * validates them before upload). */
+int ts_search_run(const ts_search* s, ts_stream stream);
+
 /* ---- Learned planner: message-passing network kernels (bf16 in, fp32 accumulate) --
  * No counterpart in the reference: the learned planner is an addition of this build.
  * Matrices are row-major; bf16 buffers are passed as void*. */

@@ -25,7 +25,7 @@ PLAN_LIB = PKG / "libtsplan.so"
 HIP_LIB = PKG / "libtensorsat_amd.so"
 
 PLAN_SOURCES = ["plan.cpp", "prog.cpp", "cubes.cpp", "packets.cpp"]
-HIP_SOURCES = ["executor.hip", "dense.hip", "batch.hip", "gnn.hip"]
+HIP_SOURCES = ["executor.hip", "dense.hip", "batch.hip", "gnn.hip", "search.hip"]
 
 HIPCC_FLAGS = [
     "-O3",

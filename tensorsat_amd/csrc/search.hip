@@ -1,0 +1,375 @@
+// Device cube search (gfx950): batched SAT decisions with the conditioned slicing done on
+// the device, one 64-lane wave per job.
+//
+// A job walks the depth-first branch tree of one instance below a fixed prefix of its
+// first `split` branches.  Branching on a variable wire fixes it in every clause tensor
+// that holds it (the reference's Box.__getitem__, tensorsat/diagrams.py:972-996 ->
+// FinRel._getitem, lang/fin_rel.py:342-343); branch results are OR-ed (Box.__add__ ->
+// FinRel._union, fin_rel.py:345-346).  A clause tensor whose literals are all false is
+// all zero and prunes the branch; one with a single free literal is one-hot and fixes
+// that variable (unit propagation).  A residual network with at most `width` (<= 6) free
+// variables R is contracted: every residual clause tensor is multiplied into one
+// rank-|R| relation over R (FinRel._contract2 keeping every wire, fin_rel.py:255-288),
+// one lane per relation entry, and the relation is reduced over all its ports by a
+// ballot (FinRel._rewire to no ports, fin_rel.py:379-393).
+//
+// Layout: lane l holds clauses l, 64 + l, ... (TS_SEARCH_SLOTS per lane) in registers as
+// packed words (three literal bytes (v << 1) | negated and a length byte); the partial
+// assignment is wave-uniform (two 128-bit masks: assigned, value) and lives in scalar
+// registers; the branch stack (assignment before the branch, branch literal, next child)
+// and the branching histogram are in LDS.  Propagation runs in rounds: every lane
+// evaluates its clauses, a ballot finds conflicts, and the unit literals of the round are
+// applied by the scalar unit one lane at a time (readlane), which also detects a
+// variable forced both ways.  The fixpoint is the same as any sequential order's.
+//
+// Every loop is bounded: propagation rounds by the number of variables (each round
+// assigns at least one), the branch stack by the number of variables, and each job by
+// its node budget.
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "tensorsat_amd.h"
+
+namespace {
+
+constexpr int SLOTS = TS_SEARCH_SLOTS;
+constexpr int MAXV = TS_SEARCH_MAX_VARS;
+constexpr uint32_t NO_CLAUSE = 0xFFu;  // length byte of an empty slot (always satisfied)
+constexpr uint32_t NO_UNIT = 0xFFFFu;
+
+__device__ __forceinline__ uint64_t splitmix(uint64_t x) {
+  uint64_t z = x + 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ uint32_t uni(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ uint64_t uni64(uint64_t x) {
+  return (uint64_t)uni((uint32_t)x) | ((uint64_t)uni((uint32_t)(x >> 32)) << 32);
+}
+
+// Reductions over the wave (every lane active): within rows of 16 lanes by DPP, then
+// across the four rows by readlane.
+template <typename Op>
+__device__ __forceinline__ uint32_t wave_reduce(uint32_t x, Op op) {
+  x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xF, 0xF, false));   // quad [1,0,3,2]
+  x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x4E, 0xF, 0xF, false));   // quad [2,3,0,1]
+  x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x141, 0xF, 0xF, false));  // row half mirror
+  x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x140, 0xF, 0xF, false));  // row mirror
+  const uint32_t a = __builtin_amdgcn_readlane(x, 0), b = __builtin_amdgcn_readlane(x, 16);
+  const uint32_t c = __builtin_amdgcn_readlane(x, 32), d = __builtin_amdgcn_readlane(x, 48);
+  return op(op(a, b), op(c, d));
+}
+struct OpMax {
+  __device__ uint32_t operator()(uint32_t a, uint32_t b) const { return a > b ? a : b; }
+};
+struct OpAnd {
+  __device__ uint32_t operator()(uint32_t a, uint32_t b) const { return a & b; }
+};
+
+__device__ __forceinline__ bool bit_of(uint64_t lo, uint64_t hi, uint32_t v) {
+  return (((v & 64) ? hi : lo) >> (v & 63)) & 1ull;
+}
+
+// Clause status under the assignment: bit 0 satisfied; bits 1..2 number of free
+// literals; bits 8..15 the last free literal byte.  An empty slot reads as satisfied.
+__device__ __forceinline__ uint32_t clause_state(uint32_t c, uint64_t a0, uint64_t a1, uint64_t v0, uint64_t v1) {
+  const uint32_t len = c >> 24;
+  if (len == NO_CLAUSE) return 1u;
+  uint32_t sat = 0, nfree = 0, lit = 0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    if ((uint32_t)i < len) {
+      const uint32_t b = (c >> (8 * i)) & 0xFFu, v = b >> 1;
+      const bool asg = bit_of(a0, a1, v);
+      const bool val = bit_of(v0, v1, v);
+      sat |= (asg && (val != (bool)(b & 1u))) ? 1u : 0u;
+      if (!asg) {
+        ++nfree;
+        lit = b;
+      }
+    }
+  }
+  return sat | (nfree << 1) | (lit << 8);
+}
+
+struct Assign {
+  uint64_t a0, a1, v0, v1;
+};
+
+// The clause words never change during a job, so the compiler would decode all of them
+// once, outside the search loop, into hundreds of registers; marking them as rewritten
+// at each use keeps the decode next to its use (a few registers per slot).
+__device__ __forceinline__ void fresh(uint32_t (&cl)[SLOTS]) {
+#pragma unroll
+  for (int k = 0; k < SLOTS; ++k) asm volatile("" : "+v"(cl[k]));
+}
+
+// Unit propagation to the fixpoint.  Returns false on a conflict.  On success `st`
+// holds every slot's clause_state under the final assignment.  Each round every lane
+// offers at most one unit literal (its first unit clause's); the scalar unit applies
+// them lane by lane, so each round assigns at least one new variable.
+__device__ __forceinline__ bool propagate(uint32_t (&cl)[SLOTS], int nslots, Assign& s, uint32_t (&st)[SLOTS]) {
+  for (int round = 0; round <= MAXV; ++round) {
+    fresh(cl);
+    bool conflict = false;
+    uint32_t unit = NO_UNIT;
+#pragma unroll
+    for (int k = 0; k < SLOTS; ++k) {
+      if (k < nslots) {
+        st[k] = clause_state(cl[k], s.a0, s.a1, s.v0, s.v1);
+        if (!(st[k] & 1u)) {
+          const uint32_t nfree = (st[k] >> 1) & 3u;
+          conflict |= nfree == 0;
+          if (nfree == 1 && unit == NO_UNIT) unit = (st[k] >> 8) & 0xFFu;
+        }
+      }
+    }
+    if (__ballot(conflict)) return false;
+    uint64_t m = __ballot(unit != NO_UNIT);
+    if (!m) return true;
+    while (m) {
+      const int ln = __builtin_ctzll(m);
+      m &= m - 1;
+      const uint32_t b = __builtin_amdgcn_readlane(unit, ln);
+      const uint32_t v = b >> 1;
+      const uint64_t bit = 1ull << (v & 63);
+      const bool hi = (v & 64) != 0;
+      const bool want = !(b & 1u);
+      const uint64_t aw = hi ? s.a1 : s.a0;
+      const uint64_t vw = hi ? s.v1 : s.v0;
+      if (aw & bit) {
+        if (((vw & bit) != 0) != want) return false;  // forced both ways
+        continue;
+      }
+      if (hi) {
+        s.a1 |= bit;
+        s.v1 |= want ? bit : 0ull;
+      } else {
+        s.a0 |= bit;
+        s.v0 |= want ? bit : 0ull;
+      }
+    }
+  }
+  return true;  // unreachable: every round with a unit assigns a new variable
+}
+
+__device__ __forceinline__ bool owns(int depth, int split, uint32_t prefix) {
+  return depth >= split || ((prefix >> depth) & ((1u << (split - depth)) - 1u)) == 0u;
+}
+
+// Entries e of a rank <= 6 relation whose bit p is 1, as a 64-bit mask.
+__device__ __forceinline__ uint64_t entries_with_bit(uint32_t p) {
+  switch (p) {
+    case 0: return 0xAAAAAAAAAAAAAAAAull;
+    case 1: return 0xCCCCCCCCCCCCCCCCull;
+    case 2: return 0xF0F0F0F0F0F0F0F0ull;
+    case 3: return 0xFF00FF00FF00FF00ull;
+    case 4: return 0xFFFF0000FFFF0000ull;
+    default: return 0xFFFFFFFF00000000ull;
+  }
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(64) void k_search(ts_search p) {
+  __shared__ uint32_t hist[MAXV];          // score << 20 | positive << 10 | negative
+  __shared__ uint64_t stk[4][MAXV + 1];    // assignment before the branch at each depth
+  __shared__ uint32_t stk_lit[MAXV + 1];   // branch literal byte | next child << 8
+
+  const int lane = (int)threadIdx.x;
+  const ts_search_job job = p.jobs[blockIdx.x];
+  const ts_search_inst inst = p.instances[job.instance];
+  const int m = inst.num_clauses;
+  const int nslots = (m + 63) >> 6;
+  const int split = job.split;
+  const uint32_t prefix = job.prefix;
+  const int width = p.width;
+
+  uint32_t cl[SLOTS];
+#pragma unroll
+  for (int k = 0; k < SLOTS; ++k) {
+    const int i = k * 64 + lane;
+    cl[k] = (k < nslots && i < m) ? p.clauses[inst.clause_begin + i] : (NO_CLAUSE << 24);
+  }
+  hist[lane] = 0;
+  hist[lane + 64] = 0;
+  __syncthreads();
+
+  int64_t nodes = 0, leaves = 0, conflicts = 0, visited = 0;
+  uint64_t digest = 0;
+  int status = 0, max_depth = 0;
+
+  Assign s{0, 0, 0, 0};
+  uint32_t st[SLOTS];
+  // The loop alternates two phases.  Propagate: `s` is the assignment of the node at
+  // depth `cd` (the root, or the child of frame cd - 1) before propagation.  Next: try
+  // the next child of the branch frame at depth `fd`.
+  int cd = 0, fd = -1;
+  bool prop = true;
+  for (;;) {
+    if (!prop) {
+      const uint32_t fr = uni(stk_lit[fd]);
+      const uint32_t t = fr >> 8;
+      if (t >= 2) {
+        if (fd == 0) break;
+        --fd;
+        continue;
+      }
+      stk_lit[fd] = (fr & 0xFFu) | ((t + 1) << 8);
+      __syncthreads();
+      if (fd < split && t != ((prefix >> fd) & 1u)) continue;
+      const uint32_t b = (fr & 0xFFu) ^ t;  // the second child negates the literal
+      const uint32_t v = b >> 1;
+      s.a0 = uni64(stk[0][fd]);
+      s.a1 = uni64(stk[1][fd]);
+      s.v0 = uni64(stk[2][fd]);
+      s.v1 = uni64(stk[3][fd]);
+      const uint64_t bit = 1ull << (v & 63);
+      const uint64_t val = (b & 1u) ? 0ull : bit;
+      if (v & 64) {
+        s.a1 |= bit;
+        s.v1 |= val;
+      } else {
+        s.a0 |= bit;
+        s.v0 |= val;
+      }
+      cd = fd + 1;
+      prop = true;
+      continue;
+    }
+    // ---- propagate the node at depth cd ----
+    if (!propagate(cl, nslots, s, st)) {
+      if (owns(cd, split, prefix)) ++conflicts;
+      if (cd == 0) break;
+      fd = cd - 1;
+      prop = false;
+      continue;
+    }
+    const int depth = cd;
+    if (p.early_exit &&
+        (__hip_atomic_load(&p.verdicts[job.instance], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & TS_SEARCH_SAT)) {
+      status = 1;
+      break;
+    }
+    if (visited >= p.max_nodes) {
+      status = 2;
+      break;
+    }
+    ++visited;
+    const bool own = owns(depth, split, prefix);
+    if (own) {
+      ++nodes;
+      max_depth = depth > max_depth ? depth : max_depth;
+    }
+    // Branching histogram over the residual clauses' free literals; it also names R.
+    fresh(cl);
+#pragma unroll
+    for (int k = 0; k < SLOTS; ++k) {
+      if (k < nslots && !(st[k] & 1u)) {
+        const uint32_t c = cl[k], len = c >> 24;
+        const uint32_t w = (((st[k] >> 1) & 3u) - 1u) << 20;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          if ((uint32_t)i < len) {
+            const uint32_t b = (c >> (8 * i)) & 0xFFu, v = b >> 1;
+            if (!bit_of(s.a0, s.a1, v)) atomicAdd(&hist[v], w | ((b & 1u) ? 1u : (1u << 10)));
+          }
+        }
+      }
+    }
+    __syncthreads();
+    const uint32_t h0 = hist[lane], h1 = hist[lane + 64];
+    hist[lane] = 0;
+    hist[lane + 64] = 0;
+    const uint64_t r0 = __ballot(h0 != 0), r1 = __ballot(h1 != 0);
+    const int rank = __popcll(r0) + __popcll(r1);
+    if (rank <= width) {
+      // ---- leaf: the residual clause tensors multiplied into the rank-|R| relation
+      // (entry e: bit j of e = value of the j-th smallest variable of R), then reduced ----
+      uint64_t bits = 1;  // no residual clause: the empty product
+      if (rank > 0) {
+        uint64_t mine = ~0ull;  // entries satisfying this lane's residual clauses
+        fresh(cl);
+#pragma unroll
+        for (int k = 0; k < SLOTS; ++k) {
+          if (k < nslots && !(st[k] & 1u)) {
+            const uint32_t c = cl[k], len = c >> 24;
+            uint64_t fals = ~0ull;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+              if ((uint32_t)i < len) {
+                const uint32_t b = (c >> (8 * i)) & 0xFFu, v = b >> 1;
+                if (!bit_of(s.a0, s.a1, v)) {  // assigned literals of a residual clause are false
+                  const uint32_t pos = (v & 64) ? (uint32_t)(__popcll(r0) + __popcll(r1 & ((1ull << (v & 63)) - 1ull)))
+                                                : (uint32_t)__popcll(r0 & ((1ull << v) - 1ull));
+                  const uint64_t ones = entries_with_bit(pos);
+                  fals &= (b & 1u) ? ones : ~ones;  // false where the variable equals the negation bit
+                }
+              }
+            }
+            mine &= ~fals;
+          }
+        }
+        const uint32_t lo = wave_reduce((uint32_t)mine, OpAnd());
+        const uint32_t hi = wave_reduce((uint32_t)(mine >> 32), OpAnd());
+        const uint64_t valid = rank == 6 ? ~0ull : ((1ull << (1u << rank)) - 1ull);
+        bits = (((uint64_t)hi << 32) | lo) & valid;
+      }
+      if (own) {
+        ++leaves;
+        digest += splitmix(splitmix(bits) + (uint64_t)rank);
+        if (bits != 0) {
+          if (lane == 0) atomicOr(&p.verdicts[job.instance], TS_SEARCH_SAT);
+          if (!p.exhaustive) break;
+        }
+      }
+      if (depth == 0) break;
+      fd = depth - 1;
+      prop = false;
+      continue;
+    }
+    // ---- branch: free variable of largest score, ties to the smaller index ----
+    const uint32_t k0 = h0 ? (((h0 >> 20) << 8) | (255u - (uint32_t)lane)) : 0u;
+    const uint32_t k1 = h1 ? (((h1 >> 20) << 8) | (255u - (uint32_t)(lane + 64))) : 0u;
+    const uint32_t key = wave_reduce(k0 > k1 ? k0 : k1, OpMax());
+    const uint32_t bv = 255u - (key & 255u);
+    const uint32_t hb = bv < 64 ? __builtin_amdgcn_readlane(h0, (int)bv) : __builtin_amdgcn_readlane(h1, (int)bv - 64);
+    const uint32_t npos = (hb >> 10) & 1023u, nneg = hb & 1023u;
+    const uint32_t first = (bv << 1) | (npos >= nneg ? 0u : 1u);
+    stk[0][depth] = s.a0;
+    stk[1][depth] = s.a1;
+    stk[2][depth] = s.v0;
+    stk[3][depth] = s.v1;
+    stk_lit[depth] = first;  // next child 0
+    __syncthreads();
+    fd = depth;
+    prop = false;
+  }
+
+  if (lane == 0) {
+    if (status == 2) atomicOr(&p.verdicts[job.instance], TS_SEARCH_INCOMPLETE);
+    if (p.stats) {
+      ts_search_stat o;
+      o.status = status;
+      o.max_depth = max_depth;
+      o.nodes = nodes;
+      o.leaves = leaves;
+      o.conflicts = conflicts;
+      o.digest = digest;
+      p.stats[blockIdx.x] = o;
+    }
+  }
+}
+
+extern "C" int ts_search_run(const ts_search* p, ts_stream stream) {
+  if (!p || p->num_jobs < 0 || p->num_instances < 0) return TS_EINVAL;
+  if (p->width < 0 || p->width > TS_SEARCH_MAX_WIDTH || p->max_nodes <= 0) return TS_EINVAL;
+  if (p->num_jobs == 0) return TS_OK;
+  if (!p->clauses || !p->instances || !p->jobs || !p->verdicts) return TS_EINVAL;
+  hipLaunchKernelGGL(k_search, dim3((unsigned)p->num_jobs), dim3(64), 0, reinterpret_cast<hipStream_t>(stream), *p);
+  return hipGetLastError() == hipSuccess ? TS_OK : TS_EHIP;
+}

@@ -23,7 +23,7 @@ Clause = tuple[int, ...]
 class CNFInstance:
     """A SAT instance in conjunctive normal form (``lib/sat.py:40``)."""
 
-    __slots__ = ("_num_vars", "_clauses")
+    __slots__ = ("_num_vars", "_clauses", "_arrays")
 
     def __init__(self, num_vars: int, clauses: Sequence[Sequence[int]]) -> None:
         """Public constructor, validating literals (``lib/sat.py:111-121``)."""
@@ -35,12 +35,14 @@ class CNFInstance:
             raise ValueError("Literals must be non-zero.")
         self._num_vars = int(num_vars)
         self._clauses = clauses
+        self._arrays = None
 
     @classmethod
-    def _new(cls, num_vars: int, clauses: tuple[Clause, ...]) -> "CNFInstance":
+    def _new(cls, num_vars: int, clauses: tuple[Clause, ...], arrays=None) -> "CNFInstance":
         self = object.__new__(cls)
         self._num_vars = num_vars
         self._clauses = clauses
+        self._arrays = arrays
         return self
 
     @classmethod
@@ -79,7 +81,8 @@ class CNFInstance:
             if clause not in seen:
                 seen.add(clause)
                 clauses.append(clause)
-        return cls._new(n, tuple(clauses))
+        flat = np.asarray(clauses, dtype=np.int32).reshape(-1)
+        return cls._new(n, tuple(clauses), (np.full(m, k, dtype=np.int32), flat))
 
     @classmethod
     def from_dimacs(cls, dimacs: str) -> "CNFInstance":
@@ -142,6 +145,16 @@ class CNFInstance:
                     f" found {len(inputs)} characters instead."
                 )
         return inputs
+
+    def literal_arrays(self) -> tuple[np.ndarray, np.ndarray]:
+        """(clause lengths, concatenated literals) as int32 arrays, computed once (the
+        random generator fills them as it draws), for batch packing without walking
+        the clause tuples again."""
+        if self._arrays is None:
+            lens = np.fromiter((len(c) for c in self._clauses), dtype=np.int32, count=len(self._clauses))
+            flat = np.fromiter((l for c in self._clauses for l in c), dtype=np.int32, count=int(lens.sum()))
+            self._arrays = (lens, flat)
+        return self._arrays
 
     def clause_matrix(self) -> np.ndarray:
         """Clauses as a dense ``(m, k_max)`` int32 array, zero-padded on the right."""

@@ -1,0 +1,298 @@
+"""
+Batched SAT decisions with the whole cube search on the device (``ts_search_run``).
+
+The batched counterpart of the reference's per-instance decision
+``bool(CotengraContraction(FinRel, wiring).contract(diagram))``
+(``examples/test_sat_solver.py:5-9``, ``contractions/cotengra.py:100-128``): each
+instance's network is sliced on its variable wires by a depth-first branch on the GPU
+(one 64-lane wave per job), clause tensors that become all-zero prune a branch and
+one-hot ones fix their variable, and every residual of at most ``width`` variables is
+contracted into one relation and reduced (``include/tensorsat_amd.h``, device cube
+search).  The host only packs the clauses and lists the jobs.
+
+Hard instances are split: a job owns the subtree below a fixed prefix of the first
+``split`` branch choices, so 2^split waves search one instance at once, and the jobs are
+launched hardest first (an estimate of the search size from the instance's size and
+clause ratio).  With early exit, the jobs of an instance stop once one of them finds a
+satisfiable leaf.  Verdicts do not depend on the split or the order.
+
+:class:`SearchSolver` decides instances the kernel's limits admit (at most 128
+variables, 768 clauses after normalisation, 3 distinct literals per clause); the others
+raise, and :func:`supported` tells them apart beforehand.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+import time
+
+import numpy as np
+
+from . import _native as N
+
+MAX_VARS = 128
+SLOTS = 12
+MAX_CLAUSES = 64 * SLOTS
+MAX_WIDTH = 6
+MAX_SPLIT = 16
+SAT = 1
+INCOMPLETE = 2
+
+INST_DTYPE = np.dtype([("clause_begin", "<i4"), ("num_clauses", "<i4"), ("num_vars", "<i4"), ("reserved", "<i4")])
+JOB_DTYPE = np.dtype([("instance", "<i4"), ("split", "<i4"), ("prefix", "<u4"), ("reserved", "<i4")])
+STAT_DTYPE = np.dtype([("status", "<i4"), ("max_depth", "<i4"), ("nodes", "<i8"), ("leaves", "<i8"),
+                       ("conflicts", "<i8"), ("digest", "<u8")])
+assert INST_DTYPE.itemsize == 16 and JOB_DTYPE.itemsize == 16 and STAT_DTYPE.itemsize == 40
+
+
+class _Search(ctypes.Structure):
+    _fields_ = [
+        ("clauses", ctypes.c_void_p), ("instances", ctypes.c_void_p), ("num_instances", ctypes.c_int32),
+        ("jobs", ctypes.c_void_p), ("num_jobs", ctypes.c_int32), ("verdicts", ctypes.c_void_p),
+        ("stats", ctypes.c_void_p), ("width", ctypes.c_int32), ("exhaustive", ctypes.c_int32),
+        ("early_exit", ctypes.c_int32), ("reserved", ctypes.c_int32), ("max_nodes", ctypes.c_int64),
+    ]
+
+
+def _pad3(lens: np.ndarray, flat: np.ndarray) -> np.ndarray:
+    """(clauses, 3) matrix of literals, 0-padded; clauses must have <= 3 literals."""
+    m = lens.size
+    mat = np.zeros((m, 3), dtype=np.int32)
+    if m:
+        starts = np.zeros(m, dtype=np.int64)
+        np.cumsum(lens[:-1], out=starts[1:])
+        cid = np.repeat(np.arange(m), lens)
+        pos = np.arange(flat.size) - np.repeat(starts, lens)
+        mat[cid, pos] = flat
+    return mat
+
+
+def pack_clauses(lens: np.ndarray, flat: np.ndarray) -> np.ndarray:
+    """Packed clause words of one or more concatenated instances (``TS_SEARCH_CLAUSE``
+    layout): duplicate literals dropped (first occurrence kept), tautologies removed
+    (a removed clause is returned as word 0xFFFFFFFF for the caller to drop), clause
+    order kept.  Clauses must have at most 3 literals."""
+    mat = _pad3(lens, flat)
+    a, b, c = mat[:, 0], mat[:, 1], mat[:, 2]
+    keep_b = (b != 0) & (b != a)
+    keep_c = (c != 0) & (c != a) & (c != b)
+    taut = ((a != 0) & ((a == -b) | (a == -c))) | ((b != 0) & (b == -c))
+    # compact the kept literals to the front
+    out = np.zeros_like(mat)
+    out[:, 0] = a
+    n1 = (a != 0).astype(np.int32)
+    rows = np.arange(mat.shape[0])
+    out[rows[keep_b], n1[keep_b]] = b[keep_b]
+    n2 = n1 + keep_b
+    out[rows[keep_c], n2[keep_c]] = c[keep_c]
+    length = n2 + keep_c
+    lit_bytes = ((np.abs(out) - 1) << 1) | (out < 0)
+    lit_bytes = np.where(out != 0, lit_bytes, 0).astype(np.uint32)
+    words = lit_bytes[:, 0] | (lit_bytes[:, 1] << 8) | (lit_bytes[:, 2] << 16) | (length.astype(np.uint32) << 24)
+    words[taut] = 0xFFFFFFFF
+    return words
+
+
+def supported(cnf) -> bool:
+    """Whether the device search takes this instance (before normalisation limits on
+    the clause count, which :func:`pack_batch` checks exactly)."""
+    lens, _ = cnf.literal_arrays()
+    return cnf.num_vars <= MAX_VARS and (lens.size == 0 or int(lens.max()) <= 3)
+
+
+def pack_batch(cnfs) -> tuple[np.ndarray, np.ndarray]:
+    """(packed clauses, per-instance records) of a batch; raises NotImplementedError for
+    an instance outside the kernel's limits."""
+    arrays = [c.literal_arrays() for c in cnfs]
+    nv = np.fromiter((c.num_vars for c in cnfs), dtype=np.int32, count=len(cnfs))
+    if nv.size and int(nv.max()) > MAX_VARS:
+        raise NotImplementedError(f"device search takes at most {MAX_VARS} variables per instance")
+    lens = np.concatenate([a[0] for a in arrays]) if arrays else np.zeros(0, np.int32)
+    flat = np.concatenate([a[1] for a in arrays]) if arrays else np.zeros(0, np.int32)
+    if lens.size and int(lens.max()) > 3:
+        raise NotImplementedError("device search takes clauses of at most 3 literals")
+    words = pack_clauses(lens, flat)
+    per = np.fromiter((a[0].size for a in arrays), dtype=np.int64, count=len(cnfs))
+    owner = np.repeat(np.arange(len(cnfs)), per)
+    keep = words != 0xFFFFFFFF
+    words, owner = words[keep], owner[keep]
+    counts = np.bincount(owner, minlength=len(cnfs)).astype(np.int64)
+    if counts.size and int(counts.max()) > MAX_CLAUSES:
+        raise NotImplementedError(f"device search takes at most {MAX_CLAUSES} clauses per instance")
+    inst = np.zeros(len(cnfs), dtype=INST_DTYPE)
+    if len(cnfs):
+        begin = np.zeros(len(cnfs), dtype=np.int64)
+        np.cumsum(counts[:-1], out=begin[1:])
+        inst["clause_begin"] = begin
+        inst["num_clauses"] = counts
+        inst["num_vars"] = nv
+    return np.ascontiguousarray(words, dtype=np.uint32), inst
+
+
+def log2_search_size(num_vars: np.ndarray, num_clauses: np.ndarray) -> np.ndarray:
+    """Estimated log2 of the branch nodes one search needs with early exit, for random
+    3-SAT-like instances: linear in n with a slope that grows with the clause ratio up
+    to the satisfiability threshold (4.26) and falls past it (measured on the batched
+    workload: slope 0.079 at ratio 4.26, 0.030 at 3.5).  Used only to order and split
+    jobs."""
+    n = np.asarray(num_vars, dtype=np.float64)
+    r = np.asarray(num_clauses, dtype=np.float64) / np.maximum(n, 1.0)
+    lo, hi = 0.030, 0.079
+    slope = np.where(r <= 4.26, lo + (hi - lo) * np.clip((r - 3.5) / 0.76, 0.0, 1.0), hi * 4.26 / np.maximum(r, 1e-9))
+    return 1.7 + slope * n
+
+
+def plan_jobs(inst: np.ndarray, *, split_target: float | None = 128.0, max_split: int = 8) -> np.ndarray:
+    """One job per subtree: instance i gets 2^split_i jobs (prefixes 0..2^split_i - 1)
+    where split_i = ceil(log2(estimated nodes / split_target)) (0 with ``split_target``
+    None), launched in decreasing order of estimated size per job."""
+    num = len(inst)
+    if num == 0:
+        return np.zeros(0, dtype=JOB_DTYPE)
+    est = log2_search_size(inst["num_vars"], inst["num_clauses"])
+    if split_target is None:
+        split = np.zeros(num, dtype=np.int64)
+    else:
+        if split_target <= 0:
+            raise ValueError("split_target must be positive")
+        split = np.clip(np.ceil(est - math.log2(split_target)), 0, min(max_split, MAX_SPLIT)).astype(np.int64)
+    nj = 1 << split
+    jobs = np.zeros(int(nj.sum()), dtype=JOB_DTYPE)
+    jobs["instance"] = np.repeat(np.arange(num), nj)
+    jobs["split"] = np.repeat(split, nj)
+    starts = np.zeros(num, dtype=np.int64)
+    np.cumsum(nj[:-1], out=starts[1:])
+    jobs["prefix"] = np.arange(jobs.size) - np.repeat(starts, nj)
+    key = np.repeat(est - split, nj)
+    order = np.lexsort((jobs["prefix"], jobs["instance"], -key))
+    return jobs[order]
+
+
+def validate(words: np.ndarray, inst: np.ndarray, jobs: np.ndarray, width: int) -> None:
+    """Host-side bounds check of everything the kernel indexes, before any upload."""
+    if not 0 <= width <= MAX_WIDTH:
+        raise ValueError(f"width must be in 0..{MAX_WIDTH}")
+    if len(inst):
+        if np.any(inst["num_clauses"] < 0) or np.any(inst["num_clauses"] > MAX_CLAUSES):
+            raise ValueError("clause count outside the kernel's limit")
+        if np.any(inst["num_vars"] < 0) or np.any(inst["num_vars"] > MAX_VARS):
+            raise ValueError("variable count outside the kernel's limit")
+        end = inst["clause_begin"].astype(np.int64) + inst["num_clauses"]
+        if np.any(inst["clause_begin"] < 0) or np.any(end > words.size):
+            raise ValueError("clause range outside the clause array")
+        if words.size:
+            length = words >> 24
+            if np.any(length > 3):
+                raise ValueError("packed clause with more than 3 literals")
+            counts = inst["num_clauses"].astype(np.int64)
+            owner = np.repeat(np.arange(len(inst)), counts)
+            begin = np.repeat(inst["clause_begin"].astype(np.int64), counts)
+            first = np.repeat(np.cumsum(counts) - counts, counts)
+            w = words[begin + np.arange(owner.size) - first]
+            nvars = inst["num_vars"][owner].astype(np.int64)
+            for i in range(3):
+                v = (w >> (8 * i + 1)) & 0x7F
+                used = (w >> 24) > i
+                if np.any(used & (v >= nvars)):
+                    raise ValueError("literal of a variable beyond the instance's count")
+    if len(jobs):
+        if np.any(jobs["instance"] < 0) or np.any(jobs["instance"] >= len(inst)):
+            raise ValueError("job names an instance outside the batch")
+        if np.any(jobs["split"] < 0) or np.any(jobs["split"] > MAX_SPLIT):
+            raise ValueError(f"split outside 0..{MAX_SPLIT}")
+        if np.any(jobs["prefix"].astype(np.int64) >> jobs["split"].astype(np.int64)):
+            raise ValueError("prefix has bits beyond its split")
+
+
+class SearchSolver:
+    """Decides a batch of CNF instances with one launch of the device cube search.
+
+    ``width``: residuals of at most this many variables are contracted (0..6).
+    ``split_target``: estimated branch nodes per job (hard instances are split into
+    2^k jobs; None: one job per instance).  ``exhaustive``: visit every branch (for
+    statistics; no early exit).  ``max_nodes``: per-job node budget; an instance whose
+    job ran out of it is reported undecided (-1) unless another job found it SAT."""
+
+    def __init__(self, *, width: int = 6, split_target: float | None = 128.0, max_split: int = 8,
+                 exhaustive: bool = False, max_nodes: int = 1 << 26, stats: bool = False, device: str = "cuda"):
+        if not 0 <= width <= MAX_WIDTH:
+            raise ValueError(f"width must be in 0..{MAX_WIDTH}")
+        self.width = width
+        self.split_target = split_target
+        self.max_split = max_split
+        self.exhaustive = exhaustive
+        self.max_nodes = int(max_nodes)
+        self.want_stats = stats or exhaustive
+        self.device = device
+        self.timing: dict = {}
+
+    def prepare(self, cnfs) -> None:
+        """Pack, plan the jobs and upload (host work, timed in ``self.timing``)."""
+        import torch
+
+        t0 = time.perf_counter()
+        words, inst = pack_batch(cnfs)
+        t1 = time.perf_counter()
+        jobs = plan_jobs(inst, split_target=self.split_target, max_split=self.max_split)
+        validate(words, inst, jobs, self.width)
+        t2 = time.perf_counter()
+        self.upload(words, inst, jobs)
+        torch.cuda.synchronize()
+        self.timing.update(pack_s=t1 - t0, plan_s=t2 - t1, upload_s=time.perf_counter() - t2)
+
+    def upload(self, words: np.ndarray, inst: np.ndarray, jobs: np.ndarray) -> None:
+        import torch
+
+        dev = self.device
+        self.num_instances = len(inst)
+        self.host_jobs = jobs
+        self.d_words = torch.from_numpy(words.view(np.int32) if words.size else np.zeros(1, np.int32)).to(dev)
+        self.d_inst = torch.from_numpy(inst.view(np.int32).reshape(-1) if len(inst) else np.zeros(4, np.int32)).to(dev)
+        self.d_jobs = torch.from_numpy(jobs.view(np.int32).reshape(-1) if len(jobs) else np.zeros(4, np.int32)).to(dev)
+        self.d_verdicts = torch.zeros(max(1, len(inst)), dtype=torch.int32, device=dev)
+        self.d_stats = torch.zeros(max(1, len(jobs)) * STAT_DTYPE.itemsize // 4, dtype=torch.int32, device=dev) \
+            if self.want_stats else None
+        self._args = _Search(
+            N.dptr(self.d_words), N.dptr(self.d_inst), len(inst), N.dptr(self.d_jobs), len(jobs),
+            N.dptr(self.d_verdicts), N.dptr(self.d_stats) if self.d_stats is not None else None, self.width,
+            int(self.exhaustive), int(not self.exhaustive), 0, self.max_nodes,
+        )
+
+    def launch(self) -> None:
+        """Clear the verdicts (a fill kernel) and launch the search on the current stream."""
+        self.d_verdicts.zero_()
+        N.check(N.lib().ts_search_run(ctypes.byref(self._args), N.stream_ptr()), "ts_search_run")
+
+    def verdicts(self) -> np.ndarray:
+        """Per instance: 1 SAT, 0 UNSAT, -1 undecided (a job ran out of its node budget)."""
+        v = self.d_verdicts.cpu().numpy()[: self.num_instances].astype(np.int64)
+        out = np.where(v & SAT, 1, 0)
+        out[((v & INCOMPLETE) != 0) & ((v & SAT) == 0)] = -1
+        return out
+
+    def job_stats(self) -> np.ndarray:
+        if self.d_stats is None:
+            raise ValueError("construct the solver with stats=True")
+        return self.d_stats.cpu().numpy().view(STAT_DTYPE)[: len(self.host_jobs)].copy()
+
+    def instance_stats(self) -> np.ndarray:
+        """Per-instance sums of the jobs' statistics (nodes, leaves, conflicts, digest
+        mod 2^64; max_depth the maximum; status the maximum)."""
+        js = self.job_stats()
+        out = np.zeros(self.num_instances, dtype=STAT_DTYPE)
+        inst = self.host_jobs["instance"]
+        for f in ("nodes", "leaves", "conflicts"):
+            np.add.at(out[f], inst, js[f])
+        np.add.at(out["digest"], inst, js["digest"])
+        np.maximum.at(out["max_depth"], inst, js["max_depth"])
+        np.maximum.at(out["status"], inst, js["status"])
+        return out
+
+    def run(self, cnfs) -> np.ndarray:
+        import torch
+
+        self.prepare(cnfs)
+        self.launch()
+        torch.cuda.synchronize()
+        return self.verdicts()

@@ -6,7 +6,8 @@ xxh64 digest of every intermediate tensor of every cube as the oracle's step loo
 (xxh64 over the concatenated 8-byte step digests, then the result's digest), in GT0
 for every cube and in MOD256 for the first cubes.  A digest of the compiled program
 (jobs, steps, leaves) is stored beside them, so a test can tell a changed plan from a
-wrong kernel.
+wrong kernel; the cube search runs on a fixed number of host threads (its parallel
+split seeds each subtree's planner, so the program depends on it).
 
 Run from the repository root: ``python tests/golden/make_config2_digests.py`` (a few
 minutes: ~3.5 s of numpy per cube).
@@ -25,7 +26,7 @@ from oracle import executor as OX  # noqa: E402
 from tensorsat_amd import program as PG  # noqa: E402
 from tensorsat_amd.workloads import single_3sat  # noqa: E402
 
-SEED, WIDTH, TRIALS, MOD256_CUBES = 0, 26, 2, 6
+SEED, WIDTH, TRIALS, THREADS, MOD256_CUBES = 0, 26, 2, 8, 6
 
 
 def program_digest(prog) -> str:
@@ -44,8 +45,8 @@ def chain_digest(step_outputs, result) -> str:
 
 
 def main() -> None:
-    prog = PG.sat_batch([single_3sat(SEED)], width=WIDTH, plan_trials=TRIALS)
-    out = {"seed": SEED, "width": WIDTH, "plan_trials": TRIALS, "program": program_digest(prog),
+    prog = PG.sat_batch([single_3sat(SEED)], width=WIDTH, plan_trials=TRIALS, threads=THREADS)
+    out = {"seed": SEED, "width": WIDTH, "plan_trials": TRIALS, "threads": THREADS, "program": program_digest(prog),
            "jobs": len(prog.jobs), "GT0": [], "MOD256": []}
     for j in range(len(prog.jobs)):
         for mode in ("GT0", "MOD256"):

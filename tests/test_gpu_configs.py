@@ -62,7 +62,8 @@ def test_n120_width26_every_intermediate_digest():
     from tensorsat_amd.executor import DeviceProgram, run_job
 
     want = json.loads((GOLDEN / "config2_w26_digests.json").read_text())
-    prog = PG.sat_batch([single_3sat(want["seed"])], width=want["width"], plan_trials=want["plan_trials"])
+    prog = PG.sat_batch([single_3sat(want["seed"])], width=want["width"], plan_trials=want["plan_trials"],
+                        threads=want["threads"])
     h = xxhash.xxh64()
     for a in (prog.jobs, prog.steps, prog.leaves):
         h.update(a.tobytes())

@@ -1,0 +1,132 @@
+"""
+GPU parity of the device cube search (``ts_search_run``) against the oracle's
+restatement (``oracle/search.c``), the independent DPLL and brute force: verdicts of
+the batched decision workload, exact per-instance statistics (branch nodes, leaves,
+conflicts and the digest of every contracted leaf relation) with and without splitting,
+edge cases (empty, unit, duplicate-literal and tautological clauses, the size limits),
+and the node budget.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import brute, dpll
+from oracle import search as OS
+from tensorsat_amd.sat import CNFInstance
+from tensorsat_amd.workloads import batched_3sat
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("nodes", "leaves", "conflicts", "digest")
+
+
+@pytest.fixture(scope="module")
+def batch():
+    return batched_3sat(4096)
+
+
+def _solve(cnfs, **kw):
+    from tensorsat_amd.search import SearchSolver
+
+    s = SearchSolver(**kw)
+    v = s.run(cnfs)
+    return s, v
+
+
+def test_batched_workload_verdicts(batch):
+    """All 4096 instances of the batched workload, split and launched hardest first with
+    early exit: verdicts equal the oracle's and DPLL's."""
+    s, v = _solve(batch, width=6, split_target=128.0)
+    print(f"jobs {len(s.host_jobs)}, max split {int(s.host_jobs['split'].max())}, sat {int(v.sum())}", flush=True)
+    ref = OS.search(batch, width=6, threads=8)["verdict"]
+    np.testing.assert_array_equal(v, ref)
+    want = np.array([dpll.solve(c.num_vars, c.clauses) for c in batch], dtype=np.int64)
+    np.testing.assert_array_equal(v, want)
+
+
+@pytest.mark.parametrize("width", [0, 3, 6])
+def test_unsplit_early_exit_statistics_exact(batch, width):
+    """One job per instance: a sequential search, so every statistic equals the oracle's."""
+    cnfs = batch[:1024]
+    s, v = _solve(cnfs, width=width, split_target=None, stats=True)
+    got = s.instance_stats()
+    ref = OS.search(cnfs, width=width, threads=8)
+    np.testing.assert_array_equal(v, ref["verdict"])
+    for f in FIELDS:
+        np.testing.assert_array_equal(got[f], ref[f], err_msg=f)
+    np.testing.assert_array_equal(got["max_depth"], ref["max_depth"])
+
+
+@pytest.mark.parametrize("target", [None, 64.0, 4.0])
+def test_split_exhaustive_statistics_sum_to_one_search(batch, target):
+    """Exhaustive searches (every branch visited): the jobs of an instance own disjoint
+    subtrees, so their statistics add up to one unsplit search's, whatever the split."""
+    cnfs = [c for c in batch[:600] if c.num_vars <= 50]
+    s, v = _solve(cnfs, width=6, split_target=target, max_split=6, exhaustive=True)
+    got = s.instance_stats()
+    ref = OS.search(cnfs, width=6, exhaustive=True, threads=8)
+    np.testing.assert_array_equal(v, ref["verdict"])
+    for f in FIELDS:
+        np.testing.assert_array_equal(got[f], ref[f], err_msg=f"{f} target {target}")
+    if target == 4.0:
+        assert int(s.host_jobs["split"].max()) >= 3
+
+
+def test_edge_cases_match_brute_force():
+    cases = [
+        CNFInstance(0, []),
+        CNFInstance(3, []),
+        CNFInstance(3, [()]),
+        CNFInstance(3, [(1,), (-1,)]),
+        CNFInstance(3, [(1,), (-1, 2), (-2, 3), (-3,)]),
+        CNFInstance(4, [(1, 1, 2), (-1, -1), (2, -2, 3), (-2, -2, -2)]),
+        CNFInstance(2, [(1, -1)]),
+        CNFInstance(5, [(1, 2), (-1, 2), (1, -2), (-1, -2), (3, 4, 5)]),
+    ]
+    rng = np.random.default_rng(77)
+    for _ in range(200):
+        n = int(rng.integers(1, 15))
+        cls = []
+        for _ in range(int(rng.integers(0, 5 * n))):
+            k = int(rng.choice([1, 2, 3, 3, 3]))
+            cls.append(tuple(int(rng.integers(1, n + 1)) * (1 if rng.random() < 0.5 else -1) for _ in range(k)))
+        cases.append(CNFInstance(n, cls))
+    want = np.array([brute.is_sat(c.num_vars, c.clauses) for c in cases], dtype=np.int64)
+    for width in (0, 6):
+        for target in (None, 2.0):
+            s, v = _solve(cases, width=width, split_target=target, max_split=4, stats=True)
+            np.testing.assert_array_equal(v, want, err_msg=f"width {width} target {target}")
+            if target is None:
+                ref = OS.search(cases, width=width)
+                got = s.instance_stats()
+                for f in FIELDS:
+                    np.testing.assert_array_equal(got[f], ref[f], err_msg=f)
+
+
+def test_size_limits_128_vars_768_clauses():
+    """The largest instances the kernel takes (all 12 clause slots of every lane, the
+    upper half of the variable masks) against the oracle."""
+    cnfs = [CNFInstance.random(3, 128, 768, rng=s) for s in range(4)]
+    cnfs += [CNFInstance.random(3, 128, 450, rng=s) for s in range(4)]
+    s, v = _solve(cnfs, width=6, split_target=None, stats=True, max_nodes=1 << 22)
+    ref = OS.search(cnfs, width=6)
+    np.testing.assert_array_equal(v, ref["verdict"])
+    got = s.instance_stats()
+    for f in FIELDS:
+        np.testing.assert_array_equal(got[f], ref[f], err_msg=f)
+
+
+def test_node_budget_reports_undecided():
+    """A job that runs out of its node budget leaves an UNSAT instance undecided (-1),
+    never a wrong verdict."""
+    cnfs = [c for c in batched_3sat(64) if c.num_vars >= 70][:6]
+    ref = OS.search(cnfs, width=6)
+    s, v = _solve(cnfs, width=6, split_target=None, max_nodes=3, stats=True)
+    for i, c in enumerate(cnfs):
+        if int(ref["nodes"][i]) > 3:
+            assert v[i] in (-1, 1)
+            if v[i] == 1:
+                assert ref["verdict"][i] == 1
+        else:
+            assert v[i] == ref["verdict"][i]
+    assert (v == -1).any()

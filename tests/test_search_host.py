@@ -1,0 +1,91 @@
+"""
+Host side of the device cube search: clause packing (normalisation as the oracle
+restates it), job planning (every subtree owned exactly once) and the bounds checks that
+run before any launch.  No GPU needed.
+"""
+
+import numpy as np
+import pytest
+
+from tensorsat_amd import search as S
+from tensorsat_amd.sat import CNFInstance
+from tensorsat_amd.workloads import batched_3sat
+
+
+def _unpack(word):
+    n = int(word) >> 24
+    out = []
+    for i in range(n):
+        b = (int(word) >> (8 * i)) & 0xFF
+        out.append(-((b >> 1) + 1) if b & 1 else (b >> 1) + 1)
+    return tuple(out)
+
+
+def test_pack_normalises_like_the_oracle():
+    cnf = CNFInstance(5, [(1, 2, 3), (2, 2, -4), (1, -1, 5), (), (-5,), (3, 3, 3), (4, -2, 4)])
+    words, inst = S.pack_batch([cnf])
+    assert [_unpack(w) for w in words] == [(1, 2, 3), (2, -4), (), (-5,), (3,), (4, -2)]
+    assert inst["num_clauses"][0] == 6 and inst["num_vars"][0] == 5 and inst["clause_begin"][0] == 0
+
+
+def test_pack_batch_offsets_and_random_instances():
+    cnfs = batched_3sat(40)
+    words, inst = S.pack_batch(cnfs)
+    for i, c in enumerate(cnfs):
+        b, n = int(inst["clause_begin"][i]), int(inst["num_clauses"][i])
+        assert [_unpack(w) for w in words[b : b + n]] == [tuple(cl) for cl in c.clauses]
+    S.validate(words, inst, S.plan_jobs(inst), 6)
+
+
+def test_limits_raise():
+    with pytest.raises(NotImplementedError):
+        S.pack_batch([CNFInstance(129, [(1, 2, 129)])])
+    with pytest.raises(NotImplementedError):
+        S.pack_batch([CNFInstance(4, [(1, 2, 3, 4)])])
+    with pytest.raises(NotImplementedError):
+        S.pack_batch([CNFInstance(100, [(1 + i % 100, 1 + (i * 7) % 100) for i in range(769)])])
+    ok = CNFInstance(128, [(1 + i % 128, -(1 + (i + 1) % 128), 1 + (i + 2) % 128) for i in range(768)])
+    words, inst = S.pack_batch([ok])
+    assert inst["num_clauses"][0] == 768 and S.supported(ok)
+
+
+def test_plan_jobs_cover_every_subtree_once():
+    cnfs = batched_3sat(200)
+    _, inst = S.pack_batch(cnfs)
+    for target in (None, 512.0, 64.0, 4.0):
+        jobs = S.plan_jobs(inst, split_target=target, max_split=6)
+        for i in range(len(cnfs)):
+            mine = jobs[jobs["instance"] == i]
+            k = int(mine["split"][0])
+            assert np.all(mine["split"] == k)
+            assert sorted(mine["prefix"].tolist()) == list(range(1 << k))
+        if target is None:
+            assert np.all(jobs["split"] == 0)
+    # hard instances (large n near the threshold) are split more and launched first
+    jobs = S.plan_jobs(inst, split_target=16.0, max_split=6)
+    est = S.log2_search_size(inst["num_vars"], inst["num_clauses"])
+    per_job = est[jobs["instance"]] - jobs["split"]
+    assert np.all(np.diff(per_job) <= 1e-9)
+
+
+def test_validate_rejects_out_of_range_records():
+    words, inst = S.pack_batch(batched_3sat(8))
+    jobs = S.plan_jobs(inst)
+    bad = jobs.copy()
+    bad["instance"][0] = 8
+    with pytest.raises(ValueError):
+        S.validate(words, inst, bad, 6)
+    bad = jobs.copy()
+    bad["split"][0], bad["prefix"][0] = 2, 4
+    with pytest.raises(ValueError):
+        S.validate(words, inst, bad, 6)
+    bad_inst = inst.copy()
+    bad_inst["num_clauses"][-1] += 1
+    with pytest.raises(ValueError):
+        S.validate(words, bad_inst, jobs, 6)
+    small = inst.copy()
+    small["num_vars"][0] = 3
+    with pytest.raises(ValueError):
+        S.validate(words, small, jobs, 6)
+    with pytest.raises(ValueError):
+        S.validate(words, inst, jobs, 7)
