@@ -2,20 +2,23 @@
 """
 bench.py — batched random 3-SAT decisions on one MI355X.
 
-``--workload dense`` instead measures one dense Boolean composition step on the int8 MFMA
-route (see ``bench_dense``); ``--workload learned`` compares time to solution of the
-learned planner against the plain planner on the learned-planner workload
-(see ``bench_learned``).
+Default workload: 4096 random 3-SAT instances (instance i from default_rng(10000 + i),
+n ~ U{20..80}, clause ratio 4.26 for even i and 3.5 for odd i) decided by the device cube
+search (``tensorsat_amd.search``): the packed clauses are resident in HBM and one timed
+"step" is the whole decision of the batch on the GPU (verdict reset + one launch of the
+search kernel: conditioned slicing, unit propagation and the contraction of every
+residual of at most W variables), with no host work.  The end-to-end rate from the list
+of CNF instances (packing, job planning, upload, launch, verdicts back) is reported beside
+it, with the search width and split chosen by the autotuner.  Verdicts are checked
+against the oracle's restatement of the search (``oracle/search.c``), the
+reference-equivalent path (the host cube planner and the oracle's numpy contraction of
+every cube, on all host cores: the timed ``cpu_baseline``) and an independent DPLL.  The
+same line nests the single n=120 decision and the n=100 #SAT count, each with its own
+roofline and CPU baseline.
 
-Workload: 4096 random 3-SAT instances (instance i from default_rng(10000 + i),
-n ~ U{20..80}, clause ratio 4.26 for even i and 3.5 for odd i).  The host front-end
-slices each instance into conditioned sub-networks (cubes) of width <= W and compiles
-contraction jobs (first pass: the first cubes of every instance; second pass: the rest of
-the cubes of instances the first pass left undecided, appended to the first pass's
-program).  Planning is reported separately as plan_s.  One timed "step" decides all 4096
-instances from device-resident jobs: the reset fill and one persistent launch.  Verdicts
-are checked bit-exactly against the CPU oracle (the reference-equivalent numpy step loop
-on the same cubes and trees) and an independent DPLL.
+``--workload batched-host`` runs the earlier pipeline (host cube planning, persistent
+contraction kernel); ``single``, ``count``, ``learned``, ``dense``, ``rewire`` and
+``copy`` run one workload on its own.
 
 Prints one JSON line.  --gpus above 1 is refused: this build runs on one GPU.
 """
@@ -114,7 +117,11 @@ def main() -> int:
                     "(default: the executor's)")
     ap.add_argument("--kernel", default="auto", choices=["auto", "lds", "global"],
                     help="batched kernel: LDS arenas, device-memory arenas, or auto (LDS when the arenas fit)")
-    ap.add_argument("--workload", default="batched", choices=["batched", "dense", "learned", "copy", "single", "count", "rewire"])
+    ap.add_argument("--workload", default="batched",
+                    choices=["batched", "batched-host", "dense", "learned", "copy", "single", "count", "rewire"])
+    ap.add_argument("--split", default="auto", help="batched: split target in branch nodes per job, 'none', "
+                    "or 'auto' (autotuned with the width)")
+    ap.add_argument("--no-nested", action="store_true", help="batched: skip the nested single / count lines")
     ap.add_argument("--seeds", type=int, default=3, help="single / count workloads: instances (seeds 0..)")
     ap.add_argument("--learned-instances", type=int, default=8)
     ap.add_argument("--learned-width", type=int, default=22, help="learned workload: cube width W")
@@ -141,7 +148,290 @@ def main() -> int:
         return bench_rewire(args)
     if args.workload == "count":
         return bench_count(args)
+    if args.workload == "batched":
+        return bench_search(args)
+    return bench_batched_host(args)
 
+
+def host_cores() -> int:
+    """Host threads this run may use: the CPU share of one GPU on a shared box (the
+    OMP_NUM_THREADS the box sets), else the affinity mask."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS")
+    return max(1, min(avail, int(env))) if env and env.isdigit() else max(1, avail)
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "host CPU"
+
+
+# ---- process pool over the oracle -----------------------------------------------------
+# Forked workers inherit the program being checked (no copy through a pipe); they only
+# run numpy and leave through os._exit, never touching the GPU state they inherit.
+_POOL_PROG = None
+
+
+def _pool_decide(job_lists, mode):
+    """Verdicts of instances, each given as its job indices in cube order (early exit)."""
+    from oracle import executor as OX
+
+    out = []
+    for jobs in job_lists:
+        v = 0
+        for j in jobs:
+            res, _ = OX.run_job(_POOL_PROG, j, mode)
+            if res.reshape(-1)[0]:
+                v = 1
+                break
+        out.append(v)
+    return out
+
+
+def _pool_job(j, mode):
+    from oracle import executor as OX
+
+    res, _ = OX.run_job(_POOL_PROG, j, mode)
+    return int(res.reshape(-1)[0])
+
+
+def _pool(prog, cores):
+    import multiprocessing as mp
+    import types
+    from concurrent.futures import ProcessPoolExecutor
+
+    global _POOL_PROG
+    _POOL_PROG = types.SimpleNamespace(jobs=prog.jobs, steps=prog.steps, leaves=prog.leaves,
+                                       num_instances=prog.num_instances)
+    return ProcessPoolExecutor(max_workers=cores, mp_context=mp.get_context("fork"))
+
+
+def reference_path_cpu(cnfs, cores: int, width: int = 6, first_pass: int = 4, mode: str = "GT0"):
+    """The reference-equivalent CPU path on all host cores: the host cube planner (C++,
+    ``cores`` threads) slices every instance into cubes of width <= W, and the oracle's
+    numpy step loop (``FinRel._contract2`` semantics, one process per core) contracts
+    them in cube order with early exit -- first the first ``first_pass`` cubes of each
+    instance, then the remaining cubes of the instances left undecided.  Returns
+    (verdicts, seconds, plan seconds)."""
+    from tensorsat_amd import program as PG
+
+    t0 = time.perf_counter()
+    verdicts = np.zeros(len(cnfs), dtype=np.int64)
+    plan_s = 0.0
+    todo = np.arange(len(cnfs))
+    begin = 0
+    for cube_cap in (first_pass, 0):
+        if todo.size == 0:
+            break
+        tp = time.perf_counter()
+        prog = PG.sat_batch([cnfs[i] for i in todo], width=width, max_cubes=cube_cap, cube_begin=begin,
+                            threads=cores, plan_trials=1)
+        plan_s += time.perf_counter() - tp
+        order = np.lexsort((prog.jobs["cube"], prog.jobs["instance"]))
+        per = [[] for _ in range(len(todo))]
+        for j in order:
+            per[int(prog.jobs["instance"][j])].append(int(j))
+        chunks = [list(range(k, len(todo), cores * 4)) for k in range(cores * 4)]
+        with _pool(prog, cores) as ex:
+            futs = [ex.submit(_pool_decide, [per[i] for i in ch], mode) for ch in chunks]
+            for ch, f in zip(chunks, futs):
+                for i, v in zip(ch, f.result()):
+                    verdicts[todo[i]] |= v
+        truncated = prog.stats[:, 3] == 1
+        todo = todo[(verdicts[todo] == 0) & truncated]
+        begin = first_pass
+    return verdicts, time.perf_counter() - t0, plan_s
+
+
+def bench_search(args) -> int:
+    """The default workload: batched decisions by the device cube search (module
+    docstring)."""
+    import torch
+
+    from tensorsat_amd import search as SE
+    from tensorsat_amd.workloads import batched_3sat
+
+    def log(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    t0 = time.perf_counter()
+    cnfs = batched_3sat(args.instances)
+    log(f"generated {len(cnfs)} instances in {time.perf_counter() - t0:.1f}s")
+    tune = None
+    if args.width == "auto" or args.split == "auto":
+        t0 = time.perf_counter()
+        widths = (4, 6) if args.width == "auto" else (int(args.width),)
+        splits = (None, 512.0, 128.0, 32.0, 8.0) if args.split == "auto" else \
+            ((None,) if args.split == "none" else (float(args.split),))
+        choice, table = SE.autotune(cnfs, widths=widths, split_targets=splits)
+        width, split = choice["width"], choice["split_target"]
+        tune = {"chosen": choice, "seconds": time.perf_counter() - t0, "candidates": list(table.values())}
+        log(f"autotuned width {width}, split target {split} in {tune['seconds']:.1f}s: " +
+            ", ".join(f"W{r['width']}/{r['split_target']}: {r['ms']:.3f}ms" for r in table.values()))
+    else:
+        width = int(args.width)
+        split = None if args.split == "none" else float(args.split)
+    solver = SE.SearchSolver(width=width, split_target=split)
+    # End to end from the CNF list: pack, plan the jobs, upload, launch, verdicts back.
+    e2e = []
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        solver.prepare(cnfs)
+        solver.launch()
+        v_e2e = solver.verdicts()
+        e2e.append(time.perf_counter() - t0)
+    e2e_s = float(np.median(e2e))
+    host = dict(solver.timing)
+    for _ in range(args.warmup):
+        solver.launch()
+    torch.cuda.synchronize()
+    # Timed steps: the whole decision of the batch from device-resident clauses.
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        solver.launch()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    ms_per_step = wall * 1000.0 / args.steps
+    # Kernel time: events on the launch stream around each launch (a loop of its own).
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    for a, b in ev:
+        a.record()
+        solver.launch()
+        b.record()
+    torch.cuda.synchronize()
+    kernel_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+    verdicts = solver.verdicts()
+    value = len(cnfs) / (ms_per_step / 1000.0)
+    jobs = solver.host_jobs
+    # Critical path: the statistics of one more launch (per job branch nodes).
+    st = SE.SearchSolver(width=width, split_target=split, stats=True)
+    st.prepare(cnfs)
+    st.launch()
+    js = st.job_stats()
+    nodes_total = int(js["nodes"].sum())
+    busiest = int(np.argmax(js["nodes"] + js["conflicts"]))
+    log(f"search: width {width}, split target {split}, {len(jobs)} jobs (max split {int(jobs['split'].max())}), "
+        f"{ms_per_step:.3f} ms/step (kernel {kernel_ms:.3f} ms) = {value / 1e6:.2f} M inst/s; end to end "
+        f"{e2e_s * 1000:.1f} ms = {len(cnfs) / e2e_s / 1e3:.1f} k inst/s; {nodes_total} branch nodes, "
+        f"busiest job {int(js['nodes'][busiest])} nodes")
+    nclauses = int(solver.d_words.numel())
+    alg_bytes = 4.0 * nclauses + 16.0 * len(cnfs) + 16.0 * len(jobs) + 4.0 * len(cnfs)
+    achieved = alg_bytes / (kernel_ms / 1000.0) / 1e9
+
+    cores = host_cores()
+    cpu = None
+    parity = {"checked": len(cnfs), "oracle_search_mismatches": None, "reference_path_mismatches": None,
+              "dpll_mismatches": None}
+    extra = {}
+    if not args.no_cpu:
+        from oracle import dpll
+        from oracle import search as OS
+
+        ref_v, ref_s, ref_plan = reference_path_cpu(cnfs, cores)
+        parity["reference_path_mismatches"] = int((ref_v != verdicts).sum())
+        cpu = {"value": len(cnfs) / ref_s, "unit": "instances/s", "cores": cores, "kind": "port",
+               "cpu_model": cpu_model(),
+               "sample": f"all {len(cnfs)} instances: host cube planner (width 6, {cores} threads, "
+                         f"{ref_plan:.2f}s) + the oracle's numpy contraction of the cubes with early exit "
+                         f"({cores} processes), {ref_s:.2f}s wall"}
+        log(f"cpu reference path: {cpu['value']:.0f} inst/s on {cores} cores ({cpu_model()}); "
+            f"mismatches {parity['reference_path_mismatches']}")
+        t0 = time.perf_counter()
+        osr = OS.search(cnfs, width=width, threads=cores)
+        os_s = time.perf_counter() - t0
+        parity["oracle_search_mismatches"] = int((osr["verdict"] != verdicts).sum())
+        extra["cpu_search_restatement"] = {"value": len(cnfs) / os_s, "unit": "instances/s", "cores": cores,
+                                           "sample": f"all {len(cnfs)} instances, oracle/search.c (the same "
+                                                     f"search on the CPU, one instance per thread), {os_s:.2f}s"}
+        log(f"cpu search restatement: {len(cnfs) / os_s:.0f} inst/s; mismatches {parity['oracle_search_mismatches']}")
+        dp = np.array([dpll.solve(c.num_vars, c.clauses) for c in cnfs], dtype=np.int64)
+        parity["dpll_mismatches"] = int((dp != verdicts).sum())
+        log(f"independent DPLL cross-check: {parity['dpll_mismatches']} mismatches over {len(cnfs)}")
+    parity["end_to_end_mismatches"] = int((v_e2e != verdicts).sum())
+    parity["undecided"] = int((verdicts < 0).sum())
+
+    nested = {}
+    if not args.no_nested:
+        nested["single_n120"] = run_single(args, seeds=range(3), log=log)
+        nested["count_n100"] = run_count(args, seeds=range(1), log=log)
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "instances/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic",
+        "config": {
+            "workload": "batched random 3-SAT decisions: 4096 instances, n~U{20..80}, ratio 4.26/3.5 "
+                        "(default_rng(10000+i)); device cube search (conditioned slicing, unit propagation, "
+                        "residuals of <= W variables contracted and reduced) from device-resident packed clauses",
+            "instances": len(cnfs),
+            "width": width,
+            "split_target": split,
+            "jobs": int(len(jobs)),
+            "max_split": int(jobs["split"].max()),
+            "autotune": tune,
+            "kernel": "k_search",
+            "kernel_ms": kernel_ms,
+            "launch": "per step: verdict reset (fill kernel) + one k_search launch of one 64-lane wave per job",
+            "sat": int((verdicts == 1).sum()),
+            "end_to_end_s": e2e_s,
+            "end_to_end_instances_per_s": len(cnfs) / e2e_s,
+            "end_to_end": "CNF list -> packed clauses -> job list -> upload -> launch -> verdicts on the host "
+                          "(median of 3)",
+            "host_s": host,
+            "branch_nodes": nodes_total,
+            "leaves": int(js["leaves"].sum()),
+            **extra,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": MEASURED_TRAFFIC.get(("search", width, len(cnfs)), (None, None))[0],
+            "traffic_source": MEASURED_TRAFFIC.get(("search", width, len(cnfs)), (None, None))[1],
+            "kernel": "k_search",
+            "bytes_per_launch": alg_bytes,
+            "binding_limit": "latency: the longest job's chain of dependent branch nodes (each a few "
+                             "propagation rounds of ballots and scalar unit steps), not HBM",
+            "critical_path": {"busiest_job_nodes": int(js["nodes"][busiest]),
+                              "busiest_job_conflicts": int(js["conflicts"][busiest]),
+                              "mean_job_nodes": float(js["nodes"].mean()),
+                              "us_per_node_on_busiest": kernel_ms * 1000.0 / max(1, int(js["nodes"][busiest]
+                                                                                    + js["conflicts"][busiest]))},
+        },
+        "cpu_baseline": cpu,
+        "parity": parity,
+        "configs": nested,
+    }
+    print(json.dumps(out), flush=True)
+    bad = [parity[k] for k in ("oracle_search_mismatches", "reference_path_mismatches", "dpll_mismatches",
+                               "end_to_end_mismatches", "undecided") if parity.get(k)]
+    nested_bad = any(v.get("status") == "mismatch" for v in nested.values())
+    return 1 if bad or nested_bad else 0
+
+
+def bench_batched_host(args) -> int:
+    """The earlier batched pipeline: host cube planning to width W, then one persistent
+    contraction launch over device-resident jobs (kernel time only in ``value``)."""
     import torch
 
     from tensorsat_amd.solver import BatchSolver
@@ -307,14 +597,16 @@ def _step_bytes(steps, elem: int = 1) -> float:
     return float(b.sum()) * elem
 
 
-def bench_single(args) -> int:
+def run_single(args, seeds, log) -> dict:
     """
     One large random 3-SAT instance on the whole GPU (n=120, m=510, seeds 0..): conditioned
     slicing into cubes of width up to 26 (tensors up to 2^26 bytes), cubes decided one
     after the other with every step a grid-wide kernel (``single.decide``'s loop), until
     the first satisfiable cube.  Reported: plan and GPU seconds per instance, and the step
     kernels' algorithmic bytes over the GPU time (HBM bound).  Verdicts are checked against
-    the independent DPLL.
+    the independent DPLL.  CPU baseline: the oracle's numpy step loop on the same cubes of
+    the first seed, one cube per process on all host cores, for a bounded wall time,
+    scaled to the cubes the GPU ran.
     """
     import torch
 
@@ -322,17 +614,16 @@ def bench_single(args) -> int:
     from tensorsat_amd.executor import DeviceProgram, run_job
     from tensorsat_amd.workloads import single_3sat
 
-    def log(msg):
-        print(f"[bench] {msg}", file=sys.stderr, flush=True)
-
     rows = []
     tot_bytes = tot_gpu = tot_plan = 0.0
     mism = 0
-    for seed in range(args.seeds):
+    progs = []
+    for seed in seeds:
         cnf = single_3sat(seed)
         t0 = time.perf_counter()
         prog = PG.sat_batch([cnf], width=26, plan_trials=2)
         plan_s = time.perf_counter() - t0
+        progs.append(prog)
         dp = DeviceProgram(prog)
         a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         torch.cuda.synchronize()
@@ -359,30 +650,57 @@ def bench_single(args) -> int:
         tot_bytes += nbytes
         tot_gpu += gpu_s
         tot_plan += plan_s
-        log(f"seed {seed}: verdict {verdict} (dpll {want}), plan {plan_s:.2f}s, gpu {gpu_s:.3f}s, "
+        log(f"n=120 seed {seed}: verdict {verdict} (dpll {want}), plan {plan_s:.2f}s, gpu {gpu_s:.3f}s, "
             f"{ran}/{len(prog.jobs)} cubes, {nbytes / gpu_s / 1e9:.0f} GB/s")
     gbs = tot_bytes / tot_gpu / 1e9
-    print(json.dumps({
+    cpu = None
+    if not args.no_cpu:
+        cores = host_cores()
+        prog, ran = progs[0], rows[0]["cubes_run"]
+        k = min(ran, cores)
+        t0 = time.perf_counter()
+        with _pool(prog, cores) as ex:
+            list(ex.map(_pool_job, range(k), [args.mode] * k))
+        dt = time.perf_counter() - t0
+        per_inst = dt * ran / k
+        cpu = {"value": per_inst, "unit": "s per instance", "cores": cores, "kind": "port", "cpu_model": cpu_model(),
+               "sample": f"seed {rows[0]['seed']}: the first {k} of the {ran} cubes the GPU ran, through the "
+                         f"oracle's numpy step loop in {cores} processes, {dt:.1f}s wall, scaled by {ran}/{k}"}
+        log(f"n=120 cpu: {k} cubes in {dt:.1f}s -> {per_inst:.1f}s per instance")
+    return {
         "metric": "plan+contract s per instance, single n=120 3-SAT decision (bit-exact)",
-        "value": (tot_plan + tot_gpu) / len(rows), "unit": "s", "n_gpus": 1, "steps": len(rows), "warmup": 0,
-        "ms_per_step": tot_gpu * 1000.0 / len(rows), "higher_is_better": False, "scaling": "weak",
-        "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+        "value": (tot_plan + tot_gpu) / len(rows), "unit": "s", "steps": len(rows),
+        "ms_per_step": tot_gpu * 1000.0 / len(rows), "higher_is_better": False, "dtype": "u8",
         "config": {"workload": "single random 3-SAT n=120 m=510 (seeds 0..), cubes of width <= 26, grid-wide steps",
                    "plan_s_mean": tot_plan / len(rows), "gpu_s_mean": tot_gpu / len(rows), "rows": rows},
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
-                     "traffic": None, "kernel": "k_step_u8 (all steps of the executed cubes, launch gaps included)"},
-        "cpu_baseline": None,
+                     "traffic": None, "kernel": "k_step_u8 / k_step_mfma (all steps of the executed cubes, "
+                                                "launch gaps included)"},
+        "cpu_baseline": cpu,
         "parity": {"checked": len(rows), "dpll_mismatches": mism},
-    }), flush=True)
-    return 0 if mism == 0 else 1
+        "status": "mismatch" if mism else "ok",
+    }
 
 
-def bench_count(args) -> int:
+def bench_single(args) -> int:
+    def log(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    line = run_single(args, range(args.seeds), log)
+    line.update({"n_gpus": 1, "warmup": 0, "scaling": "weak", "vs_baseline": None, "data": "synthetic"})
+    print(json.dumps(line), flush=True)
+    return 0 if line["status"] == "ok" else 1
+
+
+def run_count(args, seeds, log) -> dict:
     """
     Exact model counts (#SAT) of random 3-SAT n=100, m=300 (seeds 0..): cubes of width 18
     counted in uint64 by the persistent counting kernel, each cube's result times
-    2^free added per instance with overflow detection.  Parity: the same count from a
-    second decomposition (width 16) - a dropped, doubled or wrapped term would change it.
+    2^free added per instance with overflow detection.  Parity: the independent model
+    counter's count (``oracle/sharp.c``, committed in tests/golden for seeds 0..9) and a
+    second decomposition (width 16).  CPU baseline: the oracle's counting step loop on a
+    bounded sample of the same cubes, one cube per process on all host cores, scaled to
+    all cubes.
     """
     import torch
 
@@ -390,17 +708,20 @@ def bench_count(args) -> int:
     from tensorsat_amd.executor import BatchCounter
     from tensorsat_amd.workloads import count_3sat
 
-    def log(msg):
-        print(f"[bench] {msg}", file=sys.stderr, flush=True)
-
+    golden = {}
+    gpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "sharp_n100_counts.json")
+    if os.path.exists(gpath):
+        golden = {int(k): int(v) for k, v in json.load(open(gpath))["counts"].items()}
     rows = []
     mism = 0
     tot_gpu = tot_plan = tot_bytes = 0.0
-    for seed in range(args.seeds):
+    progs = []
+    for seed in seeds:
         cnf = count_3sat(seed)
         t0 = time.perf_counter()
         prog = PG.sat_batch([cnf], width=18, plan_trials=1)
         plan_s = time.perf_counter() - t0
+        progs.append(prog)
         bc = BatchCounter(prog)
         bc.run()
         a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -412,29 +733,58 @@ def bench_count(args) -> int:
         gpu_s = a.elapsed_time(b) / 1000.0
         (c18,) = bc.run()
         (c16,) = BatchCounter(PG.sat_batch([cnf], width=16, plan_trials=1)).run()
-        mism += int(c18 != c16)
+        want = golden.get(seed)
+        mism += int(c18 != c16) + int(want is not None and c18 != want)
         nbytes = _step_bytes(prog.steps, 8)
-        rows.append({"seed": seed, "count": str(c18), "count_w16": str(c16), "plan_s": plan_s, "gpu_s": gpu_s,
+        rows.append({"seed": seed, "count": str(c18), "count_w16": str(c16),
+                     "independent_count": None if want is None else str(want), "plan_s": plan_s, "gpu_s": gpu_s,
                      "cubes": len(prog.jobs), "bytes": nbytes})
         tot_gpu += gpu_s
         tot_plan += plan_s
         tot_bytes += nbytes
-        log(f"seed {seed}: count {c18} (width 16: {c16}), plan {plan_s:.2f}s, gpu {gpu_s:.3f}s, {len(prog.jobs)} cubes")
+        log(f"n=100 seed {seed}: count {c18} (width 16: {c16}, independent: {want}), plan {plan_s:.2f}s, "
+            f"gpu {gpu_s:.3f}s, {len(prog.jobs)} cubes")
     gbs = tot_bytes / tot_gpu / 1e9
-    print(json.dumps({
+    cpu = None
+    if not args.no_cpu:
+        from concurrent.futures import wait
+
+        cores = host_cores()
+        prog = progs[0]
+        nj = len(prog.jobs)
+        sample = np.random.default_rng(0).permutation(nj)[: min(nj, 64 * cores)]
+        t0 = time.perf_counter()
+        with _pool(prog, cores) as ex:
+            wait([ex.submit(_pool_job, int(j), "COUNT") for j in sample])
+        dt = time.perf_counter() - t0
+        per_inst = dt * nj / len(sample)
+        cpu = {"value": per_inst, "unit": "s per instance", "cores": cores, "kind": "port", "cpu_model": cpu_model(),
+               "sample": f"seed {rows[0]['seed']}: {len(sample)} random cubes of {nj} through the oracle's "
+                         f"counting step loop in {cores} processes, {dt:.1f}s wall, scaled by {nj}/{len(sample)}"}
+        log(f"n=100 cpu: {len(sample)} cubes in {dt:.1f}s -> {per_inst:.1f}s per instance")
+    return {
         "metric": "plan+contract s per instance, exact #SAT n=100 m=300 (uint64, checked)",
-        "value": (tot_plan + tot_gpu) / len(rows), "unit": "s", "n_gpus": 1, "steps": len(rows), "warmup": 1,
-        "ms_per_step": tot_gpu * 1000.0 / len(rows), "higher_is_better": False, "scaling": "weak",
-        "vs_baseline": None, "dtype": "u64", "data": "synthetic",
+        "value": (tot_plan + tot_gpu) / len(rows), "unit": "s", "steps": len(rows),
+        "ms_per_step": tot_gpu * 1000.0 / len(rows), "higher_is_better": False, "dtype": "u64",
         "config": {"workload": "#SAT of random 3-SAT n=100 m=300 (seeds 0..), cubes of width <= 18, "
                                "persistent uint64 counting kernel",
                    "plan_s_mean": tot_plan / len(rows), "gpu_s_mean": tot_gpu / len(rows), "rows": rows},
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
                      "traffic": None, "kernel": "k_batch_count"},
-        "cpu_baseline": None,
-        "parity": {"checked": len(rows), "decomposition_mismatches": mism},
-    }), flush=True)
-    return 0 if mism == 0 else 1
+        "cpu_baseline": cpu,
+        "parity": {"checked": len(rows), "mismatches": mism},
+        "status": "mismatch" if mism else "ok",
+    }
+
+
+def bench_count(args) -> int:
+    def log(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    line = run_count(args, range(args.seeds), log)
+    line.update({"n_gpus": 1, "warmup": 1, "scaling": "weak", "vs_baseline": None, "data": "synthetic"})
+    print(json.dumps(line), flush=True)
+    return 0 if line["status"] == "ok" else 1
 
 
 def bench_rewire(args) -> int:

@@ -185,6 +185,23 @@ int tsc_program_copy(const ts_program* p, ts_job* jobs, ts_step* steps, uint8_t*
                      int32_t* inst_stats);
 void tsc_program_free(ts_program* p);
 
+/* ---- packing for the device cube search (ts_search_run, tensorsat_amd.h) -------- */
+
+/* Pack `num` CNF instances into the device search's clause words and instance records
+ * (16-byte records {clause_begin, num_clauses, num_vars, 0}).  Instance i has
+ * nvars[i] variables and nclauses[i] clauses; its clause lengths are lens[i][c] and
+ * its literals (+-(v+1)) are lits[i][...] concatenated.  Duplicate literals are
+ * dropped, tautologies removed, clause order kept.  words_out holds at least
+ * sum(nclauses) words; *total_out receives the number written.  Returns TSP_EINVAL for
+ * a literal out of range, TSC_EUNSUPPORTED for an instance outside the kernel's limits
+ * (more than 128 variables, more than 768 clauses after normalisation, or a clause of
+ * more than 3 distinct literals); *bad_out then names the instance.  No counterpart in
+ * the reference (it builds Python diagrams, lib/sat.py:248-262). */
+#define TSC_EUNSUPPORTED (-3)
+int tsc_search_pack(int32_t num, const int32_t* nvars, const int64_t* nclauses, const int32_t* const* lens,
+                    const int32_t* const* lits, uint32_t* words_out, int32_t* inst_out, int64_t* total_out,
+                    int32_t* bad_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -24,7 +24,7 @@ INCLUDE = ROOT / "include"
 PLAN_LIB = PKG / "libtsplan.so"
 HIP_LIB = PKG / "libtensorsat_amd.so"
 
-PLAN_SOURCES = ["plan.cpp", "prog.cpp", "cubes.cpp", "packets.cpp"]
+PLAN_SOURCES = ["plan.cpp", "prog.cpp", "cubes.cpp", "packets.cpp", "searchpack.cpp"]
 HIP_SOURCES = ["executor.hip", "dense.hip", "batch.hip", "gnn.hip", "search.hip"]
 
 HIPCC_FLAGS = [

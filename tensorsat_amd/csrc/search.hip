@@ -17,10 +17,15 @@
 // packed words (three literal bytes (v << 1) | negated and a length byte); the partial
 // assignment is wave-uniform (two 128-bit masks: assigned, value) and lives in scalar
 // registers; the branch stack (assignment before the branch, branch literal, next child)
-// and the branching histogram are in LDS.  Propagation runs in rounds: every lane
-// evaluates its clauses, a ballot finds conflicts, and the unit literals of the round are
-// applied by the scalar unit one lane at a time (readlane), which also detects a
-// variable forced both ways.  The fixpoint is the same as any sequential order's.
+// and the branching histogram are in LDS.  A 256-byte table in LDS gives the status of
+// every literal byte under the current assignment (free, false, true), rebuilt from the
+// masks (one dword per lane) whenever they change, and a constant 256-entry table maps a
+// clause's length and its three literal statuses to (satisfied, free literals, position
+// of a free literal): a clause costs three byte extracts and four LDS reads.
+// Propagation runs in rounds: every lane evaluates its clauses, a ballot finds
+// conflicts, and the unit literals of the round (one per lane) are applied by the
+// scalar unit one lane at a time (readlane), which also detects a variable forced both
+// ways.  The fixpoint is the same as any sequential order's.
 //
 // Every loop is bounded: propagation rounds by the number of variables (each round
 // assigns at least one), the branch stack by the number of variables, and each job by
@@ -70,32 +75,6 @@ struct OpAnd {
   __device__ uint32_t operator()(uint32_t a, uint32_t b) const { return a & b; }
 };
 
-__device__ __forceinline__ bool bit_of(uint64_t lo, uint64_t hi, uint32_t v) {
-  return (((v & 64) ? hi : lo) >> (v & 63)) & 1ull;
-}
-
-// Clause status under the assignment: bit 0 satisfied; bits 1..2 number of free
-// literals; bits 8..15 the last free literal byte.  An empty slot reads as satisfied.
-__device__ __forceinline__ uint32_t clause_state(uint32_t c, uint64_t a0, uint64_t a1, uint64_t v0, uint64_t v1) {
-  const uint32_t len = c >> 24;
-  if (len == NO_CLAUSE) return 1u;
-  uint32_t sat = 0, nfree = 0, lit = 0;
-#pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    if ((uint32_t)i < len) {
-      const uint32_t b = (c >> (8 * i)) & 0xFFu, v = b >> 1;
-      const bool asg = bit_of(a0, a1, v);
-      const bool val = bit_of(v0, v1, v);
-      sat |= (asg && (val != (bool)(b & 1u))) ? 1u : 0u;
-      if (!asg) {
-        ++nfree;
-        lit = b;
-      }
-    }
-  }
-  return sat | (nfree << 1) | (lit << 8);
-}
-
 struct Assign {
   uint64_t a0, a1, v0, v1;
 };
@@ -108,11 +87,73 @@ __device__ __forceinline__ void fresh(uint32_t (&cl)[SLOTS]) {
   for (int k = 0; k < SLOTS; ++k) asm volatile("" : "+v"(cl[k]));
 }
 
+// Literal statuses (byte b = (v << 1) | negated): 0 free, 1 false, 2 true.
+constexpr uint32_t LIT_FREE = 0, LIT_FALSE = 1, LIT_TRUE = 2;
+// Clause status word: bit 0 satisfied, bits 1..2 free literals, bits 3..4 position of
+// the last free literal.
+__device__ __forceinline__ uint32_t cs_sat(uint32_t r) { return r & 1u; }
+__device__ __forceinline__ uint32_t cs_nfree(uint32_t r) { return (r >> 1) & 3u; }
+__device__ __forceinline__ uint32_t cs_pos(uint32_t r) { return (r >> 3) & 3u; }
+
+// Rebuild the literal table from the masks: lane l writes the statuses of variables 2l
+// and 2l + 1, both polarities (bytes 4l .. 4l + 3).
+__device__ __forceinline__ void build_lit_table(uint32_t* lt, const Assign& s, int lane) {
+  const int sh = (2 * lane) & 63;
+  const uint64_t am = lane < 32 ? s.a0 : s.a1;
+  const uint64_t vm = lane < 32 ? s.v0 : s.v1;
+  const uint32_t a = (uint32_t)(am >> sh) & 3u, v = (uint32_t)(vm >> sh) & 3u;
+  // per variable: assigned -> (positive literal, negative literal) = value ? (T, F) : (F, T)
+  const uint32_t p0 = (a & 1u) ? ((v & 1u) ? LIT_TRUE : LIT_FALSE) : LIT_FREE;
+  const uint32_t n0 = (a & 1u) ? ((v & 1u) ? LIT_FALSE : LIT_TRUE) : LIT_FREE;
+  const uint32_t p1 = (a & 2u) ? ((v & 2u) ? LIT_TRUE : LIT_FALSE) : LIT_FREE;
+  const uint32_t n1 = (a & 2u) ? ((v & 2u) ? LIT_FALSE : LIT_TRUE) : LIT_FREE;
+  lt[lane] = p0 | (n0 << 8) | (p1 << 16) | (n1 << 24);
+}
+
+// The clause-status table: entry (len << 6 | x2 << 4 | x1 << 2 | x0) for literal
+// statuses x_i of the first len literals.  Lane l writes entries 4l .. 4l + 3.
+__device__ __forceinline__ void build_clause_table(uint32_t* ct, int lane) {
+  uint32_t word = 0;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const uint32_t idx = 4u * (uint32_t)lane + (uint32_t)e, len = idx >> 6;
+    uint32_t sat = 0, nfree = 0, pos = 0;
+    for (uint32_t i = 0; i < len; ++i) {
+      const uint32_t x = (idx >> (2 * i)) & 3u;
+      sat |= x == LIT_TRUE;
+      if (x == LIT_FREE) {
+        ++nfree;
+        pos = i;
+      }
+    }
+    word |= (sat | (nfree << 1) | (pos << 3)) << (8 * e);
+  }
+  ct[lane] = word;
+}
+
+__device__ __forceinline__ uint32_t lds_byte(const uint32_t* t, uint32_t i) {
+  return reinterpret_cast<const uint8_t*>(t)[i];
+}
+
+// Status word of a clause (an empty slot reads as satisfied).
+__device__ __forceinline__ uint32_t clause_state(uint32_t c, const uint32_t* lt, const uint32_t* ct) {
+  const uint32_t x0 = lds_byte(lt, c & 0xFFu);
+  const uint32_t x1 = lds_byte(lt, (c >> 8) & 0xFFu);
+  const uint32_t x2 = lds_byte(lt, (c >> 16) & 0xFFu);
+  const uint32_t len = c >> 24;
+  const uint32_t r = lds_byte(ct, ((len & 3u) << 6) | (x2 << 4) | (x1 << 2) | x0);
+  return len == NO_CLAUSE ? 1u : r;
+}
+
 // Unit propagation to the fixpoint.  Returns false on a conflict.  On success `st`
-// holds every slot's clause_state under the final assignment.  Each round every lane
-// offers at most one unit literal (its first unit clause's); the scalar unit applies
-// them lane by lane, so each round assigns at least one new variable.
-__device__ __forceinline__ bool propagate(uint32_t (&cl)[SLOTS], int nslots, Assign& s, uint32_t (&st)[SLOTS]) {
+// holds every slot's clause status under the final assignment and the literal table
+// matches it.  Each round every lane offers at most one unit literal (its first unit
+// clause's); the scalar unit applies them lane by lane, so each round assigns at least
+// one new variable.
+__device__ __forceinline__ bool propagate(uint32_t (&cl)[SLOTS], int nslots, Assign& s, uint32_t (&st)[SLOTS],
+                                          uint32_t* lt, const uint32_t* ct, int lane) {
+  build_lit_table(lt, s, lane);
+  __syncthreads();
   for (int round = 0; round <= MAXV; ++round) {
     fresh(cl);
     bool conflict = false;
@@ -120,11 +161,11 @@ __device__ __forceinline__ bool propagate(uint32_t (&cl)[SLOTS], int nslots, Ass
 #pragma unroll
     for (int k = 0; k < SLOTS; ++k) {
       if (k < nslots) {
-        st[k] = clause_state(cl[k], s.a0, s.a1, s.v0, s.v1);
-        if (!(st[k] & 1u)) {
-          const uint32_t nfree = (st[k] >> 1) & 3u;
+        st[k] = clause_state(cl[k], lt, ct);
+        if (!cs_sat(st[k])) {
+          const uint32_t nfree = cs_nfree(st[k]);
           conflict |= nfree == 0;
-          if (nfree == 1 && unit == NO_UNIT) unit = (st[k] >> 8) & 0xFFu;
+          if (nfree == 1 && unit == NO_UNIT) unit = (cl[k] >> (8 * cs_pos(st[k]))) & 0xFFu;
         }
       }
     }
@@ -153,6 +194,8 @@ __device__ __forceinline__ bool propagate(uint32_t (&cl)[SLOTS], int nslots, Ass
         s.v0 |= want ? bit : 0ull;
       }
     }
+    build_lit_table(lt, s, lane);
+    __syncthreads();
   }
   return true;  // unreachable: every round with a unit assigns a new variable
 }
@@ -179,6 +222,8 @@ __global__ __launch_bounds__(64) void k_search(ts_search p) {
   __shared__ uint32_t hist[MAXV];          // score << 20 | positive << 10 | negative
   __shared__ uint64_t stk[4][MAXV + 1];    // assignment before the branch at each depth
   __shared__ uint32_t stk_lit[MAXV + 1];   // branch literal byte | next child << 8
+  __shared__ uint32_t lit_table[64];       // status byte of each literal byte
+  __shared__ uint32_t clause_table[64];    // status word of each (length, statuses) index
 
   const int lane = (int)threadIdx.x;
   const ts_search_job job = p.jobs[blockIdx.x];
@@ -197,6 +242,7 @@ __global__ __launch_bounds__(64) void k_search(ts_search p) {
   }
   hist[lane] = 0;
   hist[lane + 64] = 0;
+  build_clause_table(clause_table, lane);
   __syncthreads();
 
   int64_t nodes = 0, leaves = 0, conflicts = 0, visited = 0;
@@ -242,7 +288,7 @@ __global__ __launch_bounds__(64) void k_search(ts_search p) {
       continue;
     }
     // ---- propagate the node at depth cd ----
-    if (!propagate(cl, nslots, s, st)) {
+    if (!propagate(cl, nslots, s, st, lit_table, clause_table, lane)) {
       if (owns(cd, split, prefix)) ++conflicts;
       if (cd == 0) break;
       fd = cd - 1;
@@ -269,14 +315,14 @@ __global__ __launch_bounds__(64) void k_search(ts_search p) {
     fresh(cl);
 #pragma unroll
     for (int k = 0; k < SLOTS; ++k) {
-      if (k < nslots && !(st[k] & 1u)) {
+      if (k < nslots && !cs_sat(st[k])) {
         const uint32_t c = cl[k], len = c >> 24;
-        const uint32_t w = (((st[k] >> 1) & 3u) - 1u) << 20;
+        const uint32_t w = (cs_nfree(st[k]) - 1u) << 20;
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
           if ((uint32_t)i < len) {
-            const uint32_t b = (c >> (8 * i)) & 0xFFu, v = b >> 1;
-            if (!bit_of(s.a0, s.a1, v)) atomicAdd(&hist[v], w | ((b & 1u) ? 1u : (1u << 10)));
+            const uint32_t b = (c >> (8 * i)) & 0xFFu;
+            if (lds_byte(lit_table, b) == LIT_FREE) atomicAdd(&hist[b >> 1], w | ((b & 1u) ? 1u : (1u << 10)));
           }
         }
       }
@@ -296,14 +342,14 @@ __global__ __launch_bounds__(64) void k_search(ts_search p) {
         fresh(cl);
 #pragma unroll
         for (int k = 0; k < SLOTS; ++k) {
-          if (k < nslots && !(st[k] & 1u)) {
+          if (k < nslots && !cs_sat(st[k])) {
             const uint32_t c = cl[k], len = c >> 24;
             uint64_t fals = ~0ull;
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
               if ((uint32_t)i < len) {
                 const uint32_t b = (c >> (8 * i)) & 0xFFu, v = b >> 1;
-                if (!bit_of(s.a0, s.a1, v)) {  // assigned literals of a residual clause are false
+                if (lds_byte(lit_table, b) == LIT_FREE) {  // assigned literals of a residual clause are false
                   const uint32_t pos = (v & 64) ? (uint32_t)(__popcll(r0) + __popcll(r1 & ((1ull << (v & 63)) - 1ull)))
                                                 : (uint32_t)__popcll(r0 & ((1ull << v) - 1ull));
                   const uint64_t ones = entries_with_bit(pos);

@@ -55,45 +55,6 @@ class _Search(ctypes.Structure):
     ]
 
 
-def _pad3(lens: np.ndarray, flat: np.ndarray) -> np.ndarray:
-    """(clauses, 3) matrix of literals, 0-padded; clauses must have <= 3 literals."""
-    m = lens.size
-    mat = np.zeros((m, 3), dtype=np.int32)
-    if m:
-        starts = np.zeros(m, dtype=np.int64)
-        np.cumsum(lens[:-1], out=starts[1:])
-        cid = np.repeat(np.arange(m), lens)
-        pos = np.arange(flat.size) - np.repeat(starts, lens)
-        mat[cid, pos] = flat
-    return mat
-
-
-def pack_clauses(lens: np.ndarray, flat: np.ndarray) -> np.ndarray:
-    """Packed clause words of one or more concatenated instances (``TS_SEARCH_CLAUSE``
-    layout): duplicate literals dropped (first occurrence kept), tautologies removed
-    (a removed clause is returned as word 0xFFFFFFFF for the caller to drop), clause
-    order kept.  Clauses must have at most 3 literals."""
-    mat = _pad3(lens, flat)
-    a, b, c = mat[:, 0], mat[:, 1], mat[:, 2]
-    keep_b = (b != 0) & (b != a)
-    keep_c = (c != 0) & (c != a) & (c != b)
-    taut = ((a != 0) & ((a == -b) | (a == -c))) | ((b != 0) & (b == -c))
-    # compact the kept literals to the front
-    out = np.zeros_like(mat)
-    out[:, 0] = a
-    n1 = (a != 0).astype(np.int32)
-    rows = np.arange(mat.shape[0])
-    out[rows[keep_b], n1[keep_b]] = b[keep_b]
-    n2 = n1 + keep_b
-    out[rows[keep_c], n2[keep_c]] = c[keep_c]
-    length = n2 + keep_c
-    lit_bytes = ((np.abs(out) - 1) << 1) | (out < 0)
-    lit_bytes = np.where(out != 0, lit_bytes, 0).astype(np.uint32)
-    words = lit_bytes[:, 0] | (lit_bytes[:, 1] << 8) | (lit_bytes[:, 2] << 16) | (length.astype(np.uint32) << 24)
-    words[taut] = 0xFFFFFFFF
-    return words
-
-
 def supported(cnf) -> bool:
     """Whether the device search takes this instance (before normalisation limits on
     the clause count, which :func:`pack_batch` checks exactly)."""
@@ -102,32 +63,30 @@ def supported(cnf) -> bool:
 
 
 def pack_batch(cnfs) -> tuple[np.ndarray, np.ndarray]:
-    """(packed clauses, per-instance records) of a batch; raises NotImplementedError for
-    an instance outside the kernel's limits."""
+    """(packed clauses, per-instance records) of a batch, by the host library's packer
+    (``tsc_search_pack``: duplicate literals dropped, tautologies removed, clause order
+    kept); raises NotImplementedError for an instance outside the kernel's limits and
+    ValueError for a literal out of range."""
+    from . import program as PG
+
+    L = PG._lib()
+    num = len(cnfs)
     arrays = [c.literal_arrays() for c in cnfs]
-    nv = np.fromiter((c.num_vars for c in cnfs), dtype=np.int32, count=len(cnfs))
-    if nv.size and int(nv.max()) > MAX_VARS:
-        raise NotImplementedError(f"device search takes at most {MAX_VARS} variables per instance")
-    lens = np.concatenate([a[0] for a in arrays]) if arrays else np.zeros(0, np.int32)
-    flat = np.concatenate([a[1] for a in arrays]) if arrays else np.zeros(0, np.int32)
-    if lens.size and int(lens.max()) > 3:
-        raise NotImplementedError("device search takes clauses of at most 3 literals")
-    words = pack_clauses(lens, flat)
-    per = np.fromiter((a[0].size for a in arrays), dtype=np.int64, count=len(cnfs))
-    owner = np.repeat(np.arange(len(cnfs)), per)
-    keep = words != 0xFFFFFFFF
-    words, owner = words[keep], owner[keep]
-    counts = np.bincount(owner, minlength=len(cnfs)).astype(np.int64)
-    if counts.size and int(counts.max()) > MAX_CLAUSES:
-        raise NotImplementedError(f"device search takes at most {MAX_CLAUSES} clauses per instance")
-    inst = np.zeros(len(cnfs), dtype=INST_DTYPE)
-    if len(cnfs):
-        begin = np.zeros(len(cnfs), dtype=np.int64)
-        np.cumsum(counts[:-1], out=begin[1:])
-        inst["clause_begin"] = begin
-        inst["num_clauses"] = counts
-        inst["num_vars"] = nv
-    return np.ascontiguousarray(words, dtype=np.uint32), inst
+    nv = np.fromiter((c.num_vars for c in cnfs), dtype=np.int32, count=num)
+    ncl = np.fromiter((a[0].size for a in arrays), dtype=np.int64, count=num)
+    lens_p = (ctypes.c_void_p * max(1, num))(*[a[0].ctypes.data for a in arrays])
+    lits_p = (ctypes.c_void_p * max(1, num))(*[a[1].ctypes.data for a in arrays])
+    words = np.empty(max(1, int(ncl.sum())), dtype=np.uint32)
+    inst = np.zeros(num, dtype=INST_DTYPE)
+    total, bad = ctypes.c_int64(0), ctypes.c_int32(-1)
+    rc = L.tsc_search_pack(num, nv.ctypes.data, ncl.ctypes.data, lens_p, lits_p, words.ctypes.data,
+                           inst.ctypes.data if num else None, ctypes.byref(total), ctypes.byref(bad))
+    if rc == -3:
+        raise NotImplementedError(f"instance {bad.value} is outside the device search's limits ({MAX_VARS} "
+                                  f"variables, {MAX_CLAUSES} clauses, 3 distinct literals per clause)")
+    if rc != 0:
+        raise ValueError(f"instance {bad.value}: literal out of range")
+    return words[: total.value], inst
 
 
 def log2_search_size(num_vars: np.ndarray, num_clauses: np.ndarray) -> np.ndarray:
@@ -169,8 +128,10 @@ def plan_jobs(inst: np.ndarray, *, split_target: float | None = 128.0, max_split
     return jobs[order]
 
 
-def validate(words: np.ndarray, inst: np.ndarray, jobs: np.ndarray, width: int) -> None:
-    """Host-side bounds check of everything the kernel indexes, before any upload."""
+def validate(words: np.ndarray, inst: np.ndarray, jobs: np.ndarray, width: int, *, literals: bool = True) -> None:
+    """Host-side bounds check of everything the kernel indexes, before any upload
+    (``literals=False`` skips the per-literal variable check, for clause words made by
+    :func:`pack_batch`, which checks them as it packs)."""
     if not 0 <= width <= MAX_WIDTH:
         raise ValueError(f"width must be in 0..{MAX_WIDTH}")
     if len(inst):
@@ -181,7 +142,7 @@ def validate(words: np.ndarray, inst: np.ndarray, jobs: np.ndarray, width: int) 
         end = inst["clause_begin"].astype(np.int64) + inst["num_clauses"]
         if np.any(inst["clause_begin"] < 0) or np.any(end > words.size):
             raise ValueError("clause range outside the clause array")
-        if words.size:
+        if words.size and literals:
             length = words >> 24
             if np.any(length > 3):
                 raise ValueError("packed clause with more than 3 literals")
@@ -235,7 +196,7 @@ class SearchSolver:
         words, inst = pack_batch(cnfs)
         t1 = time.perf_counter()
         jobs = plan_jobs(inst, split_target=self.split_target, max_split=self.max_split)
-        validate(words, inst, jobs, self.width)
+        validate(words, inst, jobs, self.width, literals=False)  # the packer checked every literal
         t2 = time.perf_counter()
         self.upload(words, inst, jobs)
         torch.cuda.synchronize()
@@ -296,3 +257,40 @@ class SearchSolver:
         self.launch()
         torch.cuda.synchronize()
         return self.verdicts()
+
+
+def autotune(cnfs, *, widths=(4, 6), split_targets=(None, 512.0, 128.0, 32.0, 8.0), max_split: int = 8,
+             reps: int = 3) -> tuple[dict, dict]:
+    """Pick the residual width and the split target for a batch: every candidate decides
+    the batch ``reps`` times on the GPU (device-resident inputs, events around each
+    launch); the fastest mean launch wins.  Verdicts of every candidate must agree (they
+    are the same decisions), else ``RuntimeError``.  Returns (choice, table)."""
+    import torch
+
+    words, inst = pack_batch(cnfs)
+    table = {}
+    ref = None
+    for w in widths:
+        for tgt in split_targets:
+            s = SearchSolver(width=w, split_target=tgt, max_split=max_split)
+            jobs = plan_jobs(inst, split_target=tgt, max_split=max_split)
+            validate(words, inst, jobs, w)
+            s.upload(words, inst, jobs)
+            s.launch()
+            torch.cuda.synchronize()
+            v = s.verdicts()
+            if ref is None:
+                ref = v
+            elif not np.array_equal(v, ref):
+                raise RuntimeError(f"verdicts differ between search configurations (width {w}, split {tgt})")
+            ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+            for a, b in ev:
+                a.record()
+                s.launch()
+                b.record()
+            torch.cuda.synchronize()
+            ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+            table[(w, tgt)] = {"width": w, "split_target": tgt, "jobs": int(len(jobs)),
+                               "max_split": int(jobs["split"].max()) if len(jobs) else 0, "ms": ms}
+    best = min(table.values(), key=lambda r: r["ms"])
+    return {"width": best["width"], "split_target": best["split_target"]}, table
