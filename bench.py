@@ -122,6 +122,8 @@ def main() -> int:
     ap.add_argument("--split", default="auto", help="batched: split target in branch nodes per job, 'none', "
                     "or 'auto' (autotuned with the width)")
     ap.add_argument("--no-nested", action="store_true", help="batched: skip the nested single / count lines")
+    ap.add_argument("--budget-nodes", default="auto", help="batched: nodes a search job runs before it hands "
+                    "its remaining subtrees over, 'none', or 'auto'")
     ap.add_argument("--seeds", type=int, default=3, help="single / count workloads: instances (seeds 0..)")
     ap.add_argument("--learned-instances", type=int, default=8)
     ap.add_argument("--learned-width", type=int, default=22, help="learned workload: cube width W")
@@ -269,17 +271,20 @@ def bench_search(args) -> int:
     if args.width == "auto" or args.split == "auto":
         t0 = time.perf_counter()
         widths = (4, 6) if args.width == "auto" else (int(args.width),)
-        splits = (None, 512.0, 128.0, 32.0, 8.0) if args.split == "auto" else \
+        splits = (None, 32.0, 8.0, 2.0) if args.split == "auto" else \
             ((None,) if args.split == "none" else (float(args.split),))
-        choice, table = SE.autotune(cnfs, widths=widths, split_targets=splits)
-        width, split = choice["width"], choice["split_target"]
+        budgets = (None, 128, 32, 8) if args.budget_nodes == "auto" else \
+            ((None,) if args.budget_nodes == "none" else (int(args.budget_nodes),))
+        choice, table = SE.autotune(cnfs, widths=widths, split_targets=splits, budgets=budgets)
+        width, split, budget = choice["width"], choice["split_target"], choice["budget"]
         tune = {"chosen": choice, "seconds": time.perf_counter() - t0, "candidates": list(table.values())}
-        log(f"autotuned width {width}, split target {split} in {tune['seconds']:.1f}s: " +
-            ", ".join(f"W{r['width']}/{r['split_target']}: {r['ms']:.3f}ms" for r in table.values()))
+        log(f"autotuned width {width}, split target {split}, budget {budget} in {tune['seconds']:.1f}s: " +
+            ", ".join(f"W{r['width']}/{r['split_target']}/{r['budget']}: {r['ms']:.2f}ms" for r in table.values()))
     else:
         width = int(args.width)
         split = None if args.split == "none" else float(args.split)
-    solver = SE.SearchSolver(width=width, split_target=split)
+        budget = None if args.budget_nodes in ("none", "auto") else int(args.budget_nodes)
+    solver = SE.SearchSolver(width=width, split_target=split, budget=budget)
     # End to end from the CNF list: pack, plan the jobs, upload, launch, verdicts back.
     e2e = []
     for _ in range(3):
@@ -313,18 +318,21 @@ def bench_search(args) -> int:
     value = len(cnfs) / (ms_per_step / 1000.0)
     jobs = solver.host_jobs
     # Critical path: the statistics of one more launch (per job branch nodes).
-    st = SE.SearchSolver(width=width, split_target=split, stats=True)
+    st = SE.SearchSolver(width=width, split_target=split, budget=budget, stats=True)
     st.prepare(cnfs)
     st.launch()
-    js = st.job_stats()
+    all_jobs, js = st.job_stats()
     nodes_total = int(js["nodes"].sum())
     busiest = int(np.argmax(js["nodes"] + js["conflicts"]))
-    log(f"search: width {width}, split target {split}, {len(jobs)} jobs (max split {int(jobs['split'].max())}), "
+    log(f"search: width {width}, split target {split}, budget {budget}: {len(jobs)} jobs (max split "
+        f"{int(jobs['split'].max())}), {solver.rounds} rounds ({sum(solver.round_jobs)} jobs), "
         f"{ms_per_step:.3f} ms/step (kernel {kernel_ms:.3f} ms) = {value / 1e6:.2f} M inst/s; end to end "
         f"{e2e_s * 1000:.1f} ms = {len(cnfs) / e2e_s / 1e3:.1f} k inst/s; {nodes_total} branch nodes, "
         f"busiest job {int(js['nodes'][busiest])} nodes")
     nclauses = int(solver.d_words.numel())
-    alg_bytes = 4.0 * nclauses + 16.0 * len(cnfs) + 16.0 * len(jobs) + 4.0 * len(cnfs)
+    # algorithmic bytes: the clauses and instance records once, every job record of every
+    # round once, the verdicts once
+    alg_bytes = 4.0 * nclauses + 16.0 * len(cnfs) + 64.0 * sum(solver.round_jobs) + 4.0 * len(cnfs)
     achieved = alg_bytes / (kernel_ms / 1000.0) / 1e9
 
     cores = host_cores()
@@ -384,12 +392,16 @@ def bench_search(args) -> int:
             "instances": len(cnfs),
             "width": width,
             "split_target": split,
+            "budget": budget,
+            "rounds": solver.rounds,
+            "jobs_per_round": solver.round_jobs,
             "jobs": int(len(jobs)),
             "max_split": int(jobs["split"].max()),
             "autotune": tune,
             "kernel": "k_search",
             "kernel_ms": kernel_ms,
-            "launch": "per step: verdict reset (fill kernel) + one k_search launch of one 64-lane wave per job",
+            "launch": "per step: verdict reset (fill kernel) + one k_search launch of one 64-lane wave per job, "
+                      "repeated for each round of handed-over subtrees (the host reads each round's job count)",
             "sat": int((verdicts == 1).sum()),
             "end_to_end_s": e2e_s,
             "end_to_end_instances_per_s": len(cnfs) / e2e_s,

@@ -214,7 +214,6 @@ int ts_batch_max_workgroups_lds(int64_t arena_bytes, int32_t one_wave, int32_t* 
 #define TS_SEARCH_MAX_VARS 128  /* variables per instance */
 #define TS_SEARCH_SLOTS 12      /* clauses per lane: at most 64 * 12 = 768 clauses */
 #define TS_SEARCH_MAX_WIDTH 6   /* residuals of at most 2^6 entries are contracted */
-#define TS_SEARCH_MAX_SPLIT 16  /* branch levels a job's prefix can fix */
 /* Packed clause: bits 8i..8i+7 hold literal i (i < 3) as (variable << 1) | negated,
  * bits 24..31 the number of literals (0..3). */
 #define TS_SEARCH_CLAUSE(len) ((uint32_t)(len) << 24)
@@ -227,20 +226,26 @@ typedef struct {
 } ts_search_inst;
 
 typedef struct {
-  int32_t instance;
-  int32_t split;          /* branch levels fixed by prefix (0..TS_SEARCH_MAX_SPLIT) */
-  uint32_t prefix;        /* bit k: child taken at depth k < split (0 = first literal) */
-  int32_t reserved;       /* 0 */
+  int32_t instance;       /* < 0: an empty slot, skipped */
+  int32_t depth;          /* depth of the job's start node */
+  int32_t split;          /* branch levels below the start node fixed by prefix (0..31) */
+  int32_t own_from;       /* the job counts (and budgets) the nodes at depth >= own_from;
+                             shallower nodes on its prefix path belong to another job */
+  uint32_t prefix;        /* bit k: child taken at depth `depth` + k, k < split
+                             (0 = the first literal) */
+  uint32_t reserved[3];   /* 0 */
+  uint64_t assigned[2];   /* assignment of the start node before propagation: variable v */
+  uint64_t value[2];      /* is bit v & 63 of word v >> 6 (the root: all zero) */
 } ts_search_job;
 
 typedef struct {
   int32_t status;         /* 0 complete, 1 stopped (instance decided by another job),
-                             2 node budget exhausted */
+                             2 node budget exhausted (no continuation list), 3 handed its
+                             remaining subtrees to the continuation list */
   int32_t max_depth;
-  int64_t nodes, leaves, conflicts; /* counted where the job owns the node (a node above
-                             its split depth belongs to the job whose remaining prefix bits
-                             are all 0), so the sums over an instance's jobs equal one
-                             unsplit search's in exhaustive mode */
+  int64_t nodes, leaves, conflicts; /* over the nodes the job owns: the sums over all jobs
+                             of an instance (continuations included) equal one unsplit
+                             search's in exhaustive mode */
   uint64_t digest;        /* sum of mix(relation bits, rank) over owned leaves */
 } ts_search_stat;
 
@@ -258,13 +263,22 @@ typedef struct {
   int32_t width;                   /* 0..TS_SEARCH_MAX_WIDTH */
   int32_t exhaustive;              /* 1: visit every branch (statistics) */
   int32_t early_exit;              /* 1: stop a job once its instance is decided SAT */
-  int32_t reserved;
-  int64_t max_nodes;               /* per job */
+  int32_t spill_capacity;          /* continuation list capacity (jobs) */
+  int64_t max_nodes;               /* per job: owned nodes before it hands off (with a
+                                      continuation list) or stops with status 2 */
+  ts_search_job* spill;            /* device continuation list (optional): a job out of
+                                      budget appends its current subtree and every untried
+                                      sibling subtree below its split as new jobs; when they
+                                      do not fit it writes empty slots and keeps searching */
+  int32_t* spill_count;            /* device, zeroed by the caller: jobs appended (may
+                                      exceed the capacity; only the first spill_capacity
+                                      slots exist) */
 } ts_search;
 
-/* One launch of num_jobs one-wave workgroups.  Rejects (TS_EINVAL) null arrays, widths
- * or splits out of range; instance sizes are the caller's to check (the host layer
- * validates them before upload). */
+/* One launch of num_jobs one-wave workgroups.  Rejects (TS_EINVAL) null arrays and widths
+ * out of range; job and instance records are the caller's to check (the host layer
+ * validates them before upload, and the continuations the kernel writes are valid by
+ * construction). */
 int ts_search_run(const ts_search* s, ts_stream stream);
 
 /* ---- Learned planner: message-passing network kernels (bf16 in, fp32 accumulate) --

@@ -195,12 +195,12 @@ void tsc_program_free(ts_program* p);
  * sum(nclauses) words; *total_out receives the number written.  Returns TSP_EINVAL for
  * a literal out of range, TSC_EUNSUPPORTED for an instance outside the kernel's limits
  * (more than 128 variables, more than 768 clauses after normalisation, or a clause of
- * more than 3 distinct literals); *bad_out then names the instance.  No counterpart in
- * the reference (it builds Python diagrams, lib/sat.py:248-262). */
+ * more than 3 distinct literals); *bad_out then names the first such instance.  No
+ * counterpart in the reference (it builds Python diagrams, lib/sat.py:248-262). */
 #define TSC_EUNSUPPORTED (-3)
 int tsc_search_pack(int32_t num, const int32_t* nvars, const int64_t* nclauses, const int32_t* const* lens,
                     const int32_t* const* lits, uint32_t* words_out, int32_t* inst_out, int64_t* total_out,
-                    int32_t* bad_out);
+                    int32_t* bad_out, int32_t threads /* <= 0: hardware concurrency */);
 
 #ifdef __cplusplus
 }

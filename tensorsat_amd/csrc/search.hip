@@ -200,10 +200,6 @@ __device__ __forceinline__ bool propagate(uint32_t (&cl)[SLOTS], int nslots, Ass
   return true;  // unreachable: every round with a unit assigns a new variable
 }
 
-__device__ __forceinline__ bool owns(int depth, int split, uint32_t prefix) {
-  return depth >= split || ((prefix >> depth) & ((1u << (split - depth)) - 1u)) == 0u;
-}
-
 // Entries e of a rank <= 6 relation whose bit p is 1, as a 64-bit mask.
 __device__ __forceinline__ uint64_t entries_with_bit(uint32_t p) {
   switch (p) {
@@ -227,193 +223,238 @@ __global__ __launch_bounds__(64) void k_search(ts_search p) {
 
   const int lane = (int)threadIdx.x;
   const ts_search_job job = p.jobs[blockIdx.x];
-  const ts_search_inst inst = p.instances[job.instance];
-  const int m = inst.num_clauses;
-  const int nslots = (m + 63) >> 6;
-  const int split = job.split;
-  const uint32_t prefix = job.prefix;
-  const int width = p.width;
-
-  uint32_t cl[SLOTS];
-#pragma unroll
-  for (int k = 0; k < SLOTS; ++k) {
-    const int i = k * 64 + lane;
-    cl[k] = (k < nslots && i < m) ? p.clauses[inst.clause_begin + i] : (NO_CLAUSE << 24);
-  }
-  hist[lane] = 0;
-  hist[lane + 64] = 0;
-  build_clause_table(clause_table, lane);
-  __syncthreads();
-
-  int64_t nodes = 0, leaves = 0, conflicts = 0, visited = 0;
+  int64_t nodes = 0, leaves = 0, conflicts = 0, owned = 0;
   uint64_t digest = 0;
   int status = 0, max_depth = 0;
+  if (job.instance >= 0) {
+    const ts_search_inst inst = p.instances[job.instance];
+    const int m = inst.num_clauses;
+    const int nslots = (m + 63) >> 6;
+    const int base = job.depth, split = job.split, own_from = job.own_from;
+    const uint32_t prefix = job.prefix;
+    const int width = p.width;
+    int64_t budget = p.max_nodes;
 
-  Assign s{0, 0, 0, 0};
-  uint32_t st[SLOTS];
-  // The loop alternates two phases.  Propagate: `s` is the assignment of the node at
-  // depth `cd` (the root, or the child of frame cd - 1) before propagation.  Next: try
-  // the next child of the branch frame at depth `fd`.
-  int cd = 0, fd = -1;
-  bool prop = true;
-  for (;;) {
-    if (!prop) {
-      const uint32_t fr = uni(stk_lit[fd]);
-      const uint32_t t = fr >> 8;
-      if (t >= 2) {
-        if (fd == 0) break;
-        --fd;
-        continue;
-      }
-      stk_lit[fd] = (fr & 0xFFu) | ((t + 1) << 8);
-      __syncthreads();
-      if (fd < split && t != ((prefix >> fd) & 1u)) continue;
-      const uint32_t b = (fr & 0xFFu) ^ t;  // the second child negates the literal
-      const uint32_t v = b >> 1;
-      s.a0 = uni64(stk[0][fd]);
-      s.a1 = uni64(stk[1][fd]);
-      s.v0 = uni64(stk[2][fd]);
-      s.v1 = uni64(stk[3][fd]);
-      const uint64_t bit = 1ull << (v & 63);
-      const uint64_t val = (b & 1u) ? 0ull : bit;
-      if (v & 64) {
-        s.a1 |= bit;
-        s.v1 |= val;
-      } else {
-        s.a0 |= bit;
-        s.v0 |= val;
-      }
-      cd = fd + 1;
-      prop = true;
-      continue;
-    }
-    // ---- propagate the node at depth cd ----
-    if (!propagate(cl, nslots, s, st, lit_table, clause_table, lane)) {
-      if (owns(cd, split, prefix)) ++conflicts;
-      if (cd == 0) break;
-      fd = cd - 1;
-      prop = false;
-      continue;
-    }
-    const int depth = cd;
-    if (p.early_exit &&
-        (__hip_atomic_load(&p.verdicts[job.instance], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & TS_SEARCH_SAT)) {
-      status = 1;
-      break;
-    }
-    if (visited >= p.max_nodes) {
-      status = 2;
-      break;
-    }
-    ++visited;
-    const bool own = owns(depth, split, prefix);
-    if (own) {
-      ++nodes;
-      max_depth = depth > max_depth ? depth : max_depth;
-    }
-    // Branching histogram over the residual clauses' free literals; it also names R.
-    fresh(cl);
+    uint32_t cl[SLOTS];
 #pragma unroll
     for (int k = 0; k < SLOTS; ++k) {
-      if (k < nslots && !cs_sat(st[k])) {
-        const uint32_t c = cl[k], len = c >> 24;
-        const uint32_t w = (cs_nfree(st[k]) - 1u) << 20;
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-          if ((uint32_t)i < len) {
-            const uint32_t b = (c >> (8 * i)) & 0xFFu;
-            if (lds_byte(lit_table, b) == LIT_FREE) atomicAdd(&hist[b >> 1], w | ((b & 1u) ? 1u : (1u << 10)));
-          }
-        }
-      }
+      const int i = k * 64 + lane;
+      cl[k] = (k < nslots && i < m) ? p.clauses[inst.clause_begin + i] : (NO_CLAUSE << 24);
     }
-    __syncthreads();
-    const uint32_t h0 = hist[lane], h1 = hist[lane + 64];
     hist[lane] = 0;
     hist[lane + 64] = 0;
-    const uint64_t r0 = __ballot(h0 != 0), r1 = __ballot(h1 != 0);
-    const int rank = __popcll(r0) + __popcll(r1);
-    if (rank <= width) {
-      // ---- leaf: the residual clause tensors multiplied into the rank-|R| relation
-      // (entry e: bit j of e = value of the j-th smallest variable of R), then reduced ----
-      uint64_t bits = 1;  // no residual clause: the empty product
-      if (rank > 0) {
-        uint64_t mine = ~0ull;  // entries satisfying this lane's residual clauses
-        fresh(cl);
-#pragma unroll
-        for (int k = 0; k < SLOTS; ++k) {
-          if (k < nslots && !cs_sat(st[k])) {
-            const uint32_t c = cl[k], len = c >> 24;
-            uint64_t fals = ~0ull;
-#pragma unroll
-            for (int i = 0; i < 3; ++i) {
-              if ((uint32_t)i < len) {
-                const uint32_t b = (c >> (8 * i)) & 0xFFu, v = b >> 1;
-                if (lds_byte(lit_table, b) == LIT_FREE) {  // assigned literals of a residual clause are false
-                  const uint32_t pos = (v & 64) ? (uint32_t)(__popcll(r0) + __popcll(r1 & ((1ull << (v & 63)) - 1ull)))
-                                                : (uint32_t)__popcll(r0 & ((1ull << v) - 1ull));
-                  const uint64_t ones = entries_with_bit(pos);
-                  fals &= (b & 1u) ? ones : ~ones;  // false where the variable equals the negation bit
-                }
-              }
-            }
-            mine &= ~fals;
-          }
+    build_clause_table(clause_table, lane);
+    __syncthreads();
+
+    Assign s{job.assigned[0], job.assigned[1], job.value[0], job.value[1]};
+    uint32_t st[SLOTS];
+    // The loop alternates two phases.  Propagate: `s` is the assignment of the node at
+    // depth `cd` (the start node, or the child of frame cd - 1) before propagation.
+    // Next: try the next child of the branch frame at depth `fd`.
+    int cd = base, fd = base - 1;
+    bool prop = true;
+    for (;;) {
+      if (!prop) {
+        const uint32_t fr = uni(stk_lit[fd]);
+        const uint32_t t = fr >> 8;
+        if (t >= 2) {
+          if (fd == base) break;
+          --fd;
+          continue;
         }
-        const uint32_t lo = wave_reduce((uint32_t)mine, OpAnd());
-        const uint32_t hi = wave_reduce((uint32_t)(mine >> 32), OpAnd());
-        const uint64_t valid = rank == 6 ? ~0ull : ((1ull << (1u << rank)) - 1ull);
-        bits = (((uint64_t)hi << 32) | lo) & valid;
+        stk_lit[fd] = (fr & 0xFFu) | ((t + 1) << 8);
+        __syncthreads();
+        if (fd - base < split && t != ((prefix >> (fd - base)) & 1u)) continue;
+        const uint32_t b = (fr & 0xFFu) ^ t;  // the second child negates the literal
+        const uint32_t v = b >> 1;
+        s.a0 = uni64(stk[0][fd]);
+        s.a1 = uni64(stk[1][fd]);
+        s.v0 = uni64(stk[2][fd]);
+        s.v1 = uni64(stk[3][fd]);
+        const uint64_t bit = 1ull << (v & 63);
+        const uint64_t val = (b & 1u) ? 0ull : bit;
+        if (v & 64) {
+          s.a1 |= bit;
+          s.v1 |= val;
+        } else {
+          s.a0 |= bit;
+          s.v0 |= val;
+        }
+        cd = fd + 1;
+        prop = true;
+        continue;
+      }
+      // ---- propagate the node at depth cd ----
+      if (!propagate(cl, nslots, s, st, lit_table, clause_table, lane)) {
+        if (cd >= own_from) ++conflicts;
+        if (cd == base) break;
+        fd = cd - 1;
+        prop = false;
+        continue;
+      }
+      const int depth = cd;
+      if (p.early_exit &&
+          (__hip_atomic_load(&p.verdicts[job.instance], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & TS_SEARCH_SAT)) {
+        status = 1;
+        break;
+      }
+      const bool own = depth >= own_from;
+      if (own && owned >= budget) {
+        if (!p.spill) {
+          status = 2;
+          break;
+        }
+        // Hand the rest over: this node's subtree, and the untried second child of every
+        // frame the job explores freely (below its prefix levels).
+        int n_out = 1;
+        for (int k = base + split; k < depth; ++k) n_out += (uni(stk_lit[k]) >> 8) == 1u;
+        int start = 0;
+        if (lane == 0) start = atomicAdd(p.spill_count, n_out);
+        start = (int)uni((uint32_t)__builtin_amdgcn_readlane(start, 0));
+        if (start + n_out <= p.spill_capacity) {
+          if (lane == 0) {
+            ts_search_job o = {};
+            o.instance = job.instance;
+            int w = start;
+            for (int k = base + split; k < depth; ++k) {
+              const uint32_t fr = stk_lit[k];
+              if ((fr >> 8) != 1u) continue;
+              const uint32_t b = (fr & 0xFFu) ^ 1u, v = b >> 1;
+              const uint64_t bit = 1ull << (v & 63), val = (b & 1u) ? 0ull : bit;
+              o.depth = k + 1;
+              o.own_from = k + 1;
+              o.assigned[0] = stk[0][k] | ((v & 64) ? 0ull : bit);
+              o.assigned[1] = stk[1][k] | ((v & 64) ? bit : 0ull);
+              o.value[0] = stk[2][k] | ((v & 64) ? 0ull : val);
+              o.value[1] = stk[3][k] | ((v & 64) ? val : 0ull);
+              p.spill[w++] = o;
+            }
+            o.depth = depth;
+            o.own_from = depth;
+            o.assigned[0] = s.a0;
+            o.assigned[1] = s.a1;
+            o.value[0] = s.v0;
+            o.value[1] = s.v1;
+            p.spill[w] = o;
+          }
+          status = 3;
+          break;
+        }
+        // No room: empty the reserved slots that exist and finish the subtree here.
+        if (lane == 0) {
+          ts_search_job o = {};
+          o.instance = -1;
+          for (int w = start; w < p.spill_capacity && w < start + n_out; ++w) p.spill[w] = o;
+        }
+        budget = INT64_MAX;
       }
       if (own) {
-        ++leaves;
-        digest += splitmix(splitmix(bits) + (uint64_t)rank);
-        if (bits != 0) {
-          if (lane == 0) atomicOr(&p.verdicts[job.instance], TS_SEARCH_SAT);
-          if (!p.exhaustive) break;
+        ++owned;
+        ++nodes;
+        max_depth = depth > max_depth ? depth : max_depth;
+      }
+      // Branching histogram over the residual clauses' free literals; it also names R.
+      fresh(cl);
+#pragma unroll
+      for (int k = 0; k < SLOTS; ++k) {
+        if (k < nslots && !cs_sat(st[k])) {
+          const uint32_t c = cl[k], len = c >> 24;
+          const uint32_t w = (cs_nfree(st[k]) - 1u) << 20;
+#pragma unroll
+          for (int i = 0; i < 3; ++i) {
+            if ((uint32_t)i < len) {
+              const uint32_t b = (c >> (8 * i)) & 0xFFu;
+              if (lds_byte(lit_table, b) == LIT_FREE) atomicAdd(&hist[b >> 1], w | ((b & 1u) ? 1u : (1u << 10)));
+            }
+          }
         }
       }
-      if (depth == 0) break;
-      fd = depth - 1;
+      __syncthreads();
+      const uint32_t h0 = hist[lane], h1 = hist[lane + 64];
+      hist[lane] = 0;
+      hist[lane + 64] = 0;
+      const uint64_t r0 = __ballot(h0 != 0), r1 = __ballot(h1 != 0);
+      const int rank = __popcll(r0) + __popcll(r1);
+      if (rank <= width) {
+        // ---- leaf: the residual clause tensors multiplied into the rank-|R| relation
+        // (entry e: bit j of e = value of the j-th smallest variable of R), then reduced ----
+        uint64_t bits = 1;  // no residual clause: the empty product
+        if (rank > 0) {
+          uint64_t mine = ~0ull;  // entries satisfying this lane's residual clauses
+          fresh(cl);
+#pragma unroll
+          for (int k = 0; k < SLOTS; ++k) {
+            if (k < nslots && !cs_sat(st[k])) {
+              const uint32_t c = cl[k], len = c >> 24;
+              uint64_t fals = ~0ull;
+#pragma unroll
+              for (int i = 0; i < 3; ++i) {
+                if ((uint32_t)i < len) {
+                  const uint32_t b = (c >> (8 * i)) & 0xFFu, v = b >> 1;
+                  if (lds_byte(lit_table, b) == LIT_FREE) {  // assigned literals of a residual clause are false
+                    const uint32_t pos = (v & 64) ? (uint32_t)(__popcll(r0) + __popcll(r1 & ((1ull << (v & 63)) - 1ull)))
+                                                  : (uint32_t)__popcll(r0 & ((1ull << v) - 1ull));
+                    const uint64_t ones = entries_with_bit(pos);
+                    fals &= (b & 1u) ? ones : ~ones;  // false where the variable equals the negation bit
+                  }
+                }
+              }
+              mine &= ~fals;
+            }
+          }
+          const uint32_t lo = wave_reduce((uint32_t)mine, OpAnd());
+          const uint32_t hi = wave_reduce((uint32_t)(mine >> 32), OpAnd());
+          const uint64_t valid = rank == 6 ? ~0ull : ((1ull << (1u << rank)) - 1ull);
+          bits = (((uint64_t)hi << 32) | lo) & valid;
+        }
+        if (own) {
+          ++leaves;
+          digest += splitmix(splitmix(bits) + (uint64_t)rank);
+          if (bits != 0) {
+            if (lane == 0) atomicOr(&p.verdicts[job.instance], TS_SEARCH_SAT);
+            if (!p.exhaustive) break;
+          }
+        }
+        if (depth == base) break;
+        fd = depth - 1;
+        prop = false;
+        continue;
+      }
+      // ---- branch: free variable of largest score, ties to the smaller index ----
+      const uint32_t k0 = h0 ? (((h0 >> 20) << 8) | (255u - (uint32_t)lane)) : 0u;
+      const uint32_t k1 = h1 ? (((h1 >> 20) << 8) | (255u - (uint32_t)(lane + 64))) : 0u;
+      const uint32_t key = wave_reduce(k0 > k1 ? k0 : k1, OpMax());
+      const uint32_t bv = 255u - (key & 255u);
+      const uint32_t hb = bv < 64 ? __builtin_amdgcn_readlane(h0, (int)bv) : __builtin_amdgcn_readlane(h1, (int)bv - 64);
+      const uint32_t npos = (hb >> 10) & 1023u, nneg = hb & 1023u;
+      const uint32_t first = (bv << 1) | (npos >= nneg ? 0u : 1u);
+      stk[0][depth] = s.a0;
+      stk[1][depth] = s.a1;
+      stk[2][depth] = s.v0;
+      stk[3][depth] = s.v1;
+      stk_lit[depth] = first;  // next child 0
+      __syncthreads();
+      fd = depth;
       prop = false;
-      continue;
     }
-    // ---- branch: free variable of largest score, ties to the smaller index ----
-    const uint32_t k0 = h0 ? (((h0 >> 20) << 8) | (255u - (uint32_t)lane)) : 0u;
-    const uint32_t k1 = h1 ? (((h1 >> 20) << 8) | (255u - (uint32_t)(lane + 64))) : 0u;
-    const uint32_t key = wave_reduce(k0 > k1 ? k0 : k1, OpMax());
-    const uint32_t bv = 255u - (key & 255u);
-    const uint32_t hb = bv < 64 ? __builtin_amdgcn_readlane(h0, (int)bv) : __builtin_amdgcn_readlane(h1, (int)bv - 64);
-    const uint32_t npos = (hb >> 10) & 1023u, nneg = hb & 1023u;
-    const uint32_t first = (bv << 1) | (npos >= nneg ? 0u : 1u);
-    stk[0][depth] = s.a0;
-    stk[1][depth] = s.a1;
-    stk[2][depth] = s.v0;
-    stk[3][depth] = s.v1;
-    stk_lit[depth] = first;  // next child 0
-    __syncthreads();
-    fd = depth;
-    prop = false;
+    if (lane == 0 && status == 2) atomicOr(&p.verdicts[job.instance], TS_SEARCH_INCOMPLETE);
   }
-
-  if (lane == 0) {
-    if (status == 2) atomicOr(&p.verdicts[job.instance], TS_SEARCH_INCOMPLETE);
-    if (p.stats) {
-      ts_search_stat o;
-      o.status = status;
-      o.max_depth = max_depth;
-      o.nodes = nodes;
-      o.leaves = leaves;
-      o.conflicts = conflicts;
-      o.digest = digest;
-      p.stats[blockIdx.x] = o;
-    }
+  if (lane == 0 && p.stats) {
+    ts_search_stat o;
+    o.status = status;
+    o.max_depth = max_depth;
+    o.nodes = nodes;
+    o.leaves = leaves;
+    o.conflicts = conflicts;
+    o.digest = digest;
+    p.stats[blockIdx.x] = o;
   }
 }
 
 extern "C" int ts_search_run(const ts_search* p, ts_stream stream) {
   if (!p || p->num_jobs < 0 || p->num_instances < 0) return TS_EINVAL;
   if (p->width < 0 || p->width > TS_SEARCH_MAX_WIDTH || p->max_nodes <= 0) return TS_EINVAL;
+  if (p->spill && (!p->spill_count || p->spill_capacity < 0)) return TS_EINVAL;
   if (p->num_jobs == 0) return TS_OK;
   if (!p->clauses || !p->instances || !p->jobs || !p->verdicts) return TS_EINVAL;
   hipLaunchKernelGGL(k_search, dim3((unsigned)p->num_jobs), dim3(64), 0, reinterpret_cast<hipStream_t>(stream), *p);

@@ -3,7 +3,11 @@
 // into the kernel's word layout, (variable << 1 | negated) per literal byte and the
 // length in the top byte.
 
+#include <algorithm>
+#include <atomic>
 #include <cstdlib>
+#include <thread>
+#include <vector>
 
 #include "tensorsat_prog.h"
 
@@ -12,55 +16,93 @@ constexpr int kMaxVars = 128;
 constexpr int64_t kMaxClauses = 64 * 12;
 }  // namespace
 
-extern "C" int tsc_search_pack(int32_t num, const int32_t* nvars, const int64_t* nclauses, const int32_t* const* lens,
-                               const int32_t* const* lits, uint32_t* words_out, int32_t* inst_out, int64_t* total_out,
-                               int32_t* bad_out) {
-  if (num < 0 || (num > 0 && (!nvars || !nclauses || !lens || !lits || !inst_out)) || !total_out) return TSP_EINVAL;
-  int64_t w = 0;
-  for (int32_t i = 0; i < num; ++i) {
-    if (bad_out) *bad_out = i;
-    const int n = nvars[i];
-    if (n < 0) return TSP_EINVAL;
-    if (n > kMaxVars) return TSC_EUNSUPPORTED;
-    const int64_t begin = w;
-    const int32_t* L = lits[i];
-    const int32_t* len = lens[i];
-    int64_t off = 0;
-    for (int64_t c = 0; c < nclauses[i]; ++c) {
-      const int k = len[c];
-      if (k < 0) return TSP_EINVAL;
-      int keep[3];
-      int nk = 0;
-      bool taut = false;
-      for (int j = 0; j < k; ++j) {
-        const int l = L[off + j];
-        if (l == 0 || std::abs(l) > n) return TSP_EINVAL;
-        bool dup = false;
-        for (int q = 0; q < nk; ++q) {
-          dup |= keep[q] == l;
-          taut |= keep[q] == -l;
-        }
-        if (dup) continue;
-        if (nk == 3) return TSC_EUNSUPPORTED;  // more than 3 distinct literals
-        keep[nk++] = l;
+namespace {
+
+// Normalise the clauses of instance i; writes them at `out` when non-null.  Returns the
+// number of clauses kept, or a negative TSP / TSC code.
+int64_t pack_one(int n, int64_t ncl, const int32_t* len, const int32_t* L, uint32_t* out) {
+  if (n < 0) return TSP_EINVAL;
+  if (n > kMaxVars) return TSC_EUNSUPPORTED;
+  int64_t w = 0, off = 0;
+  for (int64_t c = 0; c < ncl; ++c) {
+    const int k = len[c];
+    if (k < 0) return TSP_EINVAL;
+    int keep[3];
+    int nk = 0;
+    bool taut = false;
+    for (int j = 0; j < k; ++j) {
+      const int l = L[off + j];
+      if (l == 0 || std::abs(l) > n) return TSP_EINVAL;
+      bool dup = false;
+      for (int q = 0; q < nk; ++q) {
+        dup |= keep[q] == l;
+        taut |= keep[q] == -l;
       }
-      off += k;
-      if (taut) continue;  // always satisfied: dropped
+      if (dup) continue;
+      if (nk == 3) return TSC_EUNSUPPORTED;  // more than 3 distinct literals
+      keep[nk++] = l;
+    }
+    off += k;
+    if (taut) continue;  // always satisfied: dropped
+    if (out) {
       uint32_t word = (uint32_t)nk << 24;
       for (int q = 0; q < nk; ++q) {
         const int l = keep[q];
-        const uint32_t b = ((uint32_t)(std::abs(l) - 1) << 1) | (l < 0 ? 1u : 0u);
-        word |= b << (8 * q);
+        word |= (((uint32_t)(std::abs(l) - 1) << 1) | (l < 0 ? 1u : 0u)) << (8 * q);
       }
-      if (words_out) words_out[w] = word;
-      ++w;
+      out[w] = word;
     }
-    if (w - begin > kMaxClauses) return TSC_EUNSUPPORTED;
-    inst_out[4 * i + 0] = (int32_t)begin;
-    inst_out[4 * i + 1] = (int32_t)(w - begin);
-    inst_out[4 * i + 2] = n;
-    inst_out[4 * i + 3] = 0;
+    ++w;
   }
+  return w > kMaxClauses ? (int64_t)TSC_EUNSUPPORTED : w;
+}
+
+template <typename F>
+void parallel_for(int32_t num, int threads, F&& f) {
+  threads = std::max(1, std::min<int>(threads, (num + 255) / 256));
+  if (threads == 1) {
+    for (int32_t i = 0; i < num; ++i) f(i);
+    return;
+  }
+  std::atomic<int32_t> next{0};
+  std::vector<std::thread> pool;
+  for (int t = 0; t < threads; ++t)
+    pool.emplace_back([&] {
+      for (;;) {
+        const int32_t b = next.fetch_add(256);
+        if (b >= num) break;
+        for (int32_t i = b; i < std::min(num, b + 256); ++i) f(i);
+      }
+    });
+  for (auto& th : pool) th.join();
+}
+
+}  // namespace
+
+extern "C" int tsc_search_pack(int32_t num, const int32_t* nvars, const int64_t* nclauses, const int32_t* const* lens,
+                               const int32_t* const* lits, uint32_t* words_out, int32_t* inst_out, int64_t* total_out,
+                               int32_t* bad_out, int32_t threads) {
+  if (num < 0 || (num > 0 && (!nvars || !nclauses || !lens || !lits || !inst_out)) || !total_out) return TSP_EINVAL;
+  if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+  // Pass 1: kept clauses per instance (and the first failure); pass 2: write them.
+  std::vector<int64_t> kept((size_t)num);
+  parallel_for(num, threads, [&](int32_t i) { kept[i] = pack_one(nvars[i], nclauses[i], lens[i], lits[i], nullptr); });
+  int64_t w = 0;
+  for (int32_t i = 0; i < num; ++i) {
+    if (kept[i] < 0) {
+      if (bad_out) *bad_out = i;
+      return (int)kept[i];
+    }
+    inst_out[4 * i + 0] = (int32_t)w;
+    inst_out[4 * i + 1] = (int32_t)kept[i];
+    inst_out[4 * i + 2] = nvars[i];
+    inst_out[4 * i + 3] = 0;
+    w += kept[i];
+  }
+  if (words_out)
+    parallel_for(num, threads, [&](int32_t i) {
+      pack_one(nvars[i], nclauses[i], lens[i], lits[i], words_out + inst_out[4 * i]);
+    });
   if (bad_out) *bad_out = -1;
   *total_out = w;
   return TSP_OK;

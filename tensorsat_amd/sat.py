@@ -23,7 +23,7 @@ Clause = tuple[int, ...]
 class CNFInstance:
     """A SAT instance in conjunctive normal form (``lib/sat.py:40``)."""
 
-    __slots__ = ("_num_vars", "_clauses", "_arrays")
+    __slots__ = ("_num_vars", "_clauses", "_arrays", "_addr")
 
     def __init__(self, num_vars: int, clauses: Sequence[Sequence[int]]) -> None:
         """Public constructor, validating literals (``lib/sat.py:111-121``)."""
@@ -36,6 +36,7 @@ class CNFInstance:
         self._num_vars = int(num_vars)
         self._clauses = clauses
         self._arrays = None
+        self._addr = None
 
     @classmethod
     def _new(cls, num_vars: int, clauses: tuple[Clause, ...], arrays=None) -> "CNFInstance":
@@ -43,6 +44,7 @@ class CNFInstance:
         self._num_vars = num_vars
         self._clauses = clauses
         self._arrays = arrays
+        self._addr = None
         return self
 
     @classmethod
@@ -155,6 +157,15 @@ class CNFInstance:
             flat = np.fromiter((l for c in self._clauses for l in c), dtype=np.int32, count=int(lens.sum()))
             self._arrays = (lens, flat)
         return self._arrays
+
+    def literal_addresses(self) -> tuple[int, int, int]:
+        """(clause count, address of the lengths, address of the literals) of
+        :meth:`literal_arrays`, for native batch packing (valid while the instance
+        lives; the arrays are never replaced)."""
+        if self._addr is None:
+            lens, flat = self.literal_arrays()
+            self._addr = (lens.size, lens.ctypes.data, flat.ctypes.data)
+        return self._addr
 
     def clause_matrix(self) -> np.ndarray:
         """Clauses as a dense ``(m, k_max)`` int32 array, zero-padded on the right."""

@@ -35,15 +35,16 @@ MAX_VARS = 128
 SLOTS = 12
 MAX_CLAUSES = 64 * SLOTS
 MAX_WIDTH = 6
-MAX_SPLIT = 16
+MAX_SPLIT = 31
 SAT = 1
 INCOMPLETE = 2
 
 INST_DTYPE = np.dtype([("clause_begin", "<i4"), ("num_clauses", "<i4"), ("num_vars", "<i4"), ("reserved", "<i4")])
-JOB_DTYPE = np.dtype([("instance", "<i4"), ("split", "<i4"), ("prefix", "<u4"), ("reserved", "<i4")])
+JOB_DTYPE = np.dtype([("instance", "<i4"), ("depth", "<i4"), ("split", "<i4"), ("own_from", "<i4"),
+                      ("prefix", "<u4"), ("reserved", "<u4", (3,)), ("assigned", "<u8", (2,)), ("value", "<u8", (2,))])
 STAT_DTYPE = np.dtype([("status", "<i4"), ("max_depth", "<i4"), ("nodes", "<i8"), ("leaves", "<i8"),
                        ("conflicts", "<i8"), ("digest", "<u8")])
-assert INST_DTYPE.itemsize == 16 and JOB_DTYPE.itemsize == 16 and STAT_DTYPE.itemsize == 40
+assert INST_DTYPE.itemsize == 16 and JOB_DTYPE.itemsize == 64 and STAT_DTYPE.itemsize == 40
 
 
 class _Search(ctypes.Structure):
@@ -51,7 +52,8 @@ class _Search(ctypes.Structure):
         ("clauses", ctypes.c_void_p), ("instances", ctypes.c_void_p), ("num_instances", ctypes.c_int32),
         ("jobs", ctypes.c_void_p), ("num_jobs", ctypes.c_int32), ("verdicts", ctypes.c_void_p),
         ("stats", ctypes.c_void_p), ("width", ctypes.c_int32), ("exhaustive", ctypes.c_int32),
-        ("early_exit", ctypes.c_int32), ("reserved", ctypes.c_int32), ("max_nodes", ctypes.c_int64),
+        ("early_exit", ctypes.c_int32), ("spill_capacity", ctypes.c_int32), ("max_nodes", ctypes.c_int64),
+        ("spill", ctypes.c_void_p), ("spill_count", ctypes.c_void_p),
     ]
 
 
@@ -71,16 +73,17 @@ def pack_batch(cnfs) -> tuple[np.ndarray, np.ndarray]:
 
     L = PG._lib()
     num = len(cnfs)
-    arrays = [c.literal_arrays() for c in cnfs]
     nv = np.fromiter((c.num_vars for c in cnfs), dtype=np.int32, count=num)
-    ncl = np.fromiter((a[0].size for a in arrays), dtype=np.int64, count=num)
-    lens_p = (ctypes.c_void_p * max(1, num))(*[a[0].ctypes.data for a in arrays])
-    lits_p = (ctypes.c_void_p * max(1, num))(*[a[1].ctypes.data for a in arrays])
+    addr = np.fromiter((x for c in cnfs for x in c.literal_addresses()), dtype=np.int64, count=3 * num).reshape(num, 3)
+    ncl = np.ascontiguousarray(addr[:, 0])
+    lens_p = np.ascontiguousarray(addr[:, 1])
+    lits_p = np.ascontiguousarray(addr[:, 2])
     words = np.empty(max(1, int(ncl.sum())), dtype=np.uint32)
     inst = np.zeros(num, dtype=INST_DTYPE)
     total, bad = ctypes.c_int64(0), ctypes.c_int32(-1)
-    rc = L.tsc_search_pack(num, nv.ctypes.data, ncl.ctypes.data, lens_p, lits_p, words.ctypes.data,
-                           inst.ctypes.data if num else None, ctypes.byref(total), ctypes.byref(bad))
+    rc = L.tsc_search_pack(num, nv.ctypes.data, ncl.ctypes.data, lens_p.ctypes.data, lits_p.ctypes.data,
+                           words.ctypes.data, inst.ctypes.data if num else None, ctypes.byref(total),
+                           ctypes.byref(bad), 0)
     if rc == -3:
         raise NotImplementedError(f"instance {bad.value} is outside the device search's limits ({MAX_VARS} "
                                   f"variables, {MAX_CLAUSES} clauses, 3 distinct literals per clause)")
@@ -122,10 +125,28 @@ def plan_jobs(inst: np.ndarray, *, split_target: float | None = 128.0, max_split
     jobs["split"] = np.repeat(split, nj)
     starts = np.zeros(num, dtype=np.int64)
     np.cumsum(nj[:-1], out=starts[1:])
-    jobs["prefix"] = np.arange(jobs.size) - np.repeat(starts, nj)
+    prefix = np.arange(jobs.size) - np.repeat(starts, nj)
+    jobs["prefix"] = prefix
+    # A node on the prefix levels belongs to the job whose remaining prefix bits are all
+    # 0: the job counts from one level below its highest set prefix bit.
+    jobs["own_from"] = np.where(prefix > 0, np.floor(np.log2(np.maximum(prefix, 1))).astype(np.int64) + 1, 0)
     key = np.repeat(est - split, nj)
     order = np.lexsort((jobs["prefix"], jobs["instance"], -key))
     return jobs[order]
+
+
+def _var_masks(nv: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """Per count of variables: the masks of valid bits in the two assignment words."""
+    one, ones = np.uint64(1), np.uint64(0xFFFFFFFFFFFFFFFF)
+    lo = np.where(nv >= 64, ones, (one << np.clip(nv, 0, 63).astype(np.uint64)) - one)
+    hi = np.where(nv <= 64, np.uint64(0),
+                  np.where(nv >= 128, ones, (one << np.clip(nv - 64, 0, 63).astype(np.uint64)) - one))
+    return lo.astype(np.uint64), hi.astype(np.uint64)
+
+
+def _popcount64(x: np.ndarray) -> np.ndarray:
+    b = np.ascontiguousarray(x, dtype=np.uint64).view(np.uint8).reshape(-1, 8)
+    return np.unpackbits(b, axis=1).sum(axis=1).astype(np.int64)
 
 
 def validate(words: np.ndarray, inst: np.ndarray, jobs: np.ndarray, width: int, *, literals: bool = True) -> None:
@@ -158,35 +179,63 @@ def validate(words: np.ndarray, inst: np.ndarray, jobs: np.ndarray, width: int, 
                 if np.any(used & (v >= nvars)):
                     raise ValueError("literal of a variable beyond the instance's count")
     if len(jobs):
-        if np.any(jobs["instance"] < 0) or np.any(jobs["instance"] >= len(inst)):
+        if np.any(jobs["instance"] >= len(inst)):
             raise ValueError("job names an instance outside the batch")
         if np.any(jobs["split"] < 0) or np.any(jobs["split"] > MAX_SPLIT):
             raise ValueError(f"split outside 0..{MAX_SPLIT}")
         if np.any(jobs["prefix"].astype(np.int64) >> jobs["split"].astype(np.int64)):
             raise ValueError("prefix has bits beyond its split")
+        live = jobs["instance"] >= 0
+        nv = inst["num_vars"][np.maximum(jobs["instance"], 0)].astype(np.int64)
+        lo_mask, hi_mask = _var_masks(nv)
+        asg, val = jobs["assigned"], jobs["value"]
+        if np.any(live & (((asg[:, 0] & ~lo_mask) != 0) | ((asg[:, 1] & ~hi_mask) != 0))):
+            raise ValueError("job assignment names variables beyond the instance's count")
+        if np.any(((val[:, 0] & ~asg[:, 0]) != 0) | ((val[:, 1] & ~asg[:, 1]) != 0)):
+            raise ValueError("job value bits outside its assigned variables")
+        free = nv - _popcount64(asg[:, 0]) - _popcount64(asg[:, 1])
+        # every branch below the start assigns a free variable: the branch stack reaches
+        # depth + free - 1 at most, which must stay inside its MAX_VARS frames
+        if np.any(live & ((jobs["depth"] < 0) | (jobs["depth"] + free > MAX_VARS))):
+            raise ValueError("job start depth leaves no room for its branch stack")
+        if np.any(live & ((jobs["own_from"] < 0) | (jobs["own_from"] > jobs["depth"] + jobs["split"]))):
+            raise ValueError("job owns from a depth below its prefix levels")
 
 
 class SearchSolver:
-    """Decides a batch of CNF instances with one launch of the device cube search.
+    """Decides a batch of CNF instances with the device cube search.
 
     ``width``: residuals of at most this many variables are contracted (0..6).
-    ``split_target``: estimated branch nodes per job (hard instances are split into
-    2^k jobs; None: one job per instance).  ``exhaustive``: visit every branch (for
-    statistics; no early exit).  ``max_nodes``: per-job node budget; an instance whose
-    job ran out of it is reported undecided (-1) unless another job found it SAT."""
+    ``split_target``: estimated branch nodes per job (hard instances are split into 2^k
+    jobs from the start; None: one job per instance).  ``budget``: nodes a job searches
+    before it hands its remaining subtrees over as new jobs, run by the next launch
+    (None: no hand-over).  Launches repeat until no job is handed over; the verdicts do
+    not depend on the split or the budget.  ``exhaustive``: visit every branch (for
+    statistics; no early exit).  ``max_nodes``: with ``budget`` None, the per-job node
+    cap; an instance whose job hits it is reported undecided (-1) unless another job
+    found it SAT."""
 
     def __init__(self, *, width: int = 6, split_target: float | None = 128.0, max_split: int = 8,
-                 exhaustive: bool = False, max_nodes: int = 1 << 26, stats: bool = False, device: str = "cuda"):
+                 budget: int | None = None, exhaustive: bool = False, max_nodes: int = 1 << 26,
+                 stats: bool = False, device: str = "cuda", spill_capacity: int | None = None,
+                 max_rounds: int = 100_000):
         if not 0 <= width <= MAX_WIDTH:
             raise ValueError(f"width must be in 0..{MAX_WIDTH}")
+        if budget is not None and budget < 1:
+            raise ValueError("budget must be >= 1 (or None)")
         self.width = width
         self.split_target = split_target
         self.max_split = max_split
+        self.budget = budget
         self.exhaustive = exhaustive
         self.max_nodes = int(max_nodes)
         self.want_stats = stats or exhaustive
         self.device = device
+        self.spill_capacity = spill_capacity
+        self.max_rounds = max_rounds
         self.timing: dict = {}
+        self.rounds = 0
+        self.round_jobs: list[int] = []
 
     def prepare(self, cnfs) -> None:
         """Pack, plan the jobs and upload (host work, timed in ``self.timing``)."""
@@ -210,20 +259,58 @@ class SearchSolver:
         self.host_jobs = jobs
         self.d_words = torch.from_numpy(words.view(np.int32) if words.size else np.zeros(1, np.int32)).to(dev)
         self.d_inst = torch.from_numpy(inst.view(np.int32).reshape(-1) if len(inst) else np.zeros(4, np.int32)).to(dev)
-        self.d_jobs = torch.from_numpy(jobs.view(np.int32).reshape(-1) if len(jobs) else np.zeros(4, np.int32)).to(dev)
+        self.d_jobs = torch.from_numpy(jobs.view(np.int32).reshape(-1) if len(jobs) else np.zeros(16, np.int32)).to(dev)
         self.d_verdicts = torch.zeros(max(1, len(inst)), dtype=torch.int32, device=dev)
-        self.d_stats = torch.zeros(max(1, len(jobs)) * STAT_DTYPE.itemsize // 4, dtype=torch.int32, device=dev) \
-            if self.want_stats else None
-        self._args = _Search(
-            N.dptr(self.d_words), N.dptr(self.d_inst), len(inst), N.dptr(self.d_jobs), len(jobs),
-            N.dptr(self.d_verdicts), N.dptr(self.d_stats) if self.d_stats is not None else None, self.width,
-            int(self.exhaustive), int(not self.exhaustive), 0, self.max_nodes,
+        if self.budget is not None:
+            cap = self.spill_capacity or max(1 << 18, 8 * len(jobs))
+            self.cap = int(cap)
+            self.d_spill = [torch.empty(self.cap * JOB_DTYPE.itemsize // 4, dtype=torch.int32, device=dev)
+                            for _ in range(2)]
+            self.d_count = torch.zeros(1, dtype=torch.int32, device=dev)
+        self._stats_rounds: list = []
+
+    def _args(self, jobs_ptr, num_jobs, stats, spill) -> _Search:
+        return _Search(
+            N.dptr(self.d_words), N.dptr(self.d_inst), self.num_instances, jobs_ptr, num_jobs,
+            N.dptr(self.d_verdicts), N.dptr(stats) if stats is not None else None, self.width,
+            int(self.exhaustive), int(not self.exhaustive), self.cap if spill is not None else 0,
+            self.budget if self.budget is not None else self.max_nodes,
+            N.dptr(spill) if spill is not None else None, N.dptr(self.d_count) if spill is not None else None,
         )
 
     def launch(self) -> None:
-        """Clear the verdicts (a fill kernel) and launch the search on the current stream."""
+        """Clear the verdicts (a fill kernel) and run the search on the current stream:
+        one launch, or with ``budget`` one launch per round of handed-over jobs (the host
+        reads each round's job count)."""
+        import torch
+
+        L = N.lib()
         self.d_verdicts.zero_()
-        N.check(N.lib().ts_search_run(ctypes.byref(self._args), N.stream_ptr()), "ts_search_run")
+        self._stats_rounds = []
+        jobs_ptr, n = N.dptr(self.d_jobs), len(self.host_jobs)
+        jobs_tensor = self.d_jobs
+        self.round_jobs = []
+        rounds = 0
+        while n > 0:
+            if rounds >= self.max_rounds:
+                raise RuntimeError(f"search did not finish in {self.max_rounds} rounds")
+            stats = torch.zeros(n * STAT_DTYPE.itemsize // 4, dtype=torch.int32, device=self.device) \
+                if self.want_stats else None
+            spill = None
+            if self.budget is not None:
+                spill = self.d_spill[rounds % 2]
+                self.d_count.zero_()
+            args = self._args(jobs_ptr, n, stats, spill)
+            N.check(L.ts_search_run(ctypes.byref(args), N.stream_ptr()), "ts_search_run")
+            if stats is not None:
+                self._stats_rounds.append((jobs_tensor[: n * JOB_DTYPE.itemsize // 4].clone(), stats))
+            self.round_jobs.append(n)
+            rounds += 1
+            if spill is None:
+                break
+            n = min(int(self.d_count.item()), self.cap)
+            jobs_ptr, jobs_tensor = N.dptr(spill), spill
+        self.rounds = rounds
 
     def verdicts(self) -> np.ndarray:
         """Per instance: 1 SAT, 0 UNSAT, -1 undecided (a job ran out of its node budget)."""
@@ -232,22 +319,28 @@ class SearchSolver:
         out[((v & INCOMPLETE) != 0) & ((v & SAT) == 0)] = -1
         return out
 
-    def job_stats(self) -> np.ndarray:
-        if self.d_stats is None:
+    def job_stats(self) -> tuple[np.ndarray, np.ndarray]:
+        """(jobs, statistics) of every job of every round of the last launch."""
+        if not self.want_stats:
             raise ValueError("construct the solver with stats=True")
-        return self.d_stats.cpu().numpy().view(STAT_DTYPE)[: len(self.host_jobs)].copy()
+        jobs = [j.cpu().numpy().view(JOB_DTYPE) for j, _ in self._stats_rounds]
+        st = [s.cpu().numpy().view(STAT_DTYPE) for _, s in self._stats_rounds]
+        return np.concatenate(jobs), np.concatenate(st)
 
     def instance_stats(self) -> np.ndarray:
         """Per-instance sums of the jobs' statistics (nodes, leaves, conflicts, digest
-        mod 2^64; max_depth the maximum; status the maximum)."""
-        js = self.job_stats()
+        mod 2^64; max_depth the maximum; status the maximum of complete / stopped /
+        exhausted, hand-overs counted as complete)."""
+        jobs, js = self.job_stats()
+        live = jobs["instance"] >= 0
+        jobs, js = jobs[live], js[live]
         out = np.zeros(self.num_instances, dtype=STAT_DTYPE)
-        inst = self.host_jobs["instance"]
+        inst = jobs["instance"]
         for f in ("nodes", "leaves", "conflicts"):
             np.add.at(out[f], inst, js[f])
         np.add.at(out["digest"], inst, js["digest"])
         np.maximum.at(out["max_depth"], inst, js["max_depth"])
-        np.maximum.at(out["status"], inst, js["status"])
+        np.maximum.at(out["status"], inst, np.where(js["status"] == 3, 0, js["status"]))
         return out
 
     def run(self, cnfs) -> np.ndarray:
@@ -259,12 +352,13 @@ class SearchSolver:
         return self.verdicts()
 
 
-def autotune(cnfs, *, widths=(4, 6), split_targets=(None, 512.0, 128.0, 32.0, 8.0), max_split: int = 8,
-             reps: int = 3) -> tuple[dict, dict]:
-    """Pick the residual width and the split target for a batch: every candidate decides
-    the batch ``reps`` times on the GPU (device-resident inputs, events around each
-    launch); the fastest mean launch wins.  Verdicts of every candidate must agree (they
-    are the same decisions), else ``RuntimeError``.  Returns (choice, table)."""
+def autotune(cnfs, *, widths=(4, 6), split_targets=(None, 32.0, 8.0), budgets=(None, 64, 16),
+             max_split: int = 8, reps: int = 3) -> tuple[dict, dict]:
+    """Pick the residual width, the split target and the hand-over budget for a batch:
+    every candidate decides the batch ``reps`` times on the GPU (device-resident inputs,
+    wall time of each complete decision, hand-over rounds included); the fastest mean
+    wins.  Verdicts of every candidate must agree (they are the same decisions), else
+    ``RuntimeError``.  Returns (choice, table)."""
     import torch
 
     words, inst = pack_batch(cnfs)
@@ -272,25 +366,27 @@ def autotune(cnfs, *, widths=(4, 6), split_targets=(None, 512.0, 128.0, 32.0, 8.
     ref = None
     for w in widths:
         for tgt in split_targets:
-            s = SearchSolver(width=w, split_target=tgt, max_split=max_split)
             jobs = plan_jobs(inst, split_target=tgt, max_split=max_split)
             validate(words, inst, jobs, w)
-            s.upload(words, inst, jobs)
-            s.launch()
-            torch.cuda.synchronize()
-            v = s.verdicts()
-            if ref is None:
-                ref = v
-            elif not np.array_equal(v, ref):
-                raise RuntimeError(f"verdicts differ between search configurations (width {w}, split {tgt})")
-            ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
-            for a, b in ev:
-                a.record()
+            for bud in budgets:
+                s = SearchSolver(width=w, split_target=tgt, max_split=max_split, budget=bud)
+                s.upload(words, inst, jobs)
                 s.launch()
-                b.record()
-            torch.cuda.synchronize()
-            ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
-            table[(w, tgt)] = {"width": w, "split_target": tgt, "jobs": int(len(jobs)),
-                               "max_split": int(jobs["split"].max()) if len(jobs) else 0, "ms": ms}
+                torch.cuda.synchronize()
+                v = s.verdicts()
+                if ref is None:
+                    ref = v
+                elif not np.array_equal(v, ref):
+                    raise RuntimeError(f"verdicts differ between search configurations ({w}, {tgt}, {bud})")
+                ts = []
+                for _ in range(reps):
+                    torch.cuda.synchronize()
+                    t0 = time.perf_counter()
+                    s.launch()
+                    torch.cuda.synchronize()
+                    ts.append(time.perf_counter() - t0)
+                table[(w, tgt, bud)] = {"width": w, "split_target": tgt, "budget": bud, "jobs": int(len(jobs)),
+                                        "rounds": s.rounds, "max_split": int(jobs["split"].max()) if len(jobs) else 0,
+                                        "ms": float(np.mean(ts)) * 1000.0}
     best = min(table.values(), key=lambda r: r["ms"])
-    return {"width": best["width"], "split_target": best["split_target"]}, table
+    return {"width": best["width"], "split_target": best["split_target"], "budget": best["budget"]}, table

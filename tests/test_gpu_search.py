@@ -57,19 +57,44 @@ def test_unsplit_early_exit_statistics_exact(batch, width):
     np.testing.assert_array_equal(got["max_depth"], ref["max_depth"])
 
 
-@pytest.mark.parametrize("target", [None, 64.0, 4.0])
-def test_split_exhaustive_statistics_sum_to_one_search(batch, target):
+@pytest.mark.parametrize("target,budget", [(None, None), (64.0, None), (4.0, None), (None, 4), (4.0, 2), (None, 1)])
+def test_split_exhaustive_statistics_sum_to_one_search(batch, target, budget):
     """Exhaustive searches (every branch visited): the jobs of an instance own disjoint
-    subtrees, so their statistics add up to one unsplit search's, whatever the split."""
+    subtrees -- static splits and the subtrees handed over when a job's budget runs out
+    -- so their statistics add up to one unsplit search's, whatever the split."""
     cnfs = [c for c in batch[:600] if c.num_vars <= 50]
-    s, v = _solve(cnfs, width=6, split_target=target, max_split=6, exhaustive=True)
+    s, v = _solve(cnfs, width=6, split_target=target, max_split=6, exhaustive=True, budget=budget)
+    got = s.instance_stats()
+    ref = OS.search(cnfs, width=6, exhaustive=True, threads=8)
+    print(f"target {target} budget {budget}: rounds {s.rounds}, jobs per round {s.round_jobs[:8]}", flush=True)
+    np.testing.assert_array_equal(v, ref["verdict"])
+    for f in FIELDS:
+        np.testing.assert_array_equal(got[f], ref[f], err_msg=f"{f} target {target} budget {budget}")
+    if target == 4.0:
+        assert int(s.host_jobs["split"].max()) >= 3
+    if budget is not None:
+        assert s.rounds > 3
+
+
+def test_hand_over_list_overflow_keeps_every_subtree(batch):
+    """A continuation list too small for the hand-overs: jobs that cannot hand over keep
+    searching their subtrees themselves, so no branch is lost (exact exhaustive sums)."""
+    cnfs = [c for c in batch[:300] if c.num_vars <= 45]
+    s, v = _solve(cnfs, width=6, split_target=None, exhaustive=True, budget=2, spill_capacity=16)
     got = s.instance_stats()
     ref = OS.search(cnfs, width=6, exhaustive=True, threads=8)
     np.testing.assert_array_equal(v, ref["verdict"])
     for f in FIELDS:
-        np.testing.assert_array_equal(got[f], ref[f], err_msg=f"{f} target {target}")
-    if target == 4.0:
-        assert int(s.host_jobs["split"].max()) >= 3
+        np.testing.assert_array_equal(got[f], ref[f], err_msg=f)
+    assert s.rounds >= 2
+
+
+@pytest.mark.parametrize("budget", [1, 8, 64])
+def test_batched_workload_verdicts_with_hand_over(batch, budget):
+    s, v = _solve(batch, width=6, split_target=8.0, budget=budget)
+    ref = OS.search(batch, width=6, threads=8)["verdict"]
+    print(f"budget {budget}: rounds {s.rounds}, jobs per round {s.round_jobs[:10]}", flush=True)
+    np.testing.assert_array_equal(v, ref)
 
 
 def test_edge_cases_match_brute_force():
