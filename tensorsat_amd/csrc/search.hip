@@ -329,8 +329,13 @@ __global__ __launch_bounds__(64) void k_search(ts_search p) {
               o.value[1] = stk[3][k] | ((v & 64) ? val : 0ull);
               p.spill[w++] = o;
             }
+            // The subtree keeps what is left of the job's prefix levels (the other
+            // children there belong to the jobs with the other prefixes).
+            const int rel = depth - base;
             o.depth = depth;
             o.own_from = depth;
+            o.split = rel < split ? split - rel : 0;
+            o.prefix = rel < split ? prefix >> rel : 0u;
             o.assigned[0] = s.a0;
             o.assigned[1] = s.a1;
             o.value[0] = s.v0;
