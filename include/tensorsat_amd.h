@@ -281,6 +281,21 @@ typedef struct {
  * construction). */
 int ts_search_run(const ts_search* s, ts_stream stream);
 
+/* Allocation-free variant of a whole search (no counterpart in the reference): for
+ * callers that cannot call hipMalloc -- tight serving loops, harnesses that forbid device
+ * allocation.  Runs ts_search_run on s->jobs, then on the jobs handed over, round after
+ * round, until none are left, with the continuation lists in a static device array of
+ * the library (two lists of TS_SEARCH_STATIC_JOBS jobs plus a counter, 8 MiB per GPU,
+ * held until the process ends) instead of caller buffers.  s->spill and
+ * s->spill_count must be NULL; s->spill_capacity is the list length to use (0: all of
+ * it; above TS_SEARCH_STATIC_JOBS: TS_EINVAL); s->max_nodes is the hand-over budget;
+ * s->stats (optional) receives the statistics of the caller's jobs only.  Hand-overs
+ * that do not fit are searched in place, so no input is too large.  Synchronous: it
+ * reads each round's job count on the host.  Calls that overlap on one GPU race on the
+ * static lists.  *rounds_out (optional) receives the number of launches. */
+#define TS_SEARCH_STATIC_JOBS 65536
+int ts_search_decide_noalloc(const ts_search* s, int32_t* rounds_out, ts_stream stream);
+
 /* ---- Learned planner: message-passing network kernels (bf16 in, fp32 accumulate) --
  * No counterpart in the reference: the learned planner is an addition of this build.
  * Matrices are row-major; bf16 buffers are passed as void*. */

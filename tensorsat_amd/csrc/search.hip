@@ -34,6 +34,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 
 #include "tensorsat_amd.h"
 
@@ -465,3 +466,75 @@ extern "C" int ts_search_run(const ts_search* p, ts_stream stream) {
   hipLaunchKernelGGL(k_search, dim3((unsigned)p->num_jobs), dim3(64), 0, reinterpret_cast<hipStream_t>(stream), *p);
   return hipGetLastError() == hipSuccess ? TS_OK : TS_EHIP;
 }
+
+// ---- the allocation-free entry point ----------------------------------------------
+//
+// The continuation lists of ts_search_decide_noalloc live in this file-scope device
+// array: a 256-byte counter line, then two lists of TS_SEARCH_STATIC_JOBS jobs used in
+// turn by successive rounds.  Host code never uses the array's name as an address (it
+// is a host-side stand-in there): each device's copy is located once with
+// hipGetSymbolAddress and that address is passed to the kernels.
+namespace {
+
+constexpr size_t kScratchBytes = 256 + 2 * (size_t)TS_SEARCH_STATIC_JOBS * sizeof(ts_search_job);
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch[kMaxDevices] = {};
+
+__global__ void k_zero_i32(int32_t* p) { *p = 0; }
+
+// This device's copy of `scratch` (located on the first call made on the device).
+int device_scratch(unsigned char** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return TS_EHIP;
+  if (dev < 0 || dev >= kMaxDevices) return TS_EUNSUPPORTED;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return TS_EHIP;
+    g_scratch[dev] = ptr;
+  }
+  *out = static_cast<unsigned char*>(g_scratch[dev]);
+  return TS_OK;
+}
+
+}  // namespace
+
+extern "C" int ts_search_decide_noalloc(const ts_search* s, int32_t* rounds_out, ts_stream stream) {
+  if (!s || s->spill || s->spill_count) return TS_EINVAL;
+  if (s->spill_capacity < 0 || s->spill_capacity > TS_SEARCH_STATIC_JOBS) return TS_EINVAL;
+  if (s->num_jobs < 0 || s->width < 0 || s->width > TS_SEARCH_MAX_WIDTH || s->max_nodes <= 0) return TS_EINVAL;
+  if (s->num_jobs > 0 && (!s->clauses || !s->instances || !s->jobs || !s->verdicts)) return TS_EINVAL;
+  unsigned char* base = nullptr;
+  const int rc = device_scratch(&base);
+  if (rc != TS_OK) return rc;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  int32_t* count = reinterpret_cast<int32_t*>(base);
+  ts_search_job* lists[2] = {reinterpret_cast<ts_search_job*>(base + 256),
+                             reinterpret_cast<ts_search_job*>(base + 256) + TS_SEARCH_STATIC_JOBS};
+  ts_search p = *s;
+  p.spill_capacity = s->spill_capacity ? s->spill_capacity : TS_SEARCH_STATIC_JOBS;
+  p.spill_count = count;
+  int32_t rounds = 0;
+  int32_t n = s->num_jobs;
+  while (n > 0) {
+    p.spill = lists[rounds & 1];
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, count);
+    if (hipGetLastError() != hipSuccess) return TS_EHIP;
+    const int r = ts_search_run(&p, stream);
+    if (r != TS_OK) return r;
+    ++rounds;
+    int32_t next = 0;
+    if (hipMemcpyAsync(&next, count, sizeof(next), hipMemcpyDeviceToHost, st) != hipSuccess) return TS_EHIP;
+    if (hipStreamSynchronize(st) != hipSuccess) return TS_EHIP;
+    n = next < p.spill_capacity ? next : p.spill_capacity;
+    p.jobs = lists[(rounds - 1) & 1];
+    p.num_jobs = n;
+    p.stats = nullptr;  // statistics cover the caller's own jobs only
+  }
+  if (rounds_out) *rounds_out = rounds;
+  return TS_OK;
+}
+
