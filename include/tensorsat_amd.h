@@ -273,6 +273,11 @@ typedef struct {
   int32_t* spill_count;            /* device, zeroed by the caller: jobs appended (may
                                       exceed the capacity; only the first spill_capacity
                                       slots exist) */
+  int32_t max_clauses;             /* largest num_clauses of the instances (0: unknown): the
+                                      launch uses the kernel variant with just enough clause
+                                      slots per lane; an instance beyond it is reported
+                                      TS_SEARCH_INCOMPLETE, never decided */
+  int32_t reserved2;
 } ts_search;
 
 /* One launch of num_jobs one-wave workgroups.  Rejects (TS_EINVAL) null arrays and widths

@@ -40,7 +40,7 @@
 
 namespace {
 
-constexpr int SLOTS = TS_SEARCH_SLOTS;
+
 constexpr int MAXV = TS_SEARCH_MAX_VARS;
 constexpr uint32_t NO_CLAUSE = 0xFFu;  // length byte of an empty slot (always satisfied)
 constexpr uint32_t NO_UNIT = 0xFFFFu;
@@ -83,15 +83,16 @@ struct Assign {
 // The clause words never change during a job, so the compiler would decode all of them
 // once, outside the search loop, into hundreds of registers; marking them as rewritten
 // at each use keeps the decode next to its use (a few registers per slot).
-__device__ __forceinline__ void fresh(uint32_t (&cl)[SLOTS]) {
+template <int NS>
+__device__ __forceinline__ void fresh(uint32_t (&cl)[NS]) {
 #pragma unroll
-  for (int k = 0; k < SLOTS; ++k) asm volatile("" : "+v"(cl[k]));
+  for (int k = 0; k < NS; ++k) asm volatile("" : "+v"(cl[k]));
 }
 
 // Literal statuses (byte b = (v << 1) | negated): 0 free, 1 false, 2 true.
 constexpr uint32_t LIT_FREE = 0, LIT_FALSE = 1, LIT_TRUE = 2;
-// Clause status word: bit 0 satisfied, bits 1..2 free literals, bits 3..4 position of
-// the last free literal.
+// Clause status byte: bit 0 satisfied, bits 1..2 free literals, bits 3..4 position of the
+// last free literal.
 __device__ __forceinline__ uint32_t cs_sat(uint32_t r) { return r & 1u; }
 __device__ __forceinline__ uint32_t cs_nfree(uint32_t r) { return (r >> 1) & 3u; }
 __device__ __forceinline__ uint32_t cs_pos(uint32_t r) { return (r >> 3) & 3u; }
@@ -111,64 +112,71 @@ __device__ __forceinline__ void build_lit_table(uint32_t* lt, const Assign& s, i
   lt[lane] = p0 | (n0 << 8) | (p1 << 16) | (n1 << 24);
 }
 
-// The clause-status table: entry (len << 6 | x2 << 4 | x1 << 2 | x0) for literal
-// statuses x_i of the first len literals.  Lane l writes entries 4l .. 4l + 3.
+// The clause-status table: entry ((len & 7) << 6 | x2 << 4 | x1 << 2 | x0) for literal
+// statuses x_i of the first len literals; lengths 4..7 (the empty-slot length byte 0xFF
+// lands on 7) read as satisfied.  Lane l writes entries 8l .. 8l + 7.
 __device__ __forceinline__ void build_clause_table(uint32_t* ct, int lane) {
-  uint32_t word = 0;
 #pragma unroll
-  for (int e = 0; e < 4; ++e) {
-    const uint32_t idx = 4u * (uint32_t)lane + (uint32_t)e, len = idx >> 6;
-    uint32_t sat = 0, nfree = 0, pos = 0;
-    for (uint32_t i = 0; i < len; ++i) {
-      const uint32_t x = (idx >> (2 * i)) & 3u;
-      sat |= x == LIT_TRUE;
-      if (x == LIT_FREE) {
-        ++nfree;
-        pos = i;
+  for (int h = 0; h < 2; ++h) {
+    uint32_t word = 0;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const uint32_t idx = 8u * (uint32_t)lane + 4u * (uint32_t)h + (uint32_t)e, len = idx >> 6;
+      uint32_t sat = len > 3 ? 1u : 0u, nfree = 0, pos = 0;
+      for (uint32_t i = 0; i < len && i < 3; ++i) {
+        const uint32_t x = (idx >> (2 * i)) & 3u;
+        sat |= x == LIT_TRUE;
+        if (x == LIT_FREE) {
+          ++nfree;
+          pos = i;
+        }
       }
+      word |= (sat | (nfree << 1) | (pos << 3)) << (8 * e);
     }
-    word |= (sat | (nfree << 1) | (pos << 3)) << (8 * e);
+    ct[2 * lane + h] = word;
   }
-  ct[lane] = word;
 }
 
-__device__ __forceinline__ uint32_t lds_byte(const uint32_t* t, uint32_t i) {
-  return reinterpret_cast<const uint8_t*>(t)[i];
-}
-
-// Status word of a clause (an empty slot reads as satisfied).
-__device__ __forceinline__ uint32_t clause_state(uint32_t c, const uint32_t* lt, const uint32_t* ct) {
-  const uint32_t x0 = lds_byte(lt, c & 0xFFu);
-  const uint32_t x1 = lds_byte(lt, (c >> 8) & 0xFFu);
-  const uint32_t x2 = lds_byte(lt, (c >> 16) & 0xFFu);
-  const uint32_t len = c >> 24;
-  const uint32_t r = lds_byte(ct, ((len & 3u) << 6) | (x2 << 4) | (x1 << 2) | x0);
-  return len == NO_CLAUSE ? 1u : r;
+// Status bytes of every slot's clause: all literal-status reads first, then the
+// clause-status reads, so the LDS latencies overlap.
+template <int NS>
+__device__ __forceinline__ void clause_states(const uint32_t (&cl)[NS], const uint8_t* lt, const uint8_t* ct,
+                                              uint32_t (&st)[NS]) {
+  uint32_t x[NS][3];
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) x[k][i] = lt[(cl[k] >> (8 * i)) & 0xFFu];
+  }
+#pragma unroll
+  for (int k = 0; k < NS; ++k)
+    st[k] = ct[((cl[k] >> 18) & 0x1C0u) | (x[k][2] << 4) | (x[k][1] << 2) | x[k][0]];
 }
 
 // Unit propagation to the fixpoint.  Returns false on a conflict.  On success `st`
 // holds every slot's clause status under the final assignment and the literal table
 // matches it.  Each round every lane offers at most one unit literal (its first unit
-// clause's); the scalar unit applies them lane by lane, so each round assigns at least
-// one new variable.
-__device__ __forceinline__ bool propagate(uint32_t (&cl)[SLOTS], int nslots, Assign& s, uint32_t (&st)[SLOTS],
-                                          uint32_t* lt, const uint32_t* ct, int lane) {
-  build_lit_table(lt, s, lane);
+// clause's); the scalar unit applies them lane by lane, writing the two literal-table
+// bytes of each newly assigned variable (one wave's LDS accesses are in order), so each
+// round assigns at least one new variable.
+template <int NS>
+__device__ __forceinline__ bool propagate(uint32_t (&cl)[NS], Assign& s, uint32_t (&st)[NS], uint32_t* lt32,
+                                          const uint32_t* ct32, int lane) {
+  uint8_t* lt = reinterpret_cast<uint8_t*>(lt32);
+  const uint8_t* ct = reinterpret_cast<const uint8_t*>(ct32);
+  build_lit_table(lt32, s, lane);
   __syncthreads();
   for (int round = 0; round <= MAXV; ++round) {
     fresh(cl);
+    clause_states(cl, lt, ct, st);
     bool conflict = false;
     uint32_t unit = NO_UNIT;
 #pragma unroll
-    for (int k = 0; k < SLOTS; ++k) {
-      if (k < nslots) {
-        st[k] = clause_state(cl[k], lt, ct);
-        if (!cs_sat(st[k])) {
-          const uint32_t nfree = cs_nfree(st[k]);
-          conflict |= nfree == 0;
-          if (nfree == 1 && unit == NO_UNIT) unit = (cl[k] >> (8 * cs_pos(st[k]))) & 0xFFu;
-        }
-      }
+    for (int k = 0; k < NS; ++k) {
+      const uint32_t r = st[k], nfree = cs_nfree(r);
+      const bool open = !cs_sat(r);
+      conflict |= open && nfree == 0;
+      if (open && nfree == 1 && unit == NO_UNIT) unit = (cl[k] >> (8 * cs_pos(r))) & 0xFFu;
     }
     if (__ballot(conflict)) return false;
     uint64_t m = __ballot(unit != NO_UNIT);
@@ -194,9 +202,9 @@ __device__ __forceinline__ bool propagate(uint32_t (&cl)[SLOTS], int nslots, Ass
         s.a0 |= bit;
         s.v0 |= want ? bit : 0ull;
       }
+      lt[b] = (uint8_t)LIT_TRUE;
+      lt[b ^ 1u] = (uint8_t)LIT_FALSE;
     }
-    build_lit_table(lt, s, lane);
-    __syncthreads();
   }
   return true;  // unreachable: every round with a unit assigns a new variable
 }
@@ -215,12 +223,13 @@ __device__ __forceinline__ uint64_t entries_with_bit(uint32_t p) {
 
 }  // namespace
 
+template <int NS>
 __global__ __launch_bounds__(64) void k_search(ts_search p) {
   __shared__ uint32_t hist[MAXV];          // score << 20 | positive << 10 | negative
   __shared__ uint64_t stk[4][MAXV + 1];    // assignment before the branch at each depth
   __shared__ uint32_t stk_lit[MAXV + 1];   // branch literal byte | next child << 8
   __shared__ uint32_t lit_table[64];       // status byte of each literal byte
-  __shared__ uint32_t clause_table[64];    // status word of each (length, statuses) index
+  __shared__ uint32_t clause_table[128];   // status byte of each (length, statuses) index
 
   const int lane = (int)threadIdx.x;
   const ts_search_job job = p.jobs[blockIdx.x];
@@ -230,218 +239,221 @@ __global__ __launch_bounds__(64) void k_search(ts_search p) {
   if (job.instance >= 0) {
     const ts_search_inst inst = p.instances[job.instance];
     const int m = inst.num_clauses;
-    const int nslots = (m + 63) >> 6;
-    const int base = job.depth, split = job.split, own_from = job.own_from;
-    const uint32_t prefix = job.prefix;
-    const int width = p.width;
-    int64_t budget = p.max_nodes;
+    if (m > 64 * NS) {
+      // More clauses than this variant holds (the host picks the variant from the
+      // largest instance): undecided, never a wrong verdict.
+      status = 2;
+    } else {
+      const int base = job.depth, split = job.split, own_from = job.own_from;
+      const uint32_t prefix = job.prefix;
+      const int width = p.width;
+      const uint8_t* lt8 = reinterpret_cast<const uint8_t*>(lit_table);
+      int64_t budget = p.max_nodes;
 
-    uint32_t cl[SLOTS];
+      uint32_t cl[NS];
 #pragma unroll
-    for (int k = 0; k < SLOTS; ++k) {
-      const int i = k * 64 + lane;
-      cl[k] = (k < nslots && i < m) ? p.clauses[inst.clause_begin + i] : (NO_CLAUSE << 24);
-    }
-    hist[lane] = 0;
-    hist[lane + 64] = 0;
-    build_clause_table(clause_table, lane);
-    __syncthreads();
+      for (int k = 0; k < NS; ++k) {
+        const int i = k * 64 + lane;
+        cl[k] = i < m ? p.clauses[inst.clause_begin + i] : (NO_CLAUSE << 24);
+      }
+      hist[lane] = 0;
+      hist[lane + 64] = 0;
+      build_clause_table(clause_table, lane);
+      __syncthreads();
 
-    Assign s{job.assigned[0], job.assigned[1], job.value[0], job.value[1]};
-    uint32_t st[SLOTS];
-    // The loop alternates two phases.  Propagate: `s` is the assignment of the node at
-    // depth `cd` (the start node, or the child of frame cd - 1) before propagation.
-    // Next: try the next child of the branch frame at depth `fd`.
-    int cd = base, fd = base - 1;
-    bool prop = true;
-    for (;;) {
-      if (!prop) {
-        const uint32_t fr = uni(stk_lit[fd]);
-        const uint32_t t = fr >> 8;
-        if (t >= 2) {
-          if (fd == base) break;
-          --fd;
+      Assign s{job.assigned[0], job.assigned[1], job.value[0], job.value[1]};
+      uint32_t st[NS];
+      // The loop alternates two phases.  Propagate: `s` is the assignment of the node at
+      // depth `cd` (the start node, or the child of frame cd - 1) before propagation.
+      // Next: try the next child of the branch frame at depth `fd`.
+      int cd = base, fd = base - 1;
+      bool prop = true;
+      for (;;) {
+        if (!prop) {
+          const uint32_t fr = uni(stk_lit[fd]);
+          const uint32_t t = fr >> 8;
+          if (t >= 2) {
+            if (fd == base) break;
+            --fd;
+            continue;
+          }
+          stk_lit[fd] = (fr & 0xFFu) | ((t + 1) << 8);
+          __syncthreads();
+          if (fd - base < split && t != ((prefix >> (fd - base)) & 1u)) continue;
+          const uint32_t b = (fr & 0xFFu) ^ t;  // the second child negates the literal
+          const uint32_t v = b >> 1;
+          s.a0 = uni64(stk[0][fd]);
+          s.a1 = uni64(stk[1][fd]);
+          s.v0 = uni64(stk[2][fd]);
+          s.v1 = uni64(stk[3][fd]);
+          const uint64_t bit = 1ull << (v & 63);
+          const uint64_t val = (b & 1u) ? 0ull : bit;
+          if (v & 64) {
+            s.a1 |= bit;
+            s.v1 |= val;
+          } else {
+            s.a0 |= bit;
+            s.v0 |= val;
+          }
+          cd = fd + 1;
+          prop = true;
           continue;
         }
-        stk_lit[fd] = (fr & 0xFFu) | ((t + 1) << 8);
-        __syncthreads();
-        if (fd - base < split && t != ((prefix >> (fd - base)) & 1u)) continue;
-        const uint32_t b = (fr & 0xFFu) ^ t;  // the second child negates the literal
-        const uint32_t v = b >> 1;
-        s.a0 = uni64(stk[0][fd]);
-        s.a1 = uni64(stk[1][fd]);
-        s.v0 = uni64(stk[2][fd]);
-        s.v1 = uni64(stk[3][fd]);
-        const uint64_t bit = 1ull << (v & 63);
-        const uint64_t val = (b & 1u) ? 0ull : bit;
-        if (v & 64) {
-          s.a1 |= bit;
-          s.v1 |= val;
-        } else {
-          s.a0 |= bit;
-          s.v0 |= val;
+        // ---- propagate the node at depth cd ----
+        if (!propagate<NS>(cl, s, st, lit_table, clause_table, lane)) {
+          if (cd >= own_from) ++conflicts;
+          if (cd == base) break;
+          fd = cd - 1;
+          prop = false;
+          continue;
         }
-        cd = fd + 1;
-        prop = true;
-        continue;
-      }
-      // ---- propagate the node at depth cd ----
-      if (!propagate(cl, nslots, s, st, lit_table, clause_table, lane)) {
-        if (cd >= own_from) ++conflicts;
-        if (cd == base) break;
-        fd = cd - 1;
-        prop = false;
-        continue;
-      }
-      const int depth = cd;
-      if (p.early_exit &&
-          (__hip_atomic_load(&p.verdicts[job.instance], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & TS_SEARCH_SAT)) {
-        status = 1;
-        break;
-      }
-      const bool own = depth >= own_from;
-      if (own && owned >= budget) {
-        if (!p.spill) {
-          status = 2;
+        const int depth = cd;
+        if (p.early_exit &&
+            (__hip_atomic_load(&p.verdicts[job.instance], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &
+             TS_SEARCH_SAT)) {
+          status = 1;
           break;
         }
-        // Hand the rest over: this node's subtree, and the untried second child of every
-        // frame the job explores freely (below its prefix levels).
-        int n_out = 1;
-        for (int k = base + split; k < depth; ++k) n_out += (uni(stk_lit[k]) >> 8) == 1u;
-        int start = 0;
-        if (lane == 0) start = atomicAdd(p.spill_count, n_out);
-        start = (int)uni((uint32_t)__builtin_amdgcn_readlane(start, 0));
-        if (start + n_out <= p.spill_capacity) {
+        const bool own = depth >= own_from;
+        if (own && owned >= budget) {
+          if (!p.spill) {
+            status = 2;
+            break;
+          }
+          // Hand the rest over: this node's subtree, and the untried second child of
+          // every frame the job explores freely (below its prefix levels).
+          int n_out = 1;
+          for (int k = base + split; k < depth; ++k) n_out += (uni(stk_lit[k]) >> 8) == 1u;
+          int start = 0;
+          if (lane == 0) start = atomicAdd(p.spill_count, n_out);
+          start = (int)uni((uint32_t)__builtin_amdgcn_readlane(start, 0));
+          if (start + n_out <= p.spill_capacity) {
+            if (lane == 0) {
+              ts_search_job o = {};
+              o.instance = job.instance;
+              int w = start;
+              for (int k = base + split; k < depth; ++k) {
+                const uint32_t fr = stk_lit[k];
+                if ((fr >> 8) != 1u) continue;
+                const uint32_t b = (fr & 0xFFu) ^ 1u, v = b >> 1;
+                const uint64_t bit = 1ull << (v & 63), val = (b & 1u) ? 0ull : bit;
+                o.depth = k + 1;
+                o.own_from = k + 1;
+                o.assigned[0] = stk[0][k] | ((v & 64) ? 0ull : bit);
+                o.assigned[1] = stk[1][k] | ((v & 64) ? bit : 0ull);
+                o.value[0] = stk[2][k] | ((v & 64) ? 0ull : val);
+                o.value[1] = stk[3][k] | ((v & 64) ? val : 0ull);
+                p.spill[w++] = o;
+              }
+              // The subtree keeps what is left of the job's prefix levels (the other
+              // children there belong to the jobs with the other prefixes).
+              const int rel = depth - base;
+              o.depth = depth;
+              o.own_from = depth;
+              o.split = rel < split ? split - rel : 0;
+              o.prefix = rel < split ? prefix >> rel : 0u;
+              o.assigned[0] = s.a0;
+              o.assigned[1] = s.a1;
+              o.value[0] = s.v0;
+              o.value[1] = s.v1;
+              p.spill[w] = o;
+            }
+            status = 3;
+            break;
+          }
+          // No room: empty the reserved slots that exist and finish the subtree here.
           if (lane == 0) {
             ts_search_job o = {};
-            o.instance = job.instance;
-            int w = start;
-            for (int k = base + split; k < depth; ++k) {
-              const uint32_t fr = stk_lit[k];
-              if ((fr >> 8) != 1u) continue;
-              const uint32_t b = (fr & 0xFFu) ^ 1u, v = b >> 1;
-              const uint64_t bit = 1ull << (v & 63), val = (b & 1u) ? 0ull : bit;
-              o.depth = k + 1;
-              o.own_from = k + 1;
-              o.assigned[0] = stk[0][k] | ((v & 64) ? 0ull : bit);
-              o.assigned[1] = stk[1][k] | ((v & 64) ? bit : 0ull);
-              o.value[0] = stk[2][k] | ((v & 64) ? 0ull : val);
-              o.value[1] = stk[3][k] | ((v & 64) ? val : 0ull);
-              p.spill[w++] = o;
-            }
-            // The subtree keeps what is left of the job's prefix levels (the other
-            // children there belong to the jobs with the other prefixes).
-            const int rel = depth - base;
-            o.depth = depth;
-            o.own_from = depth;
-            o.split = rel < split ? split - rel : 0;
-            o.prefix = rel < split ? prefix >> rel : 0u;
-            o.assigned[0] = s.a0;
-            o.assigned[1] = s.a1;
-            o.value[0] = s.v0;
-            o.value[1] = s.v1;
-            p.spill[w] = o;
+            o.instance = -1;
+            for (int w = start; w < p.spill_capacity && w < start + n_out; ++w) p.spill[w] = o;
           }
-          status = 3;
-          break;
+          budget = INT64_MAX;
         }
-        // No room: empty the reserved slots that exist and finish the subtree here.
-        if (lane == 0) {
-          ts_search_job o = {};
-          o.instance = -1;
-          for (int w = start; w < p.spill_capacity && w < start + n_out; ++w) p.spill[w] = o;
+        if (own) {
+          ++owned;
+          ++nodes;
+          max_depth = depth > max_depth ? depth : max_depth;
         }
-        budget = INT64_MAX;
-      }
-      if (own) {
-        ++owned;
-        ++nodes;
-        max_depth = depth > max_depth ? depth : max_depth;
-      }
-      // Branching histogram over the residual clauses' free literals; it also names R.
-      fresh(cl);
+        // Branching histogram over the residual clauses' free literals; it also names R.
+        fresh(cl);
 #pragma unroll
-      for (int k = 0; k < SLOTS; ++k) {
-        if (k < nslots && !cs_sat(st[k])) {
+        for (int k = 0; k < NS; ++k) {
           const uint32_t c = cl[k], len = c >> 24;
+          const bool open = !cs_sat(st[k]);
           const uint32_t w = (cs_nfree(st[k]) - 1u) << 20;
 #pragma unroll
           for (int i = 0; i < 3; ++i) {
-            if ((uint32_t)i < len) {
-              const uint32_t b = (c >> (8 * i)) & 0xFFu;
-              if (lds_byte(lit_table, b) == LIT_FREE) atomicAdd(&hist[b >> 1], w | ((b & 1u) ? 1u : (1u << 10)));
-            }
+            const uint32_t b = (c >> (8 * i)) & 0xFFu;
+            if (open && (uint32_t)i < len && lt8[b] == LIT_FREE)
+              atomicAdd(&hist[b >> 1], w | ((b & 1u) ? 1u : (1u << 10)));
           }
         }
-      }
-      __syncthreads();
-      const uint32_t h0 = hist[lane], h1 = hist[lane + 64];
-      hist[lane] = 0;
-      hist[lane + 64] = 0;
-      const uint64_t r0 = __ballot(h0 != 0), r1 = __ballot(h1 != 0);
-      const int rank = __popcll(r0) + __popcll(r1);
-      if (rank <= width) {
-        // ---- leaf: the residual clause tensors multiplied into the rank-|R| relation
-        // (entry e: bit j of e = value of the j-th smallest variable of R), then reduced ----
-        uint64_t bits = 1;  // no residual clause: the empty product
-        if (rank > 0) {
-          uint64_t mine = ~0ull;  // entries satisfying this lane's residual clauses
-          fresh(cl);
+        __syncthreads();
+        const uint32_t h0 = hist[lane], h1 = hist[lane + 64];
+        hist[lane] = 0;
+        hist[lane + 64] = 0;
+        const uint64_t r0 = __ballot(h0 != 0), r1 = __ballot(h1 != 0);
+        const int rank = __popcll(r0) + __popcll(r1);
+        if (rank <= width) {
+          // ---- leaf: the residual clause tensors multiplied into the rank-|R| relation
+          // (entry e: bit j of e = value of the j-th smallest variable of R), then reduced ----
+          uint64_t bits = 1;  // no residual clause: the empty product
+          if (rank > 0) {
+            uint64_t mine = ~0ull;  // entries satisfying this lane's residual clauses
+            fresh(cl);
 #pragma unroll
-          for (int k = 0; k < SLOTS; ++k) {
-            if (k < nslots && !cs_sat(st[k])) {
+            for (int k = 0; k < NS; ++k) {
               const uint32_t c = cl[k], len = c >> 24;
-              uint64_t fals = ~0ull;
+              uint64_t fals = cs_sat(st[k]) ? 0ull : ~0ull;  // a satisfied clause removes nothing
 #pragma unroll
               for (int i = 0; i < 3; ++i) {
-                if ((uint32_t)i < len) {
-                  const uint32_t b = (c >> (8 * i)) & 0xFFu, v = b >> 1;
-                  if (lds_byte(lit_table, b) == LIT_FREE) {  // assigned literals of a residual clause are false
-                    const uint32_t pos = (v & 64) ? (uint32_t)(__popcll(r0) + __popcll(r1 & ((1ull << (v & 63)) - 1ull)))
-                                                  : (uint32_t)__popcll(r0 & ((1ull << v) - 1ull));
-                    const uint64_t ones = entries_with_bit(pos);
-                    fals &= (b & 1u) ? ones : ~ones;  // false where the variable equals the negation bit
-                  }
+                const uint32_t b = (c >> (8 * i)) & 0xFFu, v = b >> 1;
+                // assigned literals of a residual clause are false
+                if ((uint32_t)i < len && lt8[b] == LIT_FREE) {
+                  const uint32_t pos = (v & 64) ? (uint32_t)(__popcll(r0) + __popcll(r1 & ((1ull << (v & 63)) - 1ull)))
+                                                : (uint32_t)__popcll(r0 & ((1ull << v) - 1ull));
+                  const uint64_t ones = entries_with_bit(pos);
+                  fals &= (b & 1u) ? ones : ~ones;  // false where the variable equals the negation bit
                 }
               }
               mine &= ~fals;
             }
+            const uint32_t lo = wave_reduce((uint32_t)mine, OpAnd());
+            const uint32_t hi = wave_reduce((uint32_t)(mine >> 32), OpAnd());
+            const uint64_t valid = rank == 6 ? ~0ull : ((1ull << (1u << rank)) - 1ull);
+            bits = (((uint64_t)hi << 32) | lo) & valid;
           }
-          const uint32_t lo = wave_reduce((uint32_t)mine, OpAnd());
-          const uint32_t hi = wave_reduce((uint32_t)(mine >> 32), OpAnd());
-          const uint64_t valid = rank == 6 ? ~0ull : ((1ull << (1u << rank)) - 1ull);
-          bits = (((uint64_t)hi << 32) | lo) & valid;
-        }
-        if (own) {
-          ++leaves;
-          digest += splitmix(splitmix(bits) + (uint64_t)rank);
-          if (bits != 0) {
-            if (lane == 0) atomicOr(&p.verdicts[job.instance], TS_SEARCH_SAT);
-            if (!p.exhaustive) break;
+          if (own) {
+            ++leaves;
+            digest += splitmix(splitmix(bits) + (uint64_t)rank);
+            if (bits != 0) {
+              if (lane == 0) atomicOr(&p.verdicts[job.instance], TS_SEARCH_SAT);
+              if (!p.exhaustive) break;
+            }
           }
+          if (depth == base) break;
+          fd = depth - 1;
+          prop = false;
+          continue;
         }
-        if (depth == base) break;
-        fd = depth - 1;
+        // ---- branch: free variable of largest score, ties to the smaller index ----
+        const uint32_t k0 = h0 ? (((h0 >> 20) << 8) | (255u - (uint32_t)lane)) : 0u;
+        const uint32_t k1 = h1 ? (((h1 >> 20) << 8) | (255u - (uint32_t)(lane + 64))) : 0u;
+        const uint32_t key = wave_reduce(k0 > k1 ? k0 : k1, OpMax());
+        const uint32_t bv = 255u - (key & 255u);
+        const uint32_t hb =
+            bv < 64 ? __builtin_amdgcn_readlane(h0, (int)bv) : __builtin_amdgcn_readlane(h1, (int)bv - 64);
+        const uint32_t npos = (hb >> 10) & 1023u, nneg = hb & 1023u;
+        const uint32_t first = (bv << 1) | (npos >= nneg ? 0u : 1u);
+        stk[0][depth] = s.a0;
+        stk[1][depth] = s.a1;
+        stk[2][depth] = s.v0;
+        stk[3][depth] = s.v1;
+        stk_lit[depth] = first;  // next child 0
+        __syncthreads();
+        fd = depth;
         prop = false;
-        continue;
       }
-      // ---- branch: free variable of largest score, ties to the smaller index ----
-      const uint32_t k0 = h0 ? (((h0 >> 20) << 8) | (255u - (uint32_t)lane)) : 0u;
-      const uint32_t k1 = h1 ? (((h1 >> 20) << 8) | (255u - (uint32_t)(lane + 64))) : 0u;
-      const uint32_t key = wave_reduce(k0 > k1 ? k0 : k1, OpMax());
-      const uint32_t bv = 255u - (key & 255u);
-      const uint32_t hb = bv < 64 ? __builtin_amdgcn_readlane(h0, (int)bv) : __builtin_amdgcn_readlane(h1, (int)bv - 64);
-      const uint32_t npos = (hb >> 10) & 1023u, nneg = hb & 1023u;
-      const uint32_t first = (bv << 1) | (npos >= nneg ? 0u : 1u);
-      stk[0][depth] = s.a0;
-      stk[1][depth] = s.a1;
-      stk[2][depth] = s.v0;
-      stk[3][depth] = s.v1;
-      stk_lit[depth] = first;  // next child 0
-      __syncthreads();
-      fd = depth;
-      prop = false;
     }
     if (lane == 0 && status == 2) atomicOr(&p.verdicts[job.instance], TS_SEARCH_INCOMPLETE);
   }
@@ -461,9 +473,23 @@ extern "C" int ts_search_run(const ts_search* p, ts_stream stream) {
   if (!p || p->num_jobs < 0 || p->num_instances < 0) return TS_EINVAL;
   if (p->width < 0 || p->width > TS_SEARCH_MAX_WIDTH || p->max_nodes <= 0) return TS_EINVAL;
   if (p->spill && (!p->spill_count || p->spill_capacity < 0)) return TS_EINVAL;
+  if (p->max_clauses < 0 || p->max_clauses > 64 * TS_SEARCH_SLOTS) return TS_EINVAL;
   if (p->num_jobs == 0) return TS_OK;
   if (!p->clauses || !p->instances || !p->jobs || !p->verdicts) return TS_EINVAL;
-  hipLaunchKernelGGL(k_search, dim3((unsigned)p->num_jobs), dim3(64), 0, reinterpret_cast<hipStream_t>(stream), *p);
+  // The variant whose clause slots per lane hold the largest instance.
+  const int need = p->max_clauses ? (p->max_clauses + 63) / 64 : TS_SEARCH_SLOTS;
+  const dim3 grid((unsigned)p->num_jobs), block(64);
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  if (need <= 2)
+    hipLaunchKernelGGL(k_search<2>, grid, block, 0, st, *p);
+  else if (need <= 4)
+    hipLaunchKernelGGL(k_search<4>, grid, block, 0, st, *p);
+  else if (need <= 6)
+    hipLaunchKernelGGL(k_search<6>, grid, block, 0, st, *p);
+  else if (need <= 8)
+    hipLaunchKernelGGL(k_search<8>, grid, block, 0, st, *p);
+  else
+    hipLaunchKernelGGL(k_search<TS_SEARCH_SLOTS>, grid, block, 0, st, *p);
   return hipGetLastError() == hipSuccess ? TS_OK : TS_EHIP;
 }
 

@@ -53,7 +53,8 @@ class _Search(ctypes.Structure):
         ("jobs", ctypes.c_void_p), ("num_jobs", ctypes.c_int32), ("verdicts", ctypes.c_void_p),
         ("stats", ctypes.c_void_p), ("width", ctypes.c_int32), ("exhaustive", ctypes.c_int32),
         ("early_exit", ctypes.c_int32), ("spill_capacity", ctypes.c_int32), ("max_nodes", ctypes.c_int64),
-        ("spill", ctypes.c_void_p), ("spill_count", ctypes.c_void_p),
+        ("spill", ctypes.c_void_p), ("spill_count", ctypes.c_void_p), ("max_clauses", ctypes.c_int32),
+        ("reserved2", ctypes.c_int32),
     ]
 
 
@@ -256,6 +257,7 @@ class SearchSolver:
 
         dev = self.device
         self.num_instances = len(inst)
+        self.max_clauses = int(inst["num_clauses"].max()) if len(inst) else 0
         self.host_jobs = jobs
         self.d_words = torch.from_numpy(words.view(np.int32) if words.size else np.zeros(1, np.int32)).to(dev)
         self.d_inst = torch.from_numpy(inst.view(np.int32).reshape(-1) if len(inst) else np.zeros(4, np.int32)).to(dev)
@@ -276,6 +278,7 @@ class SearchSolver:
             int(self.exhaustive), int(not self.exhaustive), self.cap if spill is not None else 0,
             self.budget if self.budget is not None else self.max_nodes,
             N.dptr(spill) if spill is not None else None, N.dptr(self.d_count) if spill is not None else None,
+            self.max_clauses, 0,
         )
 
     def launch(self) -> None:
