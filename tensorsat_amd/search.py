@@ -109,7 +109,8 @@ def log2_search_size(num_vars: np.ndarray, num_clauses: np.ndarray) -> np.ndarra
 def plan_jobs(inst: np.ndarray, *, split_target: float | None = 128.0, max_split: int = 8) -> np.ndarray:
     """One job per subtree: instance i gets 2^split_i jobs (prefixes 0..2^split_i - 1)
     where split_i = ceil(log2(estimated nodes / split_target)) (0 with ``split_target``
-    None), launched in decreasing order of estimated size per job."""
+    None), launched in decreasing order of estimated size per job (instances of equal
+    size in index order, the jobs of an instance in prefix order)."""
     num = len(inst)
     if num == 0:
         return np.zeros(0, dtype=JOB_DTYPE)
@@ -120,20 +121,19 @@ def plan_jobs(inst: np.ndarray, *, split_target: float | None = 128.0, max_split
         if split_target <= 0:
             raise ValueError("split_target must be positive")
         split = np.clip(np.ceil(est - math.log2(split_target)), 0, min(max_split, MAX_SPLIT)).astype(np.int64)
-    nj = 1 << split
+    order = np.argsort(-(est - split), kind="stable")
+    nj = (np.int64(1) << split)[order]
     jobs = np.zeros(int(nj.sum()), dtype=JOB_DTYPE)
-    jobs["instance"] = np.repeat(np.arange(num), nj)
-    jobs["split"] = np.repeat(split, nj)
+    jobs["instance"] = np.repeat(order, nj)
+    jobs["split"] = np.repeat(split[order], nj)
     starts = np.zeros(num, dtype=np.int64)
     np.cumsum(nj[:-1], out=starts[1:])
-    prefix = np.arange(jobs.size) - np.repeat(starts, nj)
+    prefix = np.arange(jobs.size, dtype=np.int64) - np.repeat(starts, nj)
     jobs["prefix"] = prefix
     # A node on the prefix levels belongs to the job whose remaining prefix bits are all
     # 0: the job counts from one level below its highest set prefix bit.
     jobs["own_from"] = np.where(prefix > 0, np.floor(np.log2(np.maximum(prefix, 1))).astype(np.int64) + 1, 0)
-    key = np.repeat(est - split, nj)
-    order = np.lexsort((jobs["prefix"], jobs["instance"], -key))
-    return jobs[order]
+    return jobs
 
 
 def _var_masks(nv: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
@@ -146,8 +146,7 @@ def _var_masks(nv: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
 
 
 def _popcount64(x: np.ndarray) -> np.ndarray:
-    b = np.ascontiguousarray(x, dtype=np.uint64).view(np.uint8).reshape(-1, 8)
-    return np.unpackbits(b, axis=1).sum(axis=1).astype(np.int64)
+    return np.bitwise_count(np.asarray(x, dtype=np.uint64)).astype(np.int64)
 
 
 def validate(words: np.ndarray, inst: np.ndarray, jobs: np.ndarray, width: int, *, literals: bool = True) -> None:
@@ -186,6 +185,10 @@ def validate(words: np.ndarray, inst: np.ndarray, jobs: np.ndarray, width: int, 
             raise ValueError(f"split outside 0..{MAX_SPLIT}")
         if np.any(jobs["prefix"].astype(np.int64) >> jobs["split"].astype(np.int64)):
             raise ValueError("prefix has bits beyond its split")
+        if np.any(jobs["own_from"] < 0) or np.any(jobs["own_from"] > jobs["depth"] + jobs["split"]):
+            raise ValueError("job owns from a depth below its prefix levels")
+        if not jobs["depth"].any() and not jobs["assigned"].any() and not jobs["value"].any():
+            return  # jobs from the root: nothing more to check
         live = jobs["instance"] >= 0
         nv = inst["num_vars"][np.maximum(jobs["instance"], 0)].astype(np.int64)
         lo_mask, hi_mask = _var_masks(nv)
@@ -199,8 +202,6 @@ def validate(words: np.ndarray, inst: np.ndarray, jobs: np.ndarray, width: int, 
         # depth + free - 1 at most, which must stay inside its MAX_VARS frames
         if np.any(live & ((jobs["depth"] < 0) | (jobs["depth"] + free > MAX_VARS))):
             raise ValueError("job start depth leaves no room for its branch stack")
-        if np.any(live & ((jobs["own_from"] < 0) | (jobs["own_from"] > jobs["depth"] + jobs["split"]))):
-            raise ValueError("job owns from a depth below its prefix levels")
 
 
 class SearchSolver:

@@ -89,3 +89,31 @@ def test_validate_rejects_out_of_range_records():
         S.validate(words, small, jobs, 6)
     with pytest.raises(ValueError):
         S.validate(words, inst, jobs, 7)
+
+
+def test_validate_checks_job_start_assignments():
+    """Jobs that start below the root (as hand-overs do) carry an assignment: it must name
+    only the instance's variables, keep values inside it, and leave the branch stack room
+    for every free variable."""
+    words, inst = S.pack_batch(batched_3sat(4))
+    jobs = S.plan_jobs(inst, split_target=None)
+    ok = jobs.copy()
+    ok["depth"][0] = 2
+    ok["own_from"][0] = 2
+    ok["assigned"][0, 0] = 0b11
+    ok["value"][0, 0] = 0b01
+    S.validate(words, inst, ok, 6)
+    bad = ok.copy()
+    bad["assigned"][0, 1] = 1  # variable 64 does not exist below 65 variables
+    if int(inst["num_vars"][bad["instance"][0]]) <= 64:
+        with pytest.raises(ValueError):
+            S.validate(words, inst, bad, 6)
+    bad = ok.copy()
+    bad["value"][0, 0] = 0b100  # a value for an unassigned variable
+    with pytest.raises(ValueError):
+        S.validate(words, inst, bad, 6)
+    bad = ok.copy()
+    bad["depth"][0] = 127  # 127 + free variables exceeds the 128-frame stack
+    bad["own_from"][0] = 127
+    with pytest.raises(ValueError):
+        S.validate(words, inst, bad, 6)
