@@ -127,6 +127,16 @@ int ts_program_run(const ts_job* job_host, const ts_step* steps_host, const uint
                    uint8_t* arena, int32_t mode, uint8_t* inter, const int64_t* inter_offsets,
                    ts_stream stream);
 
+/* As ts_program_run, with every run of consecutive steps whose operands and output all
+ * have at most `small_bits` bits executed by one single-workgroup launch (the steps in
+ * order, barriers between them; steps_dev is the device copy of the step array) and the
+ * larger steps launched grid-wide: one launch per run instead of one per step.  No
+ * intermediate dumps.  The per-node loop of cotengra.py:120-126 / simple.py:251-292 for
+ * networks whose small steps would leave the device idle between launches. */
+int ts_program_run_fused(const ts_job* job_host, const ts_step* steps_host, const ts_step* steps_dev,
+                         const uint8_t* leaves_dev, uint8_t* arena, int32_t mode, int32_t small_bits,
+                         ts_stream stream);
+
 /* Counting variant: arena of uint64; result scaled by 2^free_vars is added to *count
  * (device uint64); overflow sets *overflow. */
 int ts_program_count(const ts_job* job_host, const ts_step* steps_host, const uint8_t* leaves_dev,

@@ -40,6 +40,28 @@ __global__ __launch_bounds__(WG) void k_step_u8(const uint8_t* __restrict__ lhs,
   step_u8(st, L, lhs, rhs, out, (uint64_t)blockIdx.x * WG + threadIdx.x, (uint64_t)gridDim.x * WG, mode);
 }
 
+// A run of consecutive small steps of one job on one workgroup: each step's record is
+// staged in LDS, its chunk tables built, and its outputs written to the job arena before
+// the next step reads them (workgroup barriers between steps).  Saves one launch per
+// step where steps are too small to fill the device.
+__global__ __launch_bounds__(WG) void k_step_seq(uint8_t* __restrict__ arena, const ts_step* __restrict__ steps,
+                                                 int begin, int end, int mode) {
+  __shared__ Lut L;
+  __shared__ ts_step s_st;
+  const int tid = threadIdx.x;
+  for (int i = begin; i < end; ++i) {
+    if (tid < (int)(sizeof(ts_step) / 16))
+      reinterpret_cast<uint4*>(&s_st)[tid] = reinterpret_cast<const uint4*>(steps + i)[tid];
+    __syncthreads();
+    const bool need_c = !(s_st.flags & TS_STEP_KINNER) && s_st.sum_rank > 0;
+    build_lut(s_st, L, need_c, tid);
+    __syncthreads();
+    const uint8_t* rhs = arena + ((s_st.flags & TS_STEP_NORHS) ? s_st.lhs_off : s_st.rhs_off);
+    step_u8(s_st, L, arena + s_st.lhs_off, rhs, arena + s_st.out_off, (uint64_t)tid, (uint64_t)WG, mode);
+    __syncthreads();
+  }
+}
+
 __global__ __launch_bounds__(WG) void k_step_u64(const uint64_t* __restrict__ lhs,
                                                  const uint64_t* __restrict__ rhs,
                                                  uint64_t* __restrict__ out, ts_step st,
@@ -903,6 +925,47 @@ int ts_program_run(const ts_job* job, const ts_step* steps, const uint8_t* leave
                          hipMemcpyDeviceToDevice, s) != hipSuccess)
         return TS_EHIP;
     }
+  }
+  return TS_OK;
+}
+
+int ts_program_run_fused(const ts_job* job, const ts_step* steps, const ts_step* steps_dev, const uint8_t* leaves,
+                         uint8_t* arena, int32_t mode, int32_t small_bits, ts_stream stream) {
+  if (!job || !arena || !leaves || (job->num_steps > 0 && (!steps || !steps_dev))) return TS_EINVAL;
+  if (mode != TS_MODE_GT0 && mode != TS_MODE_MOD256) return TS_EINVAL;
+  if (small_bits < 0 || small_bits > TS_MAX_BITS) return TS_EINVAL;
+  if (job->num_leaf_bytes > job->arena_elems) return TS_EINVAL;
+  const ts_step* rec = steps + job->step_begin;
+  for (int i = 0; i < job->num_steps; ++i) {
+    const ts_step& st = rec[i];
+    const uint64_t hi = std::max<uint64_t>({st.lhs_off + (1ull << st.lhs_rank),
+                                             (st.flags & TS_STEP_NORHS) ? 0 : st.rhs_off + (1ull << st.rhs_rank),
+                                             st.out_off + (1ull << st.out_rank)});
+    if (hi > (uint64_t)job->arena_elems) return TS_EINVAL;
+  }
+  hipStream_t s = S(stream);
+  if (hipMemcpyAsync(arena, leaves + job->leaf_begin, (size_t)job->num_leaf_bytes, hipMemcpyDeviceToDevice, s) !=
+      hipSuccess)
+    return TS_EHIP;
+  auto small = [&](const ts_step& st) {
+    const int r = std::max<int>({st.lhs_rank, (st.flags & TS_STEP_NORHS) ? 0 : st.rhs_rank, st.out_rank});
+    return r <= small_bits;
+  };
+  int i = 0;
+  while (i < job->num_steps) {
+    if (small(rec[i])) {
+      int j = i + 1;
+      while (j < job->num_steps && small(rec[j])) ++j;
+      hipLaunchKernelGGL(k_step_seq, dim3(1), dim3(WG), 0, s, arena, steps_dev + job->step_begin, i, j, mode);
+      if (hipGetLastError() != hipSuccess) return TS_EHIP;
+      i = j;
+      continue;
+    }
+    const ts_step& st = rec[i];
+    const uint8_t* rhs = arena + ((st.flags & TS_STEP_NORHS) ? st.lhs_off : st.rhs_off);
+    const int rc = launch_step_u8(arena + st.lhs_off, rhs, arena + st.out_off, st, mode, s);
+    if (rc) return rc;
+    ++i;
   }
   return TS_OK;
 }

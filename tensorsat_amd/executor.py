@@ -88,6 +88,28 @@ def run_job(dp: DeviceProgram, j: int, mode: str = "GT0", *, keep_intermediates:
     return result, outs
 
 
+def run_job_fused(dp: DeviceProgram, j: int, mode: str = "GT0", *, arena=None, small_bits: int = 14):
+    """Run job ``j`` on the current stream with ``ts_program_run_fused`` (runs of steps of
+    at most ``small_bits`` bits on one workgroup per run, larger steps grid-wide);
+    returns a view of the result in the arena (``arena``: a uint8 tensor of at least the
+    job's arena size, else a new one)."""
+    torch = _torch()
+    L = N.lib()
+    job = dp.jobs_host[j]
+    need = max(16, int(job["arena_elems"]))
+    if arena is None:
+        arena = torch.empty(need, dtype=torch.uint8, device=dp.device)
+    elif arena.numel() < need or arena.dtype != torch.uint8:
+        raise ValueError("arena too small for the job")
+    N.check(
+        L.ts_program_run_fused(dp.job_ptr(j), N.host_ptr(dp.steps_host), N.dptr(dp.steps), N.dptr(dp.leaves),
+                               N.dptr(arena), N.MODES[mode], small_bits, N.stream_ptr()),
+        "ts_program_run_fused",
+    )
+    r0, rr = int(job["result_off"]), int(job["result_rank"])
+    return arena[r0 : r0 + (1 << rr)]
+
+
 def count_job(dp: DeviceProgram, j: int, acc=None, overflow=None):
     """Exact uint64 count of job ``j`` (times 2^free_vars) added into ``acc``."""
     torch = _torch()

@@ -3,8 +3,11 @@ One large instance on the whole GPU.
 
 * :func:`decide` — a SAT decision for one instance whose intermediates need the whole
   device (the n=120, m=510 workload: cubes of width up to 26, i.e. 2^26-entry tensors):
-  cubes run one after the other in depth-first order, every step grid-wide, stopping at
-  the first satisfiable cube.
+  cubes in depth-first order, several at a time on their own HIP streams and arenas (the
+  small steps of one cube leave the device idle that the large steps of another fill),
+  each run with its small steps fused into single-workgroup launches and its large steps
+  grid-wide; results are read in cube order and the first satisfiable cube stops the
+  submission.
 * :func:`decide_budget` — a SAT decision under a wall-clock budget for instances whose
   cube enumeration cannot be completed (the learned-planner workload, n = 150..300):
   cubes are compiled in chunks of doubling size in depth-first order and each chunk is
@@ -46,28 +49,56 @@ class SingleResult:
 
 
 def decide(cnf: CNFInstance, *, width: int = 26, mode: str = "GT0", plan_trials: int = 2,
-           program: PG.Program | None = None) -> SingleResult:
+           program: PG.Program | None = None, streams: int = 4, small_bits: int = 14) -> SingleResult:
+    """See the module docstring.  ``stats["cubes_launched"]`` counts the cubes that ran
+    (up to ``streams - 1`` past the deciding one); ``cubes_run`` those up to it."""
+    import collections
+
     import torch
+
+    from .executor import run_job_fused
 
     t0 = time.perf_counter()
     prog = program if program is not None else PG.sat_batch([cnf], width=width, plan_trials=plan_trials)
     plan_s = time.perf_counter() - t0
     if len(prog.jobs) == 0:
         return SingleResult(0, plan_s, 0.0, 0, 0, {"host_unsat": True})
+    if streams < 1:
+        raise ValueError("streams must be >= 1")
     dp = DeviceProgram(prog)
+    nj = len(prog.jobs)
+    size = max(16, int(prog.jobs["arena_elems"].max()))
+    lanes = [(torch.cuda.Stream(), torch.empty(size, dtype=torch.uint8, device=dp.device))
+             for _ in range(min(streams, nj))]
+    results = torch.zeros(nj, dtype=torch.uint8, device=dp.device)
+    main = torch.cuda.current_stream()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    verdict = 0
-    ran = 0
-    for j in range(len(prog.jobs)):
-        res, _ = run_job(dp, j, mode)
-        ran += 1
-        if bool(res.any().item()):
+    verdict, ran, launched = 0, 0, 0
+    inflight: collections.deque = collections.deque()
+    while True:
+        while launched < nj and len(inflight) < len(lanes):
+            st, arena = lanes[launched % len(lanes)]
+            st.wait_stream(main)
+            with torch.cuda.stream(st):
+                res = run_job_fused(dp, launched, mode, arena=arena, small_bits=small_bits)
+                results[launched : launched + 1].copy_(res.amax().reshape(1))
+                ev = torch.cuda.Event()
+                ev.record(st)
+            inflight.append((launched, ev))
+            launched += 1
+        if not inflight:
+            break
+        j, ev = inflight.popleft()
+        ev.synchronize()
+        ran = j + 1
+        if int(results[j].item()):
             verdict = 1
             break
     torch.cuda.synchronize()
-    return SingleResult(verdict, plan_s, time.perf_counter() - t0, len(prog.jobs), ran,
-                        {"max_rank": int(prog.stats[0, 4]), "branch_nodes": int(prog.stats[0, 1])})
+    return SingleResult(verdict, plan_s, time.perf_counter() - t0, nj, ran,
+                        {"max_rank": int(prog.stats[0, 4]), "branch_nodes": int(prog.stats[0, 1]),
+                         "cubes_launched": launched, "streams": len(lanes)})
 
 
 def decide_budget(cnf: CNFInstance, *, width: int = 22, budget_s: float = 20.0, mode: str = "GT0",

@@ -76,3 +76,31 @@ def test_count_overflow_raises():
     prog = PG.sat_batch([cnf], width=8)
     with pytest.raises(OverflowError):
         BatchCounter(prog).run()
+
+
+@pytest.mark.parametrize("width,small_bits", [(14, 10), (26, 14), (20, 27)])
+def test_fused_runner_leaves_the_same_arena(width, small_bits):
+    """ts_program_run_fused (runs of small steps on one workgroup, large steps grid-wide)
+    executes the same steps in the same order as the per-step runner: the whole final
+    arena of every cube is byte-identical, in both modes."""
+    import ctypes
+
+    import torch
+
+    from tensorsat_amd import _native as N
+    from tensorsat_amd.executor import DeviceProgram, run_job_fused
+
+    prog = PG.sat_batch([single_3sat(1)], width=width, max_cubes=4, plan_trials=2, threads=8)
+    dp = DeviceProgram(prog)
+    L = N.lib()
+    for mode in ("GT0", "MOD256"):
+        for j in range(len(prog.jobs)):
+            size = max(16, int(prog.jobs["arena_elems"][j]))
+            a = torch.zeros(size, dtype=torch.uint8, device="cuda")
+            b = torch.zeros(size, dtype=torch.uint8, device="cuda")
+            N.check(L.ts_program_run(dp.job_ptr(j), N.host_ptr(dp.steps_host), N.dptr(dp.leaves), N.dptr(a),
+                                     N.MODES[mode], None, None, N.stream_ptr()), "ts_program_run")
+            res = run_job_fused(dp, j, mode, arena=b, small_bits=small_bits)
+            torch.cuda.synchronize()
+            print(f"width {width} small {small_bits} {mode} cube {j}: result {int(res.item())}", flush=True)
+            assert torch.equal(a, b), f"cube {j}"
