@@ -287,8 +287,23 @@ typedef struct {
                                       launch uses the kernel variant with just enough clause
                                       slots per lane; an instance beyond it is reported
                                       TS_SEARCH_INCOMPLETE, never decided */
-  int32_t reserved2;
+  int32_t order;                   /* TS_SEARCH_ORDER_SCORE or TS_SEARCH_ORDER_LEX */
+  uint64_t* witness;               /* device, 2 words per instance (optional): the assignment
+                                      of the satisfiable leaf a job finds (variable v is bit
+                                      v & 63 of word v >> 6; variables the leaf leaves free
+                                      are 0, its relation's entry the lexicographically
+                                      smallest), written by each job that finds one */
 } ts_search;
+
+/* Branching orders.  SCORE: the free variable of largest weighted occurrence in the
+ * residual clauses, its more frequent polarity first (the decision default).  LEX: the
+ * smallest variable of the residual, false first: with one job per instance, no hand-over
+ * and no exhaustive mode, the first satisfiable leaf gives the lexicographically smallest
+ * satisfying assignment (variables in index order, 0 before 1) -- the assignment the
+ * reference's search over one contraction with 2n input selections returns
+ * (lib/sat.py:231-246 contraction_wiring; notebooks/02-random-ksat.ipynb cell 36). */
+#define TS_SEARCH_ORDER_SCORE 0
+#define TS_SEARCH_ORDER_LEX 1
 
 /* One launch of num_jobs one-wave workgroups.  Rejects (TS_EINVAL) null arrays and widths
  * out of range; job and instance records are the caller's to check (the host layer

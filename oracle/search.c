@@ -29,7 +29,11 @@
  *   - leaves += 1 at every leaf; digest += mix(relation bits, rank) (mod 2^64), the
  *     relation's entry e (bit j of e = value of the j-th smallest residual variable)
  *     being bit e of the 64-bit word;
- *   - without `exhaustive`, the search stops at the first leaf whose result is 1.
+ *   - without `exhaustive`, the search stops at the first leaf whose result is 1;
+ *   - order LEX instead branches on the smallest residual variable, false first, and the
+ *     first satisfiable leaf's witness (its assignment, R's variables from the relation
+ *     entry that is smallest with R's smallest variable most significant, 0 for the other
+ *     free variables) is the lexicographically smallest satisfying assignment.
  * Unit propagation to the fixpoint has one result whatever the order units are taken
  * in, so the device's parallel rounds and this sequential loop agree.
  *
@@ -73,7 +77,9 @@ typedef struct {
   int* cl;   /* m x 3 literals (+-(v+1)), 0 = absent */
   int* len;
   int8_t* val;
-  int width, exhaustive;
+  int width, exhaustive, order;
+  uint64_t witness[2];
+  int have_witness;
   int64_t max_nodes;
   search_stat st;
   leaf_rec* trace;
@@ -197,12 +203,37 @@ static int dfs(Ctx* c, int* trail, int* nt, int depth) {
         }
     }
     result = bits != 0;
+    if (result && !c->have_witness) {
+      int best = 0;
+      uint32_t best_key = 0xFFFFFFFFu;
+      for (int e = 0; e < (1 << r); ++e) {
+        if (!((bits >> e) & 1)) continue;
+        uint32_t key = 0; /* e with bit 0 (R's smallest variable) most significant */
+        for (int j = 0; j < r; ++j) key |= ((e >> j) & 1u) << (r - 1 - j);
+        if (key < best_key) {
+          best_key = key;
+          best = e;
+        }
+      }
+      c->witness[0] = c->witness[1] = 0;
+      for (int v = 0; v < c->n && v < 128; ++v)
+        if (c->val[v] > 0) c->witness[v >> 6] |= 1ull << (v & 63);
+      for (int j = 0; j < r; ++j)
+        if ((best >> j) & 1) c->witness[vars[j] >> 6] |= 1ull << (vars[j] & 63);
+      c->have_witness = 1;
+    }
     if (result && !c->exhaustive) c->stop = 1;
   } else {
-    int bv = -1;
-    for (int v = 0; v < c->n; ++v)
-      if (inr[v] && (bv < 0 || score[v] > score[bv])) bv = v;
-    int first = pos[bv] >= neg[bv] ? bv + 1 : -(bv + 1);
+    int bv = -1, first;
+    if (c->order == 1) {
+      for (int v = 0; v < c->n && bv < 0; ++v)
+        if (inr[v]) bv = v;
+      first = -(bv + 1);
+    } else {
+      for (int v = 0; v < c->n; ++v)
+        if (inr[v] && (bv < 0 || score[v] > score[bv])) bv = v;
+      first = pos[bv] >= neg[bv] ? bv + 1 : -(bv + 1);
+    }
     for (int t = 0; t < 2 && !c->stop; ++t) {
       int l = t ? -first : first;
       int mark = *nt;
@@ -223,9 +254,9 @@ static int dfs(Ctx* c, int* trail, int* nt, int depth) {
   return result;
 }
 
-static void run_one(int n, int m, const int64_t* loff, const int32_t* lits, int width, int exhaustive,
-                    int64_t max_nodes, search_stat* out, leaf_rec* trace, int trace_cap, int* trace_len,
-                    int instance) {
+static void run_one(int n, int m, const int64_t* loff, const int32_t* lits, int width, int exhaustive, int order,
+                    int64_t max_nodes, search_stat* out, uint64_t* witness, leaf_rec* trace, int trace_cap,
+                    int* trace_len, int instance) {
   Ctx c;
   memset(&c, 0, sizeof(c));
   c.n = n;
@@ -234,6 +265,7 @@ static void run_one(int n, int m, const int64_t* loff, const int32_t* lits, int 
   c.val = (int8_t*)calloc((size_t)(n > 0 ? n : 1), 1);
   c.width = width;
   c.exhaustive = exhaustive;
+  c.order = order;
   c.max_nodes = max_nodes;
   c.trace = trace;
   c.trace_cap = trace_cap;
@@ -283,6 +315,10 @@ static void run_one(int n, int m, const int64_t* loff, const int32_t* lits, int 
     }
   }
   *out = c.st;
+  if (witness) {
+    witness[0] = c.have_witness ? c.witness[0] : 0;
+    witness[1] = c.have_witness ? c.witness[1] : 0;
+  }
   free(trail);
   free(c.val);
   free(c.len);
@@ -290,11 +326,12 @@ static void run_one(int n, int m, const int64_t* loff, const int32_t* lits, int 
 }
 
 typedef struct {
-  int num, width, exhaustive, threads;
+  int num, width, exhaustive, order, threads;
   int64_t max_nodes;
   const int32_t *nvars, *lits;
   const int64_t *coff, *loff;
   search_stat* out;
+  uint64_t* witness;
   int next;
   pthread_mutex_t mu;
 } Pool;
@@ -307,8 +344,8 @@ static void* worker(void* arg) {
     pthread_mutex_unlock(&p->mu);
     if (i >= p->num) break;
     int m = (int)(p->coff[i + 1] - p->coff[i]);
-    run_one(p->nvars[i], m, p->loff + p->coff[i], p->lits, p->width, p->exhaustive, p->max_nodes, &p->out[i], NULL,
-            0, NULL, i);
+    run_one(p->nvars[i], m, p->loff + p->coff[i], p->lits, p->width, p->exhaustive, p->order, p->max_nodes,
+            &p->out[i], p->witness ? p->witness + 2 * (size_t)i : NULL, NULL, 0, NULL, i);
   }
   return NULL;
 }
@@ -318,9 +355,9 @@ static void* worker(void* arg) {
  * removing duplicates).  threads <= 1 runs on the calling thread; with `trace`, the
  * first trace_cap leaves (single-threaded runs only) are recorded. */
 int search_batch(int num, const int32_t* nvars, const int64_t* coff, const int64_t* loff, const int32_t* lits,
-                 int width, int exhaustive, int64_t max_nodes, int threads, search_stat* out, leaf_rec* trace,
-                 int trace_cap, int* trace_len) {
-  if (num < 0 || width < 0 || width > 6 || !out) return -2;
+                 int width, int exhaustive, int order, int64_t max_nodes, int threads, search_stat* out,
+                 uint64_t* witness, leaf_rec* trace, int trace_cap, int* trace_len) {
+  if (num < 0 || width < 0 || width > 6 || !out || order < 0 || order > 1) return -2;
   for (int i = 0; i < num; ++i) {
     if (nvars[i] < 0) return -2;
     for (int64_t c = coff[i]; c < coff[i + 1]; ++c) {
@@ -346,8 +383,8 @@ int search_batch(int num, const int32_t* nvars, const int64_t* coff, const int64
   if (threads <= 1 || trace) {
     for (int i = 0; i < num; ++i) {
       int m = (int)(coff[i + 1] - coff[i]);
-      run_one(nvars[i], m, loff + coff[i], lits, width, exhaustive, max_nodes, &out[i], trace, trace_cap, trace_len,
-              i);
+      run_one(nvars[i], m, loff + coff[i], lits, width, exhaustive, order, max_nodes, &out[i],
+              witness ? witness + 2 * (size_t)i : NULL, trace, trace_cap, trace_len, i);
     }
     return 0;
   }
@@ -356,6 +393,8 @@ int search_batch(int num, const int32_t* nvars, const int64_t* coff, const int64
   p.num = num;
   p.width = width;
   p.exhaustive = exhaustive;
+  p.order = order;
+  p.witness = witness;
   p.max_nodes = max_nodes;
   p.nvars = nvars;
   p.lits = lits;

@@ -35,8 +35,8 @@ def _load():
             subprocess.run(["make", "-C", str(_DIR)], check=True, capture_output=True)
         L = ctypes.CDLL(str(_LIB))
         vp = ctypes.c_void_p
-        L.search_batch.argtypes = [ctypes.c_int, vp, vp, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int64,
-                                   ctypes.c_int, vp, vp, ctypes.c_int, vp]
+        L.search_batch.argtypes = [ctypes.c_int, vp, vp, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_int64, ctypes.c_int, vp, vp, vp, ctypes.c_int, vp]
         L.search_batch.restype = ctypes.c_int
         L.search_leaf_mix.argtypes = [ctypes.c_uint64, ctypes.c_int]
         L.search_leaf_mix.restype = ctypes.c_uint64
@@ -57,22 +57,40 @@ def pack(cnfs):
 
 
 def search(cnfs, *, width: int = 6, exhaustive: bool = False, max_nodes: int = 1 << 40, threads: int = 1,
-           trace: int = 0, packed=None):
-    """Per-instance statistics (STAT_DTYPE) and, with ``trace``, the first leaves."""
+           trace: int = 0, packed=None, order: str = "score", witness: bool = False):
+    """Per-instance statistics (STAT_DTYPE); with ``trace``, also the first leaves; with
+    ``witness``, also each instance's first satisfiable leaf's assignment (a string of
+    "0"/"1" per variable, None if none) -- with ``order="lex"`` the lexicographically
+    smallest satisfying assignment."""
     nv, coff, loff, lits = packed if packed is not None else pack(cnfs)
     num = len(nv)
     out = np.zeros(num, dtype=STAT_DTYPE)
     tr = np.zeros(max(trace, 1), dtype=LEAF_DTYPE)
+    wit = np.zeros((max(num, 1), 2), dtype=np.uint64)
     tl = ctypes.c_int(0)
     lp = lits if lits.size else np.zeros(1, np.int32)
+    if order not in ("score", "lex"):
+        raise ValueError("order must be 'score' or 'lex'")
     rc = _load().search_batch(num, nv.ctypes.data, coff.ctypes.data, loff.ctypes.data, lp.ctypes.data, width,
-                              int(exhaustive), max_nodes, threads, out.ctypes.data,
-                              tr.ctypes.data if trace else None, trace, ctypes.byref(tl))
+                              int(exhaustive), 1 if order == "lex" else 0, max_nodes, threads, out.ctypes.data,
+                              wit.ctypes.data, tr.ctypes.data if trace else None, trace, ctypes.byref(tl))
     if rc == -2:
         raise ValueError("invalid instance (literal out of range, or width outside 0..6)")
     if rc == -3:
         raise ValueError("a clause has more than 3 distinct literals")
-    return (out, tr[: tl.value]) if trace else out
+    res = [out]
+    if trace:
+        res.append(tr[: tl.value])
+    if witness:
+        ws = []
+        for i in range(num):
+            if out["verdict"][i] != 1:
+                ws.append(None)
+                continue
+            bits = int(wit[i, 0]) | (int(wit[i, 1]) << 64)
+            ws.append("".join("1" if (bits >> j) & 1 else "0" for j in range(int(nv[i]))))
+        res.append(ws)
+    return res[0] if len(res) == 1 else tuple(res)
 
 
 def leaf_mix(bits: int, rank: int) -> int:

@@ -427,6 +427,33 @@ __global__ __launch_bounds__(64) void k_search(ts_search p) {
             ++leaves;
             digest += splitmix(splitmix(bits) + (uint64_t)rank);
             if (bits != 0) {
+              if (p.witness && lane == 0) {
+                // The satisfying entry that is smallest with the smallest variable of R
+                // most significant (lexicographic order of the assignment), then the
+                // assignment: its values, R's variables from that entry, 0 elsewhere.
+                uint32_t best = 0, best_key = 0xFFFFFFFFu;
+                for (uint64_t mbits = bits; mbits; mbits &= mbits - 1) {
+                  const uint32_t e = (uint32_t)__builtin_ctzll(mbits);
+                  const uint32_t key = __builtin_bitreverse32(e) >> (32 - (rank > 0 ? rank : 1));
+                  if (key < best_key) {
+                    best_key = key;
+                    best = e;
+                  }
+                }
+                uint64_t w0 = s.v0, w1 = s.v1, q0 = r0, q1 = r1;
+                for (int j = 0; j < rank; ++j) {
+                  const bool in_lo = q0 != 0;
+                  const uint32_t v = in_lo ? (uint32_t)__builtin_ctzll(q0) : 64u + (uint32_t)__builtin_ctzll(q1);
+                  if (in_lo) q0 &= q0 - 1;
+                  else q1 &= q1 - 1;
+                  if ((best >> j) & 1u) {
+                    if (v < 64) w0 |= 1ull << v;
+                    else w1 |= 1ull << (v - 64);
+                  }
+                }
+                p.witness[2 * (size_t)job.instance] = w0;
+                p.witness[2 * (size_t)job.instance + 1] = w1;
+              }
               if (lane == 0) atomicOr(&p.verdicts[job.instance], TS_SEARCH_SAT);
               if (!p.exhaustive) break;
             }
@@ -436,15 +463,22 @@ __global__ __launch_bounds__(64) void k_search(ts_search p) {
           prop = false;
           continue;
         }
-        // ---- branch: free variable of largest score, ties to the smaller index ----
-        const uint32_t k0 = h0 ? (((h0 >> 20) << 8) | (255u - (uint32_t)lane)) : 0u;
-        const uint32_t k1 = h1 ? (((h1 >> 20) << 8) | (255u - (uint32_t)(lane + 64))) : 0u;
-        const uint32_t key = wave_reduce(k0 > k1 ? k0 : k1, OpMax());
-        const uint32_t bv = 255u - (key & 255u);
-        const uint32_t hb =
-            bv < 64 ? __builtin_amdgcn_readlane(h0, (int)bv) : __builtin_amdgcn_readlane(h1, (int)bv - 64);
-        const uint32_t npos = (hb >> 10) & 1023u, nneg = hb & 1023u;
-        const uint32_t first = (bv << 1) | (npos >= nneg ? 0u : 1u);
+        // ---- branch: free variable of largest score (ties to the smaller index), its
+        // more frequent polarity first; or (LEX) the smallest variable of R, false first ----
+        uint32_t first;
+        if (p.order == TS_SEARCH_ORDER_LEX) {
+          const uint32_t bv = r0 ? (uint32_t)__builtin_ctzll(r0) : 64u + (uint32_t)__builtin_ctzll(r1);
+          first = (bv << 1) | 1u;
+        } else {
+          const uint32_t k0 = h0 ? (((h0 >> 20) << 8) | (255u - (uint32_t)lane)) : 0u;
+          const uint32_t k1 = h1 ? (((h1 >> 20) << 8) | (255u - (uint32_t)(lane + 64))) : 0u;
+          const uint32_t key = wave_reduce(k0 > k1 ? k0 : k1, OpMax());
+          const uint32_t bv = 255u - (key & 255u);
+          const uint32_t hb =
+              bv < 64 ? __builtin_amdgcn_readlane(h0, (int)bv) : __builtin_amdgcn_readlane(h1, (int)bv - 64);
+          const uint32_t npos = (hb >> 10) & 1023u, nneg = hb & 1023u;
+          first = (bv << 1) | (npos >= nneg ? 0u : 1u);
+        }
         stk[0][depth] = s.a0;
         stk[1][depth] = s.a1;
         stk[2][depth] = s.v0;
@@ -474,6 +508,7 @@ extern "C" int ts_search_run(const ts_search* p, ts_stream stream) {
   if (p->width < 0 || p->width > TS_SEARCH_MAX_WIDTH || p->max_nodes <= 0) return TS_EINVAL;
   if (p->spill && (!p->spill_count || p->spill_capacity < 0)) return TS_EINVAL;
   if (p->max_clauses < 0 || p->max_clauses > 64 * TS_SEARCH_SLOTS) return TS_EINVAL;
+  if (p->order != TS_SEARCH_ORDER_SCORE && p->order != TS_SEARCH_ORDER_LEX) return TS_EINVAL;
   if (p->num_jobs == 0) return TS_OK;
   if (!p->clauses || !p->instances || !p->jobs || !p->verdicts) return TS_EINVAL;
   // The variant whose clause slots per lane hold the largest instance.

@@ -54,7 +54,7 @@ class _Search(ctypes.Structure):
         ("stats", ctypes.c_void_p), ("width", ctypes.c_int32), ("exhaustive", ctypes.c_int32),
         ("early_exit", ctypes.c_int32), ("spill_capacity", ctypes.c_int32), ("max_nodes", ctypes.c_int64),
         ("spill", ctypes.c_void_p), ("spill_count", ctypes.c_void_p), ("max_clauses", ctypes.c_int32),
-        ("reserved2", ctypes.c_int32),
+        ("order", ctypes.c_int32), ("witness", ctypes.c_void_p),
     ]
 
 
@@ -220,11 +220,19 @@ class SearchSolver:
     def __init__(self, *, width: int = 6, split_target: float | None = 128.0, max_split: int = 8,
                  budget: int | None = None, exhaustive: bool = False, max_nodes: int = 1 << 26,
                  stats: bool = False, device: str = "cuda", spill_capacity: int | None = None,
-                 max_rounds: int = 100_000):
+                 max_rounds: int = 100_000, order: str = "score", witness: bool = False):
+        """``order``: "score" (the decision default) or "lex" (smallest residual variable,
+        false first); ``witness``: keep each satisfiable instance's assignment
+        (:meth:`witnesses`).  "lex" with one job per instance (``split_target`` None,
+        ``budget`` None) makes that assignment the lexicographically smallest."""
         if not 0 <= width <= MAX_WIDTH:
             raise ValueError(f"width must be in 0..{MAX_WIDTH}")
         if budget is not None and budget < 1:
             raise ValueError("budget must be >= 1 (or None)")
+        if order not in ("score", "lex"):
+            raise ValueError("order must be 'score' or 'lex'")
+        self.order = order
+        self.want_witness = witness
         self.width = width
         self.split_target = split_target
         self.max_split = max_split
@@ -264,6 +272,9 @@ class SearchSolver:
         self.d_inst = torch.from_numpy(inst.view(np.int32).reshape(-1) if len(inst) else np.zeros(4, np.int32)).to(dev)
         self.d_jobs = torch.from_numpy(jobs.view(np.int32).reshape(-1) if len(jobs) else np.zeros(16, np.int32)).to(dev)
         self.d_verdicts = torch.zeros(max(1, len(inst)), dtype=torch.int32, device=dev)
+        self.d_witness = torch.zeros(2 * max(1, len(inst)), dtype=torch.int64, device=dev) \
+            if self.want_witness else None
+        self.num_vars = inst["num_vars"].copy()
         if self.budget is not None:
             cap = self.spill_capacity or max(1 << 18, 8 * len(jobs))
             self.cap = int(cap)
@@ -279,7 +290,8 @@ class SearchSolver:
             int(self.exhaustive), int(not self.exhaustive), self.cap if spill is not None else 0,
             self.budget if self.budget is not None else self.max_nodes,
             N.dptr(spill) if spill is not None else None, N.dptr(self.d_count) if spill is not None else None,
-            self.max_clauses, 0,
+            self.max_clauses, 1 if self.order == "lex" else 0,
+            N.dptr(self.d_witness) if self.d_witness is not None else None,
         )
 
     def launch(self) -> None:
@@ -290,6 +302,8 @@ class SearchSolver:
 
         L = N.lib()
         self.d_verdicts.zero_()
+        if self.d_witness is not None:
+            self.d_witness.zero_()
         self._stats_rounds = []
         jobs_ptr, n = N.dptr(self.d_jobs), len(self.host_jobs)
         jobs_tensor = self.d_jobs
@@ -321,6 +335,23 @@ class SearchSolver:
         v = self.d_verdicts.cpu().numpy()[: self.num_instances].astype(np.int64)
         out = np.where(v & SAT, 1, 0)
         out[((v & INCOMPLETE) != 0) & ((v & SAT) == 0)] = -1
+        return out
+
+    def witnesses(self) -> list:
+        """Per instance: the assignment of the satisfiable leaf found ("0"/"1" per
+        variable, in index order), or None for an instance not found satisfiable."""
+        if self.d_witness is None:
+            raise ValueError("construct the solver with witness=True")
+        v = self.verdicts()
+        w = self.d_witness.cpu().numpy().view(np.uint64).reshape(-1, 2)
+        out = []
+        for i in range(self.num_instances):
+            if v[i] != 1:
+                out.append(None)
+                continue
+            n = int(self.num_vars[i])
+            bits = int(w[i, 0]) | (int(w[i, 1]) << 64)
+            out.append("".join("1" if (bits >> j) & 1 else "0" for j in range(n)))
         return out
 
     def job_stats(self) -> tuple[np.ndarray, np.ndarray]:

@@ -103,3 +103,36 @@ def test_search_rejects_bad_input():
         OS.search([CNFInstance(4, [(1, 2, 3, 4)])])  # more than 3 distinct literals
     with pytest.raises(ValueError):
         OS.search([CNFInstance(3, [(1, 2)])], width=7)
+
+
+def test_lex_witness_is_the_smallest_model():
+    """order="lex": the witness is the lexicographically smallest satisfying assignment
+    (variables in index order, 0 before 1), by enumeration; and it satisfies."""
+    rng = np.random.default_rng(13)
+    cnfs = [_random_mixed(rng, int(rng.integers(1, 15))) for _ in range(300)]
+    cnfs += [CNFInstance.random(3, n, int(round(r * n)), rng=70 + n) for n in range(8, 19) for r in (3.0, 4.0)]
+    for width in (0, 3, 6):
+        st, ws = OS.search(cnfs, width=width, order="lex", witness=True)
+        for c, v, w in zip(cnfs, st["verdict"], ws):
+            want = None
+            for a in range(1 << c.num_vars):  # a's bit n-1-j is variable j: lexicographic order
+                bits = [(a >> (c.num_vars - 1 - j)) & 1 for j in range(c.num_vars)]
+                if all(any((l > 0) == bool(bits[abs(l) - 1]) for l in cl) for cl in c.clauses):
+                    want = "".join(map(str, bits))
+                    break
+            assert (v == 1) == (want is not None)
+            assert w == want, (c.clauses, w, want)
+
+
+def test_lex_witness_reference_golden_traces():
+    """The reference's assignment searches (notebook 01 cell 113, 02 cell 36)."""
+    import json
+    from pathlib import Path
+
+    gold = json.loads((Path(__file__).parent / "golden" / "reference_goldens.json").read_text())
+    for g in gold["assignment_search"]:
+        cnf = CNFInstance.random(g["k"], g["n"], g["m"], rng=g["rng"])
+        if max(len(c) for c in cnf.clauses) > 3:
+            continue  # the device search packs at most 3 literals per clause
+        _, ws = OS.search([cnf], width=6, order="lex", witness=True)
+        assert ws[0] == g["assignment"]
