@@ -306,14 +306,19 @@ def bench_search(args) -> int:
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_start
     ms_per_step = wall * 1000.0 / args.steps
-    # Kernel time: events on the launch stream around each launch (a loop of its own).
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
-    for a, b in ev:
-        a.record()
-        solver.launch()
-        b.record()
+    # Kernel time: events on the launch stream around each k_search launch (a loop of its
+    # own): per step the sum over its rounds, per launch the mean over every launch.
+    per_step = []
+    for _ in range(args.steps):
+        ev = []
+        solver.launch(events=ev)
+        per_step.append(ev)
     torch.cuda.synchronize()
-    kernel_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+    launch_ms = [[a.elapsed_time(b) for a, b in ev] for ev in per_step]
+    kernel_ms = float(np.mean([sum(x) for x in launch_ms]))
+    launch_avg_us = 1000.0 * float(np.mean([t for x in launch_ms for t in x]))
+    round_us = [round(1000.0 * float(np.mean([x[r] for x in launch_ms if r < len(x)])), 1)
+                for r in range(max(len(x) for x in launch_ms))]
     verdicts = solver.verdicts()
     value = len(cnfs) / (ms_per_step / 1000.0)
     jobs = solver.host_jobs
@@ -330,9 +335,12 @@ def bench_search(args) -> int:
         f"{e2e_s * 1000:.1f} ms = {len(cnfs) / e2e_s / 1e3:.1f} k inst/s; {nodes_total} branch nodes, "
         f"busiest job {int(js['nodes'][busiest])} nodes")
     nclauses = int(solver.d_words.numel())
-    # algorithmic bytes: the clauses and instance records once, every job record of every
-    # round once, the verdicts once
-    alg_bytes = 4.0 * nclauses + 16.0 * len(cnfs) + 64.0 * sum(solver.round_jobs) + 4.0 * len(cnfs)
+    # algorithmic bytes per step: the packed clauses (4 B each) and instance records (16 B)
+    # once, every job record (64 B) read once in its round and, for handed-over jobs,
+    # written once in the round before, the verdicts (4 B) once.  Over the summed k_search
+    # durations of the step this is the mean bytes per launch over the mean launch time.
+    alg_bytes = (4.0 * nclauses + 16.0 * len(cnfs) + 64.0 * sum(solver.round_jobs)
+                 + 64.0 * sum(solver.round_jobs[1:]) + 4.0 * len(cnfs))
     achieved = alg_bytes / (kernel_ms / 1000.0) / 1e9
 
     cores = host_cores()
@@ -400,6 +408,8 @@ def bench_search(args) -> int:
             "autotune": tune,
             "kernel": "k_search",
             "kernel_ms": kernel_ms,
+            "k_search_launch_avg_us": launch_avg_us,
+            "k_search_round_us": round_us,
             "launch": "per step: verdict reset (fill kernel) + one k_search launch of one 64-lane wave per job, "
                       "repeated for each round of handed-over subtrees (the host reads each round's job count)",
             "sat": int((verdicts == 1).sum()),
@@ -421,7 +431,9 @@ def bench_search(args) -> int:
             "traffic": MEASURED_TRAFFIC.get(("search", width, len(cnfs)), (None, None))[0],
             "traffic_source": MEASURED_TRAFFIC.get(("search", width, len(cnfs)), (None, None))[1],
             "kernel": "k_search",
-            "bytes_per_launch": alg_bytes,
+            "bytes_per_step": alg_bytes,
+            "bytes_per_launch_avg": alg_bytes / max(1, solver.rounds),
+            "launch_avg_us": launch_avg_us,
             "binding_limit": "latency: the longest job's chain of dependent branch nodes (each a few "
                              "propagation rounds of ballots and scalar unit steps), not HBM",
             "critical_path": {"busiest_job_nodes": int(js["nodes"][busiest]),
@@ -1094,7 +1106,9 @@ def bench_dense(args) -> int:
         "dtype": "i8", "data": "synthetic",
         "config": {"workload": f"dense composition 2^{mb} x 2^{kb} by 2^{kb} x 2^{nb}, K innermost, "
                                f"density 2^-{kb / 2:g}, {args.mode}",
-                   "kernel_ms": kernel_ms, "valu_ms": valu_ms, "bytes": nbytes,
+                   "kernel_ms": kernel_ms,
+            "k_search_launch_avg_us": launch_avg_us,
+            "k_search_round_us": round_us, "valu_ms": valu_ms, "bytes": nbytes,
                    "hbm_gbs": nbytes / (kernel_ms / 1000.0) / 1e9,
                    "tile_autotune": None if tune is None else {
                        "chosen": "x".join(map(str, DT.TILES[tune["tile"]])),

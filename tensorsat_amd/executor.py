@@ -88,7 +88,7 @@ def run_job(dp: DeviceProgram, j: int, mode: str = "GT0", *, keep_intermediates:
     return result, outs
 
 
-def run_job_fused(dp: DeviceProgram, j: int, mode: str = "GT0", *, arena=None, small_bits: int = 14):
+def run_job_fused(dp: DeviceProgram, j: int, mode: str = "GT0", *, arena=None, small_bits: int = 12):
     """Run job ``j`` on the current stream with ``ts_program_run_fused`` (runs of steps of
     at most ``small_bits`` bits on one workgroup per run, larger steps grid-wide);
     returns a view of the result in the arena (``arena``: a uint8 tensor of at least the

@@ -91,7 +91,9 @@ class CNFInstance:
         """
         Parses DIMACS text (``lib/sat.py:80-100``): comment lines start with ``c``,
         the header is ``p cnf <vars> <clauses>``, then one clause per line with its
-        terminating ``0`` dropped.
+        terminating ``0`` dropped.  Literals are separated by single spaces, as the
+        reference reads them: other separators (tabs, runs of spaces inside a line)
+        raise ``ValueError``, and so does an instance without literals.
         """
         lines = [
             s for line in dimacs.strip().split("\n") if (s := line.strip()) and not s.startswith("c")
@@ -102,11 +104,13 @@ class CNFInstance:
         if not match:
             raise ValueError("DIMACS code must start with 'p cnf <num vars> <num clauses>'.")
         num_vars, num_clauses = map(int, match.groups())
-        clauses = tuple(tuple(map(int, line.split()))[:-1] for line in lines[1:])
+        clauses = tuple(tuple(map(int, line.split(" ")))[:-1] for line in lines[1:])
         if len(clauses) != num_clauses:
             raise ValueError("Number of clauses does not match the specified number.")
         lits = [abs(l) for c in clauses for l in c]
-        if lits and num_vars < max(lits):
+        if not lits:
+            raise ValueError("DIMACS code holds no literals.")
+        if num_vars < max(lits):
             raise ValueError("Clauses contain invalid variables.")
         return cls._new(num_vars, clauses)
 

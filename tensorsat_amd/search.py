@@ -294,10 +294,11 @@ class SearchSolver:
             N.dptr(self.d_witness) if self.d_witness is not None else None,
         )
 
-    def launch(self) -> None:
+    def launch(self, events: list | None = None) -> None:
         """Clear the verdicts (a fill kernel) and run the search on the current stream:
         one launch, or with ``budget`` one launch per round of handed-over jobs (the host
-        reads each round's job count)."""
+        reads each round's job count).  With ``events`` (a list), a pair of timing events
+        recorded on that stream around each search launch is appended to it."""
         import torch
 
         L = N.lib()
@@ -319,7 +320,13 @@ class SearchSolver:
                 spill = self.d_spill[rounds % 2]
                 self.d_count.zero_()
             args = self._args(jobs_ptr, n, stats, spill)
+            if events is not None:
+                pair = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+                pair[0].record()
             N.check(L.ts_search_run(ctypes.byref(args), N.stream_ptr()), "ts_search_run")
+            if events is not None:
+                pair[1].record()
+                events.append(pair)
             if stats is not None:
                 self._stats_rounds.append((jobs_tensor[: n * JOB_DTYPE.itemsize // 4].clone(), stats))
             self.round_jobs.append(n)

@@ -49,7 +49,7 @@ class SingleResult:
 
 
 def decide(cnf: CNFInstance, *, width: int = 26, mode: str = "GT0", plan_trials: int = 2,
-           program: PG.Program | None = None, streams: int = 4, small_bits: int = 14) -> SingleResult:
+           program: PG.Program | None = None, streams: int = 4, small_bits: int = 12) -> SingleResult:
     """See the module docstring.  ``stats["cubes_launched"]`` counts the cubes that ran
     (up to ``streams - 1`` past the deciding one); ``cubes_run`` those up to it."""
     import collections

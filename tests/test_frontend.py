@@ -61,6 +61,13 @@ def test_dimacs_roundtrip_and_errors():
         CNFInstance.from_dimacs("p cnf 1 2\n1 0")
     with pytest.raises(ValueError):
         CNFInstance.from_dimacs("p cnf 1 1\n2 0")
+    # literals separated by single spaces only, as the reference splits them
+    with pytest.raises(ValueError):
+        CNFInstance.from_dimacs("p cnf 2 1\n1  2 0")
+    with pytest.raises(ValueError):
+        CNFInstance.from_dimacs("p cnf 2 1\n1\t2 0")
+    with pytest.raises(ValueError):
+        CNFInstance.from_dimacs("p cnf 2 0")
     with pytest.raises(ValueError):
         CNFInstance.random(3, 2, 4)
     with pytest.raises(ValueError):
