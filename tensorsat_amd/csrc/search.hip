@@ -15,17 +15,18 @@
 //
 // Layout: lane l holds clauses l, 64 + l, ... (TS_SEARCH_SLOTS per lane) in registers as
 // packed words (three literal bytes (v << 1) | negated and a length byte); the partial
-// assignment is wave-uniform (two 128-bit masks: assigned, value) and lives in scalar
-// registers; the branch stack (assignment before the branch, branch literal, next child)
+// assignment is wave-uniform (two 128-bit masks: assigned, value, split into even and
+// odd variables) and lives in scalar registers; the branch stack (assignment before the branch, branch literal, next child)
 // and the branching histogram are in LDS.  A 256-byte table in LDS gives the status of
 // every literal byte under the current assignment (free, false, true), rebuilt from the
 // masks (one dword per lane) whenever they change, and a constant 256-entry table maps a
 // clause's length and its three literal statuses to (satisfied, free literals, position
 // of a free literal): a clause costs three byte extracts and four LDS reads.
 // Propagation runs in rounds: every lane evaluates its clauses, a ballot finds
-// conflicts, and the unit literals of the round (one per lane) are applied by the
-// scalar unit one lane at a time (readlane), which also detects a variable forced both
-// ways.  The fixpoint is the same as any sequential order's.
+// conflicts, and the unit literals of the round (one per lane) are written into the
+// literal table by all lanes at once, a read-back detecting a variable forced both
+// ways; the masks are rebuilt from the table at the fixpoint, which is the same as any
+// sequential order's.
 //
 // Every loop is bounded: propagation rounds by the number of variables (each round
 // assigns at least one), the branch stack by the number of variables, and each job by
@@ -76,9 +77,40 @@ struct OpAnd {
   __device__ uint32_t operator()(uint32_t a, uint32_t b) const { return a & b; }
 };
 
+// A partial assignment in the kernel's layout: variable v is bit v >> 1 of the even word
+// (v even) or of the odd word (v odd), so lane l's variables 2l and 2l + 1 are bit l of
+// each.  Job records and witnesses use the natural layout (variable v = bit v & 63 of
+// word v >> 6); the conversions below run on every lane (they ballot).
 struct Assign {
-  uint64_t a0, a1, v0, v1;
+  uint64_t ae, ao, ve, vo;  // assigned, value; even and odd variables
 };
+
+__device__ __forceinline__ uint32_t natural_bit(uint64_t n0, uint64_t n1, uint32_t v) {
+  return (uint32_t)(((v < 64 ? n0 : n1) >> (v & 63)) & 1ull);
+}
+__device__ __forceinline__ uint32_t split_bit(uint64_t e, uint64_t o, uint32_t v) {
+  return (uint32_t)((((v & 1u) ? o : e) >> (v >> 1)) & 1ull);
+}
+__device__ __forceinline__ void from_natural(uint64_t n0, uint64_t n1, uint64_t& e, uint64_t& o, int lane) {
+  e = __ballot(natural_bit(n0, n1, 2u * (uint32_t)lane));
+  o = __ballot(natural_bit(n0, n1, 2u * (uint32_t)lane + 1u));
+}
+__device__ __forceinline__ void to_natural(uint64_t e, uint64_t o, uint64_t& n0, uint64_t& n1, int lane) {
+  n0 = __ballot(split_bit(e, o, (uint32_t)lane));
+  n1 = __ballot(split_bit(e, o, (uint32_t)lane + 64u));
+}
+// Make literal byte b = (v << 1) | negated true.
+__device__ __forceinline__ void set_literal(Assign& s, uint32_t b) {
+  const uint32_t v = b >> 1;
+  const uint64_t bit = 1ull << (v >> 1), val = (b & 1u) ? 0ull : bit;
+  if (v & 1u) {
+    s.ao |= bit;
+    s.vo |= val;
+  } else {
+    s.ae |= bit;
+    s.ve |= val;
+  }
+}
 
 // The clause words never change during a job, so the compiler would decode all of them
 // once, outside the search loop, into hundreds of registers; marking them as rewritten
@@ -100,16 +132,25 @@ __device__ __forceinline__ uint32_t cs_pos(uint32_t r) { return (r >> 3) & 3u; }
 // Rebuild the literal table from the masks: lane l writes the statuses of variables 2l
 // and 2l + 1, both polarities (bytes 4l .. 4l + 3).
 __device__ __forceinline__ void build_lit_table(uint32_t* lt, const Assign& s, int lane) {
-  const int sh = (2 * lane) & 63;
-  const uint64_t am = lane < 32 ? s.a0 : s.a1;
-  const uint64_t vm = lane < 32 ? s.v0 : s.v1;
-  const uint32_t a = (uint32_t)(am >> sh) & 3u, v = (uint32_t)(vm >> sh) & 3u;
+  const uint32_t a = (uint32_t)((s.ae >> lane) & 1ull) | ((uint32_t)((s.ao >> lane) & 1ull) << 1);
+  const uint32_t v = (uint32_t)((s.ve >> lane) & 1ull) | ((uint32_t)((s.vo >> lane) & 1ull) << 1);
   // per variable: assigned -> (positive literal, negative literal) = value ? (T, F) : (F, T)
   const uint32_t p0 = (a & 1u) ? ((v & 1u) ? LIT_TRUE : LIT_FALSE) : LIT_FREE;
   const uint32_t n0 = (a & 1u) ? ((v & 1u) ? LIT_FALSE : LIT_TRUE) : LIT_FREE;
   const uint32_t p1 = (a & 2u) ? ((v & 2u) ? LIT_TRUE : LIT_FALSE) : LIT_FREE;
   const uint32_t n1 = (a & 2u) ? ((v & 2u) ? LIT_FALSE : LIT_TRUE) : LIT_FREE;
   lt[lane] = p0 | (n0 << 8) | (p1 << 16) | (n1 << 24);
+}
+
+// The masks back from the literal table: lane l reads the statuses of the positive
+// literals of variables 2l and 2l + 1.
+__device__ __forceinline__ void masks_from_table(Assign& s, const uint32_t* lt, int lane) {
+  const uint32_t w = lt[lane];
+  const uint32_t pe = w & 0xFFu, po = (w >> 16) & 0xFFu;
+  s.ae = __ballot(pe != LIT_FREE);
+  s.ve = __ballot(pe == LIT_TRUE);
+  s.ao = __ballot(po != LIT_FREE);
+  s.vo = __ballot(po == LIT_TRUE);
 }
 
 // The clause-status table: entry ((len & 7) << 6 | x2 << 4 | x1 << 2 | x0) for literal
@@ -153,12 +194,16 @@ __device__ __forceinline__ void clause_states(const uint32_t (&cl)[NS], const ui
     st[k] = ct[((cl[k] >> 18) & 0x1C0u) | (x[k][2] << 4) | (x[k][1] << 2) | x[k][0]];
 }
 
-// Unit propagation to the fixpoint.  Returns false on a conflict.  On success `st`
-// holds every slot's clause status under the final assignment and the literal table
-// matches it.  Each round every lane offers at most one unit literal (its first unit
-// clause's); the scalar unit applies them lane by lane, writing the two literal-table
-// bytes of each newly assigned variable (one wave's LDS accesses are in order), so each
-// round assigns at least one new variable.
+// Unit propagation to the fixpoint.  Returns false on a conflict.  On success `s` is the
+// final assignment, `st` holds every slot's clause status under it and the literal
+// table matches it.  Each round every lane offers at most one unit literal (its first
+// unit clause's, whose variable is free) and all lanes apply theirs at once: first each
+// writes TRUE at its literal's byte, then FALSE at the opposite literal's byte.  One
+// wave's LDS accesses complete in order, so a variable offered with both signs in the
+// round ends with equal bytes at both literals (the later write's value at each) and
+// every lane that offered it reads a mismatch back: a conflict.  A variable offered with
+// one sign ends TRUE/FALSE however many lanes offered it.  Each round with a unit
+// assigns at least one new variable.
 template <int NS>
 __device__ __forceinline__ bool propagate(uint32_t (&cl)[NS], Assign& s, uint32_t (&st)[NS], uint32_t* lt32,
                                           const uint32_t* ct32, int lane) {
@@ -179,32 +224,19 @@ __device__ __forceinline__ bool propagate(uint32_t (&cl)[NS], Assign& s, uint32_
       if (open && nfree == 1 && unit == NO_UNIT) unit = (cl[k] >> (8 * cs_pos(r))) & 0xFFu;
     }
     if (__ballot(conflict)) return false;
-    uint64_t m = __ballot(unit != NO_UNIT);
-    if (!m) return true;
-    while (m) {
-      const int ln = __builtin_ctzll(m);
-      m &= m - 1;
-      const uint32_t b = __builtin_amdgcn_readlane(unit, ln);
-      const uint32_t v = b >> 1;
-      const uint64_t bit = 1ull << (v & 63);
-      const bool hi = (v & 64) != 0;
-      const bool want = !(b & 1u);
-      const uint64_t aw = hi ? s.a1 : s.a0;
-      const uint64_t vw = hi ? s.v1 : s.v0;
-      if (aw & bit) {
-        if (((vw & bit) != 0) != want) return false;  // forced both ways
-        continue;
-      }
-      if (hi) {
-        s.a1 |= bit;
-        s.v1 |= want ? bit : 0ull;
-      } else {
-        s.a0 |= bit;
-        s.v0 |= want ? bit : 0ull;
-      }
-      lt[b] = (uint8_t)LIT_TRUE;
-      lt[b ^ 1u] = (uint8_t)LIT_FALSE;
+    const bool has = unit != NO_UNIT;
+    if (!__ballot(has)) {
+      masks_from_table(s, lt32, lane);
+      return true;
     }
+    // Compiler barriers keep the two write phases and the reads in this order (a read's
+    // address differs from its lane's other write, so it could otherwise be hoisted).
+    if (has) lt[unit] = (uint8_t)LIT_TRUE;
+    asm volatile("" ::: "memory");
+    if (has) lt[unit ^ 1u] = (uint8_t)LIT_FALSE;
+    asm volatile("" ::: "memory");
+    const bool clash = has && (lt[unit] != (uint8_t)LIT_TRUE || lt[unit ^ 1u] != (uint8_t)LIT_FALSE);
+    if (__ballot(clash)) return false;  // a variable forced both ways
   }
   return true;  // unreachable: every round with a unit assigns a new variable
 }
@@ -221,284 +253,317 @@ __device__ __forceinline__ uint64_t entries_with_bit(uint32_t p) {
   }
 }
 
-}  // namespace
+struct Shared {
+  uint32_t hist[MAXV];          // score << 20 | positive << 10 | negative
+  uint64_t stk[4][MAXV + 1];    // assignment before the branch at each depth
+  uint32_t stk_lit[MAXV + 1];   // branch literal byte | next child << 8
+  uint32_t lit_table[64];       // status byte of each literal byte
+  uint32_t clause_table[128];   // status byte of each (length, statuses) index
+};
 
+struct JobResult {
+  int64_t nodes, leaves, conflicts;
+  uint64_t digest;
+  int status, max_depth;
+};
+
+// One job's search with NS clause slots per lane (the instance has at most 64 * NS
+// clauses).  The clause-status table and the zeroed histogram are in `sh` on entry.
 template <int NS>
-__global__ __launch_bounds__(64) void k_search(ts_search p) {
-  __shared__ uint32_t hist[MAXV];          // score << 20 | positive << 10 | negative
-  __shared__ uint64_t stk[4][MAXV + 1];    // assignment before the branch at each depth
-  __shared__ uint32_t stk_lit[MAXV + 1];   // branch literal byte | next child << 8
-  __shared__ uint32_t lit_table[64];       // status byte of each literal byte
-  __shared__ uint32_t clause_table[128];   // status byte of each (length, statuses) index
+__device__ __forceinline__ void search_job(const ts_search& p, const ts_search_job& job, const ts_search_inst& inst,
+                                           Shared& sh, int lane, JobResult& r) {
+  const int m = inst.num_clauses;
+  int64_t owned = 0;
+  const int base = job.depth, split = job.split, own_from = job.own_from;
+  const uint32_t prefix = job.prefix;
+  const int width = p.width;
+  const uint8_t* lt8 = reinterpret_cast<const uint8_t*>(sh.lit_table);
+  int64_t budget = p.max_nodes;
 
-  const int lane = (int)threadIdx.x;
-  const ts_search_job job = p.jobs[blockIdx.x];
-  int64_t nodes = 0, leaves = 0, conflicts = 0, owned = 0;
-  uint64_t digest = 0;
-  int status = 0, max_depth = 0;
-  if (job.instance >= 0) {
-    const ts_search_inst inst = p.instances[job.instance];
-    const int m = inst.num_clauses;
-    if (m > 64 * NS) {
-      // More clauses than this variant holds (the host picks the variant from the
-      // largest instance): undecided, never a wrong verdict.
-      status = 2;
-    } else {
-      const int base = job.depth, split = job.split, own_from = job.own_from;
-      const uint32_t prefix = job.prefix;
-      const int width = p.width;
-      const uint8_t* lt8 = reinterpret_cast<const uint8_t*>(lit_table);
-      int64_t budget = p.max_nodes;
-
-      uint32_t cl[NS];
+  uint32_t cl[NS];
 #pragma unroll
-      for (int k = 0; k < NS; ++k) {
-        const int i = k * 64 + lane;
-        cl[k] = i < m ? p.clauses[inst.clause_begin + i] : (NO_CLAUSE << 24);
-      }
-      hist[lane] = 0;
-      hist[lane + 64] = 0;
-      build_clause_table(clause_table, lane);
-      __syncthreads();
+  for (int k = 0; k < NS; ++k) {
+    const int i = k * 64 + lane;
+    cl[k] = i < m ? p.clauses[inst.clause_begin + i] : (NO_CLAUSE << 24);
+  }
 
-      Assign s{job.assigned[0], job.assigned[1], job.value[0], job.value[1]};
-      uint32_t st[NS];
-      // The loop alternates two phases.  Propagate: `s` is the assignment of the node at
-      // depth `cd` (the start node, or the child of frame cd - 1) before propagation.
-      // Next: try the next child of the branch frame at depth `fd`.
-      int cd = base, fd = base - 1;
-      bool prop = true;
-      for (;;) {
-        if (!prop) {
-          const uint32_t fr = uni(stk_lit[fd]);
-          const uint32_t t = fr >> 8;
-          if (t >= 2) {
-            if (fd == base) break;
-            --fd;
-            continue;
-          }
-          stk_lit[fd] = (fr & 0xFFu) | ((t + 1) << 8);
-          __syncthreads();
-          if (fd - base < split && t != ((prefix >> (fd - base)) & 1u)) continue;
-          const uint32_t b = (fr & 0xFFu) ^ t;  // the second child negates the literal
-          const uint32_t v = b >> 1;
-          s.a0 = uni64(stk[0][fd]);
-          s.a1 = uni64(stk[1][fd]);
-          s.v0 = uni64(stk[2][fd]);
-          s.v1 = uni64(stk[3][fd]);
-          const uint64_t bit = 1ull << (v & 63);
-          const uint64_t val = (b & 1u) ? 0ull : bit;
-          if (v & 64) {
-            s.a1 |= bit;
-            s.v1 |= val;
-          } else {
-            s.a0 |= bit;
-            s.v0 |= val;
-          }
-          cd = fd + 1;
-          prop = true;
-          continue;
+  Assign s;
+  from_natural(job.assigned[0], job.assigned[1], s.ae, s.ao, lane);
+  from_natural(job.value[0], job.value[1], s.ve, s.vo, lane);
+  uint32_t st[NS];
+  // The loop alternates two phases.  Propagate: `s` is the assignment of the node at
+  // depth `cd` (the start node, or the child of frame cd - 1) before propagation.
+  // Next: try the next child of the branch frame at depth `fd`.
+  int cd = base, fd = base - 1;
+  bool prop = true;
+  for (;;) {
+    if (!prop) {
+      const uint32_t fr = uni(sh.stk_lit[fd]);
+      const uint32_t t = fr >> 8;
+      if (t >= 2) {
+        if (fd == base) break;
+        --fd;
+        continue;
+      }
+      sh.stk_lit[fd] = (fr & 0xFFu) | ((t + 1) << 8);
+      __syncthreads();
+      if (fd - base < split && t != ((prefix >> (fd - base)) & 1u)) continue;
+      s.ae = uni64(sh.stk[0][fd]);
+      s.ao = uni64(sh.stk[1][fd]);
+      s.ve = uni64(sh.stk[2][fd]);
+      s.vo = uni64(sh.stk[3][fd]);
+      set_literal(s, (fr & 0xFFu) ^ t);  // the second child negates the literal
+      cd = fd + 1;
+      prop = true;
+      continue;
+    }
+    // ---- propagate the node at depth cd ----
+    if (!propagate<NS>(cl, s, st, sh.lit_table, sh.clause_table, lane)) {
+      if (cd >= own_from) ++r.conflicts;
+      if (cd == base) break;
+      fd = cd - 1;
+      prop = false;
+      continue;
+    }
+    const int depth = cd;
+    if (p.early_exit &&
+        (__hip_atomic_load(&p.verdicts[job.instance], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &
+         TS_SEARCH_SAT)) {
+      r.status = 1;
+      break;
+    }
+    const bool own = depth >= own_from;
+    if (own && owned >= budget) {
+      if (!p.spill) {
+        r.status = 2;
+        break;
+      }
+      // Hand the rest over: this node's subtree, and the untried second child of
+      // every frame the job explores freely (below its prefix levels).
+      int n_out = 1;
+      for (int k = base + split; k < depth; ++k) n_out += (uni(sh.stk_lit[k]) >> 8) == 1u;
+      int start = 0;
+      if (lane == 0) start = atomicAdd(p.spill_count, n_out);
+      start = (int)uni((uint32_t)__builtin_amdgcn_readlane(start, 0));
+      if (start + n_out <= p.spill_capacity) {
+        // Records are built on every lane (the layout conversion ballots) and
+        // written by lane 0.
+        ts_search_job o = {};
+        o.instance = job.instance;
+        int w = start;
+        for (int k = base + split; k < depth; ++k) {
+          const uint32_t fr = uni(sh.stk_lit[k]);
+          if ((fr >> 8) != 1u) continue;
+          Assign c{uni64(sh.stk[0][k]), uni64(sh.stk[1][k]), uni64(sh.stk[2][k]), uni64(sh.stk[3][k])};
+          set_literal(c, (fr & 0xFFu) ^ 1u);
+          o.depth = k + 1;
+          o.own_from = k + 1;
+          to_natural(c.ae, c.ao, o.assigned[0], o.assigned[1], lane);
+          to_natural(c.ve, c.vo, o.value[0], o.value[1], lane);
+          if (lane == 0) p.spill[w] = o;
+          ++w;
         }
-        // ---- propagate the node at depth cd ----
-        if (!propagate<NS>(cl, s, st, lit_table, clause_table, lane)) {
-          if (cd >= own_from) ++conflicts;
-          if (cd == base) break;
-          fd = cd - 1;
-          prop = false;
-          continue;
-        }
-        const int depth = cd;
-        if (p.early_exit &&
-            (__hip_atomic_load(&p.verdicts[job.instance], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &
-             TS_SEARCH_SAT)) {
-          status = 1;
-          break;
-        }
-        const bool own = depth >= own_from;
-        if (own && owned >= budget) {
-          if (!p.spill) {
-            status = 2;
-            break;
-          }
-          // Hand the rest over: this node's subtree, and the untried second child of
-          // every frame the job explores freely (below its prefix levels).
-          int n_out = 1;
-          for (int k = base + split; k < depth; ++k) n_out += (uni(stk_lit[k]) >> 8) == 1u;
-          int start = 0;
-          if (lane == 0) start = atomicAdd(p.spill_count, n_out);
-          start = (int)uni((uint32_t)__builtin_amdgcn_readlane(start, 0));
-          if (start + n_out <= p.spill_capacity) {
-            if (lane == 0) {
-              ts_search_job o = {};
-              o.instance = job.instance;
-              int w = start;
-              for (int k = base + split; k < depth; ++k) {
-                const uint32_t fr = stk_lit[k];
-                if ((fr >> 8) != 1u) continue;
-                const uint32_t b = (fr & 0xFFu) ^ 1u, v = b >> 1;
-                const uint64_t bit = 1ull << (v & 63), val = (b & 1u) ? 0ull : bit;
-                o.depth = k + 1;
-                o.own_from = k + 1;
-                o.assigned[0] = stk[0][k] | ((v & 64) ? 0ull : bit);
-                o.assigned[1] = stk[1][k] | ((v & 64) ? bit : 0ull);
-                o.value[0] = stk[2][k] | ((v & 64) ? 0ull : val);
-                o.value[1] = stk[3][k] | ((v & 64) ? val : 0ull);
-                p.spill[w++] = o;
-              }
-              // The subtree keeps what is left of the job's prefix levels (the other
-              // children there belong to the jobs with the other prefixes).
-              const int rel = depth - base;
-              o.depth = depth;
-              o.own_from = depth;
-              o.split = rel < split ? split - rel : 0;
-              o.prefix = rel < split ? prefix >> rel : 0u;
-              o.assigned[0] = s.a0;
-              o.assigned[1] = s.a1;
-              o.value[0] = s.v0;
-              o.value[1] = s.v1;
-              p.spill[w] = o;
-            }
-            status = 3;
-            break;
-          }
-          // No room: empty the reserved slots that exist and finish the subtree here.
-          if (lane == 0) {
-            ts_search_job o = {};
-            o.instance = -1;
-            for (int w = start; w < p.spill_capacity && w < start + n_out; ++w) p.spill[w] = o;
-          }
-          budget = INT64_MAX;
-        }
-        if (own) {
-          ++owned;
-          ++nodes;
-          max_depth = depth > max_depth ? depth : max_depth;
-        }
-        // Branching histogram over the residual clauses' free literals; it also names R.
+        // The subtree keeps what is left of the job's prefix levels (the other
+        // children there belong to the jobs with the other prefixes).
+        const int rel = depth - base;
+        o.depth = depth;
+        o.own_from = depth;
+        o.split = rel < split ? split - rel : 0;
+        o.prefix = rel < split ? prefix >> rel : 0u;
+        to_natural(s.ae, s.ao, o.assigned[0], o.assigned[1], lane);
+        to_natural(s.ve, s.vo, o.value[0], o.value[1], lane);
+        if (lane == 0) p.spill[w] = o;
+        r.status = 3;
+        break;
+      }
+      // No room: empty the reserved slots that exist and finish the subtree here.
+      if (lane == 0) {
+        ts_search_job o = {};
+        o.instance = -1;
+        for (int w = start; w < p.spill_capacity && w < start + n_out; ++w) p.spill[w] = o;
+      }
+      budget = INT64_MAX;
+    }
+    if (own) {
+      ++owned;
+      ++r.nodes;
+      r.max_depth = depth > r.max_depth ? depth : r.max_depth;
+    }
+    // Branching histogram over the residual clauses' free literals; it also names R.
+    fresh(cl);
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      const uint32_t c = cl[k], len = c >> 24;
+      const bool open = !cs_sat(st[k]);
+      const uint32_t w = (cs_nfree(st[k]) - 1u) << 20;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        const uint32_t b = (c >> (8 * i)) & 0xFFu;
+        if (open && (uint32_t)i < len && lt8[b] == LIT_FREE)
+          atomicAdd(&sh.hist[b >> 1], w | ((b & 1u) ? 1u : (1u << 10)));
+      }
+    }
+    __syncthreads();
+    const uint32_t h0 = sh.hist[lane], h1 = sh.hist[lane + 64];
+    sh.hist[lane] = 0;
+    sh.hist[lane + 64] = 0;
+    const uint64_t r0 = __ballot(h0 != 0), r1 = __ballot(h1 != 0);
+    const int rank = __popcll(r0) + __popcll(r1);
+    if (rank <= width) {
+      // ---- leaf: the residual clause tensors multiplied into the rank-|R| relation
+      // (entry e: bit j of e = value of the j-th smallest variable of R), then reduced ----
+      uint64_t bits = 1;  // no residual clause: the empty product
+      if (rank > 0) {
+        uint64_t mine = ~0ull;  // entries satisfying this lane's residual clauses
         fresh(cl);
 #pragma unroll
         for (int k = 0; k < NS; ++k) {
           const uint32_t c = cl[k], len = c >> 24;
-          const bool open = !cs_sat(st[k]);
-          const uint32_t w = (cs_nfree(st[k]) - 1u) << 20;
+          uint64_t fals = cs_sat(st[k]) ? 0ull : ~0ull;  // a satisfied clause removes nothing
 #pragma unroll
           for (int i = 0; i < 3; ++i) {
-            const uint32_t b = (c >> (8 * i)) & 0xFFu;
-            if (open && (uint32_t)i < len && lt8[b] == LIT_FREE)
-              atomicAdd(&hist[b >> 1], w | ((b & 1u) ? 1u : (1u << 10)));
-          }
-        }
-        __syncthreads();
-        const uint32_t h0 = hist[lane], h1 = hist[lane + 64];
-        hist[lane] = 0;
-        hist[lane + 64] = 0;
-        const uint64_t r0 = __ballot(h0 != 0), r1 = __ballot(h1 != 0);
-        const int rank = __popcll(r0) + __popcll(r1);
-        if (rank <= width) {
-          // ---- leaf: the residual clause tensors multiplied into the rank-|R| relation
-          // (entry e: bit j of e = value of the j-th smallest variable of R), then reduced ----
-          uint64_t bits = 1;  // no residual clause: the empty product
-          if (rank > 0) {
-            uint64_t mine = ~0ull;  // entries satisfying this lane's residual clauses
-            fresh(cl);
-#pragma unroll
-            for (int k = 0; k < NS; ++k) {
-              const uint32_t c = cl[k], len = c >> 24;
-              uint64_t fals = cs_sat(st[k]) ? 0ull : ~0ull;  // a satisfied clause removes nothing
-#pragma unroll
-              for (int i = 0; i < 3; ++i) {
-                const uint32_t b = (c >> (8 * i)) & 0xFFu, v = b >> 1;
-                // assigned literals of a residual clause are false
-                if ((uint32_t)i < len && lt8[b] == LIT_FREE) {
-                  const uint32_t pos = (v & 64) ? (uint32_t)(__popcll(r0) + __popcll(r1 & ((1ull << (v & 63)) - 1ull)))
-                                                : (uint32_t)__popcll(r0 & ((1ull << v) - 1ull));
-                  const uint64_t ones = entries_with_bit(pos);
-                  fals &= (b & 1u) ? ones : ~ones;  // false where the variable equals the negation bit
-                }
-              }
-              mine &= ~fals;
-            }
-            const uint32_t lo = wave_reduce((uint32_t)mine, OpAnd());
-            const uint32_t hi = wave_reduce((uint32_t)(mine >> 32), OpAnd());
-            const uint64_t valid = rank == 6 ? ~0ull : ((1ull << (1u << rank)) - 1ull);
-            bits = (((uint64_t)hi << 32) | lo) & valid;
-          }
-          if (own) {
-            ++leaves;
-            digest += splitmix(splitmix(bits) + (uint64_t)rank);
-            if (bits != 0) {
-              if (p.witness && lane == 0) {
-                // The satisfying entry that is smallest with the smallest variable of R
-                // most significant (lexicographic order of the assignment), then the
-                // assignment: its values, R's variables from that entry, 0 elsewhere.
-                uint32_t best = 0, best_key = 0xFFFFFFFFu;
-                for (uint64_t mbits = bits; mbits; mbits &= mbits - 1) {
-                  const uint32_t e = (uint32_t)__builtin_ctzll(mbits);
-                  const uint32_t key = __builtin_bitreverse32(e) >> (32 - (rank > 0 ? rank : 1));
-                  if (key < best_key) {
-                    best_key = key;
-                    best = e;
-                  }
-                }
-                uint64_t w0 = s.v0, w1 = s.v1, q0 = r0, q1 = r1;
-                for (int j = 0; j < rank; ++j) {
-                  const bool in_lo = q0 != 0;
-                  const uint32_t v = in_lo ? (uint32_t)__builtin_ctzll(q0) : 64u + (uint32_t)__builtin_ctzll(q1);
-                  if (in_lo) q0 &= q0 - 1;
-                  else q1 &= q1 - 1;
-                  if ((best >> j) & 1u) {
-                    if (v < 64) w0 |= 1ull << v;
-                    else w1 |= 1ull << (v - 64);
-                  }
-                }
-                p.witness[2 * (size_t)job.instance] = w0;
-                p.witness[2 * (size_t)job.instance + 1] = w1;
-              }
-              if (lane == 0) atomicOr(&p.verdicts[job.instance], TS_SEARCH_SAT);
-              if (!p.exhaustive) break;
+            const uint32_t b = (c >> (8 * i)) & 0xFFu, v = b >> 1;
+            // assigned literals of a residual clause are false
+            if ((uint32_t)i < len && lt8[b] == LIT_FREE) {
+              const uint32_t pos = (v & 64) ? (uint32_t)(__popcll(r0) + __popcll(r1 & ((1ull << (v & 63)) - 1ull)))
+                                            : (uint32_t)__popcll(r0 & ((1ull << v) - 1ull));
+              const uint64_t ones = entries_with_bit(pos);
+              fals &= (b & 1u) ? ones : ~ones;  // false where the variable equals the negation bit
             }
           }
-          if (depth == base) break;
-          fd = depth - 1;
-          prop = false;
-          continue;
+          mine &= ~fals;
         }
-        // ---- branch: free variable of largest score (ties to the smaller index), its
-        // more frequent polarity first; or (LEX) the smallest variable of R, false first ----
-        uint32_t first;
-        if (p.order == TS_SEARCH_ORDER_LEX) {
-          const uint32_t bv = r0 ? (uint32_t)__builtin_ctzll(r0) : 64u + (uint32_t)__builtin_ctzll(r1);
-          first = (bv << 1) | 1u;
-        } else {
-          const uint32_t k0 = h0 ? (((h0 >> 20) << 8) | (255u - (uint32_t)lane)) : 0u;
-          const uint32_t k1 = h1 ? (((h1 >> 20) << 8) | (255u - (uint32_t)(lane + 64))) : 0u;
-          const uint32_t key = wave_reduce(k0 > k1 ? k0 : k1, OpMax());
-          const uint32_t bv = 255u - (key & 255u);
-          const uint32_t hb =
-              bv < 64 ? __builtin_amdgcn_readlane(h0, (int)bv) : __builtin_amdgcn_readlane(h1, (int)bv - 64);
-          const uint32_t npos = (hb >> 10) & 1023u, nneg = hb & 1023u;
-          first = (bv << 1) | (npos >= nneg ? 0u : 1u);
-        }
-        stk[0][depth] = s.a0;
-        stk[1][depth] = s.a1;
-        stk[2][depth] = s.v0;
-        stk[3][depth] = s.v1;
-        stk_lit[depth] = first;  // next child 0
-        __syncthreads();
-        fd = depth;
-        prop = false;
+        const uint32_t lo = wave_reduce((uint32_t)mine, OpAnd());
+        const uint32_t hi = wave_reduce((uint32_t)(mine >> 32), OpAnd());
+        const uint64_t valid = rank == 6 ? ~0ull : ((1ull << (1u << rank)) - 1ull);
+        bits = (((uint64_t)hi << 32) | lo) & valid;
       }
+      if (own) {
+        ++r.leaves;
+        r.digest += splitmix(splitmix(bits) + (uint64_t)rank);
+        if (bits != 0) {
+          uint64_t w0 = 0, w1 = 0;  // the leaf's assignment's values, natural layout
+          if (p.witness) to_natural(s.ve, s.vo, w0, w1, lane);
+          if (p.witness && lane == 0) {
+            // The satisfying entry that is smallest with the smallest variable of R
+            // most significant (lexicographic order of the assignment), then the
+            // assignment: its values, R's variables from that entry, 0 elsewhere.
+            uint32_t best = 0, best_key = 0xFFFFFFFFu;
+            for (uint64_t mbits = bits; mbits; mbits &= mbits - 1) {
+              const uint32_t e = (uint32_t)__builtin_ctzll(mbits);
+              const uint32_t key = __builtin_bitreverse32(e) >> (32 - (rank > 0 ? rank : 1));
+              if (key < best_key) {
+                best_key = key;
+                best = e;
+              }
+            }
+            uint64_t q0 = r0, q1 = r1;
+            for (int j = 0; j < rank; ++j) {
+              const bool in_lo = q0 != 0;
+              const uint32_t v = in_lo ? (uint32_t)__builtin_ctzll(q0) : 64u + (uint32_t)__builtin_ctzll(q1);
+              if (in_lo) q0 &= q0 - 1;
+              else q1 &= q1 - 1;
+              if ((best >> j) & 1u) {
+                if (v < 64) w0 |= 1ull << v;
+                else w1 |= 1ull << (v - 64);
+              }
+            }
+            p.witness[2 * (size_t)job.instance] = w0;
+            p.witness[2 * (size_t)job.instance + 1] = w1;
+          }
+          if (lane == 0) atomicOr(&p.verdicts[job.instance], TS_SEARCH_SAT);
+          if (!p.exhaustive) break;
+        }
+      }
+      if (depth == base) break;
+      fd = depth - 1;
+      prop = false;
+      continue;
     }
-    if (lane == 0 && status == 2) atomicOr(&p.verdicts[job.instance], TS_SEARCH_INCOMPLETE);
+    // ---- branch: free variable of largest score (ties to the smaller index), its
+    // more frequent polarity first; or (LEX) the smallest variable of R, false first ----
+    uint32_t first;
+    if (p.order == TS_SEARCH_ORDER_LEX) {
+      const uint32_t bv = r0 ? (uint32_t)__builtin_ctzll(r0) : 64u + (uint32_t)__builtin_ctzll(r1);
+      first = (bv << 1) | 1u;
+    } else {
+      const uint32_t k0 = h0 ? (((h0 >> 20) << 8) | (255u - (uint32_t)lane)) : 0u;
+      const uint32_t k1 = h1 ? (((h1 >> 20) << 8) | (255u - (uint32_t)(lane + 64))) : 0u;
+      const uint32_t key = wave_reduce(k0 > k1 ? k0 : k1, OpMax());
+      const uint32_t bv = 255u - (key & 255u);
+      const uint32_t hb =
+          bv < 64 ? __builtin_amdgcn_readlane(h0, (int)bv) : __builtin_amdgcn_readlane(h1, (int)bv - 64);
+      const uint32_t npos = (hb >> 10) & 1023u, nneg = hb & 1023u;
+      first = (bv << 1) | (npos >= nneg ? 0u : 1u);
+    }
+    sh.stk[0][depth] = s.ae;
+    sh.stk[1][depth] = s.ao;
+    sh.stk[2][depth] = s.ve;
+    sh.stk[3][depth] = s.vo;
+    sh.stk_lit[depth] = first;  // next child 0
+    __syncthreads();
+    fd = depth;
+    prop = false;
+  }
+}
+
+// The job runs the first of the variants NS, Rest... (increasing clause slots per lane)
+// that holds its clauses.
+template <int NS, int... Rest>
+__device__ __forceinline__ void search_variant(int need, const ts_search& p, const ts_search_job& job,
+                                               const ts_search_inst& inst, Shared& sh, int lane, JobResult& r) {
+  if constexpr (sizeof...(Rest) == 0) {
+    search_job<NS>(p, job, inst, sh, lane, r);
+  } else {
+    if (need <= NS)
+      search_job<NS>(p, job, inst, sh, lane, r);
+    else
+      search_variant<Rest...>(need, p, job, inst, sh, lane, r);
+  }
+}
+
+template <int... V>
+constexpr int last_of() {
+  int x = 0;
+  ((x = V), ...);
+  return x;
+}
+
+}  // namespace
+
+// Variants V... (increasing clause slots per lane); the host picks the kernel whose
+// largest variant holds the batch's largest instance.
+template <int... V>
+__global__ __launch_bounds__(64) void k_search(ts_search p) {
+  __shared__ Shared sh;
+  constexpr int NSMAX = last_of<V...>();
+  const int lane = (int)threadIdx.x;
+  const ts_search_job job = p.jobs[blockIdx.x];
+  JobResult r{};
+  if (job.instance >= 0) {
+    const ts_search_inst inst = p.instances[job.instance];
+    const int need = (int)uni((uint32_t)((inst.num_clauses + 63) / 64));
+    if (inst.num_clauses > 64 * NSMAX) {
+      // More clauses than this kernel holds (the host picks the kernel from the largest
+      // instance): undecided, never a wrong verdict.
+      r.status = 2;
+    } else {
+      sh.hist[lane] = 0;
+      sh.hist[lane + 64] = 0;
+      build_clause_table(sh.clause_table, lane);
+      __syncthreads();
+      search_variant<V...>(need, p, job, inst, sh, lane, r);
+    }
+    if (lane == 0 && r.status == 2) atomicOr(&p.verdicts[job.instance], TS_SEARCH_INCOMPLETE);
   }
   if (lane == 0 && p.stats) {
     ts_search_stat o;
-    o.status = status;
-    o.max_depth = max_depth;
-    o.nodes = nodes;
-    o.leaves = leaves;
-    o.conflicts = conflicts;
-    o.digest = digest;
+    o.status = r.status;
+    o.max_depth = r.max_depth;
+    o.nodes = r.nodes;
+    o.leaves = r.leaves;
+    o.conflicts = r.conflicts;
+    o.digest = r.digest;
     p.stats[blockIdx.x] = o;
   }
 }
@@ -511,20 +576,21 @@ extern "C" int ts_search_run(const ts_search* p, ts_stream stream) {
   if (p->order != TS_SEARCH_ORDER_SCORE && p->order != TS_SEARCH_ORDER_LEX) return TS_EINVAL;
   if (p->num_jobs == 0) return TS_OK;
   if (!p->clauses || !p->instances || !p->jobs || !p->verdicts) return TS_EINVAL;
-  // The variant whose clause slots per lane hold the largest instance.
+  // The kernel whose largest variant holds the largest instance; each job runs the
+  // smallest of its variants that holds its own.
   const int need = p->max_clauses ? (p->max_clauses + 63) / 64 : TS_SEARCH_SLOTS;
   const dim3 grid((unsigned)p->num_jobs), block(64);
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   if (need <= 2)
-    hipLaunchKernelGGL(k_search<2>, grid, block, 0, st, *p);
+    hipLaunchKernelGGL((k_search<2>), grid, block, 0, st, *p);
   else if (need <= 4)
-    hipLaunchKernelGGL(k_search<4>, grid, block, 0, st, *p);
+    hipLaunchKernelGGL((k_search<2, 3, 4>), grid, block, 0, st, *p);
   else if (need <= 6)
-    hipLaunchKernelGGL(k_search<6>, grid, block, 0, st, *p);
+    hipLaunchKernelGGL((k_search<2, 3, 4, 5, 6>), grid, block, 0, st, *p);
   else if (need <= 8)
-    hipLaunchKernelGGL(k_search<8>, grid, block, 0, st, *p);
+    hipLaunchKernelGGL((k_search<4, 6, 8>), grid, block, 0, st, *p);
   else
-    hipLaunchKernelGGL(k_search<TS_SEARCH_SLOTS>, grid, block, 0, st, *p);
+    hipLaunchKernelGGL((k_search<4, 8, TS_SEARCH_SLOTS>), grid, block, 0, st, *p);
   return hipGetLastError() == hipSuccess ? TS_OK : TS_EHIP;
 }
 
