@@ -295,8 +295,10 @@ typedef struct {
                                       smallest), written by each job that finds one */
 } ts_search;
 
-/* Branching orders.  SCORE: the free variable of largest weighted occurrence in the
- * residual clauses, its more frequent polarity first (the decision default).  LEX: the
+/* Branching orders.  SCORE: the free variable of largest (P + 1) * (N + 1), P and N its
+ * occurrences in the residual clauses as a positive and as a negative literal, each
+ * weighted 5 in a clause with two free literals and 1 in one with three (ties to the
+ * smaller index), its more frequent polarity first (the decision default).  LEX: the
  * smallest variable of the residual, false first: with one job per instance, no hand-over
  * and no exhaustive mode, the first satisfiable leaf gives the lexicographically smallest
  * satisfying assignment (variables in index order, 0 before 1) -- the assignment the

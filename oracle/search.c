@@ -21,9 +21,10 @@
  *   - node (after propagation without conflict): nodes += 1; residual = clauses with no
  *     true literal; R = their free variables; no residual clause -> leaf with result 1
  *     (the empty product); |R| <= width -> leaf, result = OR over the rank-|R| relation;
- *     else branch on the free variable of largest score = sum over residual clauses
- *     holding it of (free literals - 1), ties to the smaller index, its more frequent
- *     polarity among residual clauses first (positive on a tie);
+ *     else branch on the free variable of largest score = (P + 1) * (N + 1), P and N
+ *     the sums over residual clauses holding it as a positive / negative literal of a
+ *     weight 5 (two free literals) or 1 (three), ties to the smaller index, its more
+ *     frequent polarity among residual clauses (by plain count) first, positive on a tie;
  *   - a branch assigns the literal and propagates to the fixpoint (conflict: conflicts
  *     += 1, the branch is skipped);
  *   - leaves += 1 at every leaf; digest += mix(relation bits, rank) (mod 2^64), the
@@ -135,7 +136,8 @@ static int dfs(Ctx* c, int* trail, int* nt, int depth) {
   }
   c->st.nodes++;
   if (depth > c->st.max_depth) c->st.max_depth = depth;
-  int* score = (int*)calloc((size_t)c->n, sizeof(int));
+  int* pw = (int*)calloc((size_t)c->n, sizeof(int));
+  int* nw = (int*)calloc((size_t)c->n, sizeof(int));
   int* pos = (int*)calloc((size_t)c->n, sizeof(int));
   int* neg = (int*)calloc((size_t)c->n, sizeof(int));
   char* inr = (char*)calloc((size_t)c->n, 1);
@@ -152,10 +154,15 @@ static int dfs(Ctx* c, int* trail, int* nt, int depth) {
     for (int k = 0; k < c->len[i]; ++k) {
       int v = abs(q[k]) - 1;
       if (c->val[v] != 0) continue;
-      score[v] += nfree - 1;
+      const int w = nfree == 2 ? 5 : 1;
       inr[v] = 1;
-      if (q[k] > 0) pos[v]++;
-      else neg[v]++;
+      if (q[k] > 0) {
+        pos[v]++;
+        pw[v] += w;
+      } else {
+        neg[v]++;
+        nw[v] += w;
+      }
     }
   }
   int r = 0;
@@ -230,8 +237,14 @@ static int dfs(Ctx* c, int* trail, int* nt, int depth) {
         if (inr[v]) bv = v;
       first = -(bv + 1);
     } else {
-      for (int v = 0; v < c->n; ++v)
-        if (inr[v] && (bv < 0 || score[v] > score[bv])) bv = v;
+      int64_t best = -1;
+      for (int v = 0; v < c->n; ++v) {
+        const int64_t sc = (int64_t)(pw[v] + 1) * (nw[v] + 1);
+        if (inr[v] && sc > best) {
+          best = sc;
+          bv = v;
+        }
+      }
       first = pos[bv] >= neg[bv] ? bv + 1 : -(bv + 1);
     }
     for (int t = 0; t < 2 && !c->stop; ++t) {
@@ -250,7 +263,8 @@ static int dfs(Ctx* c, int* trail, int* nt, int depth) {
   free(inr);
   free(neg);
   free(pos);
-  free(score);
+  free(nw);
+  free(pw);
   return result;
 }
 

@@ -241,6 +241,13 @@ __device__ __forceinline__ bool propagate(uint32_t (&cl)[NS], Assign& s, uint32_
   return true;  // unreachable: every round with a unit assigns a new variable
 }
 
+// Weight of a residual clause with two free literals in the branching histogram (one
+// with three free literals weighs 1), and the score of a histogram entry.
+constexpr uint32_t HIST_W2 = 5;
+__device__ __forceinline__ uint32_t score_of(uint64_t h) {
+  return ((uint32_t)(h & 0xFFFFu) + 1u) * ((uint32_t)((h >> 16) & 0xFFFFu) + 1u);
+}
+
 // Entries e of a rank <= 6 relation whose bit p is 1, as a 64-bit mask.
 __device__ __forceinline__ uint64_t entries_with_bit(uint32_t p) {
   switch (p) {
@@ -254,7 +261,7 @@ __device__ __forceinline__ uint64_t entries_with_bit(uint32_t p) {
 }
 
 struct Shared {
-  uint32_t hist[MAXV];          // score << 20 | positive << 10 | negative
+  unsigned long long hist[MAXV];  // per variable: weighted positive, weighted negative, counts
   uint64_t stk[4][MAXV + 1];    // assignment before the branch at each depth
   uint32_t stk_lit[MAXV + 1];   // branch literal byte | next child << 8
   uint32_t lit_table[64];       // status byte of each literal byte
@@ -390,21 +397,24 @@ __device__ __forceinline__ void search_job(const ts_search& p, const ts_search_j
       r.max_depth = depth > r.max_depth ? depth : r.max_depth;
     }
     // Branching histogram over the residual clauses' free literals; it also names R.
+    // Per variable, 16-bit fields: the weighted occurrences as a positive literal, as a
+    // negative literal (weight HIST_W2 in a clause with two free literals, 1 with three),
+    // and the two plain counts.  With at most 768 clauses no field overflows.
     fresh(cl);
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
       const uint32_t c = cl[k], len = c >> 24;
       const bool open = !cs_sat(st[k]);
-      const uint32_t w = (cs_nfree(st[k]) - 1u) << 20;
+      const unsigned long long w = cs_nfree(st[k]) == 2u ? HIST_W2 : 1u;
+      const unsigned long long inc_pos = w | (1ull << 32), inc_neg = (w << 16) | (1ull << 48);
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
         const uint32_t b = (c >> (8 * i)) & 0xFFu;
-        if (open && (uint32_t)i < len && lt8[b] == LIT_FREE)
-          atomicAdd(&sh.hist[b >> 1], w | ((b & 1u) ? 1u : (1u << 10)));
+        if (open && (uint32_t)i < len && lt8[b] == LIT_FREE) atomicAdd(&sh.hist[b >> 1], (b & 1u) ? inc_neg : inc_pos);
       }
     }
     __syncthreads();
-    const uint32_t h0 = sh.hist[lane], h1 = sh.hist[lane + 64];
+    const uint64_t h0 = sh.hist[lane], h1 = sh.hist[lane + 64];
     sh.hist[lane] = 0;
     sh.hist[lane + 64] = 0;
     const uint64_t r0 = __ballot(h0 != 0), r1 = __ballot(h1 != 0);
@@ -480,21 +490,23 @@ __device__ __forceinline__ void search_job(const ts_search& p, const ts_search_j
       prop = false;
       continue;
     }
-    // ---- branch: free variable of largest score (ties to the smaller index), its
-    // more frequent polarity first; or (LEX) the smallest variable of R, false first ----
+    // ---- branch: free variable of largest score (weighted positive + 1) x (weighted
+    // negative + 1) (ties to the smaller index), its more frequent polarity first; or
+    // (LEX) the smallest variable of R, false first ----
     uint32_t first;
     if (p.order == TS_SEARCH_ORDER_LEX) {
       const uint32_t bv = r0 ? (uint32_t)__builtin_ctzll(r0) : 64u + (uint32_t)__builtin_ctzll(r1);
       first = (bv << 1) | 1u;
     } else {
-      const uint32_t k0 = h0 ? (((h0 >> 20) << 8) | (255u - (uint32_t)lane)) : 0u;
-      const uint32_t k1 = h1 ? (((h1 >> 20) << 8) | (255u - (uint32_t)(lane + 64))) : 0u;
+      // scores < 3841^2 < 2^24, so (score << 8 | 255 - variable) orders scores, then indices
+      const uint32_t k0 = h0 ? ((score_of(h0) << 8) | (255u - (uint32_t)lane)) : 0u;
+      const uint32_t k1 = h1 ? ((score_of(h1) << 8) | (255u - (uint32_t)(lane + 64))) : 0u;
       const uint32_t key = wave_reduce(k0 > k1 ? k0 : k1, OpMax());
       const uint32_t bv = 255u - (key & 255u);
-      const uint32_t hb =
-          bv < 64 ? __builtin_amdgcn_readlane(h0, (int)bv) : __builtin_amdgcn_readlane(h1, (int)bv - 64);
-      const uint32_t npos = (hb >> 10) & 1023u, nneg = hb & 1023u;
-      first = (bv << 1) | (npos >= nneg ? 0u : 1u);
+      const uint64_t neg0 = __ballot(((h0 >> 32) & 0xFFFFu) < (h0 >> 48));
+      const uint64_t neg1 = __ballot(((h1 >> 32) & 0xFFFFu) < (h1 >> 48));
+      const uint32_t negfirst = (uint32_t)(((bv < 64 ? neg0 : neg1) >> (bv & 63)) & 1ull);
+      first = (bv << 1) | negfirst;
     }
     sh.stk[0][depth] = s.ae;
     sh.stk[1][depth] = s.ao;

@@ -94,16 +94,16 @@ def pack_batch(cnfs) -> tuple[np.ndarray, np.ndarray]:
 
 
 def log2_search_size(num_vars: np.ndarray, num_clauses: np.ndarray) -> np.ndarray:
-    """Estimated log2 of the branch nodes one search needs with early exit, for random
-    3-SAT-like instances: linear in n with a slope that grows with the clause ratio up
-    to the satisfiability threshold (4.26) and falls past it (measured on the batched
-    workload: slope 0.079 at ratio 4.26, 0.030 at 3.5).  Used only to order and split
-    jobs."""
+    """Estimated log2 of the branch nodes and conflicts one search needs with early exit,
+    for random 3-SAT-like instances: linear in n with a slope that grows with the clause
+    ratio up to the satisfiability threshold (4.26) and falls past it (least squares on
+    the batched workload: slope 0.054 at ratio 4.26, 0.024 at 3.5).  Used only to order
+    and split jobs."""
     n = np.asarray(num_vars, dtype=np.float64)
     r = np.asarray(num_clauses, dtype=np.float64) / np.maximum(n, 1.0)
-    lo, hi = 0.030, 0.079
+    lo, hi = 0.024, 0.054
     slope = np.where(r <= 4.26, lo + (hi - lo) * np.clip((r - 3.5) / 0.76, 0.0, 1.0), hi * 4.26 / np.maximum(r, 1e-9))
-    return 1.7 + slope * n
+    return 2.5 + slope * n
 
 
 def plan_jobs(inst: np.ndarray, *, split_target: float | None = 128.0, max_split: int = 8) -> np.ndarray:
