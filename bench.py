@@ -287,7 +287,7 @@ def bench_search(args) -> int:
     solver = SE.SearchSolver(width=width, split_target=split, budget=budget)
     # End to end from the CNF list: pack, plan the jobs, upload, launch, verdicts back.
     e2e = []
-    for _ in range(3):
+    for _ in range(5):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         solver.prepare(cnfs)
@@ -416,7 +416,7 @@ def bench_search(args) -> int:
             "end_to_end_s": e2e_s,
             "end_to_end_instances_per_s": len(cnfs) / e2e_s,
             "end_to_end": "CNF list -> packed clauses -> job list -> upload -> launch -> verdicts on the host "
-                          "(median of 3)",
+                          "(median of 5)",
             "host_s": host,
             "branch_nodes": nodes_total,
             "leaves": int(js["leaves"].sum()),

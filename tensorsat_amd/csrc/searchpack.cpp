@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cstdlib>
+#include <cstring>
 #include <thread>
 #include <vector>
 
@@ -24,9 +25,21 @@ int64_t pack_one(int n, int64_t ncl, const int32_t* len, const int32_t* L, uint3
   if (n < 0) return TSP_EINVAL;
   if (n > kMaxVars) return TSC_EUNSUPPORTED;
   int64_t w = 0, off = 0;
+  const auto code = [](int l) { return ((uint32_t)(std::abs(l) - 1) << 1) | (l < 0 ? 1u : 0u); };
   for (int64_t c = 0; c < ncl; ++c) {
     const int k = len[c];
     if (k < 0) return TSP_EINVAL;
+    if (k == 3) {
+      // the common case: three literals of distinct variables, kept as they are
+      const int a = L[off], b = L[off + 1], d = L[off + 2];
+      const int va = std::abs(a), vb = std::abs(b), vd = std::abs(d);
+      if (va != vb && va != vd && vb != vd && va && vb && vd && va <= n && vb <= n && vd <= n) {
+        if (out) out[w] = (3u << 24) | code(a) | (code(b) << 8) | (code(d) << 16);
+        ++w;
+        off += 3;
+        continue;
+      }
+    }
     int keep[3];
     int nk = 0;
     bool taut = false;
@@ -46,10 +59,7 @@ int64_t pack_one(int n, int64_t ncl, const int32_t* len, const int32_t* L, uint3
     if (taut) continue;  // always satisfied: dropped
     if (out) {
       uint32_t word = (uint32_t)nk << 24;
-      for (int q = 0; q < nk; ++q) {
-        const int l = keep[q];
-        word |= (((uint32_t)(std::abs(l) - 1) << 1) | (l < 0 ? 1u : 0u)) << (8 * q);
-      }
+      for (int q = 0; q < nk; ++q) word |= code(keep[q]) << (8 * q);
       out[w] = word;
     }
     ++w;
@@ -84,25 +94,34 @@ extern "C" int tsc_search_pack(int32_t num, const int32_t* nvars, const int64_t*
                                int32_t* bad_out, int32_t threads) {
   if (num < 0 || (num > 0 && (!nvars || !nclauses || !lens || !lits || !inst_out)) || !total_out) return TSP_EINVAL;
   if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
-  // Pass 1: kept clauses per instance (and the first failure); pass 2: write them.
-  std::vector<int64_t> kept((size_t)num);
-  parallel_for(num, threads, [&](int32_t i) { kept[i] = pack_one(nvars[i], nclauses[i], lens[i], lits[i], nullptr); });
+  // One parallel pass packs instance i at its input clause offset (packing never adds
+  // clauses, and words_out holds at least the input clause count when given); the
+  // kept ranges are then moved down in order over the clauses dropped before them.
+  std::vector<int64_t> in_off((size_t)num + 1, 0), kept((size_t)num);
+  for (int32_t i = 0; i < num; ++i) {
+    if (nclauses[i] < 0) {
+      if (bad_out) *bad_out = i;
+      return TSP_EINVAL;
+    }
+    in_off[i + 1] = in_off[i] + nclauses[i];
+  }
+  parallel_for(num, threads, [&](int32_t i) {
+    kept[i] = pack_one(nvars[i], nclauses[i], lens[i], lits[i], words_out ? words_out + in_off[i] : nullptr);
+  });
   int64_t w = 0;
   for (int32_t i = 0; i < num; ++i) {
     if (kept[i] < 0) {
       if (bad_out) *bad_out = i;
       return (int)kept[i];
     }
+    if (words_out && w != in_off[i] && kept[i] > 0)
+      std::memmove(words_out + w, words_out + in_off[i], (size_t)kept[i] * sizeof(uint32_t));
     inst_out[4 * i + 0] = (int32_t)w;
     inst_out[4 * i + 1] = (int32_t)kept[i];
     inst_out[4 * i + 2] = nvars[i];
     inst_out[4 * i + 3] = 0;
     w += kept[i];
   }
-  if (words_out)
-    parallel_for(num, threads, [&](int32_t i) {
-      pack_one(nvars[i], nclauses[i], lens[i], lits[i], words_out + inst_out[4 * i]);
-    });
   if (bad_out) *bad_out = -1;
   *total_out = w;
   return TSP_OK;

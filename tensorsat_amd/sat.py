@@ -23,7 +23,7 @@ Clause = tuple[int, ...]
 class CNFInstance:
     """A SAT instance in conjunctive normal form (``lib/sat.py:40``)."""
 
-    __slots__ = ("_num_vars", "_clauses", "_arrays", "_addr")
+    __slots__ = ("_num_vars", "_clauses", "_arrays", "_addr", "_head")
 
     def __init__(self, num_vars: int, clauses: Sequence[Sequence[int]]) -> None:
         """Public constructor, validating literals (``lib/sat.py:111-121``)."""
@@ -37,6 +37,7 @@ class CNFInstance:
         self._clauses = clauses
         self._arrays = None
         self._addr = None
+        self._head = None
 
     @classmethod
     def _new(cls, num_vars: int, clauses: tuple[Clause, ...], arrays=None) -> "CNFInstance":
@@ -45,6 +46,7 @@ class CNFInstance:
         self._clauses = clauses
         self._arrays = arrays
         self._addr = None
+        self._head = None
         return self
 
     @classmethod
@@ -170,6 +172,13 @@ class CNFInstance:
             lens, flat = self.literal_arrays()
             self._addr = (lens.size, lens.ctypes.data, flat.ctypes.data)
         return self._addr
+
+    def pack_header(self) -> tuple[int, int, int, int]:
+        """(variable count, clause count, address of the lengths, address of the
+        literals): what native batch packing reads per instance."""
+        if self._head is None:
+            self._head = (self._num_vars, *self.literal_addresses())
+        return self._head
 
     def clause_matrix(self) -> np.ndarray:
         """Clauses as a dense ``(m, k_max)`` int32 array, zero-padded on the right."""

@@ -24,6 +24,7 @@ raise, and :func:`supported` tells them apart beforehand.
 from __future__ import annotations
 
 import ctypes
+import itertools
 import math
 import time
 
@@ -74,11 +75,12 @@ def pack_batch(cnfs) -> tuple[np.ndarray, np.ndarray]:
 
     L = PG._lib()
     num = len(cnfs)
-    nv = np.fromiter((c.num_vars for c in cnfs), dtype=np.int32, count=num)
-    addr = np.fromiter((x for c in cnfs for x in c.literal_addresses()), dtype=np.int64, count=3 * num).reshape(num, 3)
-    ncl = np.ascontiguousarray(addr[:, 0])
-    lens_p = np.ascontiguousarray(addr[:, 1])
-    lits_p = np.ascontiguousarray(addr[:, 2])
+    head = np.fromiter(itertools.chain.from_iterable(c.pack_header() for c in cnfs), dtype=np.int64,
+                       count=4 * num).reshape(num, 4)
+    nv = head[:, 0].astype(np.int32)
+    ncl = np.ascontiguousarray(head[:, 1])
+    lens_p = np.ascontiguousarray(head[:, 2])
+    lits_p = np.ascontiguousarray(head[:, 3])
     words = np.empty(max(1, int(ncl.sum())), dtype=np.uint32)
     inst = np.zeros(num, dtype=INST_DTYPE)
     total, bad = ctypes.c_int64(0), ctypes.c_int32(-1)
