@@ -313,6 +313,25 @@ typedef struct {
  * construction). */
 int ts_search_run(const ts_search* s, ts_stream stream);
 
+/* The whole search in one launch (queue mode): resident one-wave workgroups take job
+ * records from a device queue in order, and a job that exhausts its s->max_nodes owned
+ * nodes publishes its remaining subtrees (as in round mode) at the queue's end for any
+ * idle workgroup to take -- no host round trip between hand-overs.  records holds
+ * `capacity` jobs, the first s->num_jobs of them the start jobs (s->jobs, s->spill,
+ * s->spill_count and s->spill_capacity are ignored); state (TS_SEARCH_QUEUE_STATE
+ * int32) and ready (capacity uint32 flags) are device scratch that the call
+ * initialises (a fill kernel, no memset).  After the launch completes: state[0] =
+ * records reserved (those beyond capacity were searched in place instead),
+ * state[192] != 0 if a workgroup gave up waiting for a record (a safety limit of 30 s;
+ * the verdicts are then incomplete).  s->stats (optional, capacity entries) is indexed by record.  workers:
+ * workgroups to launch (<= 0 or more than fit on the device at once: as many as fit);
+ * records are claimed in order by whichever workgroup is free, so the search completes
+ * however many of them the device runs at once.  Generalises
+ * ts_search_run's hand-over rounds; no counterpart in the reference. */
+#define TS_SEARCH_QUEUE_STATE 256
+int ts_search_run_queue(const ts_search* s, ts_search_job* records, int32_t capacity, int32_t* state,
+                        uint32_t* ready, int32_t workers, ts_stream stream);
+
 /* Allocation-free variant of a whole search (no counterpart in the reference): for
  * callers that cannot call hipMalloc -- tight serving loops, harnesses that forbid device
  * allocation.  Runs ts_search_run on s->jobs, then on the jobs handed over, round after

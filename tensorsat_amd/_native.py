@@ -37,7 +37,7 @@ EXPORTS = (
     "ts_program_run", "ts_program_count", "ts_batch_run", "ts_batch_max_workgroups",
     "ts_batch_compact", "ts_batch_count", "ts_batch_max_workgroups_lds", "ts_gnn_gemm", "ts_gnn_segsum", "ts_gnn_relu_back",
     "ts_gnn_head", "ts_gnn_adam", "ts_gnn_cast_bf16", "ts_dense_config", "ts_dense_tiles", "ts_dense_tile_of",
-    "ts_search_run", "ts_search_decide_noalloc", "ts_program_run_fused",
+    "ts_search_run", "ts_search_run_queue", "ts_search_decide_noalloc", "ts_program_run_fused",
 )
 
 _lib = None
@@ -99,6 +99,7 @@ def load() -> ctypes.CDLL:
         L.ts_dense_tiles.argtypes = [i32, i32, i32, i32, i32]
         L.ts_dense_tile_of.argtypes = [i32, i32, i32, i32, P(i32)]
         L.ts_search_run.argtypes = [vp, vp]
+        L.ts_search_run_queue.argtypes = [vp, vp, i32, vp, vp, i32, vp]
         L.ts_search_decide_noalloc.argtypes = [vp, P(i32), vp]
         _lib = L
     return _lib

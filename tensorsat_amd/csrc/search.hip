@@ -274,11 +274,58 @@ struct JobResult {
   int status, max_depth;
 };
 
+// Queue mode (ts_search_run_queue): the continuation list is the queue itself; `state`
+// holds, one 256-byte line each, the records reserved, the next record to claim, the
+// records pending (published -- the start records from the outset -- and not finished)
+// and an error flag; `ready` has one flag per record beyond the start records.  Null:
+// round mode.
+constexpr int Q_TAIL = 0, Q_HEAD = 64, Q_PENDING = 128, Q_ERROR = 192;
+struct Queue {
+  int32_t* state;
+  uint32_t* ready;
+};
+
+// A continuation record written by lanes 0..15, one dword each, with system-scope
+// stores: they reach memory past every cache, so a reader on another XCD never meets a
+// stale copy of the line (records of different producers share 128-byte lines).
+__device__ __forceinline__ void store_record(ts_search_job* rec, const ts_search_job& o, int lane) {
+  uint32_t d;
+  switch (lane) {
+    case 0: d = (uint32_t)o.instance; break;
+    case 1: d = (uint32_t)o.depth; break;
+    case 2: d = (uint32_t)o.split; break;
+    case 3: d = (uint32_t)o.own_from; break;
+    case 4: d = o.prefix; break;
+    case 5: d = o.reserved[0]; break;
+    case 6: d = o.reserved[1]; break;
+    case 7: d = o.reserved[2]; break;
+    case 8: d = (uint32_t)o.assigned[0]; break;
+    case 9: d = (uint32_t)(o.assigned[0] >> 32); break;
+    case 10: d = (uint32_t)o.assigned[1]; break;
+    case 11: d = (uint32_t)(o.assigned[1] >> 32); break;
+    case 12: d = (uint32_t)o.value[0]; break;
+    case 13: d = (uint32_t)(o.value[0] >> 32); break;
+    case 14: d = (uint32_t)o.value[1]; break;
+    default: d = (uint32_t)(o.value[1] >> 32); break;
+  }
+  if (lane < 16)
+    __hip_atomic_store(reinterpret_cast<uint32_t*>(rec) + lane, d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// Queue mode: make records [first, first + n) runnable -- one release fence for their
+// stores, counted as pending (lane 0 adds), then flagged (a lane per record).
+__device__ __forceinline__ void publish(const Queue& q, int first, int n, int lane) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  atomicAdd(&q.state[Q_PENDING], lane == 0 ? n : 0);
+  for (int b = 0; b < n; b += 64)
+    if (b + lane < n) __hip_atomic_store(&q.ready[first + b + lane], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // One job's search with NS clause slots per lane (the instance has at most 64 * NS
 // clauses).  The clause-status table and the zeroed histogram are in `sh` on entry.
 template <int NS>
 __device__ __forceinline__ void search_job(const ts_search& p, const ts_search_job& job, const ts_search_inst& inst,
-                                           Shared& sh, int lane, JobResult& r) {
+                                           Shared& sh, int lane, JobResult& r, const Queue& q) {
   const int m = inst.num_clauses;
   int64_t owned = 0;
   const int base = job.depth, split = job.split, own_from = job.own_from;
@@ -349,9 +396,9 @@ __device__ __forceinline__ void search_job(const ts_search& p, const ts_search_j
       // every frame the job explores freely (below its prefix levels).
       int n_out = 1;
       for (int k = base + split; k < depth; ++k) n_out += (uni(sh.stk_lit[k]) >> 8) == 1u;
-      int start = 0;
-      if (lane == 0) start = atomicAdd(p.spill_count, n_out);
-      start = (int)uni((uint32_t)__builtin_amdgcn_readlane(start, 0));
+      // one reservation for the wave (lane 0 adds, every lane takes part: no branch on
+      // the lane inside the search loops)
+      const int start = (int)uni((uint32_t)atomicAdd(p.spill_count, lane == 0 ? n_out : 0));
       if (start + n_out <= p.spill_capacity) {
         // Records are built on every lane (the layout conversion ballots) and
         // written by lane 0.
@@ -367,7 +414,7 @@ __device__ __forceinline__ void search_job(const ts_search& p, const ts_search_j
           o.own_from = k + 1;
           to_natural(c.ae, c.ao, o.assigned[0], o.assigned[1], lane);
           to_natural(c.ve, c.vo, o.value[0], o.value[1], lane);
-          if (lane == 0) p.spill[w] = o;
+          store_record(p.spill + w, o, lane);
           ++w;
         }
         // The subtree keeps what is left of the job's prefix levels (the other
@@ -379,15 +426,18 @@ __device__ __forceinline__ void search_job(const ts_search& p, const ts_search_j
         o.prefix = rel < split ? prefix >> rel : 0u;
         to_natural(s.ae, s.ao, o.assigned[0], o.assigned[1], lane);
         to_natural(s.ve, s.vo, o.value[0], o.value[1], lane);
-        if (lane == 0) p.spill[w] = o;
+        store_record(p.spill + w, o, lane);
+        if (q.ready) publish(q, start, n_out, lane);
         r.status = 3;
         break;
       }
       // No room: empty the reserved slots that exist and finish the subtree here.
-      if (lane == 0) {
+      {
         ts_search_job o = {};
         o.instance = -1;
-        for (int w = start; w < p.spill_capacity && w < start + n_out; ++w) p.spill[w] = o;
+        const int end = start + n_out < p.spill_capacity ? start + n_out : p.spill_capacity;
+        for (int w = start; w < end; ++w) store_record(p.spill + w, o, lane);
+        if (q.ready && end > start) publish(q, start, end - start, lane);
       }
       budget = INT64_MAX;
     }
@@ -454,7 +504,8 @@ __device__ __forceinline__ void search_job(const ts_search& p, const ts_search_j
         if (bits != 0) {
           uint64_t w0 = 0, w1 = 0;  // the leaf's assignment's values, natural layout
           if (p.witness) to_natural(s.ve, s.vo, w0, w1, lane);
-          if (p.witness && lane == 0) {
+          if (p.witness) {
+            // computed on every lane (the values are uniform), stored by lanes 0 and 1
             // The satisfying entry that is smallest with the smallest variable of R
             // most significant (lexicographic order of the assignment), then the
             // assignment: its values, R's variables from that entry, 0 elsewhere.
@@ -478,10 +529,9 @@ __device__ __forceinline__ void search_job(const ts_search& p, const ts_search_j
                 else w1 |= 1ull << (v - 64);
               }
             }
-            p.witness[2 * (size_t)job.instance] = w0;
-            p.witness[2 * (size_t)job.instance + 1] = w1;
+            if (lane < 2) p.witness[2 * (size_t)job.instance + lane] = lane ? w1 : w0;
           }
-          if (lane == 0) atomicOr(&p.verdicts[job.instance], TS_SEARCH_SAT);
+          atomicOr(&p.verdicts[job.instance], lane == 0 ? TS_SEARCH_SAT : 0u);
           if (!p.exhaustive) break;
         }
       }
@@ -523,14 +573,15 @@ __device__ __forceinline__ void search_job(const ts_search& p, const ts_search_j
 // that holds its clauses.
 template <int NS, int... Rest>
 __device__ __forceinline__ void search_variant(int need, const ts_search& p, const ts_search_job& job,
-                                               const ts_search_inst& inst, Shared& sh, int lane, JobResult& r) {
+                                               const ts_search_inst& inst, Shared& sh, int lane, JobResult& r,
+                                               const Queue& q) {
   if constexpr (sizeof...(Rest) == 0) {
-    search_job<NS>(p, job, inst, sh, lane, r);
+    search_job<NS>(p, job, inst, sh, lane, r, q);
   } else {
     if (need <= NS)
-      search_job<NS>(p, job, inst, sh, lane, r);
+      search_job<NS>(p, job, inst, sh, lane, r, q);
     else
-      search_variant<Rest...>(need, p, job, inst, sh, lane, r);
+      search_variant<Rest...>(need, p, job, inst, sh, lane, r, q);
   }
 }
 
@@ -541,43 +592,160 @@ constexpr int last_of() {
   return x;
 }
 
+// One job record: its search (the smallest variant of V... that holds its instance),
+// verdict bits and statistics in r.
+template <int... V>
+__device__ __forceinline__ void run_record(const ts_search& p, const ts_search_job& job, Shared& sh, int lane,
+                                           JobResult& r, const Queue& q) {
+  constexpr int NSMAX = last_of<V...>();
+  if (job.instance < 0) return;
+  const ts_search_inst inst = p.instances[job.instance];
+  const int need = (int)uni((uint32_t)((inst.num_clauses + 63) / 64));
+  if (inst.num_clauses > 64 * NSMAX) {
+    // More clauses than this kernel holds (the host picks the kernel from the largest
+    // instance): undecided, never a wrong verdict.
+    r.status = 2;
+  } else if (p.early_exit && (uni(__hip_atomic_load(&p.verdicts[job.instance], __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT)) & TS_SEARCH_SAT)) {
+    r.status = 1;  // decided by another job already
+  } else {
+    sh.hist[lane] = 0;
+    sh.hist[lane + 64] = 0;
+    build_clause_table(sh.clause_table, lane);
+    __syncthreads();
+    search_variant<V...>(need, p, job, inst, sh, lane, r, q);
+  }
+  if (r.status == 2) atomicOr(&p.verdicts[job.instance], lane == 0 ? TS_SEARCH_INCOMPLETE : 0u);
+}
+
+// A job's statistics, written by lanes 0..9 (one dword each).
+__device__ __forceinline__ void write_stat(ts_search_stat* out, const JobResult& r, int lane) {
+  uint32_t d;
+  switch (lane) {
+    case 0: d = (uint32_t)r.status; break;
+    case 1: d = (uint32_t)r.max_depth; break;
+    case 2: d = (uint32_t)r.nodes; break;
+    case 3: d = (uint32_t)((uint64_t)r.nodes >> 32); break;
+    case 4: d = (uint32_t)r.leaves; break;
+    case 5: d = (uint32_t)((uint64_t)r.leaves >> 32); break;
+    case 6: d = (uint32_t)r.conflicts; break;
+    case 7: d = (uint32_t)((uint64_t)r.conflicts >> 32); break;
+    case 8: d = (uint32_t)r.digest; break;
+    default: d = (uint32_t)(r.digest >> 32); break;
+  }
+  if (lane < 10) reinterpret_cast<uint32_t*>(out)[lane] = d;
+}
+
 }  // namespace
 
-// Variants V... (increasing clause slots per lane); the host picks the kernel whose
-// largest variant holds the batch's largest instance.
+// Round mode: one workgroup per job of p.jobs.  Variants V... (increasing clause slots
+// per lane); the host picks the kernel whose largest variant holds the batch's largest
+// instance.
 template <int... V>
 __global__ __launch_bounds__(64) void k_search(ts_search p) {
   __shared__ Shared sh;
-  constexpr int NSMAX = last_of<V...>();
   const int lane = (int)threadIdx.x;
   const ts_search_job job = p.jobs[blockIdx.x];
   JobResult r{};
-  if (job.instance >= 0) {
-    const ts_search_inst inst = p.instances[job.instance];
-    const int need = (int)uni((uint32_t)((inst.num_clauses + 63) / 64));
-    if (inst.num_clauses > 64 * NSMAX) {
-      // More clauses than this kernel holds (the host picks the kernel from the largest
-      // instance): undecided, never a wrong verdict.
-      r.status = 2;
-    } else {
-      sh.hist[lane] = 0;
-      sh.hist[lane + 64] = 0;
-      build_clause_table(sh.clause_table, lane);
-      __syncthreads();
-      search_variant<V...>(need, p, job, inst, sh, lane, r);
+  run_record<V...>(p, job, sh, lane, r, Queue{nullptr, nullptr});
+  if (p.stats) write_stat(&p.stats[blockIdx.x], r, lane);
+}
+
+namespace {
+
+// A queue record as written by other workgroups in this launch: read past every cache
+// (system-scope loads, lanes 0..15 one dword each) after its flag was acquired, then
+// made uniform.
+__device__ __forceinline__ ts_search_job load_record(const ts_search_job* rec, int lane) {
+  const uint32_t* w = reinterpret_cast<const uint32_t*>(rec);
+  const uint32_t x = lane < 16 ? __hip_atomic_load(w + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) : 0u;
+  uint32_t d[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) d[i] = (uint32_t)__builtin_amdgcn_readlane((int)x, i);
+  ts_search_job j;
+  j.instance = (int32_t)d[0];
+  j.depth = (int32_t)d[1];
+  j.split = (int32_t)d[2];
+  j.own_from = (int32_t)d[3];
+  j.prefix = d[4];
+  j.reserved[0] = d[5];
+  j.reserved[1] = d[6];
+  j.reserved[2] = d[7];
+  j.assigned[0] = (uint64_t)d[8] | ((uint64_t)d[9] << 32);
+  j.assigned[1] = (uint64_t)d[10] | ((uint64_t)d[11] << 32);
+  j.value[0] = (uint64_t)d[12] | ((uint64_t)d[13] << 32);
+  j.value[1] = (uint64_t)d[14] | ((uint64_t)d[15] << 32);
+  return j;
+}
+
+// Wall-clock ticks (100 MHz) a worker waits for a queue record before it gives up with
+// the error flag: a safety net of 30 s, far beyond any job's length in this search.
+constexpr uint64_t kWaitTicks = 3000000000ull;
+
+}  // namespace
+
+// Queue mode: the next record to take, claimed in order: a start record at once, a
+// handed-over one once it is flagged (then after an acquire); -1 when the queue is
+// drained (nothing pending: no job runs and none will ever be written at the claim) or
+// the claim lies beyond the capacity.  Every lane takes part in each atomic (lane 0
+// adds), so no branch in the worker loop depends on the lane; the result is
+// wave-uniform.  Polls back off (sleeps growing to about 4 us, the pending count read
+// every fourth poll) so that idle workers do not load the memory system the running
+// ones use.
+__device__ __forceinline__ int next_record(const ts_search& p, const Queue& q, int lane) {
+  const int idx = (int)uni((uint32_t)atomicAdd(&q.state[Q_HEAD], lane == 0 ? 1 : 0));
+  if (idx >= p.spill_capacity) return -1;
+  if (idx < p.num_jobs) return idx;
+  const uint64_t t0 = wall_clock64();
+  int result = -1;
+  for (uint32_t poll = 0;; ++poll) {
+    if (uni(__hip_atomic_load(&q.ready[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+      result = idx;
+      break;
     }
-    if (lane == 0 && r.status == 2) atomicOr(&p.verdicts[job.instance], TS_SEARCH_INCOMPLETE);
+    if ((poll & 3u) == 3u) {
+      if (uni((uint32_t)__hip_atomic_load(&q.state[Q_PENDING], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) == 0u)
+        break;
+      if (wall_clock64() - t0 > kWaitTicks) {
+        atomicOr(&q.state[Q_ERROR], lane == 0 ? 1 : 0);
+        break;
+      }
+    }
+    if (poll < 8)
+      __builtin_amdgcn_s_sleep(8);
+    else if (poll < 32)
+      __builtin_amdgcn_s_sleep(32);
+    else
+      __builtin_amdgcn_s_sleep(127);
   }
-  if (lane == 0 && p.stats) {
-    ts_search_stat o;
-    o.status = r.status;
-    o.max_depth = r.max_depth;
-    o.nodes = r.nodes;
-    o.leaves = r.leaves;
-    o.conflicts = r.conflicts;
-    o.digest = r.digest;
-    p.stats[blockIdx.x] = o;
+  if (result >= 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the record's stores, after its flag
+  return (int)uni((uint32_t)result);
+}
+
+// Queue mode: workers take records in claim order -- the start records (hardest first)
+// and then the hand-overs in order of publication -- until the queue is drained.  No
+// record belongs to a worker before the worker claims it, so the launch needs no
+// assumption about how many workgroups are resident at once: a waiting worker waits for
+// a record whose producer, or whose claimer, is running.  Every job finishes (its nodes
+// are bounded), so every wait ends; a wall-clock limit turns a wait that would not into
+// the error flag.
+template <int... V>
+__global__ __launch_bounds__(64) void k_search_queue(ts_search p, Queue q) {
+  __shared__ Shared sh;
+  const int lane = (int)threadIdx.x;
+  for (int idx = next_record(p, q, lane); idx >= 0; idx = next_record(p, q, lane)) {
+    const ts_search_job job = idx < p.num_jobs ? p.jobs[idx] : load_record(p.spill + idx, lane);
+    JobResult r{};
+    run_record<V...>(p, job, sh, lane, r, q);
+    if (p.stats) write_stat(&p.stats[idx], r, lane);
+    atomicAdd(&q.state[Q_PENDING], lane == 0 ? -1 : 0);  // continuations were published before
   }
+}
+
+__global__ void k_queue_init(int32_t* state, uint32_t* ready, int32_t capacity, int32_t num_jobs) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i < capacity) ready[i] = 0u;
+  if (i < TS_SEARCH_QUEUE_STATE) state[i] = (i == Q_TAIL || i == Q_PENDING) ? num_jobs : 0;
 }
 
 extern "C" int ts_search_run(const ts_search* p, ts_stream stream) {
@@ -604,6 +772,56 @@ extern "C" int ts_search_run(const ts_search* p, ts_stream stream) {
   else
     hipLaunchKernelGGL((k_search<4, 8, TS_SEARCH_SLOTS>), grid, block, 0, st, *p);
   return hipGetLastError() == hipSuccess ? TS_OK : TS_EHIP;
+}
+
+namespace {
+
+// Workers: the workgroups the device holds at once (more would only queue behind them;
+// a worker that is not resident yet owns no record, so none waits for it).
+template <int... V>
+int launch_queue(const ts_search& p, const Queue& q, int workers, hipStream_t st) {
+  int dev = 0, per_cu = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return TS_EHIP;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_search_queue<V...>, 64, 0) != hipSuccess)
+    return TS_EHIP;
+  const int resident = per_cu * cus;
+  if (resident < 1) return TS_EHIP;
+  if (workers <= 0 || workers > resident) workers = resident;
+  workers = workers < p.spill_capacity ? workers : p.spill_capacity;
+  if (workers < 1) workers = 1;
+  const int cap = p.spill_capacity;
+  const int n = cap > TS_SEARCH_QUEUE_STATE ? cap : TS_SEARCH_QUEUE_STATE;
+  hipLaunchKernelGGL(k_queue_init, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, q.state, q.ready, cap,
+                     p.num_jobs);
+  hipLaunchKernelGGL((k_search_queue<V...>), dim3((unsigned)workers), dim3(64), 0, st, p, q);
+  return hipGetLastError() == hipSuccess ? TS_OK : TS_EHIP;
+}
+
+}  // namespace
+
+extern "C" int ts_search_run_queue(const ts_search* s, ts_search_job* records, int32_t capacity, int32_t* state,
+                                   uint32_t* ready, int32_t workers, ts_stream stream) {
+  if (!s || !records || !state || !ready || capacity <= 0) return TS_EINVAL;
+  if (s->num_jobs < 0 || s->num_jobs > capacity || s->num_instances < 0) return TS_EINVAL;
+  if (s->width < 0 || s->width > TS_SEARCH_MAX_WIDTH || s->max_nodes <= 0) return TS_EINVAL;
+  if (s->max_clauses < 0 || s->max_clauses > 64 * TS_SEARCH_SLOTS) return TS_EINVAL;
+  if (s->order != TS_SEARCH_ORDER_SCORE && s->order != TS_SEARCH_ORDER_LEX) return TS_EINVAL;
+  if (s->num_jobs > 0 && (!s->clauses || !s->instances || !s->verdicts)) return TS_EINVAL;
+  ts_search p = *s;
+  p.jobs = records;
+  p.spill = records;
+  p.spill_count = state + Q_TAIL;
+  p.spill_capacity = capacity;
+  const Queue q{state, ready};
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  const int need = p.max_clauses ? (p.max_clauses + 63) / 64 : TS_SEARCH_SLOTS;
+  if (need <= 2) return launch_queue<2>(p, q, workers, st);
+  if (need <= 4) return launch_queue<2, 3, 4>(p, q, workers, st);
+  if (need <= 6) return launch_queue<2, 3, 4, 5, 6>(p, q, workers, st);
+  if (need <= 8) return launch_queue<4, 6, 8>(p, q, workers, st);
+  return launch_queue<4, 8, TS_SEARCH_SLOTS>(p, q, workers, st);
 }
 
 // ---- the allocation-free entry point ----------------------------------------------

@@ -57,41 +57,54 @@ def test_unsplit_early_exit_statistics_exact(batch, width):
     np.testing.assert_array_equal(got["max_depth"], ref["max_depth"])
 
 
-@pytest.mark.parametrize("target,budget", [(None, None), (64.0, None), (4.0, None), (None, 4), (4.0, 2), (None, 1)])
-def test_split_exhaustive_statistics_sum_to_one_search(batch, target, budget):
+@pytest.mark.parametrize("target,budget,queue", [(None, None, False), (64.0, None, False), (4.0, None, False),
+                                                (None, 4, False), (4.0, 2, False), (None, 1, False),
+                                                (None, 4, True), (4.0, 2, True), (None, 1, True)])
+def test_split_exhaustive_statistics_sum_to_one_search(batch, target, budget, queue):
     """Exhaustive searches (every branch visited): the jobs of an instance own disjoint
-    subtrees -- static splits and the subtrees handed over when a job's budget runs out
-    -- so their statistics add up to one unsplit search's, whatever the split."""
+    subtrees -- static splits and the subtrees handed over when a job's budget runs out,
+    to the next launch (rounds) or to the same launch's queue -- so their statistics add
+    up to one unsplit search's, whatever the split."""
     cnfs = [c for c in batch[:600] if c.num_vars <= 50]
-    s, v = _solve(cnfs, width=6, split_target=target, max_split=6, exhaustive=True, budget=budget)
+    s, v = _solve(cnfs, width=6, split_target=target, max_split=6, exhaustive=True, budget=budget, queue=queue)
     got = s.instance_stats()
     ref = OS.search(cnfs, width=6, exhaustive=True, threads=8)
-    print(f"target {target} budget {budget}: rounds {s.rounds}, jobs per round {s.round_jobs[:8]}", flush=True)
+    extra = f", queue records {s.queue_records()}" if s.queue else ""
+    print(f"target {target} budget {budget} queue {queue}: rounds {s.rounds}, jobs per round {s.round_jobs[:8]}"
+          f"{extra}", flush=True)
     np.testing.assert_array_equal(v, ref["verdict"])
     for f in FIELDS:
         np.testing.assert_array_equal(got[f], ref[f], err_msg=f"{f} target {target} budget {budget}")
     if target == 4.0:
         assert int(s.host_jobs["split"].max()) >= 3
-    if budget is not None:
+    if budget is not None and not queue:
         assert s.rounds > 3
+    if queue:
+        assert s.queue_records() > 2 * len(s.host_jobs)  # hand-overs ran in the one launch
 
 
-def test_hand_over_list_overflow_keeps_every_subtree(batch):
-    """A continuation list too small for the hand-overs: jobs that cannot hand over keep
-    searching their subtrees themselves, so no branch is lost (exact exhaustive sums)."""
+@pytest.mark.parametrize("queue", [False, True])
+def test_hand_over_list_overflow_keeps_every_subtree(batch, queue):
+    """A continuation list (or queue) too small for the hand-overs: jobs that cannot hand
+    over keep searching their subtrees themselves, so no branch is lost (exact
+    exhaustive sums)."""
     cnfs = [c for c in batch[:300] if c.num_vars <= 45]
-    s, v = _solve(cnfs, width=6, split_target=None, exhaustive=True, budget=2, spill_capacity=16)
+    cap = 16 if not queue else len(cnfs) + 16
+    s, v = _solve(cnfs, width=6, split_target=None, exhaustive=True, budget=2, spill_capacity=cap, queue=queue)
     got = s.instance_stats()
     ref = OS.search(cnfs, width=6, exhaustive=True, threads=8)
     np.testing.assert_array_equal(v, ref["verdict"])
     for f in FIELDS:
         np.testing.assert_array_equal(got[f], ref[f], err_msg=f)
-    assert s.rounds >= 2
+    if queue:
+        assert s.queue_records() > cap  # more reserved than fit: searched in place
+    else:
+        assert s.rounds >= 2
 
 
-@pytest.mark.parametrize("budget", [1, 8, 64])
-def test_batched_workload_verdicts_with_hand_over(batch, budget):
-    s, v = _solve(batch, width=6, split_target=8.0, budget=budget)
+@pytest.mark.parametrize("budget,queue", [(1, False), (8, False), (64, False), (1, True), (8, True), (64, True)])
+def test_batched_workload_verdicts_with_hand_over(batch, budget, queue):
+    s, v = _solve(batch, width=6, split_target=8.0, budget=budget, queue=queue)
     ref = OS.search(batch, width=6, threads=8)["verdict"]
     print(f"budget {budget}: rounds {s.rounds}, jobs per round {s.round_jobs[:10]}", flush=True)
     np.testing.assert_array_equal(v, ref)

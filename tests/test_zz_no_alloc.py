@@ -43,7 +43,7 @@ def _noalloc(cnfs, width, split_target, budget, spill_capacity=0):
 def _reference(cnfs, width, split_target, budget):
     from tensorsat_amd.search import SearchSolver
 
-    s = SearchSolver(width=width, split_target=split_target, budget=budget)
+    s = SearchSolver(width=width, split_target=split_target, budget=budget, queue=False)
     v = s.run(cnfs)
     return v, s.rounds
 
