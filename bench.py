@@ -42,6 +42,7 @@ HBM_PEAK_GBS = 8000.0
 # other configurations report null.
 MEASURED_TRAFFIC = {
     ("batched", 6, 4096, 4): (44.4e6, "profiles/k_batch_wave_batched3sat_w6.md"),
+    ("search", 4, 4096): (25.2e6, "profiles/k_search_batched3sat.md"),
     ("dense", 13, 13, 13, 6): (1.120e9, "profiles/k_step_mfma_dense_composition_13_13_13.md"),
     ("dense", 13, 13, 13, 4): (1.468e9, "profiles/k_step_mfma_dense_composition_13_13_13.md"),
 }
@@ -118,13 +119,17 @@ def main() -> int:
     ap.add_argument("--kernel", default="auto", choices=["auto", "lds", "global"],
                     help="batched kernel: LDS arenas, device-memory arenas, or auto (LDS when the arenas fit)")
     ap.add_argument("--workload", default="batched",
-                    choices=["batched", "batched-host", "dense", "learned", "copy", "single", "count", "rewire"])
+                    choices=["batched", "batched-host", "dense", "learned", "copy", "single", "count", "rewire", "plain"])
     ap.add_argument("--split", default="auto", help="batched: split target in branch nodes per job, 'none', "
                     "or 'auto' (autotuned with the width)")
     ap.add_argument("--no-nested", action="store_true", help="batched: skip the nested single / count lines")
     ap.add_argument("--budget-nodes", default="auto", help="batched: nodes a search job runs before it hands "
                     "its remaining subtrees over, 'none', or 'auto'")
     ap.add_argument("--seeds", type=int, default=3, help="single / count workloads: instances (seeds 0..)")
+    ap.add_argument("--streams", type=int, default=0,
+                    help="single / plain: cubes (slices) in flight on their own streams (0: the workload's default)")
+    ap.add_argument("--min-sum-bits", type=int, default=6,
+                    help="dense steps with at least this many summed bits take the int8 MFMA route (0..27)")
     ap.add_argument("--learned-instances", type=int, default=8)
     ap.add_argument("--learned-width", type=int, default=22, help="learned workload: cube width W")
     ap.add_argument("--learned-k", type=int, default=0, help="learned workload: keep only clauses of this k (0: all)")
@@ -144,6 +149,15 @@ def main() -> int:
         return bench_learned(args)
     if args.workload == "copy":
         return bench_copy(args)
+    if args.min_sum_bits != 6:
+        import torch
+
+        from tensorsat_amd import _native as N
+
+        torch.zeros(1, device="cuda")  # the device context first
+        N.check(N.lib().ts_dense_config(1, args.min_sum_bits), "ts_dense_config")
+    if args.workload == "plain":
+        return bench_plain(args)
     if args.workload == "single":
         return bench_single(args)
     if args.workload == "rewire":
@@ -377,6 +391,7 @@ def bench_search(args) -> int:
 
     nested = {}
     if not args.no_nested:
+        nested["plain_n40"] = run_plain(args, seeds=range(3), log=log)
         nested["single_n120"] = run_single(args, seeds=range(3), log=log)
         nested["count_n100"] = run_count(args, seeds=range(1), log=log)
 
@@ -624,9 +639,10 @@ def _step_bytes(steps, elem: int = 1) -> float:
 def run_single(args, seeds, log) -> dict:
     """
     One large random 3-SAT instance on the whole GPU (n=120, m=510, seeds 0..): conditioned
-    slicing into cubes of width up to 26 (tensors up to 2^26 bytes), cubes decided one
-    after the other with every step a grid-wide kernel (``single.decide``'s loop), until
-    the first satisfiable cube.  Reported: plan and GPU seconds per instance, and the step
+    slicing into cubes of width up to 26 (tensors up to 2^26 bytes), cubes in depth-first
+    order, four in flight on their own streams, runs of small steps fused into one
+    workgroup and large steps grid-wide (``single.decide``), until the first satisfiable
+    cube.  Reported: plan and GPU seconds per instance, and the step
     kernels' algorithmic bytes over the GPU time (HBM bound).  Verdicts are checked against
     the independent DPLL.  CPU baseline: the oracle's numpy step loop on the same cubes of
     the first seed, one cube per process on all host cores, for a bounded wall time,
@@ -650,8 +666,9 @@ def run_single(args, seeds, log) -> dict:
         prog = PG.sat_batch([cnf], width=26, plan_trials=2)
         plan_s = time.perf_counter() - t0
         progs.append(prog)
-        decide(cnf, program=prog, mode=args.mode)  # warm-up: code objects, arenas
-        r = decide(cnf, program=prog, mode=args.mode)
+        streams = args.streams or 4
+        decide(cnf, program=prog, mode=args.mode, streams=streams)  # warm-up: code objects, arenas
+        r = decide(cnf, program=prog, mode=args.mode, streams=streams)
         gpu_s = r.gpu_s
         launched = r.stats.get("cubes_launched", r.cubes_run)
         nbytes = float(sum(_step_bytes(prog.job_steps(j)) for j in range(launched)))
@@ -698,6 +715,98 @@ def run_single(args, seeds, log) -> dict:
         "parity": {"checked": len(rows), "dpll_mismatches": mism},
         "status": "mismatch" if mism else "ok",
     }
+
+
+def run_plain(args, seeds, log) -> dict:
+    """
+    Config 1, the reference's own call shape on one GPU: a random 3-SAT instance with n=40,
+    m=170 (``workloads.plain_cpu_3sat``), its flattened bintree network
+    (``(cnf.inputs(None) >> cnf.diagram()).flatten()``) contracted along a greedy tree
+    (``PlannedContraction(..., optimize="greedy")``, ``examples/test_sat_solver.py:5-9``)
+    sliced to intermediates of at most 2^26 entries, every slice compiled into one job of
+    step records (contracted wires innermost) and the jobs decided by the step executors
+    with early exit (``single.decide``).  Verdicts against the independent DPLL.  CPU: the
+    oracle's numpy step loop on the same jobs on all host cores, sampled and scaled.
+    """
+    from tensorsat_amd import program as PG
+    from tensorsat_amd.contractions import PlannedContraction
+    from tensorsat_amd.rel import DeviceRel
+    from tensorsat_amd.single import decide
+    from tensorsat_amd.workloads import plain_cpu_3sat
+
+    rows, progs = [], []
+    mism = 0
+    for seed in seeds:
+        cnf = plain_cpu_3sat(seed)
+        d = (cnf.inputs(None) >> cnf.diagram()).flatten()
+        t0 = time.perf_counter()
+        c = PlannedContraction(DeviceRel, d.wiring, optimize="greedy", slicing_target=26)
+        plan_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        prog = c.contract_program(d)
+        compile_s = time.perf_counter() - t0
+        progs.append(prog)
+        streams = args.streams or 4
+        decide(cnf, program=prog, mode=args.mode, streams=streams)  # warm-up
+        r = decide(cnf, program=prog, mode=args.mode, streams=streams)
+        launched = r.stats.get("cubes_launched", r.cubes_run)
+        st = prog.steps[: int(prog.jobs["step_begin"][launched - 1] + prog.jobs["num_steps"][launched - 1])]
+        dense, ops, nbytes = PG.step_work(st, args.min_sum_bits)
+        want = None
+        if not args.no_cpu:
+            from oracle import dpll
+
+            want = int(dpll.solve(cnf.num_vars, cnf.clauses))
+            mism += int(r.value != want)
+        rows.append({"seed": seed, "verdict": r.value, "dpll": want, "plan_s": plan_s, "compile_s": compile_s,
+                     "gpu_s": r.gpu_s, "slices": len(prog.jobs), "slices_run": r.cubes_run,
+                     "slices_launched": launched, "sliced_wires": len(c.tree.sliced),
+                     "log2_max_size": float(c.tree.cost.log2_max_size), "ops": float(ops.sum()),
+                     "bytes": float(nbytes.sum()), "mfma_op_share": float(ops[dense].sum() / ops.sum())})
+        log(f"n=40 seed {seed}: verdict {r.value} (dpll {want}), plan {plan_s:.2f}s + compile {compile_s:.2f}s, "
+            f"gpu {r.gpu_s:.4f}s, {r.cubes_run}/{len(prog.jobs)} slices ({launched} launched), "
+            f"{nbytes.sum() / r.gpu_s / 1e9:.0f} GB/s, MFMA-routed ops {ops[dense].sum() / ops.sum():.1%}")
+    gpu = float(np.mean([r["gpu_s"] for r in rows]))
+    host = float(np.mean([r["plan_s"] + r["compile_s"] for r in rows]))
+    gbs = sum(r["bytes"] for r in rows) / sum(r["gpu_s"] for r in rows) / 1e9
+    cpu = None
+    if not args.no_cpu:
+        cores = host_cores()
+        prog, ran = progs[0], rows[0]["slices_run"]
+        k = min(ran, cores)
+        t0 = time.perf_counter()
+        with _pool(prog, cores) as ex:
+            list(ex.map(_pool_job, range(k), [args.mode] * k))
+        dt = time.perf_counter() - t0
+        per_inst = dt * ran / k
+        cpu = {"value": per_inst + rows[0]["plan_s"] + rows[0]["compile_s"], "unit": "s per instance",
+               "cores": cores, "kind": "port", "cpu_model": cpu_model(),
+               "sample": f"seed {rows[0]['seed']}: {k} of the {ran} slices the GPU ran, through the oracle's numpy "
+                         f"step loop in {cores} processes, {dt:.1f}s wall, scaled by {ran}/{k}, plus the same "
+                         f"plan and compile time"}
+        log(f"n=40 cpu: {k} slices in {dt:.1f}s -> {per_inst:.1f}s per instance")
+    return {
+        "metric": "plan+contract s per instance, plain n=40 3-SAT decision through the reference's call shape "
+                  "(greedy tree, bit-exact)",
+        "value": host + gpu, "unit": "s", "steps": len(rows), "ms_per_step": gpu * 1000.0,
+        "higher_is_better": False, "dtype": "u8",
+        "config": {"workload": "plain random 3-SAT n=40 m=170 (seeds 0..), bintree network, greedy tree sliced to "
+                               "2^26 entries, one compiled job per slice, early exit",
+                   "host_s_mean": host, "gpu_s_mean": gpu, "rows": rows},
+        "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                     "traffic": None, "kernel": "k_step_u8 / k_step_seq / k_step_mfma: algorithmic bytes of every "
+                                                "step of the launched slices over the decision's wall time"},
+        "cpu_baseline": cpu,
+        "parity": {"checked": len(rows), "dpll_mismatches": mism},
+        "status": "mismatch" if mism else "ok",
+    }
+
+
+def bench_plain(args) -> int:
+    out = run_plain(args, range(args.seeds), lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True))
+    out.update({"n_gpus": 1, "warmup": 1, "scaling": "weak", "vs_baseline": None, "data": "synthetic"})
+    print(json.dumps(out), flush=True)
+    return 1 if out["status"] == "mismatch" else 0
 
 
 def bench_single(args) -> int:

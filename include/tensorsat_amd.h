@@ -91,11 +91,14 @@ int ts_dense_tiles(int32_t mbits, int32_t nbits, int32_t bbits, int32_t kbits, i
 /* The tile steps of that shape use now (1..6). */
 int ts_dense_tile_of(int32_t mbits, int32_t nbits, int32_t bbits, int32_t kbits, int32_t* tile);
 
-/* Routing of dense steps (ts_contract2 and ts_program_run) to the int8 MFMA kernel
- * (v_mfma_i32_16x16x64_i8, exact int32 counts): a step goes there when its summed wires
- * are innermost in both operands, number at least max(6, min_sum_bits), and at least 4
- * out wires belong to lhs only and 4 to rhs only; other steps run on the VALU path.
- * Default: enabled, min_sum_bits = 6.  Results are identical on both paths; the switch
+/* Routing of dense steps (ts_contract2 and ts_program_run) to the int8 MFMA kernels
+ * (v_mfma_i32_16x16x64_i8, exact int32 counts): a binary step goes there when its summed
+ * wires are innermost in both operands, number at least min_sum_bits, and at least 4 out
+ * wires belong to lhs only and 4 to rhs only -- 6 or more summed wires to the pipelined
+ * kernel, 0..5 to the one-stage kernel (K zero-padded to 64 bytes in LDS); other steps
+ * run on the VALU path.  Default: enabled, min_sum_bits = 6 (the one-stage kernel is
+ * opt-in: on the measured workloads its tiles' 64-byte output runs were no faster than
+ * the VALU path's whole-line stores).  Results are identical on every path; the switch
  * exists to time one against the other.  Not thread-safe (process-wide setting). */
 int ts_dense_config(int32_t enable, int32_t min_sum_bits);
 

@@ -169,12 +169,32 @@ class PlannedContraction(Contraction):
         return result
 
     def contract_program(self, diagram: Diagram) -> _prog.Program:
-        """Compile the (unsliced) tree with the diagram's host leaves into one job."""
+        """
+        Compile the tree with the diagram's host leaves for the device step executors:
+        one job, or with sliced wires one job per slice (the sliced wires fixed in every
+        leaf that holds them, ``Box.__getitem__``; the jobs' results are to be OR-ed, or
+        summed when counting).  Every job has the tree's steps with the sliced wires
+        removed, in the compiler's layouts (contracted wires innermost in both operands,
+        so dense steps take the MFMA route).  Needs extent-2 wires, unique wires per slot
+        and a scalar result.
+        """
         self.validate(diagram)
-        if self._tree.sliced:
-            raise NotImplementedError("compile sliced trees with program.sat_batch / per slice")
+        if self.wiring.out_wires:
+            raise NotImplementedError("compiled programs of sliced trees need a scalar result")
         leaves = [np.asarray(getattr(b, "tensor", b), dtype=np.uint8) for b in diagram.blocks]
-        return _prog.compile_network(self.wiring, leaves, self._tree)
+        sliced = list(self._tree.sliced)
+        if not sliced:
+            return _prog.compile_network(self.wiring, leaves, self._tree)
+        w = self.wiring
+        slots = [tuple(x for x in ws if x not in set(sliced)) for ws in w.slot_wires_list]
+        sub = Wiring(w.wire_dims, slots, ())
+        progs = []
+        for vals in itertools.product(range(2), repeat=len(sliced)):
+            fixed = dict(zip(sliced, vals))
+            cut = [np.ascontiguousarray(t[tuple(fixed.get(x, slice(None)) for x in ws)])
+                   for t, ws in zip(leaves, w.slot_wires_list)]
+            progs.append(_prog.compile_network(sub, cut, self._tree))
+        return _prog.concat(progs, [np.zeros(1, np.int64)] * len(progs), 1)
 
     def __repr__(self) -> str:
         return "<PlannedContraction>"

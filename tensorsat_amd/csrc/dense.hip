@@ -231,8 +231,101 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
       }
 }
 
+// Short-K steps (0..5 summed bits, K = 1..32 bytes per row; opt-in, ts_dense_config):
+// one K stage, K zero-padded to 64 in LDS.  Such products are output-bound: one
+// 16 x 16 x 64 int8 MFMA gives 256 outputs every 16 cycles of a SIMD, about five times
+// the HBM write rate of the whole device, so the padding costs nothing that shows.  What
+// does show is the output stores: a tile writes runs of 64 bytes per row, where the step
+// engine's consecutive lanes write whole lines, and on the measured workloads (the
+// reference-style trees of n=40 instances, the n=120 cube programs) the step engine was
+// as fast or faster -- hence off by default.  Row r's K
+// bytes are read by thread r (rows are K-aligned: the K bits are the operand's lowest),
+// written to LDS at the swizzled segment position with zeros after them.
+template <int TM, int TN>
+__global__ __launch_bounds__(256) void k_step_mfma_short(const uint8_t* __restrict__ lhs, const uint8_t* __restrict__ rhs,
+                                                         uint8_t* __restrict__ out, DenseArgs a) {
+  constexpr int BM = 32 * TM, BN = 32 * TN;
+  __shared__ __attribute__((aligned(16))) uint8_t sA[BM][LDSW];
+  __shared__ __attribute__((aligned(16))) uint8_t sB[BN][LDSW];
+  __shared__ uint32_t rowO[BM], colO[BN];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  // Tile order as in k_step_mfma: a contiguous run of tile ids per XCD, groups of 8 tile
+  // rows walked column by column within a batch index.
+  const uint64_t nwg = gridDim.x;
+  uint64_t w = blockIdx.x;
+  if ((nwg & 7) == 0) w = (w & 7) * (nwg >> 3) + (w >> 3);
+  const uint64_t tiles = (uint64_t)a.mtiles * (uint64_t)a.ntiles;
+  const uint64_t b = w / tiles;
+  const uint64_t t = w % tiles;
+  constexpr uint64_t G = 8;
+  const uint64_t gspan = G * (uint64_t)a.ntiles;
+  const uint64_t first_m = (t / gspan) * G;
+  const uint64_t gsize = ((uint64_t)a.mtiles - first_m) < G ? ((uint64_t)a.mtiles - first_m) : G;
+  const uint64_t mt = first_m + (t % gspan) % gsize;
+  const uint64_t nt = (t % gspan) / gsize;
+  const uint64_t m0 = mt * BM, n0 = nt * BN;
+  const uint64_t mcount = 1ull << a.mbits, ncount = 1ull << a.nbits;
+  const uint32_t bL = scatter(b, a.bbits, a.b_lpos), bR = scatter(b, a.bbits, a.b_rpos);
+  const uint32_t bO = scatter(b, a.bbits, a.b_opos);
+  const int K = 1 << a.kbits;
+  // One row (of A, then of B) per thread and pass: its K bytes into registers (rows past
+  // the operand's end read its first row; their outputs are discarded), then the whole
+  // 64-byte LDS row, zeros after K.
+  for (int i = tid; i < BM + BN; i += 256) {
+    const bool isA = i < BM;
+    const int r = isA ? i : i - BM;
+    const uint64_t idx = (isA ? m0 : n0) + r;
+    const bool live = idx < (isA ? mcount : ncount);
+    uint32_t off = 0;
+    if (live) off = isA ? scatter(idx, a.mbits, a.m_lpos) : scatter(idx, a.nbits, a.n_rpos);
+    const uint8_t* src = (isA ? lhs + bL : rhs + bR) + off;
+    if (isA) rowO[r] = live ? scatter(idx, a.mbits, a.m_opos) : 0u;
+    else colO[r] = live ? scatter(idx, a.nbits, a.n_opos) : 0u;
+    i4 seg0 = (i4){0, 0, 0, 0}, seg1 = (i4){0, 0, 0, 0};
+    if (K >= 16) {
+      seg0 = *reinterpret_cast<const i4*>(src);
+      if (K == 32) seg1 = *reinterpret_cast<const i4*>(src + 16);
+    } else if (K == 8) {
+      const uint2 v = *reinterpret_cast<const uint2*>(src);
+      seg0 = (i4){(int)v.x, (int)v.y, 0, 0};
+    } else if (K == 4) {
+      seg0 = (i4){(int)*reinterpret_cast<const uint32_t*>(src), 0, 0, 0};
+    } else if (K == 2) {
+      seg0 = (i4){(int)*reinterpret_cast<const uint16_t*>(src), 0, 0, 0};
+    } else {
+      seg0 = (i4){(int)*src, 0, 0, 0};
+    }
+    uint8_t* row = isA ? &sA[r][0] : &sB[r][0];
+    const int sw = seg_swz(r);
+    *reinterpret_cast<i4*>(row + 16 * (0 ^ sw)) = seg0;
+    *reinterpret_cast<i4*>(row + 16 * (1 ^ sw)) = seg1;
+    *reinterpret_cast<i4*>(row + 16 * (2 ^ sw)) = (i4){0, 0, 0, 0};
+    *reinterpret_cast<i4*>(row + 16 * (3 ^ sw)) = (i4){0, 0, 0, 0};
+  }
+  __syncthreads();
+  const int wm = (wave >> 1) * 16 * TM, wn = (wave & 1) * 16 * TN;
+  const int rr = lane & 15, kk = ((lane >> 4) ^ seg_swz(rr)) * 16;
+  i4 fa[TM], fb[TN];
+  for (int i = 0; i < TM; ++i) fa[i] = *reinterpret_cast<const i4*>(&sA[wm + 16 * i + rr][kk]);
+  for (int j = 0; j < TN; ++j) fb[j] = *reinterpret_cast<const i4*>(&sB[wn + 16 * j + rr][kk]);
+  for (int i = 0; i < TM; ++i)
+    for (int j = 0; j < TN; ++j) {
+      const i4 acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[i], fb[j], (i4){0, 0, 0, 0}, 0, 0, 0);
+      // Accumulator layout: lane l holds rows 4 * (l / 16) + q, column l % 16.
+      for (int q = 0; q < 4; ++q) {
+        const int r = wm + 16 * i + 4 * (lane >> 4) + q;
+        const int col = wn + 16 * j + (lane & 15);
+        if (m0 + r >= mcount || n0 + col >= ncount) continue;
+        const int v = acc[q];
+        const uint8_t bit = a.mode == TS_MODE_MOD256 ? (uint8_t)((v & 255) != 0) : (uint8_t)(v > 0);
+        out[(uint64_t)bO + rowO[r] + colO[col]] = bit;
+      }
+    }
+}
+
 bool g_enabled = true;
 int g_min_kbits = 6;
+int g_short_min = 6;  // steps of g_short_min..5 summed bits take k_step_mfma_short (6: none)
 
 // Tile per step shape (mbits, nbits, bbits, kbits): 0 the built-in rule, 1 64x64,
 // 2 128x64, 3 64x128, 4 128x128, 5 256x128, 6 128x256 (rows x columns); set by the tile
@@ -260,7 +353,7 @@ int tile_for(int m, int n, int b, int k) {
 bool dense_eligible(const ts_step& st, const void* lhs, const void* rhs) {
   if (!g_enabled) return false;
   if ((st.flags & TS_STEP_NORHS) || !(st.flags & TS_STEP_KINNER)) return false;
-  if (st.sum_rank < g_min_kbits || st.sum_rank < 6) return false;
+  if (st.sum_rank < (st.sum_rank < 6 ? g_short_min : g_min_kbits)) return false;
   if ((reinterpret_cast<uintptr_t>(lhs) | reinterpret_cast<uintptr_t>(rhs)) & 15) return false;
   int mb = 0, nb = 0;
   for (int j = 0; j < st.out_rank; ++j) {
@@ -290,6 +383,23 @@ int launch_dense(const uint8_t* lhs, const uint8_t* rhs, uint8_t* out, const ts_
       a.n_rpos[a.nbits] = st.o_rpos[j];
       a.n_opos[a.nbits++] = (uint8_t)j;
     }
+  }
+  if (a.kbits < 6) {
+    // one K stage: tile by the built-in rule
+    const int rule = tile_rule(a.mbits, a.nbits);
+    const int64_t BM = rule == 6 ? 128 : rule == 4 ? 128 : 64, BN = rule == 6 ? 256 : rule == 4 ? 128 : 64;
+    a.mtiles = ((1ll << a.mbits) + BM - 1) / BM;
+    a.ntiles = ((1ll << a.nbits) + BN - 1) / BN;
+    const int64_t blocks = a.mtiles * a.ntiles * (1ll << a.bbits);
+    if (blocks > (1ll << 24)) return TS_EUNSUPPORTED;
+    const dim3 grid((unsigned)blocks), block(256);
+    if (rule == 6)
+      hipLaunchKernelGGL((k_step_mfma_short<4, 8>), grid, block, 0, s, lhs, rhs, out, a);
+    else if (rule == 4)
+      hipLaunchKernelGGL((k_step_mfma_short<4, 4>), grid, block, 0, s, lhs, rhs, out, a);
+    else
+      hipLaunchKernelGGL((k_step_mfma_short<2, 2>), grid, block, 0, s, lhs, rhs, out, a);
+    return hipGetLastError() == hipSuccess ? TS_OK : TS_EHIP;
   }
   // Tile from the autotuned table, else the built-in rule (tile_rule)
   // entries and 64 x 64 otherwise.
@@ -335,8 +445,9 @@ extern "C" int ts_dense_tile_of(int32_t mbits, int32_t nbits, int32_t bbits, int
 }
 
 extern "C" int ts_dense_config(int32_t enable, int32_t min_sum_bits) {
-  if (min_sum_bits < 6 || min_sum_bits > TS_MAX_BITS) return TS_EINVAL;
+  if (min_sum_bits < 0 || min_sum_bits > TS_MAX_BITS) return TS_EINVAL;
   tsd::g_enabled = enable != 0;
-  tsd::g_min_kbits = min_sum_bits;
+  tsd::g_short_min = min_sum_bits < 6 ? min_sum_bits : 6;
+  tsd::g_min_kbits = min_sum_bits < 6 ? 6 : min_sum_bits;
   return TS_OK;
 }

@@ -175,6 +175,30 @@ def order_cost(wiring: Wiring, order) -> tuple[_plan.Plan, _plan.Cost]:
     return _plan.Plan(S, path[: max(0, 2 * (S - 1))].reshape(-1, 2), cost=cost), cost
 
 
+def step_work(steps: np.ndarray, min_sum_bits: int = 6) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """
+    Per step record: (takes the int8 MFMA route, operations 2^(out + summed + 1), bytes
+    2^lhs + 2^rhs + 2^out).  The route rule is the executor's (``ts_dense_config``'s
+    ``min_sum_bits``, 6 by default): a binary step whose summed bits are innermost in both
+    operands (TS_STEP_KINNER), with at least ``min_sum_bits`` summed bits and at least 4
+    out bits held by only one operand on each side.
+    """
+    out_rank = steps["out_rank"].astype(np.int64)
+    sum_rank = steps["sum_rank"].astype(np.int64)
+    binary = (steps["flags"] & 2) == 0
+    kinner = (steps["flags"] & STEP_KINNER) != 0
+    bit = np.arange(MAX_BITS)[None, :] < out_rank[:, None]
+    has_l = steps["o_lpos"] != NOPOS
+    has_r = steps["o_rpos"] != NOPOS
+    mb = (bit & has_l & ~has_r).sum(axis=1)
+    nb = (bit & has_r & ~has_l).sum(axis=1)
+    dense = binary & kinner & (sum_rank >= min_sum_bits) & (mb >= 4) & (nb >= 4)
+    ops = np.ldexp(1.0, out_rank + sum_rank + 1)
+    rhs = np.where(binary, np.ldexp(1.0, steps["rhs_rank"].astype(np.int64)), 0.0)
+    nbytes = np.ldexp(1.0, steps["lhs_rank"].astype(np.int64)) + rhs + np.ldexp(1.0, out_rank)
+    return dense, ops, nbytes
+
+
 def compile_network(wiring: Wiring, leaves, plan: _plan.Plan) -> Program:
     """
     Compile a bit network (every wire extent 2, unique wires per slot, unique out wires)

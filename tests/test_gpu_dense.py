@@ -1,6 +1,7 @@
 """
 GPU parity of the int8 MFMA route for dense Boolean steps (ts_contract2 with the summed
-wires innermost in both operands): bit-exact against the oracle in both epilogues, over
+wires innermost in both operands; 6 or more summed bits on the pipelined kernel, 0..5 on
+the one-stage kernel): bit-exact against the oracle in both epilogues, over
 interleaved row / column / batch bits, partial tiles, sparse and all-ones operands (the
 count 256 and 512 cases where MOD256 differs from GT0), and against the VALU route on
 larger shapes.
@@ -61,6 +62,14 @@ CASES = [
     (7, 8, 7, 0, 0.3),
     (8, 7, 7, 0, 1.0),   # 128 x 128 tiles, every count 256
     (12, 7, 8, 0, 0.05),  # 64 K stages: several unrolled loop trips, then the tail
+    # one-stage kernel: K of 1..32 bytes zero-padded to 64
+    (0, 4, 4, 0, 0.5),   # outer product
+    (1, 5, 6, 2, 0.5),
+    (2, 7, 7, 1, 0.5),   # 128 x 128 tiles
+    (3, 6, 5, 3, 0.4),   # partial tiles, interleaved batch bits
+    (4, 10, 11, 0, 0.3),  # 128 x 256 tiles
+    (5, 8, 9, 1, 1.0),   # every count 32
+    (5, 4, 4, 4, 0.05),  # sparse: many zero counts
 ]
 
 
@@ -71,12 +80,19 @@ def test_dense_route_matches_oracle(case, mode):
     kb, mb, nb, bb, dens = CASES[case]
     lhs, lw, rhs, rw, ow = _case(rng, kb, mb, nb, bb, dens)
     print(f"case {CASES[case]} {mode}", flush=True)
-    got = _run(lhs, lw, rhs, rw, ow, mode)
+    from tensorsat_amd import _native as N
+
+    if kb < 6:  # the one-stage kernel is opt-in: route the step there for the test
+        N.check(N.lib().ts_dense_config(1, 0), "ts_dense_config")
+    try:
+        got = _run(lhs, lw, rhs, rw, ow, mode)
+    finally:
+        N.check(N.lib().ts_dense_config(1, 6), "ts_dense_config")
     ref = F.contract2(lhs, lw, rhs, rw, ow, mode)
     np.testing.assert_array_equal(got, ref)
 
 
-@pytest.mark.parametrize("shape", [(12, 8, 8, 0), (10, 7, 7, 3), (6, 10, 10, 0)])
+@pytest.mark.parametrize("shape", [(12, 8, 8, 0), (10, 7, 7, 3), (6, 10, 10, 0), (4, 10, 10, 2), (2, 11, 9, 1)])
 def test_dense_route_matches_valu_route(shape):
     from tensorsat_amd import _native as N
 
@@ -84,7 +100,11 @@ def test_dense_route_matches_valu_route(shape):
     lhs, lw, rhs, rw, ow = _case(rng, *shape, 2.0 ** (-shape[0] / 2))
     print(f"shape {shape}", flush=True)
     for mode in ("GT0", "MOD256"):
-        mfma = _run(lhs, lw, rhs, rw, ow, mode)
+        N.check(N.lib().ts_dense_config(1, 0), "ts_dense_config")  # every eligible step on MFMA
+        try:
+            mfma = _run(lhs, lw, rhs, rw, ow, mode)
+        finally:
+            N.check(N.lib().ts_dense_config(1, 6), "ts_dense_config")
         N.check(N.lib().ts_dense_config(0, 6), "ts_dense_config")
         try:
             valu = _run(lhs, lw, rhs, rw, ow, mode)
