@@ -140,6 +140,20 @@ int ts_program_run_fused(const ts_job* job_host, const ts_step* steps_host, cons
                          const uint8_t* leaves_dev, uint8_t* arena, int32_t mode, int32_t small_bits,
                          ts_stream stream);
 
+/* As ts_program_run_fused for `count` jobs at once (jobs_host: their records, contiguous
+ * on the host), job g in arena + g * arena_stride (arena_stride a multiple of 256 and at
+ * least every job's arena_elems): the jobs advance phase by phase -- phase p is each
+ * job's p-th segment, a run of small steps or one large step -- with one launch per
+ * phase for all the runs (one workgroup per job), one for the large steps (blocks shared
+ * out among the jobs; steps with few outputs and many summed terms get one workgroup
+ * per output) and one per MFMA-routed step.  results[g] (device) = 1 if any entry of job
+ * g's result is nonzero, else 0.  The slices of a sliced tree (cotengra.py:100-128) and
+ * the cubes of one instance are gangs: their small steps, one workgroup each when run
+ * alone, share the device.  Enqueued on `stream`; no allocation. */
+int ts_program_run_gang(const ts_job* jobs_host, int32_t count, const ts_step* steps_host,
+                        const ts_step* steps_dev, const uint8_t* leaves_dev, uint8_t* arena, int64_t arena_stride,
+                        uint8_t* results, int32_t mode, int32_t small_bits, ts_stream stream);
+
 /* Counting variant: arena of uint64; result scaled by 2^free_vars is added to *count
  * (device uint64); overflow sets *overflow. */
 int ts_program_count(const ts_job* job_host, const ts_step* steps_host, const uint8_t* leaves_dev,

@@ -37,7 +37,7 @@ EXPORTS = (
     "ts_program_run", "ts_program_count", "ts_batch_run", "ts_batch_max_workgroups",
     "ts_batch_compact", "ts_batch_count", "ts_batch_max_workgroups_lds", "ts_gnn_gemm", "ts_gnn_segsum", "ts_gnn_relu_back",
     "ts_gnn_head", "ts_gnn_adam", "ts_gnn_cast_bf16", "ts_dense_config", "ts_dense_tiles", "ts_dense_tile_of",
-    "ts_search_run", "ts_search_run_queue", "ts_search_decide_noalloc", "ts_program_run_fused",
+    "ts_search_run", "ts_search_run_queue", "ts_search_decide_noalloc", "ts_program_run_fused", "ts_program_run_gang",
 )
 
 _lib = None
@@ -82,6 +82,7 @@ def load() -> ctypes.CDLL:
         L.ts_program_run.argtypes = [vp, vp, vp, vp, i32, vp, vp, vp]
         L.ts_program_count.argtypes = [vp, vp, vp, vp, vp, vp, vp]
         L.ts_program_run_fused.argtypes = [vp, vp, vp, vp, vp, i32, i32, vp]
+        L.ts_program_run_gang.argtypes = [vp, i32, vp, vp, vp, vp, i64, vp, i32, i32, vp]
         L.ts_batch_run.argtypes = [P(Batch), vp]
         L.ts_batch_compact.argtypes = [vp, i32, vp, vp, vp, vp, vp]
         L.ts_batch_count.argtypes = [P(Batch), vp, vp, vp]

@@ -40,19 +40,93 @@ __global__ __launch_bounds__(WG) void k_step_u8(const uint8_t* __restrict__ lhs,
   step_u8(st, L, lhs, rhs, out, (uint64_t)blockIdx.x * WG + threadIdx.x, (uint64_t)gridDim.x * WG, mode);
 }
 
+// Steps with few outputs and many summed terms (a scalar closing a contraction tree, a
+// reduction to a few bits): one workgroup per output, its 2^sum_rank terms split over
+// the threads (16-byte runs when the summed bits are innermost, chunk tables otherwise:
+// L.cl / L.cr built by the caller), the partial counts added across the workgroup.
+// Counts are exact in uint32 (at most 2^27 terms) before the epilogue.
+__device__ void dot_block(const ts_step& st, const Lut& L, const uint8_t* __restrict__ lhs,
+                          const uint8_t* __restrict__ rhs, uint8_t* __restrict__ out, uint32_t o, int mode,
+                          uint32_t* part) {
+  const int tid = threadIdx.x;
+  const bool kinner = st.flags & TS_STEP_KINNER, norhs = st.flags & TS_STEP_NORHS;
+  uint32_t offl = 0, offr = 0;
+  for (int j = 0; j < st.out_rank; ++j)
+    if ((o >> j) & 1u) {
+      if (st.o_lpos[j] != TS_NOPOS) offl |= 1u << st.o_lpos[j];
+      if (st.o_rpos[j] != TS_NOPOS) offr |= 1u << st.o_rpos[j];
+    }
+  const uint32_t nterm = 1u << st.sum_rank;
+  uint32_t acc = 0;
+  if (kinner) {
+    const uint4* a = reinterpret_cast<const uint4*>(lhs + offl);
+    const uint4* b = reinterpret_cast<const uint4*>(rhs + offr);
+    for (uint32_t i = tid; i < (nterm >> 4); i += WG) {
+      uint4 x = a[i];
+      if (!norhs) {
+        const uint4 y = b[i];
+        x.x &= y.x;
+        x.y &= y.y;
+        x.z &= y.z;
+        x.w &= y.w;
+      }
+      acc += __popc(x.x) + __popc(x.y) + __popc(x.z) + __popc(x.w);
+    }
+  } else {
+    const int nqc = nchunks(st.sum_rank);
+    for (uint32_t c = tid; c < nterm; c += WG) {
+      const uint8_t x = lhs[offl + c_off<4>(L.cl, c, nqc)];
+      const uint8_t y = norhs ? (uint8_t)1 : rhs[offr + c_off<4>(L.cr, c, nqc)];
+      acc += x & y;
+    }
+  }
+  for (int m = 32; m >= 1; m >>= 1) acc += __shfl_xor(acc, m);
+  if ((tid & 63) == 0) part[tid >> 6] = acc;
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t t = 0;
+    for (int w = 0; w < WG / 64; ++w) t += part[w];
+    out[o] = (uint8_t)(mode == TS_MODE_MOD256 ? ((t & 255u) != 0) : (t != 0));
+  }
+}
+
+// Rule shared by the host dispatch: at least 2^8 terms per output, at most 2^12 outputs,
+// summed bits innermost (16-byte runs) or not.
+inline bool dot_shaped(const ts_step& st) { return st.sum_rank >= 8 && st.out_rank <= 12; }
+
+__global__ __launch_bounds__(WG) void k_step_dot(const uint8_t* __restrict__ lhs, const uint8_t* __restrict__ rhs,
+                                                 uint8_t* __restrict__ out, ts_step st, int mode) {
+  __shared__ Lut L;
+  __shared__ uint32_t part[WG / 64];
+  if (!(st.flags & TS_STEP_KINNER)) {
+    build_lut(st, L, true, threadIdx.x);
+    __syncthreads();
+  }
+  dot_block(st, L, lhs, rhs, out, blockIdx.x, mode, part);
+}
+
+// Descriptor structs passed by value are read through the kernel-argument segment
+// (uniform scalar loads, no private copy of their arrays).
+template <class T>
+__device__ __forceinline__ const T& kernarg() {
+  return *(const T*)__builtin_amdgcn_kernarg_segment_ptr();  // constant to generic address space
+}
+
 // A run of consecutive small steps of one job on one workgroup: each step's record is
 // staged in LDS, its chunk tables built, and its outputs written to the job arena before
 // the next step reads them (workgroup barriers between steps).  Saves one launch per
 // step where steps are too small to fill the device.
-__global__ __launch_bounds__(WG) void k_step_seq(uint8_t* __restrict__ arena, const ts_step* __restrict__ steps,
-                                                 int begin, int end, int mode) {
-  __shared__ Lut L;
-  __shared__ ts_step s_st;
+// The next record is loaded while the current step runs.
+__device__ void seq_run(uint8_t* __restrict__ arena, const ts_step* __restrict__ steps, int n, int mode, Lut& L,
+                        ts_step& s_st) {
+  constexpr int RW = (int)(sizeof(ts_step) / 16);
   const int tid = threadIdx.x;
-  for (int i = begin; i < end; ++i) {
-    if (tid < (int)(sizeof(ts_step) / 16))
-      reinterpret_cast<uint4*>(&s_st)[tid] = reinterpret_cast<const uint4*>(steps + i)[tid];
+  uint4 r = {0u, 0u, 0u, 0u};
+  if (tid < RW && n > 0) r = reinterpret_cast<const uint4*>(steps)[tid];
+  for (int i = 0; i < n; ++i) {
+    if (tid < RW) reinterpret_cast<uint4*>(&s_st)[tid] = r;
     __syncthreads();
+    if (tid < RW && i + 1 < n) r = reinterpret_cast<const uint4*>(steps + i + 1)[tid];
     const bool need_c = !(s_st.flags & TS_STEP_KINNER) && s_st.sum_rank > 0;
     build_lut(s_st, L, need_c, tid);
     __syncthreads();
@@ -60,6 +134,112 @@ __global__ __launch_bounds__(WG) void k_step_seq(uint8_t* __restrict__ arena, co
     step_u8(s_st, L, arena + s_st.lhs_off, rhs, arena + s_st.out_off, (uint64_t)tid, (uint64_t)WG, mode);
     __syncthreads();
   }
+}
+
+__global__ __launch_bounds__(WG) void k_step_seq(uint8_t* __restrict__ arena, const ts_step* __restrict__ steps,
+                                                 int begin, int end, int mode) {
+  __shared__ Lut L;
+  __shared__ ts_step s_st;
+  seq_run(arena, steps + begin, end - begin, mode, L, s_st);
+}
+
+// ---- gangs: several jobs of one program advanced together --------------------------
+//
+// Each job has its own arena; a launch serves up to GANG jobs, its descriptors passed by
+// value (kernarg above).
+
+constexpr int GANG = 64;
+
+struct GangSeq {  // one workgroup per job: a run of small steps
+  const ts_step* steps[GANG];
+  uint8_t* arena[GANG];
+  int32_t count[GANG];
+  int32_t mode;
+};
+
+// kind[j]: 0 = step_u8 over the job's blocks (grid-stride), 1 = one block per output
+// (few outputs, many terms).
+struct GangStep {
+  const ts_step* rec[GANG];
+  uint8_t* arena[GANG];
+  uint32_t blk[GANG + 1];
+  uint8_t kind[GANG];
+  int32_t n, mode;
+};
+
+struct GangCopy {
+  const uint8_t* src[GANG];
+  uint8_t* dst[GANG];
+  int64_t bytes[GANG];
+};
+
+struct GangAny {
+  const uint8_t* res[GANG];
+  int64_t bytes[GANG];
+  uint8_t* out;
+};
+
+__global__ __launch_bounds__(WG) void k_gang_seq(GangSeq) {
+  __shared__ Lut L;
+  __shared__ ts_step s_st;
+  const GangSeq& g = kernarg<GangSeq>();
+  const int j = blockIdx.x;
+  seq_run(g.arena[j], g.steps[j], g.count[j], g.mode, L, s_st);
+}
+
+__global__ __launch_bounds__(WG) void k_gang_step(GangStep) {
+  __shared__ Lut L;
+  __shared__ ts_step s_st;
+  __shared__ uint32_t part[WG / 64];
+  const GangStep& g = kernarg<GangStep>();
+  const uint32_t b = blockIdx.x;
+  int j = 0;
+  while (j + 1 < g.n && g.blk[j + 1] <= b) ++j;
+  const int tid = threadIdx.x;
+  if (tid < (int)(sizeof(ts_step) / 16))
+    reinterpret_cast<uint4*>(&s_st)[tid] = reinterpret_cast<const uint4*>(g.rec[j])[tid];
+  __syncthreads();
+  uint8_t* arena = g.arena[j];
+  const bool kinner = s_st.flags & TS_STEP_KINNER;
+  const uint8_t* rhs = arena + ((s_st.flags & TS_STEP_NORHS) ? s_st.lhs_off : s_st.rhs_off);
+  const uint32_t first = b - g.blk[j], nb = g.blk[j + 1] - g.blk[j];
+  if (g.kind[j]) {
+    if (!kinner) {
+      build_lut(s_st, L, true, tid);
+      __syncthreads();
+    }
+    dot_block(s_st, L, arena + s_st.lhs_off, rhs, arena + s_st.out_off, first, g.mode, part);
+    return;
+  }
+  build_lut(s_st, L, !kinner && s_st.sum_rank > 0, tid);
+  __syncthreads();
+  step_u8(s_st, L, arena + s_st.lhs_off, rhs, arena + s_st.out_off, (uint64_t)first * WG + tid, (uint64_t)nb * WG,
+          g.mode);
+}
+
+// grid (x: 4 KiB chunks, y: job)
+__global__ __launch_bounds__(WG) void k_gang_copy(GangCopy) {
+  const GangCopy& g = kernarg<GangCopy>();
+  const int j = blockIdx.y;
+  const int64_t n = g.bytes[j];
+  const uint8_t* src = g.src[j];
+  uint8_t* dst = g.dst[j];
+  for (int64_t i = (int64_t)blockIdx.x * WG + threadIdx.x; i < n; i += (int64_t)gridDim.x * WG) dst[i] = src[i];
+}
+
+// One workgroup per job: out[j] = 1 if any byte of the job's result is nonzero.
+__global__ __launch_bounds__(WG) void k_gang_any(GangAny) {
+  __shared__ int any;
+  const GangAny& g = kernarg<GangAny>();
+  const int j = blockIdx.x;
+  if (threadIdx.x == 0) any = 0;
+  __syncthreads();
+  const uint8_t* r = g.res[j];
+  int v = 0;
+  for (int64_t i = threadIdx.x; i < g.bytes[j]; i += WG) v |= r[i];
+  if (v) any = 1;
+  __syncthreads();
+  if (threadIdx.x == 0) g.out[j] = (uint8_t)(any != 0);
 }
 
 __global__ __launch_bounds__(WG) void k_step_u64(const uint64_t* __restrict__ lhs,
@@ -577,6 +757,10 @@ int make_step(int rl, const int32_t* lw, int rr, const int32_t* rw, int ro, cons
 int launch_step_u8(const uint8_t* lhs, const uint8_t* rhs, uint8_t* out, const ts_step& st, int mode,
                    hipStream_t s) {
   if (tsd::dense_eligible(st, lhs, rhs)) return tsd::launch_dense(lhs, rhs, out, st, mode, s);
+  if (dot_shaped(st)) {
+    hipLaunchKernelGGL(k_step_dot, dim3(1u << st.out_rank), dim3(WG), 0, s, lhs, rhs, out, st, mode);
+    return hip_rc(hipGetLastError());
+  }
   hipLaunchKernelGGL(k_step_u8, dim3(grid_for(1ull << st.out_rank)), dim3(WG), 0, s, lhs, rhs, out, st, mode);
   return hip_rc(hipGetLastError());
 }
@@ -966,6 +1150,164 @@ int ts_program_run_fused(const ts_job* job, const ts_step* steps, const ts_step*
     const int rc = launch_step_u8(arena + st.lhs_off, rhs, arena + st.out_off, st, mode, s);
     if (rc) return rc;
     ++i;
+  }
+  return TS_OK;
+}
+
+int ts_program_run_gang(const ts_job* jobs, int32_t count, const ts_step* steps, const ts_step* steps_dev,
+                        const uint8_t* leaves, uint8_t* arena, int64_t arena_stride, uint8_t* results, int32_t mode,
+                        int32_t small_bits, ts_stream stream) {
+  if (!jobs || count < 0 || !arena || !leaves || !results) return TS_EINVAL;
+  if (mode != TS_MODE_GT0 && mode != TS_MODE_MOD256) return TS_EINVAL;
+  if (small_bits < 0 || small_bits > TS_MAX_BITS || arena_stride <= 0 || (arena_stride & 255)) return TS_EINVAL;
+  if (count == 0) return TS_OK;
+  auto small = [&](const ts_step& st) {
+    const int r = std::max<int>({st.lhs_rank, (st.flags & TS_STEP_NORHS) ? 0 : st.rhs_rank, st.out_rank});
+    return r <= small_bits;
+  };
+  // Segments per job: [begin, end) runs of small steps (kind 0) or one large step (kind 1).
+  struct Seg {
+    int begin, end, kind;
+  };
+  std::vector<std::vector<Seg>> segs(count);
+  size_t phases = 0;
+  for (int g = 0; g < count; ++g) {
+    const ts_job& job = jobs[g];
+    if (job.num_steps > 0 && !(steps && steps_dev)) return TS_EINVAL;
+    if (job.arena_elems > arena_stride || job.num_leaf_bytes > job.arena_elems || job.num_leaf_bytes < 0)
+      return TS_EINVAL;
+    if (job.result_rank < 0 || job.result_rank > TS_MAX_BITS ||
+        job.result_off + (1ll << job.result_rank) > job.arena_elems)
+      return TS_EINVAL;
+    const ts_step* rec = steps + job.step_begin;
+    for (int i = 0; i < job.num_steps; ++i) {
+      const ts_step& st = rec[i];
+      const uint64_t hi = std::max<uint64_t>({st.lhs_off + (1ull << st.lhs_rank),
+                                               (st.flags & TS_STEP_NORHS) ? 0 : st.rhs_off + (1ull << st.rhs_rank),
+                                               st.out_off + (1ull << st.out_rank)});
+      if (hi > (uint64_t)job.arena_elems) return TS_EINVAL;
+    }
+    int i = 0;
+    while (i < job.num_steps) {
+      if (small(rec[i])) {
+        int k = i + 1;
+        while (k < job.num_steps && small(rec[k])) ++k;
+        segs[g].push_back({i, k, 0});
+        i = k;
+      } else {
+        segs[g].push_back({i, i + 1, 1});
+        ++i;
+      }
+    }
+    phases = std::max(phases, segs[g].size());
+  }
+  hipStream_t s = S(stream);
+  auto job_arena = [&](int g) { return arena + (int64_t)g * arena_stride; };
+  for (int g0 = 0; g0 < count; g0 += GANG) {
+    const int n = std::min(GANG, count - g0);
+    GangCopy c;
+    std::memset(&c, 0, sizeof(c));
+    int64_t most = 0;
+    for (int k = 0; k < n; ++k) {
+      c.src[k] = leaves + jobs[g0 + k].leaf_begin;
+      c.dst[k] = job_arena(g0 + k);
+      c.bytes[k] = jobs[g0 + k].num_leaf_bytes;
+      most = std::max(most, c.bytes[k]);
+    }
+    if (most > 0) {
+      const int gx = (int)std::min<int64_t>((most + 4095) / 4096, 1024);
+      hipLaunchKernelGGL(k_gang_copy, dim3(gx, n), dim3(WG), 0, s, c);
+      if (hipGetLastError() != hipSuccess) return TS_EHIP;
+    }
+  }
+  for (size_t p = 0; p < phases; ++p) {
+    // runs of small steps: one workgroup per job
+    GangSeq q;
+    int nq = 0;
+    auto flush_seq = [&]() {
+      if (!nq) return TS_OK;
+      q.mode = mode;
+      hipLaunchKernelGGL(k_gang_seq, dim3(nq), dim3(WG), 0, s, q);
+      nq = 0;
+      return hip_rc(hipGetLastError());
+    };
+    // large steps: MFMA-routed ones alone, the others shared out by block ranges
+    GangStep t;
+    int nt = 0;
+    uint64_t want = 0;
+    std::vector<uint64_t> wants;
+    auto flush_step = [&]() {
+      if (!nt) return TS_OK;
+      // at most 65536 blocks per launch, every job keeping at least one (grid-stride
+      // loops take any count); one-block-per-output jobs keep theirs
+      const uint64_t cap = 65536;
+      uint32_t at = 0;
+      for (int k = 0; k < nt; ++k) {
+        t.blk[k] = at;
+        uint64_t b = wants[k];
+        if (!t.kind[k] && want > cap) b = std::max<uint64_t>(1, b * cap / want);
+        at += (uint32_t)b;
+      }
+      t.blk[nt] = at;
+      t.n = nt;
+      t.mode = mode;
+      hipLaunchKernelGGL(k_gang_step, dim3(at), dim3(WG), 0, s, t);
+      nt = 0;
+      want = 0;
+      wants.clear();
+      return hip_rc(hipGetLastError());
+    };
+    for (int g = 0; g < count; ++g) {
+      if (p >= segs[g].size()) continue;
+      const Seg& sg = segs[g][p];
+      const ts_step* rec = steps + jobs[g].step_begin;
+      const ts_step* rec_dev = steps_dev + jobs[g].step_begin;
+      uint8_t* ar = job_arena(g);
+      if (sg.kind == 0) {
+        q.steps[nq] = rec_dev + sg.begin;
+        q.arena[nq] = ar;
+        q.count[nq] = sg.end - sg.begin;
+        if (++nq == GANG) {
+          const int rc = flush_seq();
+          if (rc) return rc;
+        }
+        continue;
+      }
+      const ts_step& st = rec[sg.begin];
+      const uint8_t* lhs = ar + st.lhs_off;
+      const uint8_t* rhs = ar + ((st.flags & TS_STEP_NORHS) ? st.lhs_off : st.rhs_off);
+      if (tsd::dense_eligible(st, lhs, rhs)) {
+        const int rc = tsd::launch_dense(lhs, rhs, ar + st.out_off, st, mode, s);
+        if (rc) return rc;
+        continue;
+      }
+      t.rec[nt] = rec_dev + sg.begin;
+      t.arena[nt] = ar;
+      t.kind[nt] = dot_shaped(st) ? 1 : 0;
+      const uint64_t b = t.kind[nt] ? (1ull << st.out_rank) : (uint64_t)grid_for(1ull << st.out_rank);
+      wants.push_back(b);
+      if (!t.kind[nt]) want += b;
+      if (++nt == GANG) {
+        const int rc = flush_step();
+        if (rc) return rc;
+      }
+    }
+    int rc = flush_seq();
+    if (rc) return rc;
+    rc = flush_step();
+    if (rc) return rc;
+  }
+  for (int g0 = 0; g0 < count; g0 += GANG) {
+    const int n = std::min(GANG, count - g0);
+    GangAny a;
+    std::memset(&a, 0, sizeof(a));
+    for (int k = 0; k < n; ++k) {
+      a.res[k] = job_arena(g0 + k) + jobs[g0 + k].result_off;
+      a.bytes[k] = 1ll << jobs[g0 + k].result_rank;
+    }
+    a.out = results + g0;
+    hipLaunchKernelGGL(k_gang_any, dim3(n), dim3(WG), 0, s, a);
+    if (hipGetLastError() != hipSuccess) return TS_EHIP;
   }
   return TS_OK;
 }

@@ -110,6 +110,39 @@ def run_job_fused(dp: DeviceProgram, j: int, mode: str = "GT0", *, arena=None, s
     return arena[r0 : r0 + (1 << rr)]
 
 
+def gang_stride(dp: DeviceProgram) -> int:
+    """Bytes per job arena in a gang: the largest job arena, rounded up to 256."""
+    return (max(16, int(dp.jobs_host["arena_elems"].max())) + 255) & ~255
+
+
+def run_gang(dp: DeviceProgram, first: int, count: int, mode: str = "GT0", *, arena=None, results=None,
+             small_bits: int = 12):
+    """Run jobs ``first .. first+count-1`` together with ``ts_program_run_gang`` on the
+    current stream; returns a uint8 tensor of ``count`` flags (1: the job's result has a
+    nonzero entry).  ``arena``: uint8 tensor of at least ``count * gang_stride(dp)``
+    bytes, else a new one; ``results``: uint8 tensor of at least ``count`` entries."""
+    torch = _torch()
+    L = N.lib()
+    if first < 0 or count < 0 or first + count > len(dp.jobs_host):
+        raise ValueError("job range out of the program")
+    stride = gang_stride(dp)
+    if arena is None:
+        arena = torch.empty(max(1, count) * stride, dtype=torch.uint8, device=dp.device)
+    elif arena.numel() < count * stride or arena.dtype != torch.uint8:
+        raise ValueError("arena too small for the gang")
+    if results is None:
+        results = torch.empty(max(1, count), dtype=torch.uint8, device=dp.device)
+    elif results.numel() < count or results.dtype != torch.uint8:
+        raise ValueError("results too small for the gang")
+    N.check(
+        L.ts_program_run_gang(dp.job_ptr(first), count, N.host_ptr(dp.steps_host), N.dptr(dp.steps),
+                              N.dptr(dp.leaves), N.dptr(arena), stride, N.dptr(results), N.MODES[mode], small_bits,
+                              N.stream_ptr()),
+        "ts_program_run_gang",
+    )
+    return results[:count]
+
+
 def count_job(dp: DeviceProgram, j: int, acc=None, overflow=None):
     """Exact uint64 count of job ``j`` (times 2^free_vars) added into ``acc``."""
     torch = _torch()

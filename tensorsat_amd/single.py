@@ -3,11 +3,10 @@ One large instance on the whole GPU.
 
 * :func:`decide` — a SAT decision for one instance whose intermediates need the whole
   device (the n=120, m=510 workload: cubes of width up to 26, i.e. 2^26-entry tensors):
-  cubes in depth-first order, several at a time on their own HIP streams and arenas (the
-  small steps of one cube leave the device idle that the large steps of another fill),
-  each run with its small steps fused into single-workgroup launches and its large steps
-  grid-wide; results are read in cube order and the first satisfiable cube stops the
-  submission.
+  cubes in depth-first order, in gangs (16 cubes, then 32) that advance together phase
+  by phase: the runs of small steps of all the gang's cubes in one launch (one
+  workgroup per cube), their large steps in one grid-wide launch; the flags are read
+  after each gang and the first satisfiable cube stops the submission.
 * :func:`decide_budget` — a SAT decision under a wall-clock budget for instances whose
   cube enumeration cannot be completed (the learned-planner workload, n = 150..300):
   cubes are compiled in chunks of doubling size in depth-first order and each chunk is
@@ -34,7 +33,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import program as PG
-from .executor import BatchCounter, DeviceProgram, run_job
+from .executor import BatchCounter, DeviceProgram, gang_stride, run_gang, run_job
 from .sat import CNFInstance
 
 
@@ -49,25 +48,81 @@ class SingleResult:
 
 
 def decide(cnf: CNFInstance, *, width: int = 26, mode: str = "GT0", plan_trials: int = 2,
-           program: PG.Program | None = None, streams: int = 4, small_bits: int = 12) -> SingleResult:
-    """See the module docstring.  ``stats["cubes_launched"]`` counts the cubes that ran
-    (up to ``streams - 1`` past the deciding one); ``cubes_run`` those up to it."""
-    import collections
-
+           program: PG.Program | None = None, streams: int = 4, small_bits: int = 12, gang: int = 32,
+           first_gang: int = 16, lookahead: bool = False) -> SingleResult:
+    """See the module docstring.  With ``gang`` > 0 the cubes (or slices) run in gangs
+    (``ts_program_run_gang``) of ``first_gang`` jobs, doubling up to ``gang``; with
+    ``lookahead`` each gang is enqueued before the previous one's flags are read (the
+    device never waits for the host, at the price of one more gang past the deciding
+    one), else the flags are read between gangs.  ``gang=0`` runs the cubes one by one on
+    ``streams`` streams.  ``stats["cubes_launched"]`` counts the cubes that ran (the rest
+    of the deciding gang, and the gang after it with ``lookahead``); ``cubes_run`` those up
+    to the deciding one."""
     import torch
-
-    from .executor import run_job_fused
 
     t0 = time.perf_counter()
     prog = program if program is not None else PG.sat_batch([cnf], width=width, plan_trials=plan_trials)
     plan_s = time.perf_counter() - t0
     if len(prog.jobs) == 0:
         return SingleResult(0, plan_s, 0.0, 0, 0, {"host_unsat": True})
-    if streams < 1:
-        raise ValueError("streams must be >= 1")
+    if streams < 1 or gang < 0 or first_gang < 1:
+        raise ValueError("streams and first_gang must be >= 1, gang >= 0")
     dp = DeviceProgram(prog)
-    nj = len(prog.jobs)
-    size = max(16, int(prog.jobs["arena_elems"].max()))
+    stats = {"max_rank": int(prog.stats[0, 4]), "branch_nodes": int(prog.stats[0, 1])}
+    if gang == 0:
+        verdict, ran, launched, t = _decide_streams(dp, mode, streams, small_bits)
+        stats.update(cubes_launched=launched, streams=min(streams, len(prog.jobs)))
+    else:
+        verdict, ran, launched, t = _decide_gangs(dp, mode, gang, first_gang, small_bits, lookahead)
+        stats.update(cubes_launched=launched, gang=gang)
+    return SingleResult(verdict, plan_s, t, len(prog.jobs), ran, stats)
+
+
+def _decide_gangs(dp: DeviceProgram, mode: str, gang: int, first_gang: int, small_bits: int, lookahead: bool):
+    import collections
+
+    import torch
+
+    nj = len(dp.jobs_host)
+    size = min(first_gang, gang)
+    arena = torch.empty(min(gang, nj) * gang_stride(dp), dtype=torch.uint8, device=dp.device)
+    flags_dev = torch.zeros(nj, dtype=torch.uint8, device=dp.device)
+    flags = torch.zeros(nj, dtype=torch.uint8, pin_memory=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    verdict, ran, begin = 0, 0, 0
+    pending: collections.deque = collections.deque()
+    while begin < nj or pending:
+        if begin < nj and len(pending) < (2 if lookahead else 1):
+            n = min(size, nj - begin)
+            run_gang(dp, begin, n, mode, arena=arena, results=flags_dev[begin : begin + n], small_bits=small_bits)
+            flags[begin : begin + n].copy_(flags_dev[begin : begin + n], non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record()
+            pending.append((begin, n, ev))
+            begin += n
+            size = min(gang, 2 * size)
+            continue
+        b, n, ev = pending.popleft()
+        ev.synchronize()
+        hits = np.flatnonzero(flags[b : b + n].numpy())
+        if hits.size:
+            verdict, ran = 1, b + int(hits[0]) + 1
+            break
+        ran = b + n
+    torch.cuda.synchronize()
+    return verdict, ran, begin, time.perf_counter() - t0
+
+
+def _decide_streams(dp: DeviceProgram, mode: str, streams: int, small_bits: int):
+    import collections
+
+    import torch
+
+    from .executor import run_job_fused
+
+    nj = len(dp.jobs_host)
+    size = max(16, int(dp.jobs_host["arena_elems"].max()))
     lanes = [(torch.cuda.Stream(), torch.empty(size, dtype=torch.uint8, device=dp.device))
              for _ in range(min(streams, nj))]
     results = torch.zeros(nj, dtype=torch.uint8, device=dp.device)
@@ -96,9 +151,7 @@ def decide(cnf: CNFInstance, *, width: int = 26, mode: str = "GT0", plan_trials:
             verdict = 1
             break
     torch.cuda.synchronize()
-    return SingleResult(verdict, plan_s, time.perf_counter() - t0, nj, ran,
-                        {"max_rank": int(prog.stats[0, 4]), "branch_nodes": int(prog.stats[0, 1]),
-                         "cubes_launched": launched, "streams": len(lanes)})
+    return verdict, ran, launched, time.perf_counter() - t0
 
 
 def decide_budget(cnf: CNFInstance, *, width: int = 22, budget_s: float = 20.0, mode: str = "GT0",

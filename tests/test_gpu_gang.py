@@ -1,0 +1,91 @@
+"""
+GPU parity of gang execution (ts_program_run_gang): several jobs of one program advanced
+together phase by phase.  Each job's result, read from its arena, equals the oracle's
+byte for byte and its flag equals "the result has a nonzero entry", for gangs of cubes
+with different step structures (cubes of twelve n=70 instances, more than two launches'
+worth of jobs) and of slices with one structure (the n=40 greedy tree), across the small-step
+threshold; decisions through gangs equal DPLL.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import dpll
+from oracle import executor as OX
+from tensorsat_amd import program as PG
+from tensorsat_amd.sat import CNFInstance
+from tensorsat_amd.workloads import plain_cpu_3sat, single_3sat
+
+pytestmark = pytest.mark.gpu
+
+
+def _check_gang(prog, first, count, mode, small_bits, oracle_jobs):
+    import torch
+
+    from tensorsat_amd.executor import DeviceProgram, gang_stride, run_gang, run_job_fused
+
+    dp = DeviceProgram(prog)
+    stride = gang_stride(dp)
+    arena = torch.full((count * stride,), 5, dtype=torch.uint8, device="cuda")
+    flags = run_gang(dp, first, count, mode, arena=arena, small_bits=small_bits).cpu().numpy()
+    torch.cuda.synchronize()
+    jobs = prog.jobs
+    for g in range(count):
+        j = first + g
+        r0, rr = int(jobs["result_off"][j]), int(jobs["result_rank"][j])
+        got = arena[g * stride + r0 : g * stride + r0 + (1 << rr)].cpu().numpy()
+        one = run_job_fused(dp, j, mode, small_bits=12).cpu().numpy()
+        np.testing.assert_array_equal(got, one, err_msg=f"job {j}: gang vs one job at a time")
+        assert flags[g] == int(got.any()), f"job {j} flag"
+        if j in oracle_jobs:
+            ref, _ = OX.run_job(prog, j, mode)
+            np.testing.assert_array_equal(got, ref.reshape(-1), err_msg=f"job {j}: gang vs oracle")
+
+
+@pytest.mark.parametrize("small_bits", [12, 6, 0])
+@pytest.mark.parametrize("mode", ["GT0", "MOD256"])
+def test_gang_of_cubes_with_different_structures(mode, small_bits):
+    cnfs = [CNFInstance.random(3, 70, 290, rng=900 + i) for i in range(12)]
+    prog = PG.sat_batch(cnfs, width=14, plan_trials=1)
+    n = len(prog.jobs)
+    print(f"cubes {n} mode {mode} small_bits {small_bits}", flush=True)
+    assert n > 128 and np.unique(prog.jobs["num_steps"]).size > 10  # three launches' worth, mixed shapes
+    _check_gang(prog, 0, n, mode, small_bits, oracle_jobs={0, 1, 63, 64, 128, n - 1})
+
+
+@pytest.mark.parametrize("mode", ["GT0", "MOD256"])
+def test_gang_of_slices_with_one_structure(mode):
+    from tensorsat_amd.contractions import PlannedContraction
+    from tensorsat_amd.rel import DeviceRel
+
+    cnf = plain_cpu_3sat(0)
+    d = (cnf.inputs(None) >> cnf.diagram()).flatten()
+    c = PlannedContraction(DeviceRel, d.wiring, optimize="greedy", slicing_target=22)
+    prog = c.contract_program(d)
+    print(f"slices {len(prog.jobs)} sliced wires {len(c.tree.sliced)}", flush=True)
+    _check_gang(prog, 3, 70, mode, 12, oracle_jobs={3, 40, 72})
+
+
+@pytest.mark.parametrize("gang,lookahead", [(0, False), (4, True), (32, False), (64, True)])
+def test_gang_decisions_match_dpll(gang, lookahead):
+    from tensorsat_amd.single import decide
+
+    for seed in range(3):
+        cnf = single_3sat(seed)
+        r = decide(cnf, width=22, gang=gang, first_gang=min(8, gang) if gang else 8, lookahead=lookahead)
+        print(f"seed {seed} gang {gang}: verdict {r.value} cubes {r.cubes_run}/{r.cubes}", flush=True)
+        assert r.value == int(dpll.solve(cnf.num_vars, cnf.clauses))
+        assert r.cubes_run <= r.stats["cubes_launched"] <= r.cubes
+
+
+def test_gang_rejects_bad_arguments():
+    import torch
+
+    from tensorsat_amd.executor import DeviceProgram, run_gang
+
+    prog = PG.sat_batch([single_3sat(0)], width=12, max_cubes=4, plan_trials=1)
+    dp = DeviceProgram(prog)
+    with pytest.raises(ValueError):
+        run_gang(dp, 2, 10)
+    with pytest.raises(ValueError):
+        run_gang(dp, 0, 2, arena=torch.empty(16, dtype=torch.uint8, device="cuda"))
