@@ -151,6 +151,131 @@ __device__ __forceinline__ uint64_t load_row(const uint8_t* p, int nc) {
   return nc == 2 ? *reinterpret_cast<const uint32_t*>(p) : *reinterpret_cast<const uint64_t*>(p);
 }
 
+// ---- vector path for short rows ------------------------------------------------------
+//
+// When the low t out bits of a step are, in each operand, either the bits right above
+// its summed bits in order (a run: the rows of 2^t consecutive outputs are 2^t
+// consecutive rows) or not held at all (one row for all of them), a lane computes 2^t
+// consecutive outputs from one load per operand and one store.
+
+// t (0 = no vector path): at most 4 bits, at most 32 row bytes per operand and lane.
+__host__ __device__ inline int vec_bits(const ts_step& st) {
+  const int nc = st.sum_rank;
+  if (nc > 3 || !(nc == 0 || (st.flags & TS_STEP_KINNER))) return 0;
+  const bool norhs = st.flags & TS_STEP_NORHS;
+  int tmax = 5 - nc < 4 ? 5 - nc : 4;
+  if (tmax > st.out_rank) tmax = st.out_rank;
+  const bool lrun = st.o_lpos[0] != TS_NOPOS, rrun = !norhs && st.o_rpos[0] != TS_NOPOS;
+  int t = 0;
+  for (; t < tmax; ++t) {
+    const bool lok = lrun ? st.o_lpos[t] == nc + t : st.o_lpos[t] == TS_NOPOS;
+    const bool rok = norhs || (rrun ? st.o_rpos[t] == nc + t : st.o_rpos[t] == TS_NOPOS);
+    if (!lok || !rok) break;
+  }
+  return t;
+}
+
+template <int BYTES>
+__device__ __forceinline__ void load_bytes(uint64_t* w, const uint8_t* p) {
+  if constexpr (BYTES >= 16) {
+#pragma unroll
+    for (int i = 0; i < BYTES / 16; ++i) {
+      const uint4 v = reinterpret_cast<const uint4*>(p)[i];
+      w[2 * i] = ((uint64_t)v.y << 32) | v.x;
+      w[2 * i + 1] = ((uint64_t)v.w << 32) | v.z;
+    }
+  } else if constexpr (BYTES == 8) {
+    w[0] = *reinterpret_cast<const uint64_t*>(p);
+  } else if constexpr (BYTES == 4) {
+    w[0] = *reinterpret_cast<const uint32_t*>(p);
+  } else if constexpr (BYTES == 2) {
+    w[0] = *reinterpret_cast<const uint16_t*>(p);
+  } else {
+    w[0] = *p;
+  }
+}
+
+// Row i (2^K bytes) of a chunk held as 64-bit words.
+template <int K>
+__device__ __forceinline__ uint64_t row_of(const uint64_t* w, int i) {
+  if constexpr (K == 3) {
+    return w[i];
+  } else {
+    constexpr int PER = 8 >> K;  // rows per word
+    constexpr uint64_t MASK = (1ull << (8 << K)) - 1;
+    return (w[i / PER] >> (8 * ((i % PER) << K))) & MASK;
+  }
+}
+
+template <int K, int V, int NQ>
+__device__ void step_vec_kv(const ts_step& st, const LutT<NQ>& L, const uint8_t* __restrict__ lhs,
+                            const uint8_t* __restrict__ rhs, uint8_t* __restrict__ out, uint64_t first,
+                            uint64_t stride) {
+  constexpr int T = V == 2 ? 1 : V == 4 ? 2 : V == 8 ? 3 : 4;
+  constexpr int RUNB = V << K, ROWB = 1 << K;
+  constexpr int NW = (RUNB + 7) / 8;
+  const bool norhs = st.flags & TS_STEP_NORHS;
+  const bool lrun = st.o_lpos[0] != TS_NOPOS, rrun = !norhs && st.o_rpos[0] != TS_NOPOS;
+  const int nqo = nchunks(st.out_rank);
+  const uint64_t nq = 1ull << (st.out_rank - T);
+  if (first >= nq) return;
+  const uint32_t lo = (uint32_t)((first << T) & 255);  // the same for every q of this lane
+  const uint32_t lo_l = L.ol[0][lo], lo_r = L.orr[0][lo];
+  for (uint64_t q = first; q < nq; q += stride) {
+    const uint64_t o = q << T;
+    const uint32_t hi = (uint32_t)(o >> 8);
+    const uint8_t* pl = lhs + lo_l + hi_off<NQ>(L.ol, hi, nqo);
+    uint64_t a[NW], b[NW];
+    if (lrun) load_bytes<RUNB>(a, pl);
+    else load_bytes<ROWB>(a, pl);
+    if (!norhs) {
+      const uint8_t* pr = rhs + lo_r + hi_off<NQ>(L.orr, hi, nqo);
+      if (rrun) load_bytes<RUNB>(b, pr);
+      else load_bytes<ROWB>(b, pr);
+    }
+    uint32_t res[(V + 3) / 4];
+#pragma unroll
+    for (int i = 0; i < (V + 3) / 4; ++i) res[i] = 0;
+#pragma unroll
+    for (int i = 0; i < V; ++i) {
+      uint64_t x = row_of<K>(a, lrun ? i : 0);
+      if (!norhs) x &= row_of<K>(b, rrun ? i : 0);
+      res[i / 4] |= (uint32_t)(x != 0) << (8 * (i % 4));
+    }
+    uint8_t* po = out + o;
+    if constexpr (V == 16) *reinterpret_cast<uint4*>(po) = make_uint4(res[0], res[1], res[2], res[3]);
+    else if constexpr (V == 8) *reinterpret_cast<uint2*>(po) = make_uint2(res[0], res[1]);
+    else if constexpr (V == 4) *reinterpret_cast<uint32_t*>(po) = res[0];
+    else *reinterpret_cast<uint16_t*>(po) = (uint16_t)res[0];
+  }
+}
+
+// Grid-stride over groups of 2^vec_bits(st) outputs (first / stride as for step_u8).
+// Counts are at most 8, so both epilogues reduce to count != 0.
+template <int NQ>
+__device__ void step_u8_vec(const ts_step& st, const LutT<NQ>& L, const uint8_t* __restrict__ lhs,
+                            const uint8_t* __restrict__ rhs, uint8_t* __restrict__ out, uint64_t first,
+                            uint64_t stride, int t) {
+#define TS_VEC(K, V) step_vec_kv<K, V, NQ>(st, L, lhs, rhs, out, first, stride)
+  switch (st.sum_rank * 8 + t) {
+    case 0 * 8 + 1: TS_VEC(0, 2); break;
+    case 0 * 8 + 2: TS_VEC(0, 4); break;
+    case 0 * 8 + 3: TS_VEC(0, 8); break;
+    case 0 * 8 + 4: TS_VEC(0, 16); break;
+    case 1 * 8 + 1: TS_VEC(1, 2); break;
+    case 1 * 8 + 2: TS_VEC(1, 4); break;
+    case 1 * 8 + 3: TS_VEC(1, 8); break;
+    case 1 * 8 + 4: TS_VEC(1, 16); break;
+    case 2 * 8 + 1: TS_VEC(2, 2); break;
+    case 2 * 8 + 2: TS_VEC(2, 4); break;
+    case 2 * 8 + 3: TS_VEC(2, 8); break;
+    case 3 * 8 + 1: TS_VEC(3, 2); break;
+    case 3 * 8 + 2: TS_VEC(3, 4); break;
+    default: break;
+  }
+#undef TS_VEC
+}
+
 // Boolean step over out indices o = first + k * stride (first = block * 256 + tid and
 // stride = blocks * 256: the callers' grid-stride launch shape).
 template <int NQ>

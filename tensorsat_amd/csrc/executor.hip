@@ -37,7 +37,11 @@ __global__ __launch_bounds__(WG) void k_step_u8(const uint8_t* __restrict__ lhs,
   const bool need_c = !(st.flags & TS_STEP_KINNER) && st.sum_rank > 0;
   build_lut(st, L, need_c, threadIdx.x);
   __syncthreads();
-  step_u8(st, L, lhs, rhs, out, (uint64_t)blockIdx.x * WG + threadIdx.x, (uint64_t)gridDim.x * WG, mode);
+  const int t = vec_bits(st);
+  if (t > 0)
+    step_u8_vec(st, L, lhs, rhs, out, (uint64_t)blockIdx.x * WG + threadIdx.x, (uint64_t)gridDim.x * WG, t);
+  else
+    step_u8(st, L, lhs, rhs, out, (uint64_t)blockIdx.x * WG + threadIdx.x, (uint64_t)gridDim.x * WG, mode);
 }
 
 // Steps with few outputs and many summed terms (a scalar closing a contraction tree, a
@@ -213,8 +217,13 @@ __global__ __launch_bounds__(WG) void k_gang_step(GangStep) {
   }
   build_lut(s_st, L, !kinner && s_st.sum_rank > 0, tid);
   __syncthreads();
-  step_u8(s_st, L, arena + s_st.lhs_off, rhs, arena + s_st.out_off, (uint64_t)first * WG + tid, (uint64_t)nb * WG,
-          g.mode);
+  const int t = vec_bits(s_st);
+  if (t > 0)
+    step_u8_vec(s_st, L, arena + s_st.lhs_off, rhs, arena + s_st.out_off, (uint64_t)first * WG + tid,
+                (uint64_t)nb * WG, t);
+  else
+    step_u8(s_st, L, arena + s_st.lhs_off, rhs, arena + s_st.out_off, (uint64_t)first * WG + tid, (uint64_t)nb * WG,
+            g.mode);
 }
 
 // grid (x: 4 KiB chunks, y: job)

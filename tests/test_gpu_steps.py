@@ -3,8 +3,8 @@ GPU parity of the step kernels outside the MFMA route, through ts_contract2, in 
 epilogues, bit-exact against the oracle: steps with few outputs and many summed terms
 (one workgroup per output), with the summed wires innermost in both operands or
 scattered -- all-ones operands make every count a multiple of 256, where MOD256 gives 0
-and GT0 gives 1 -- and LDS-tiled steps (summed wires innermost, at most 6 of them, at
-least 2^12 outputs) over out orders that keep an operand's bits in runs or scatter them.
+and GT0 gives 1 -- and wide steps (summed wires innermost, at least 2^12 outputs) over
+out orders that keep the operands' rows in runs (the vector path) or scatter them.
 """
 
 import numpy as np
@@ -71,13 +71,18 @@ def test_few_outputs_many_terms(case, mode):
         assert want.all() == (mode == "GT0") and (want.any() == (mode == "GT0"))
 
 
-TILE_CASES = [
+WIDE_CASES = [
     # summed bits, lhs-only, rhs-only, shared out bits, density, out order
-    (1, 4, 4, 6, 0.5, "lhs"),     # the n=120 cubes' common shape: one summed bit, runs in lhs
+    (1, 4, 4, 6, 0.5, "lhs"),     # the n=120 cubes' common shape: one summed bit
     (1, 4, 4, 6, 0.5, "random"),
+    (1, 4, 4, 6, 0.5, "shared"),  # low out bits: rows in runs in both operands (vector path)
+    (1, 4, 5, 6, 0.5, "rhs-only"),  # low out bits: a run in rhs, absent from lhs
     (0, 6, 6, 2, 0.5, "lhs"),     # outer product with shared bits
-    (3, 5, 3, 6, 0.5, "rhs"),
-    (5, 2, 7, 5, 0.3, "random"),  # 32 terms: LDS budget forces smaller tiles
+    (0, 5, 6, 4, 0.5, "shared"),
+    (2, 4, 4, 5, 0.5, "shared"),
+    (3, 5, 3, 6, 0.5, "rhs"),     # 8-byte rows: eight outputs' loads per trip
+    (3, 3, 3, 6, 0.5, "shared"),
+    (5, 2, 7, 5, 0.3, "random"),
     (6, 3, 3, 8, 0.2, "lhs"),     # 64 terms, three lhs-only bits (not MFMA-shaped)
     (2, 0, 8, 6, 0.5, "rhs"),     # lhs holds only shared bits
     (4, 7, 0, 7, 0.5, "lhs"),     # rhs holds only shared bits
@@ -88,9 +93,11 @@ TILE_CASES = [
 
 
 @pytest.mark.parametrize("mode", ["GT0", "MOD256"])
-@pytest.mark.parametrize("case", TILE_CASES, ids=[str(c) for c in TILE_CASES])
-def test_tiled_steps(case, mode):
-    """Summed wires innermost in both operands, at least 2^12 outputs: the LDS-tiled path."""
+@pytest.mark.parametrize("case", WIDE_CASES, ids=[str(c) for c in WIDE_CASES])
+def test_wide_steps(case, mode):
+    """Summed wires innermost in both operands, at least 2^12 outputs, out orders that keep
+    the operands' rows in runs (one load per operand for 2..16 consecutive outputs) or
+    scatter them."""
     kbits, mbits, nbits, bbits, density, order = case
     rng = np.random.default_rng(kbits * 1009 + mbits * 101 + nbits * 11 + bbits + len(order))
     K = list(range(kbits))
@@ -99,14 +106,20 @@ def test_tiled_steps(case, mode):
     B = [300 + i for i in range(bbits)]
     lhead = [int(x) for x in rng.permutation(M + B)]
     rhead = [int(x) for x in rng.permutation(Nw + B)]
-    lw = lhead + K[::-1]
-    rw = rhead + K[::-1]
-    if order == "lhs":  # low out bits are the lhs bits right above the summed ones, in order
+    if order == "shared":  # shared bits lowest in both operands, same order, lowest in out
+        lhead, rhead = M + B, Nw + B
+        ow = M + Nw + B
+    elif order == "rhs-only":  # rhs-only bits lowest in rhs and in out
+        rhead = B + Nw
+        ow = M + B + Nw
+    elif order == "lhs":  # low out bits are the lhs bits right above the summed ones, in order
         ow = [w for w in rhead if w not in lhead] + lhead
     elif order == "rhs":
         ow = [w for w in lhead if w not in rhead] + rhead
     else:
         ow = [int(x) for x in rng.permutation(M + Nw + B)]
+    lw = lhead + K[::-1]
+    rw = rhead + K[::-1]
     if density >= 1.0:
         lhs = np.ones((2,) * len(lw), np.uint8)
         rhs = np.ones((2,) * len(rw), np.uint8)
