@@ -126,8 +126,10 @@ def main() -> int:
     ap.add_argument("--budget-nodes", default="auto", help="batched: nodes a search job runs before it hands "
                     "its remaining subtrees over, 'none', or 'auto'")
     ap.add_argument("--seeds", type=int, default=3, help="single / count workloads: instances (seeds 0..)")
-    ap.add_argument("--streams", type=int, default=0,
-                    help="single / plain: cubes (slices) in flight on their own streams (0: the workload's default)")
+    ap.add_argument("--gang", type=int, default=32,
+                    help="single / plain: largest gang of cubes (slices) advanced together (0: one at a time on "
+                         "--streams streams)")
+    ap.add_argument("--streams", type=int, default=4, help="single / plain with --gang 0: streams")
     ap.add_argument("--min-sum-bits", type=int, default=6,
                     help="dense steps with at least this many summed bits take the int8 MFMA route (0..27)")
     ap.add_argument("--learned-instances", type=int, default=8)
@@ -640,9 +642,9 @@ def run_single(args, seeds, log) -> dict:
     """
     One large random 3-SAT instance on the whole GPU (n=120, m=510, seeds 0..): conditioned
     slicing into cubes of width up to 26 (tensors up to 2^26 bytes), cubes in depth-first
-    order, four in flight on their own streams, runs of small steps fused into one
-    workgroup and large steps grid-wide (``single.decide``), until the first satisfiable
-    cube.  Reported: plan and GPU seconds per instance, and the step
+    order in gangs of 16 then 32 advanced together (``ts_program_run_gang``: the runs of
+    small steps of a gang's cubes in one launch, one workgroup per cube, their large steps
+    in one grid-wide launch; ``single.decide``), until the first satisfiable cube.  Reported: plan and GPU seconds per instance, and the step
     kernels' algorithmic bytes over the GPU time (HBM bound).  Verdicts are checked against
     the independent DPLL.  CPU baseline: the oracle's numpy step loop on the same cubes of
     the first seed, one cube per process on all host cores, for a bounded wall time,
@@ -666,9 +668,9 @@ def run_single(args, seeds, log) -> dict:
         prog = PG.sat_batch([cnf], width=26, plan_trials=2)
         plan_s = time.perf_counter() - t0
         progs.append(prog)
-        streams = args.streams or 4
-        decide(cnf, program=prog, mode=args.mode, streams=streams)  # warm-up: code objects, arenas
-        r = decide(cnf, program=prog, mode=args.mode, streams=streams)
+        kw = dict(program=prog, mode=args.mode, streams=args.streams, gang=args.gang)
+        decide(cnf, **kw)  # warm-up: code objects, arenas
+        r = decide(cnf, **kw)
         gpu_s = r.gpu_s
         launched = r.stats.get("cubes_launched", r.cubes_run)
         nbytes = float(sum(_step_bytes(prog.job_steps(j)) for j in range(launched)))
@@ -705,12 +707,12 @@ def run_single(args, seeds, log) -> dict:
         "metric": "plan+contract s per instance, single n=120 3-SAT decision (bit-exact)",
         "value": (tot_plan + tot_gpu) / len(rows), "unit": "s", "steps": len(rows),
         "ms_per_step": tot_gpu * 1000.0 / len(rows), "higher_is_better": False, "dtype": "u8",
-        "config": {"workload": "single random 3-SAT n=120 m=510 (seeds 0..), cubes of width <= 26, 4 cubes in flight "
-                               "on their own streams, small steps fused per workgroup, large steps grid-wide",
+        "config": {"workload": "single random 3-SAT n=120 m=510 (seeds 0..), cubes of width <= 26 in gangs of up to "
+                               f"{args.gang} advanced together (0: one at a time on {args.streams} streams)",
                    "plan_s_mean": tot_plan / len(rows), "gpu_s_mean": tot_gpu / len(rows), "rows": rows},
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
-                     "traffic": None, "kernel": "k_step_u8 / k_step_seq / k_step_mfma: algorithmic bytes of every "
-                                                "step of the launched cubes over the decision's wall time"},
+                     "traffic": None, "kernel": "k_gang_step / k_gang_seq: algorithmic bytes of every step of the "
+                                                "launched cubes over the decision's wall time"},
         "cpu_baseline": cpu,
         "parity": {"checked": len(rows), "dpll_mismatches": mism},
         "status": "mismatch" if mism else "ok",
@@ -724,8 +726,8 @@ def run_plain(args, seeds, log) -> dict:
     (``(cnf.inputs(None) >> cnf.diagram()).flatten()``) contracted along a greedy tree
     (``PlannedContraction(..., optimize="greedy")``, ``examples/test_sat_solver.py:5-9``)
     sliced to intermediates of at most 2^26 entries, every slice compiled into one job of
-    step records (contracted wires innermost) and the jobs decided by the step executors
-    with early exit (``single.decide``).  Verdicts against the independent DPLL.  CPU: the
+    step records (contracted wires innermost) and the slices decided in gangs advanced
+    together with early exit (``single.decide``).  Verdicts against the independent DPLL.  CPU: the
     oracle's numpy step loop on the same jobs on all host cores, sampled and scaled.
     """
     from tensorsat_amd import program as PG
@@ -746,9 +748,9 @@ def run_plain(args, seeds, log) -> dict:
         prog = c.contract_program(d)
         compile_s = time.perf_counter() - t0
         progs.append(prog)
-        streams = args.streams or 4
-        decide(cnf, program=prog, mode=args.mode, streams=streams)  # warm-up
-        r = decide(cnf, program=prog, mode=args.mode, streams=streams)
+        kw = dict(program=prog, mode=args.mode, streams=args.streams, gang=args.gang)
+        decide(cnf, **kw)  # warm-up
+        r = decide(cnf, **kw)
         launched = r.stats.get("cubes_launched", r.cubes_run)
         st = prog.steps[: int(prog.jobs["step_begin"][launched - 1] + prog.jobs["num_steps"][launched - 1])]
         dense, ops, nbytes = PG.step_work(st, args.min_sum_bits)
@@ -791,11 +793,12 @@ def run_plain(args, seeds, log) -> dict:
         "value": host + gpu, "unit": "s", "steps": len(rows), "ms_per_step": gpu * 1000.0,
         "higher_is_better": False, "dtype": "u8",
         "config": {"workload": "plain random 3-SAT n=40 m=170 (seeds 0..), bintree network, greedy tree sliced to "
-                               "2^26 entries, one compiled job per slice, early exit",
+                               f"2^26 entries, one compiled job per slice, gangs of up to {args.gang} slices, "
+                               "early exit",
                    "host_s_mean": host, "gpu_s_mean": gpu, "rows": rows},
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
-                     "traffic": None, "kernel": "k_step_u8 / k_step_seq / k_step_mfma: algorithmic bytes of every "
-                                                "step of the launched slices over the decision's wall time"},
+                     "traffic": None, "kernel": "k_gang_step / k_gang_seq: algorithmic bytes of every step of the "
+                                                "launched slices over the decision's wall time"},
         "cpu_baseline": cpu,
         "parity": {"checked": len(rows), "dpll_mismatches": mism},
         "status": "mismatch" if mism else "ok",
