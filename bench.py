@@ -119,7 +119,7 @@ def main() -> int:
     ap.add_argument("--kernel", default="auto", choices=["auto", "lds", "global"],
                     help="batched kernel: LDS arenas, device-memory arenas, or auto (LDS when the arenas fit)")
     ap.add_argument("--workload", default="batched",
-                    choices=["batched", "batched-host", "dense", "learned", "copy", "single", "count", "rewire", "plain"])
+                    choices=["batched", "batched-host", "dense", "learned", "copy", "single", "count", "rewire", "plain", "gnn"])
     ap.add_argument("--split", default="auto", help="batched: split target in branch nodes per job, 'none', "
                     "or 'auto' (autotuned with the width)")
     ap.add_argument("--no-nested", action="store_true", help="batched: skip the nested single / count lines")
@@ -162,6 +162,14 @@ def main() -> int:
         return bench_plain(args)
     if args.workload == "single":
         return bench_single(args)
+    if args.workload == "gnn":
+        def log(msg):
+            print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+        line = run_gnn(args, log)
+        line.update({"n_gpus": 1, "warmup": 2, "scaling": "weak", "vs_baseline": None, "data": "synthetic"})
+        print(json.dumps(line), flush=True)
+        return 0
     if args.workload == "rewire":
         return bench_rewire(args)
     if args.workload == "count":
@@ -396,6 +404,7 @@ def bench_search(args) -> int:
         nested["plain_n40"] = run_plain(args, seeds=range(3), log=log)
         nested["single_n120"] = run_single(args, seeds=range(3), log=log)
         nested["count_n100"] = run_count(args, seeds=range(1), log=log)
+        nested["gnn_train"] = run_gnn(args, log=log)
 
     out = {
         "metric": METRIC,
@@ -810,6 +819,67 @@ def bench_plain(args) -> int:
     out.update({"n_gpus": 1, "warmup": 1, "scaling": "weak", "vs_baseline": None, "data": "synthetic"})
     print(json.dumps(out), flush=True)
     return 1 if out["status"] == "mismatch" else 0
+
+
+def run_gnn(args, log, batch: int = 32, steps: int = 10) -> dict:
+    """
+    The learned planner network's training step (6 message-passing layers of width 128,
+    bf16 operands, fp32 accumulation; ``learn.model.GNN``) on one batch of 32 training
+    instances of the config-5 distribution (``learn.data.ksat``, seeds 10^6..): forward,
+    backward and Adam, all ``ts_gnn_*`` kernels.  Roofline of the GEMM kernel (``k_gemm``,
+    v_mfma_f32_16x16x32_bf16) over the step's GEMMs, each timed with HIP events on the launch
+    stream: their products are skinny (N = 128, K = 384 or the row count), 97 flops per byte
+    at most against the 312 of the bf16 ridge (2.5 PFLOP/s over 8 TB/s), so the bound is
+    HBM: achieved = algorithmic bytes (operands once, output once, twice when accumulated)
+    over the summed durations; the flop rate is reported beside it.
+    """
+    import torch
+
+    from tensorsat_amd.learn import data as D
+    from tensorsat_amd.learn import graph as G
+    from tensorsat_amd.learn.model import GNN
+
+    g = G.build([D.ksat(10**6 + i) for i in range(batch)])
+    dg = G.DeviceGraph(g)
+    target = torch.rand(dg.V, device="cuda")
+    model = GNN(seed=0)
+    for _ in range(2):  # warm-up
+        model.forward(dg, target=target, train=True)
+        model.adam_step(lr=1e-4)
+    torch.cuda.synchronize()
+    model.gemm_log = []
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        model.forward(dg, target=target, train=True)
+        model.adam_step(lr=1e-4)
+    e1.record()
+    torch.cuda.synchronize()
+    step_ms = e0.elapsed_time(e1) / steps
+    flops = sum(f for f, _, _, _ in model.gemm_log)
+    nbytes = sum(b for _, b, _, _ in model.gemm_log)
+    gemm_s = sum(a.elapsed_time(b) for _, _, a, b in model.gemm_log) / 1000.0
+    calls = len(model.gemm_log) // steps
+    model.gemm_log = None
+    tflops = flops / gemm_s / 1e12
+    gbs = nbytes / gemm_s / 1e9
+    log(f"gnn: {dg.V} variables, {dg.C} clauses, step {step_ms:.2f} ms, {calls} GEMMs per step, "
+        f"GEMM {gemm_s / steps * 1e3:.2f} ms per step at {gbs:.0f} GB/s, {tflops:.1f} TFLOP/s")
+    return {
+        "metric": "ms per training step, learned planner network (batch of 32 instances)",
+        "value": step_ms, "unit": "ms", "steps": steps, "ms_per_step": step_ms, "higher_is_better": False,
+        "dtype": "bf16",
+        "config": {"workload": f"message-passing network, 6 layers x 128, batch of {batch} config-5 training "
+                               f"instances ({dg.V} variables, {dg.C} clauses), forward + backward + Adam",
+                   "gemm_calls_per_step": calls, "gemm_ms_per_step": gemm_s / steps * 1e3,
+                   "gemm_tflops": tflops, "gemm_tflops_frac_of_bf16_peak": tflops / 2500.0,
+                   "gemm_flops_per_byte": flops / nbytes},
+        "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                     "traffic": None, "kernel": "k_gemm: algorithmic bytes of the step's GEMMs over their summed "
+                                                "durations"},
+        "cpu_baseline": None,
+        "status": "ok",
+    }
 
 
 def bench_single(args) -> int:

@@ -4,7 +4,8 @@
 //  * k_gemm: C = A x B (+ bias) (ReLU) with arbitrary strides, so one kernel covers the
 //    forward product X W and the backward products dX = dZ W^T and dW = X^T dZ.
 //    64x64 block tiles, 4 waves each computing 32x32 with 2x2 v_mfma_f32_16x16x32_bf16,
-//    K in steps of 32 staged through LDS.  Output fp32 or bf16, optionally accumulated.
+//    K in steps of 32 staged through LDS with 16-byte loads along whichever index is
+//    contiguous.  Output fp32 or bf16, optionally accumulated.
 //  * k_segsum: out[r, off + f] = sum over incidences e of row r of |w[e]| X[col[e], f],
 //    positive and negative literals (the sign of w[e]) into separate column blocks: the
 //    normalised signed neighbour sums of the clause-variable incidence graph.  The edge
@@ -34,6 +35,87 @@ constexpr int TM = 64, TN = 64, TK = 32;
 __device__ __forceinline__ float bf2f(__bf16 x) { return (float)x; }
 __device__ __forceinline__ __bf16 f2bf(float x) { return (__bf16)x; }
 
+// Staging of one 64 x 32 operand tile into LDS, dst[row][k] = src[row * s_row + k * s_k]
+// (zero outside rows < nrows, k < ke): 16-byte loads of 8 elements along the contiguous
+// index when it is k (vec 1: one 16-byte LDS store) or the row (vec 2: eight 2-byte LDS
+// stores, transposing), element loads otherwise and for ragged edges.
+__device__ __forceinline__ int vec_mode(const __bf16* p, int64_t s_row, int64_t s_k) {
+  if (reinterpret_cast<uintptr_t>(p) & 15) return 0;
+  if (s_k == 1 && (s_row & 7) == 0) return 1;
+  if (s_row == 1 && (s_k & 7) == 0) return 2;
+  return 0;
+}
+
+__device__ __forceinline__ void stage_tile(__bf16 (*dst)[TK + 8], const __bf16* __restrict__ src, int64_t s_row,
+                                           int64_t s_k, int row0, int nrows, int k0, int ke, int vec, int tid) {
+  if (vec == 1) {  // 64 rows x 4 vectors of 8 k
+    const int r = tid >> 2, c = (tid & 3) * 8;
+    const int m = row0 + r, k = k0 + c;
+    if (m < nrows && k + 8 <= ke) {
+      *reinterpret_cast<uint4*>(&dst[r][c]) = *reinterpret_cast<const uint4*>(src + (int64_t)m * s_row + k);
+    } else {
+#pragma unroll
+      for (int t = 0; t < 8; ++t)
+        dst[r][c + t] = (m < nrows && k + t < ke) ? src[(int64_t)m * s_row + k + t] : f2bf(0.f);
+    }
+    return;
+  }
+  if (vec == 2) {  // 32 k x 8 vectors of 8 rows
+    const int c = tid >> 3, r = (tid & 7) * 8;
+    const int m = row0 + r, k = k0 + c;
+    if (k < ke && m + 8 <= nrows) {
+      const uint4 v = *reinterpret_cast<const uint4*>(src + m + (int64_t)k * s_k);
+      const __bf16* e = reinterpret_cast<const __bf16*>(&v);
+#pragma unroll
+      for (int t = 0; t < 8; ++t) dst[r + t][c] = e[t];
+    } else {
+#pragma unroll
+      for (int t = 0; t < 8; ++t)
+        dst[r + t][c] = (k < ke && m + t < nrows) ? src[m + t + (int64_t)k * s_k] : f2bf(0.f);
+    }
+    return;
+  }
+  const bool kfast = s_k == 1;  // coalesce along the contiguous index
+  for (int e = tid; e < 64 * TK; e += 256) {
+    const int r = kfast ? e / TK : e % 64, c = kfast ? e % TK : e / 64;
+    const int m = row0 + r, k = k0 + c;
+    dst[r][c] = (m < nrows && k < ke) ? src[(int64_t)m * s_row + (int64_t)k * s_k] : f2bf(0.f);
+  }
+}
+
+// The vector modes split into a global load (the thread's 8 elements, zero outside the
+// bounds) and an LDS store, so that the next K tile's loads are in flight while the
+// current tile's MFMAs run.
+__device__ __forceinline__ uint4 load_vec(const __bf16* __restrict__ src, int64_t s_row, int64_t s_k, int row0,
+                                          int nrows, int k0, int ke, int vec, int tid) {
+  bf8 x;
+  if (vec == 1) {
+    const int r = tid >> 2, c = (tid & 3) * 8;
+    const int m = row0 + r, k = k0 + c;
+    if (m < nrows && k + 8 <= ke) return *reinterpret_cast<const uint4*>(src + (int64_t)m * s_row + k);
+#pragma unroll
+    for (int t = 0; t < 8; ++t) x[t] = (m < nrows && k + t < ke) ? src[(int64_t)m * s_row + k + t] : f2bf(0.f);
+  } else {
+    const int c = tid >> 3, r = (tid & 7) * 8;
+    const int m = row0 + r, k = k0 + c;
+    if (k < ke && m + 8 <= nrows) return *reinterpret_cast<const uint4*>(src + m + (int64_t)k * s_k);
+#pragma unroll
+    for (int t = 0; t < 8; ++t) x[t] = (k < ke && m + t < nrows) ? src[m + t + (int64_t)k * s_k] : f2bf(0.f);
+  }
+  return __builtin_bit_cast(uint4, x);
+}
+
+__device__ __forceinline__ void store_vec(__bf16 (*dst)[TK + 8], uint4 v, int vec, int tid) {
+  if (vec == 1) {
+    *reinterpret_cast<uint4*>(&dst[tid >> 2][(tid & 3) * 8]) = v;
+    return;
+  }
+  const bf8 x = __builtin_bit_cast(bf8, v);
+  const int c = tid >> 3, r = (tid & 7) * 8;
+#pragma unroll
+  for (int t = 0; t < 8; ++t) dst[r + t][c] = x[t];
+}
+
 // C[m, n] (+)= sum_k A(m, k) B(k, n); A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn],
 // C at C[m*ldc + n].  Bounds are checked per element.
 __global__ __launch_bounds__(256) void k_gemm(const __bf16* __restrict__ A, int64_t sam, int64_t sak,
@@ -50,24 +132,13 @@ __global__ __launch_bounds__(256) void k_gemm(const __bf16* __restrict__ A, int6
   // Split-K: block z covers k in [kb, ke); partial sums are added atomically (fp32 C).
   const int kb = blockIdx.z * kchunk;
   const int ke = min(K, kb + kchunk);
-  const bool a_kfast = sak == 1, b_kfast = sbk == 1;  // coalesce along the contiguous index
+  const int vec_a = vec_mode(A, sam, sak), vec_b = vec_mode(B, sbn, sbk);
   f4 acc[2][2];
   for (int i = 0; i < 2; ++i)
     for (int j = 0; j < 2; ++j) acc[i][j] = (f4){0.f, 0.f, 0.f, 0.f};
-  for (int k0 = kb; k0 < ke; k0 += TK) {
-    for (int e = tid; e < TM * TK; e += 256) {
-      const int r = a_kfast ? e / TK : e % TM, c = a_kfast ? e % TK : e / TM;
-      const int m = m0 + r, k = k0 + c;
-      sA[r][c] = (m < M && k < ke) ? A[(int64_t)m * sam + (int64_t)k * sak] : f2bf(0.f);
-    }
-    for (int e = tid; e < TN * TK; e += 256) {
-      const int r = b_kfast ? e / TK : e % TN, c = b_kfast ? e % TK : e / TN;  // r = n, c = k
-      const int n = n0 + r, k = k0 + c;
-      sB[r][c] = (n < N && k < ke) ? B[(int64_t)k * sbk + (int64_t)n * sbn] : f2bf(0.f);
-    }
-    __syncthreads();
-    // Lane l of a 16x16x32 tile holds row/col (l % 16) and k = 8*(l/16) .. +7 of both.
-    const int rr = lane & 15, kk = (lane >> 4) * 8;
+  // Lane l of a 16x16x32 tile holds row/col (l % 16) and k = 8*(l/16) .. +7 of both.
+  const int rr = lane & 15, kk = (lane >> 4) * 8;
+  auto mfma_tile = [&]() {
     bf8 a[2], b[2];
     for (int i = 0; i < 2; ++i)
       for (int t = 0; t < 8; ++t) a[i][t] = sA[wm + 16 * i + rr][kk + t];
@@ -75,7 +146,29 @@ __global__ __launch_bounds__(256) void k_gemm(const __bf16* __restrict__ A, int6
       for (int t = 0; t < 8; ++t) b[j][t] = sB[wn + 16 * j + rr][kk + t];
     for (int i = 0; i < 2; ++i)
       for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
-    __syncthreads();
+  };
+  if (vec_a && vec_b) {
+    uint4 ra = load_vec(A, sam, sak, m0, M, kb, ke, vec_a, tid);
+    uint4 rb = load_vec(B, sbn, sbk, n0, N, kb, ke, vec_b, tid);
+    for (int k0 = kb; k0 < ke; k0 += TK) {
+      store_vec(sA, ra, vec_a, tid);
+      store_vec(sB, rb, vec_b, tid);
+      __syncthreads();
+      if (k0 + TK < ke) {
+        ra = load_vec(A, sam, sak, m0, M, k0 + TK, ke, vec_a, tid);
+        rb = load_vec(B, sbn, sbk, n0, N, k0 + TK, ke, vec_b, tid);
+      }
+      mfma_tile();
+      __syncthreads();
+    }
+  } else {
+    for (int k0 = kb; k0 < ke; k0 += TK) {
+      stage_tile(sA, A, sam, sak, m0, M, k0, ke, vec_a, tid);
+      stage_tile(sB, B, sbn, sbk, n0, N, k0, ke, vec_b, tid);
+      __syncthreads();
+      mfma_tile();
+      __syncthreads();
+    }
   }
   // Accumulator layout: lane l holds rows 4*(l/16) + t (t = 0..3), column l % 16.
   for (int i = 0; i < 2; ++i)
@@ -114,29 +207,45 @@ __global__ __launch_bounds__(256) void k_segsum(const int32_t* __restrict__ rowp
   const int r = wave;
   float p0 = 0.f, p1 = 0.f, q0 = 0.f, q1 = 0.f;
   const int f0 = lane, f1 = lane + 64;
-  for (int e = rowptr[r]; e < rowptr[r + 1]; ++e) {
-    const int c = col[e];
-    const float w = weight[e];
-    const bool pos = w > 0.f;
-    const float aw = fabsf(w);
-    const int64_t base = (int64_t)c * ldx + (pos ? xoff_pos : xoff_neg);
-    float v0 = 0.f, v1 = 0.f;
+  auto load = [&](int e, float& v0, float& v1, float& w) {
+    w = weight[e];
+    const int64_t base = (int64_t)col[e] * ldx + (w > 0.f ? xoff_pos : xoff_neg);
+    v0 = v1 = 0.f;
     if (x_f32) {
       const float* x = reinterpret_cast<const float*>(X) + base;
-      if (f0 < F) v0 = aw * x[f0];
-      if (f1 < F) v1 = aw * x[f1];
+      if (f0 < F) v0 = x[f0];
+      if (f1 < F) v1 = x[f1];
     } else {
       const __bf16* x = reinterpret_cast<const __bf16*>(X) + base;
-      if (f0 < F) v0 = aw * bf2f(x[f0]);
-      if (f1 < F) v1 = aw * bf2f(x[f1]);
+      if (f0 < F) v0 = bf2f(x[f0]);
+      if (f1 < F) v1 = bf2f(x[f1]);
     }
-    if (pos) {
-      p0 += v0;
-      p1 += v1;
+  };
+  auto add = [&](float v0, float v1, float w) {
+    const float aw = fabsf(w);
+    if (w > 0.f) {
+      p0 += aw * v0;
+      p1 += aw * v1;
     } else {
-      q0 += v0;
-      q1 += v1;
+      q0 += aw * v0;
+      q1 += aw * v1;
     }
+  };
+  // Four incidences per trip, their rows loaded before any is added (four row gathers in
+  // flight per wave instead of one).
+  int e = rowptr[r];
+  const int end = rowptr[r + 1];
+  for (; e + 4 <= end; e += 4) {
+    float v0[4], v1[4], w[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) load(e + u, v0[u], v1[u], w[u]);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) add(v0[u], v1[u], w[u]);
+  }
+  for (; e < end; ++e) {
+    float v0, v1, w;
+    load(e, v0, v1, w);
+    add(v0, v1, w);
   }
   if (out_f32_accumulate) {
     float* o = reinterpret_cast<float*>(out) + (int64_t)r * ldo;
@@ -184,6 +293,7 @@ __global__ __launch_bounds__(128) void k_head(const __bf16* __restrict__ H, int6
                                               int F, int train) {
   __shared__ float red[128];
   const int f = threadIdx.x;
+  float dw_acc = 0.f, db_acc = 0.f, loss_acc = 0.f;  // this block's rows, added once at the end
   for (int r = blockIdx.x; r < rows; r += gridDim.x) {
     const float h = f < F ? bf2f(H[(int64_t)r * ldh + f]) : 0.f;
     red[f] = h * (f < F ? w[f] : 0.f);
@@ -199,15 +309,20 @@ __global__ __launch_bounds__(128) void k_head(const __bf16* __restrict__ H, int6
       const float m = mask ? mask[r] : 1.f;
       const float d = yr - target[r];
       const float g = 2.f * m * d * inv_count;
-      if (f == 0) {
-        atomicAdd(loss, m * d * d * inv_count);
-        atomicAdd(db, g);
-      }
+      loss_acc += m * d * d * inv_count;
+      db_acc += g;
       if (f < F) {
         dH[(int64_t)r * F + f] = g * w[f];
-        atomicAdd(&dw[f], g * h);
+        dw_acc += g * h;
       }
     }
+  }
+  if (train) {
+    if (f == 0) {
+      atomicAdd(loss, loss_acc);
+      atomicAdd(db, db_acc);
+    }
+    if (f < F) atomicAdd(&dw[f], dw_acc);
   }
 }
 
@@ -290,7 +405,7 @@ int ts_gnn_head(const void* H, int64_t ldh, int32_t rows, const float* w, const 
   if (!H || !w || !b || !y || rows < 0 || F < 0 || F > 128) return TS_EINVAL;
   if (train && (!target || !loss || !dH || !dw || !db)) return TS_EINVAL;
   if (rows == 0) return TS_OK;
-  hipLaunchKernelGGL(k_head, dim3(std::min(rows, 8192)), dim3(128), 0, reinterpret_cast<hipStream_t>(stream),
+  hipLaunchKernelGGL(k_head, dim3(std::min(rows, 512)), dim3(128), 0, reinterpret_cast<hipStream_t>(stream),
                      reinterpret_cast<const __bf16*>(H), ldh, rows, w, b, target, mask, inv_count, y, loss, dH, dw,
                      db, F, train);
   return hipGetLastError() == hipSuccess ? TS_OK : TS_EHIP;

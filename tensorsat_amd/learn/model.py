@@ -84,6 +84,7 @@ class GNN:
         self.m = torch.zeros(self.total, dtype=torch.float32, device=device)
         self.v = torch.zeros(self.total, dtype=torch.float32, device=device)
         self.steps = 0
+        self.gemm_log = None  # a list to record every GEMM's flops, bytes and events into
 
     # -- views ---------------------------------------------------------------------
     def _ptr(self, buf, name: str, row: int = 0, elem: int = 4):
@@ -102,8 +103,16 @@ class GNN:
 
     # -- kernels ---------------------------------------------------------------------
     def _gemm(self, A, sam, sak, B, sbk, sbn, C, ldc, M, Nn, K, bias=None, relu=0, out_bf16=1, acc=0):
+        log = self.gemm_log
+        if log is not None:  # measurement: (flops, bytes, start, end), events on the current stream
+            e0, e1 = self.torch.cuda.Event(enable_timing=True), self.torch.cuda.Event(enable_timing=True)
+            e0.record()
         N.check(N.lib().ts_gnn_gemm(A, sam, sak, B, sbk, sbn, C, ldc, M, Nn, K, bias, relu, out_bf16, acc,
                                     N.stream_ptr()), "ts_gnn_gemm")
+        if log is not None:
+            e1.record()
+            out_b = M * Nn * (2 if out_bf16 else 4) * (2 if acc else 1)  # read back when accumulating
+            log.append((2.0 * M * Nn * K, 2.0 * (M * K + K * Nn) + out_b, e0, e1))
 
     def _segsum(self, rowptr, col, weight, nrows, X, ldx, xp, xn, xf32, out, ldo, op, on, acc):
         N.check(N.lib().ts_gnn_segsum(rowptr, col, weight, nrows, X, ldx, xp, xn, xf32, out, ldo, op, on, HID, acc,
