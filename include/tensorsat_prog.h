@@ -202,6 +202,19 @@ int tsc_search_pack(int32_t num, const int32_t* nvars, const int64_t* nclauses, 
                     const int32_t* const* lits, uint32_t* words_out, int32_t* inst_out, int64_t* total_out,
                     int32_t* bad_out, int32_t threads /* <= 0: hardware concurrency */);
 
+/* Start jobs of a batch for the device search (ts_search_job records, 64 bytes):
+ * instance i gets 2^s_i jobs with prefixes 0 .. 2^s_i - 1, where s_i = clamp(ceil(e_i -
+ * log2(split_target)), 0, min(max_split, 31)) and e_i = 2.5 + slope(m_i / n_i) n_i
+ * estimates log2 of its branch nodes (slope 0.024 at clause ratio <= 3.5 rising linearly
+ * to 0.054 at 4.26, then 0.054 * 4.26 / ratio); split_target <= 0: s_i = 0.  Jobs in
+ * increasing -(e_i - s_i), ties in instance order, an instance's jobs in prefix order;
+ * own_from = 1 + the index of the prefix's highest set bit (0 for prefix 0).  inst:
+ * `num` records of 4 int32 (clause_begin, num_clauses, num_vars, reserved), as
+ * tsc_search_pack writes them.  *njobs_out = the job count; TSP_ENOMEM (nothing written)
+ * when it exceeds `capacity`.  No counterpart in the reference. */
+int tsc_search_plan(int32_t num, const int32_t* inst, double split_target, int32_t max_split, void* jobs_out,
+                    int64_t capacity, int64_t* njobs_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <thread>
@@ -124,5 +125,67 @@ extern "C" int tsc_search_pack(int32_t num, const int32_t* nvars, const int64_t*
   }
   if (bad_out) *bad_out = -1;
   *total_out = w;
+  return TSP_OK;
+}
+
+// Start jobs of a batch (the search's job records, 64 bytes each).
+namespace {
+
+struct StartJob {
+  int32_t instance, depth, split, own_from;
+  uint32_t prefix, reserved[3];
+  uint64_t assigned[2], value[2];
+};
+static_assert(sizeof(StartJob) == 64, "job record layout");
+
+// Estimated log2 branch nodes of a random 3-SAT-like instance: linear in n with a slope
+// that grows with the clause ratio up to the threshold (4.26) and falls past it.
+double log2_search_size(int32_t num_vars, int32_t num_clauses) {
+  const double n = num_vars, r = (double)num_clauses / std::max(n, 1.0);
+  const double lo = 0.024, hi = 0.054;
+  const double slope = r <= 4.26 ? lo + (hi - lo) * std::min(std::max((r - 3.5) / 0.76, 0.0), 1.0)
+                                 : hi * 4.26 / std::max(r, 1e-9);
+  return 2.5 + slope * n;
+}
+
+}  // namespace
+
+extern "C" int tsc_search_plan(int32_t num, const int32_t* inst, double split_target, int32_t max_split,
+                               void* jobs_out, int64_t capacity, int64_t* njobs_out) {
+  if (num < 0 || (num > 0 && !inst) || !njobs_out || max_split < 0 || capacity < 0) return TSP_EINVAL;
+  const int32_t cap_split = std::min<int32_t>(max_split, 31);
+  std::vector<int32_t> split((size_t)num, 0);
+  std::vector<double> key((size_t)num);
+  const double lt = split_target > 0 ? std::log2(split_target) : 0.0;
+  int64_t total = 0;
+  for (int32_t i = 0; i < num; ++i) {
+    const double est = log2_search_size(inst[4 * i + 2], inst[4 * i + 1]);
+    if (split_target > 0) {
+      const double s = std::ceil(est - lt);
+      split[i] = s <= 0 ? 0 : (s >= cap_split ? cap_split : (int32_t)s);
+    }
+    key[i] = -(est - (double)split[i]);
+    total += int64_t(1) << split[i];
+  }
+  *njobs_out = total;
+  if (total > capacity) return TSP_ENOMEM;
+  std::vector<int32_t> order((size_t)num);
+  for (int32_t i = 0; i < num; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return key[a] < key[b]; });
+  StartJob* out = static_cast<StartJob*>(jobs_out);
+  int64_t j = 0;
+  for (int32_t i : order) {
+    const uint32_t nj = 1u << split[i];
+    for (uint32_t p = 0; p < nj; ++p, ++j) {
+      StartJob& r = out[j];
+      std::memset(&r, 0, sizeof(r));
+      r.instance = i;
+      r.split = split[i];
+      r.prefix = p;
+      // A node on the prefix levels belongs to the job whose remaining prefix bits are
+      // all 0: the job counts from one level below its highest set prefix bit.
+      r.own_from = p ? 32 - __builtin_clz(p) : 0;
+    }
+  }
   return TSP_OK;
 }

@@ -67,11 +67,12 @@ def supported(cnf) -> bool:
     return cnf.num_vars <= MAX_VARS and (lens.size == 0 or int(lens.max()) <= 3)
 
 
-def pack_batch(cnfs) -> tuple[np.ndarray, np.ndarray]:
+def pack_batch(cnfs, alloc=None) -> tuple[np.ndarray, np.ndarray]:
     """(packed clauses, per-instance records) of a batch, by the host library's packer
     (``tsc_search_pack``: duplicate literals dropped, tautologies removed, clause order
     kept); raises NotImplementedError for an instance outside the kernel's limits and
-    ValueError for a literal out of range."""
+    ValueError for a literal out of range.  ``alloc(words, instances)``, if given, returns
+    the (uint32 [>= words], INST_DTYPE [instances]) arrays to pack into."""
     from . import program as PG
 
     L = PG._lib()
@@ -82,8 +83,11 @@ def pack_batch(cnfs) -> tuple[np.ndarray, np.ndarray]:
     ncl = np.ascontiguousarray(head[:, 1])
     lens_p = np.ascontiguousarray(head[:, 2])
     lits_p = np.ascontiguousarray(head[:, 3])
-    words = np.empty(max(1, int(ncl.sum())), dtype=np.uint32)
-    inst = np.zeros(num, dtype=INST_DTYPE)
+    if alloc is not None:
+        words, inst = alloc(max(1, int(ncl.sum())), num)
+    else:
+        words = np.empty(max(1, int(ncl.sum())), dtype=np.uint32)
+        inst = np.zeros(num, dtype=INST_DTYPE)
     total, bad = ctypes.c_int64(0), ctypes.c_int32(-1)
     rc = L.tsc_search_pack(num, nv.ctypes.data, ncl.ctypes.data, lens_p.ctypes.data, lits_p.ctypes.data,
                            words.ctypes.data, inst.ctypes.data if num else None, ctypes.byref(total),
@@ -94,6 +98,10 @@ def pack_batch(cnfs) -> tuple[np.ndarray, np.ndarray]:
     if rc != 0:
         raise ValueError(f"instance {bad.value}: literal out of range")
     return words[: total.value], inst
+
+
+def _align(n: int) -> int:
+    return (n + 255) & ~255
 
 
 def log2_search_size(num_vars: np.ndarray, num_clauses: np.ndarray) -> np.ndarray:
@@ -109,34 +117,35 @@ def log2_search_size(num_vars: np.ndarray, num_clauses: np.ndarray) -> np.ndarra
     return 2.5 + slope * n
 
 
-def plan_jobs(inst: np.ndarray, *, split_target: float | None = 128.0, max_split: int = 8) -> np.ndarray:
+def plan_jobs(inst: np.ndarray, *, split_target: float | None = 128.0, max_split: int = 8,
+              out: np.ndarray | None = None) -> np.ndarray:
     """One job per subtree: instance i gets 2^split_i jobs (prefixes 0..2^split_i - 1)
-    where split_i = ceil(log2(estimated nodes / split_target)) (0 with ``split_target``
-    None), launched in decreasing order of estimated size per job (instances of equal
-    size in index order, the jobs of an instance in prefix order)."""
+    where split_i = ceil(log2(estimated nodes / split_target)) (:func:`log2_search_size`;
+    0 with ``split_target`` None), launched in decreasing order of estimated size per job
+    (instances of equal size in index order, the jobs of an instance in prefix order).
+    Built by the host library (``tsc_search_plan``), into ``out`` when it has room."""
+    from . import program as PG
+
+    if split_target is not None and split_target <= 0:
+        raise ValueError("split_target must be positive")
     num = len(inst)
     if num == 0:
         return np.zeros(0, dtype=JOB_DTYPE)
-    est = log2_search_size(inst["num_vars"], inst["num_clauses"])
-    if split_target is None:
-        split = np.zeros(num, dtype=np.int64)
-    else:
-        if split_target <= 0:
-            raise ValueError("split_target must be positive")
-        split = np.clip(np.ceil(est - math.log2(split_target)), 0, min(max_split, MAX_SPLIT)).astype(np.int64)
-    order = np.argsort(-(est - split), kind="stable")
-    nj = (np.int64(1) << split)[order]
-    jobs = np.zeros(int(nj.sum()), dtype=JOB_DTYPE)
-    jobs["instance"] = np.repeat(order, nj)
-    jobs["split"] = np.repeat(split[order], nj)
-    starts = np.zeros(num, dtype=np.int64)
-    np.cumsum(nj[:-1], out=starts[1:])
-    prefix = np.arange(jobs.size, dtype=np.int64) - np.repeat(starts, nj)
-    jobs["prefix"] = prefix
-    # A node on the prefix levels belongs to the job whose remaining prefix bits are all
-    # 0: the job counts from one level below its highest set prefix bit.
-    jobs["own_from"] = np.where(prefix > 0, np.floor(np.log2(np.maximum(prefix, 1))).astype(np.int64) + 1, 0)
-    return jobs
+    L = PG._lib()
+    rec = np.ascontiguousarray(inst)
+    target = 0.0 if split_target is None else float(split_target)
+    cap = 0 if out is None else len(out)
+    count = ctypes.c_int64(0)
+    buf = out if out is not None else np.zeros(0, dtype=JOB_DTYPE)
+    rc = L.tsc_search_plan(num, rec.ctypes.data, target, max_split, buf.ctypes.data if cap else None, cap,
+                           ctypes.byref(count))
+    if rc == -2:  # no room: the count is known now
+        buf = np.empty(count.value, dtype=JOB_DTYPE)
+        rc = L.tsc_search_plan(num, rec.ctypes.data, target, max_split, buf.ctypes.data, count.value,
+                               ctypes.byref(count))
+    if rc != 0:
+        raise ValueError("tsc_search_plan: invalid arguments")
+    return buf[: count.value]
 
 
 def _var_masks(nv: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
@@ -152,10 +161,13 @@ def _popcount64(x: np.ndarray) -> np.ndarray:
     return np.bitwise_count(np.asarray(x, dtype=np.uint64)).astype(np.int64)
 
 
-def validate(words: np.ndarray, inst: np.ndarray, jobs: np.ndarray, width: int, *, literals: bool = True) -> None:
+def validate(words: np.ndarray, inst: np.ndarray, jobs: np.ndarray | None, width: int, *,
+             literals: bool = True) -> None:
     """Host-side bounds check of everything the kernel indexes, before any upload
     (``literals=False`` skips the per-literal variable check, for clause words made by
-    :func:`pack_batch`, which checks them as it packs)."""
+    :func:`pack_batch`, which checks them as it packs; ``jobs=None`` skips the job checks,
+    for start jobs made by :func:`plan_jobs`, whose planner writes only instances of the
+    batch, splits of 0..31, prefixes below 2^split and owners at or above the root)."""
     if not 0 <= width <= MAX_WIDTH:
         raise ValueError(f"width must be in 0..{MAX_WIDTH}")
     if len(inst):
@@ -181,7 +193,7 @@ def validate(words: np.ndarray, inst: np.ndarray, jobs: np.ndarray, width: int, 
                 used = (w >> 24) > i
                 if np.any(used & (v >= nvars)):
                     raise ValueError("literal of a variable beyond the instance's count")
-    if len(jobs):
+    if jobs is not None and len(jobs):
         if np.any(jobs["instance"] >= len(inst)):
             raise ValueError("job names an instance outside the batch")
         if np.any(jobs["split"] < 0) or np.any(jobs["split"] > MAX_SPLIT):
@@ -254,33 +266,90 @@ class SearchSolver:
         self.spill_capacity = spill_capacity
         self.max_rounds = max_rounds
         self.timing: dict = {}
+        self._pin = self._dev_stage = self._staged = None
         self.rounds = 0
         self.round_jobs: list[int] = []
 
     def prepare(self, cnfs) -> None:
-        """Pack, plan the jobs and upload (host work, timed in ``self.timing``)."""
+        """Pack, plan the jobs and upload (host work, timed in ``self.timing``).  Clauses,
+        instance records and jobs are written straight into one pinned staging buffer
+        (kept by the solver, grown as needed) and sent with one host-to-device copy."""
         import torch
 
+        if self._staged is not None:
+            self._staged.synchronize()  # the previous batch's copy has read the staging buffer
         t0 = time.perf_counter()
-        words, inst = pack_batch(cnfs)
+        layout = {}
+
+        def alloc(nwords, ninst):
+            layout["words"] = (0, nwords * 4)
+            layout["inst"] = (_align(nwords * 4), ninst * INST_DTYPE.itemsize)
+            self._stage(layout["inst"][0] + ninst * INST_DTYPE.itemsize)
+            h = self._pin.numpy()
+            return (h[: nwords * 4].view(np.uint32),
+                    h[layout["inst"][0] : layout["inst"][0] + layout["inst"][1]].view(INST_DTYPE))
+
+        words, inst = pack_batch(cnfs, alloc)
         t1 = time.perf_counter()
-        jobs = plan_jobs(inst, split_target=self.split_target, max_split=self.max_split)
-        validate(words, inst, jobs, self.width, literals=False)  # the packer checked every literal
+        jo = _align(layout["inst"][0] + layout["inst"][1])
+        room = (self._pin.numel() - jo) // JOB_DTYPE.itemsize
+        h = self._pin.numpy()
+        jobs = plan_jobs(inst, split_target=self.split_target, max_split=self.max_split,
+                         out=h[jo : jo + room * JOB_DTYPE.itemsize].view(JOB_DTYPE) if room > 0 else None)
+        if room <= 0 or not np.may_share_memory(jobs, h):  # grown: stage the jobs too
+            need = jo + jobs.nbytes
+            old = (words.copy(), inst.copy())
+            self._stage(need)
+            h = self._pin.numpy()
+            words = h[: old[0].nbytes].view(np.uint32)
+            words[:] = old[0]
+            inst = h[layout["inst"][0] : layout["inst"][0] + old[1].nbytes].view(INST_DTYPE)
+            inst[:] = old[1]
+            staged = h[jo : jo + jobs.nbytes].view(JOB_DTYPE)
+            staged[:] = jobs
+            jobs = staged
+        layout["jobs"] = (jo, jobs.nbytes)
+        validate(words, inst, None, self.width, literals=False)  # packer and planner checked the rest
         t2 = time.perf_counter()
-        self.upload(words, inst, jobs)
+        self.upload(words, inst, jobs, staged=layout)
         torch.cuda.synchronize()
         self.timing.update(pack_s=t1 - t0, plan_s=t2 - t1, upload_s=time.perf_counter() - t2)
 
-    def upload(self, words: np.ndarray, inst: np.ndarray, jobs: np.ndarray) -> None:
+    def _stage(self, nbytes: int) -> None:
+        """Pinned staging buffer (and its device twin) of at least ``nbytes``."""
+        import torch
+
+        if self._pin is None or self._pin.numel() < nbytes:
+            size = max(int(nbytes * 1.25), 1 << 20)
+            self._pin = torch.empty(size, dtype=torch.uint8, pin_memory=True)
+            self._dev_stage = torch.empty(size, dtype=torch.uint8, device=self.device)
+
+    def upload(self, words: np.ndarray, inst: np.ndarray, jobs: np.ndarray, staged: dict | None = None) -> None:
+        """Device copies of the records; ``staged`` (from :meth:`prepare`): their byte
+        ranges in the pinned staging buffer, sent with one copy."""
         import torch
 
         dev = self.device
         self.num_instances = len(inst)
         self.max_clauses = int(inst["num_clauses"].max()) if len(inst) else 0
         self.host_jobs = jobs
-        self.d_words = torch.from_numpy(words.view(np.int32) if words.size else np.zeros(1, np.int32)).to(dev)
-        self.d_inst = torch.from_numpy(inst.view(np.int32).reshape(-1) if len(inst) else np.zeros(4, np.int32)).to(dev)
-        self.d_jobs = torch.from_numpy(jobs.view(np.int32).reshape(-1) if len(jobs) else np.zeros(16, np.int32)).to(dev)
+        if staged is not None and len(inst) and len(jobs) and words.size:
+            end = staged["jobs"][0] + staged["jobs"][1]
+            self._dev_stage[:end].copy_(self._pin[:end], non_blocking=True)
+            self._staged = torch.cuda.Event()
+            self._staged.record()
+
+            def view(key):
+                a, n = staged[key]
+                return self._dev_stage[a : a + n].view(torch.int32)
+
+            self.d_words, self.d_inst, self.d_jobs = view("words"), view("inst"), view("jobs")
+        else:
+            self.d_words = torch.from_numpy(words.view(np.int32) if words.size else np.zeros(1, np.int32)).to(dev)
+            self.d_inst = torch.from_numpy(inst.view(np.int32).reshape(-1) if len(inst)
+                                           else np.zeros(4, np.int32)).to(dev)
+            self.d_jobs = torch.from_numpy(jobs.view(np.int32).reshape(-1) if len(jobs)
+                                           else np.zeros(16, np.int32)).to(dev)
         self.d_verdicts = torch.zeros(max(1, len(inst)), dtype=torch.int32, device=dev)
         self.d_witness = torch.zeros(2 * max(1, len(inst)), dtype=torch.int64, device=dev) \
             if self.want_witness else None

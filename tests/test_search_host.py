@@ -117,3 +117,42 @@ def test_validate_checks_job_start_assignments():
     bad["own_from"][0] = 127
     with pytest.raises(ValueError):
         S.validate(words, inst, bad, 6)
+
+
+def _plan_jobs_numpy(inst, split_target, max_split):
+    """Restatement of the start-job rule in numpy (the C planner's specification)."""
+    num = len(inst)
+    est = S.log2_search_size(inst["num_vars"], inst["num_clauses"])
+    if split_target is None:
+        split = np.zeros(num, dtype=np.int64)
+    else:
+        split = np.clip(np.ceil(est - np.log2(split_target)), 0, min(max_split, S.MAX_SPLIT)).astype(np.int64)
+    order = np.argsort(-(est - split), kind="stable")
+    nj = (np.int64(1) << split)[order]
+    jobs = np.zeros(int(nj.sum()), dtype=S.JOB_DTYPE)
+    jobs["instance"] = np.repeat(order, nj)
+    jobs["split"] = np.repeat(split[order], nj)
+    starts = np.zeros(num, dtype=np.int64)
+    np.cumsum(nj[:-1], out=starts[1:])
+    prefix = np.arange(jobs.size, dtype=np.int64) - np.repeat(starts, nj)
+    jobs["prefix"] = prefix
+    jobs["own_from"] = np.where(prefix > 0, np.floor(np.log2(np.maximum(prefix, 1))).astype(np.int64) + 1, 0)
+    return jobs
+
+
+def test_plan_jobs_match_the_numpy_rule():
+    """The host library's start jobs equal the numpy restatement record for record, over
+    clause ratios either side of the threshold and every split setting."""
+    from tensorsat_amd.sat import CNFInstance
+
+    cnfs = batched_3sat(300) + [CNFInstance.random(3, n, int(r * n), rng=7 + n) for n in (20, 60, 100)
+                               for r in (3.0, 4.26, 5.5)]
+    _, inst = S.pack_batch(cnfs)
+    for target, ms in ((None, 8), (512.0, 8), (64.0, 6), (4.0, 8), (1.0, 3), (0.25, 40)):
+        got = S.plan_jobs(inst, split_target=target, max_split=ms)
+        want = _plan_jobs_numpy(inst, target, ms)
+        assert got.tobytes() == want.tobytes(), (target, ms)
+    out = np.zeros(4, dtype=S.JOB_DTYPE)  # too small: a new array comes back
+    assert S.plan_jobs(inst, split_target=64.0, out=out).tobytes() == _plan_jobs_numpy(inst, 64.0, 8).tobytes()
+    with pytest.raises(ValueError):
+        S.plan_jobs(inst, split_target=0.0)
