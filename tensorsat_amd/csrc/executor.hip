@@ -94,6 +94,27 @@ __device__ void dot_block(const ts_step& st, const Lut& L, const uint8_t* __rest
   }
 }
 
+// Large steps whose rows (2^4..2^10 bytes, summed bits innermost in both operands) are
+// read again and again (at least two out bits held by one operand only): packed into bit
+// masks first by gang execution when the job's arena stride leaves room for them.  On
+// the masks the step has 3 summed bits fewer; rows of at most 8 bytes (at most 64 terms)
+// may take the != 0 shortcut, longer ones keep exact popcounts for the MOD256 epilogue.
+__host__ __device__ inline bool packable(const ts_step& st) {
+  if (!(st.flags & TS_STEP_KINNER) || (st.flags & TS_STEP_NORHS)) return false;
+  if (st.sum_rank < 4 || st.sum_rank > 10 || st.out_rank < 12) return false;
+  int one_side = 0;
+  for (int q = 0; q < st.out_rank; ++q) one_side += (st.o_lpos[q] == TS_NOPOS) != (st.o_rpos[q] == TS_NOPOS);
+  return one_side >= 2;
+}
+
+// Offset of the packed rhs after the packed lhs (16-byte aligned), and the bytes both take.
+__host__ __device__ inline uint32_t packed_rhs_at(const ts_step& st) {
+  return ((1u << (st.lhs_rank - 3)) + 15u) & ~15u;
+}
+__host__ __device__ inline uint64_t packed_bytes(const ts_step& st) {
+  return packed_rhs_at(st) + (1ull << (st.rhs_rank - 3));
+}
+
 // Rule shared by the host dispatch: at least 2^8 terms per output, at most 2^12 outputs,
 // summed bits innermost (16-byte runs) or not.
 inline bool dot_shaped(const ts_step& st) { return st.sum_rank >= 8 && st.out_rank <= 12; }
@@ -163,12 +184,24 @@ struct GangSeq {  // one workgroup per job: a run of small steps
 
 // kind[j]: 0 = step_u8 over the job's blocks (grid-stride), 1 = one block per output
 // (few outputs, many terms).
+// kind 2: the step's rows (2^4..2^10 bytes, summed bits innermost) were packed into bit
+// masks at arena offset pk[j] (lhs) and pk[j] + packed_rhs_at(record) (rhs): the step
+// runs on them as on rows of 2..128 bytes, each byte holding 8 terms (popcount of the
+// AND is still the exact count).
 struct GangStep {
   const ts_step* rec[GANG];
   uint8_t* arena[GANG];
   uint32_t blk[GANG + 1];
+  uint32_t pk[GANG];
   uint8_t kind[GANG];
   int32_t n, mode;
+};
+
+struct GangPack {  // pack tasks: rows of 2^k bytes -> 2^k-bit masks (k = 4..10)
+  const uint8_t* src[2 * GANG];
+  uint8_t* dst[2 * GANG];
+  uint32_t rows[2 * GANG];
+  uint8_t k[2 * GANG];
 };
 
 struct GangCopy {
@@ -203,11 +236,26 @@ __global__ __launch_bounds__(WG) void k_gang_step(GangStep) {
   if (tid < (int)(sizeof(ts_step) / 16))
     reinterpret_cast<uint4*>(&s_st)[tid] = reinterpret_cast<const uint4*>(g.rec[j])[tid];
   __syncthreads();
+  if (g.kind[j] == 2) {  // the same step on the packed rows
+    if (tid == 0) {
+      const uint32_t pk = g.pk[j];
+      s_st.rhs_off = pk + packed_rhs_at(s_st);
+      s_st.lhs_off = pk;
+      s_st.lhs_rank -= 3;
+      s_st.rhs_rank -= 3;
+      s_st.sum_rank -= 3;
+      for (int q = 0; q < s_st.out_rank; ++q) {
+        if (s_st.o_lpos[q] != TS_NOPOS) s_st.o_lpos[q] -= 3;
+        if (s_st.o_rpos[q] != TS_NOPOS) s_st.o_rpos[q] -= 3;
+      }
+    }
+    __syncthreads();
+  }
   uint8_t* arena = g.arena[j];
   const bool kinner = s_st.flags & TS_STEP_KINNER;
   const uint8_t* rhs = arena + ((s_st.flags & TS_STEP_NORHS) ? s_st.lhs_off : s_st.rhs_off);
   const uint32_t first = b - g.blk[j], nb = g.blk[j + 1] - g.blk[j];
-  if (g.kind[j]) {
+  if (g.kind[j] == 1) {
     if (!kinner) {
       build_lut(s_st, L, true, tid);
       __syncthreads();
@@ -224,6 +272,38 @@ __global__ __launch_bounds__(WG) void k_gang_step(GangStep) {
   else
     step_u8(s_st, L, arena + s_st.lhs_off, rhs, arena + s_st.out_off, (uint64_t)first * WG + tid, (uint64_t)nb * WG,
             g.mode);
+}
+
+// Nonzero bytes of a 32-bit word as 4 bits.
+__device__ __forceinline__ uint32_t nz_bits4(uint32_t w) {
+  const uint32_t m = (((w & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | w) & 0x80808080u;
+  return ((m >> 7) | (m >> 14) | (m >> 21) | (m >> 28)) & 0xFu;
+}
+
+// grid (x: row chunks, y: task); one row (2^k bytes, k = 4..10) per thread, packed to a
+// 2^k-bit mask (bit i = byte i nonzero) stored little-endian.
+__global__ __launch_bounds__(WG) void k_gang_pack(GangPack) {
+  const GangPack& g = kernarg<GangPack>();
+  const int t = blockIdx.y;
+  const uint32_t rows = g.rows[t];
+  const int k = g.k[t];
+  const uint4* src = reinterpret_cast<const uint4*>(g.src[t]);
+  uint8_t* dst = g.dst[t];
+  const int nv = 1 << (k - 4);               // 16-byte vectors per row
+  const int nw = k >= 6 ? 1 << (k - 6) : 1;  // 64-bit mask words per row
+  for (uint32_t r = blockIdx.x * WG + threadIdx.x; r < rows; r += gridDim.x * WG) {
+    for (int w = 0; w < nw; ++w) {
+      uint64_t bits = 0;
+      for (int v = 0; v < (nv < 4 ? nv : 4); ++v) {
+        const uint4 x = src[(uint64_t)r * nv + 4 * w + v];
+        const uint64_t b16 = nz_bits4(x.x) | (nz_bits4(x.y) << 4) | (nz_bits4(x.z) << 8) | (nz_bits4(x.w) << 12);
+        bits |= b16 << (16 * v);
+      }
+      if (k == 4) reinterpret_cast<uint16_t*>(dst)[r] = (uint16_t)bits;
+      else if (k == 5) reinterpret_cast<uint32_t*>(dst)[r] = (uint32_t)bits;
+      else reinterpret_cast<uint64_t*>(dst)[(uint64_t)r * nw + w] = bits;
+    }
+  }
 }
 
 // grid (x: 4 KiB chunks, y: job)
@@ -1245,8 +1325,22 @@ int ts_program_run_gang(const ts_job* jobs, int32_t count, const ts_step* steps,
     int nt = 0;
     uint64_t want = 0;
     std::vector<uint64_t> wants;
+    // row packing for kind-2 steps, launched just before the step launch that uses it
+    GangPack pkd;
+    int npk = 0;
+    uint32_t pk_rows = 0;
+    auto flush_pack = [&]() {
+      if (!npk) return TS_OK;
+      const int gx = (int)std::min<uint32_t>((pk_rows + WG - 1) / WG, 1024);
+      hipLaunchKernelGGL(k_gang_pack, dim3(gx, npk), dim3(WG), 0, s, pkd);
+      npk = 0;
+      pk_rows = 0;
+      return hip_rc(hipGetLastError());
+    };
     auto flush_step = [&]() {
       if (!nt) return TS_OK;
+      const int prc = flush_pack();
+      if (prc) return prc;
       // at most 65536 blocks per launch, every job keeping at least one (grid-stride
       // loops take any count); one-block-per-output jobs keep theirs
       const uint64_t cap = 65536;
@@ -1254,7 +1348,7 @@ int ts_program_run_gang(const ts_job* jobs, int32_t count, const ts_step* steps,
       for (int k = 0; k < nt; ++k) {
         t.blk[k] = at;
         uint64_t b = wants[k];
-        if (!t.kind[k] && want > cap) b = std::max<uint64_t>(1, b * cap / want);
+        if (t.kind[k] != 1 && want > cap) b = std::max<uint64_t>(1, b * cap / want);
         at += (uint32_t)b;
       }
       t.blk[nt] = at;
@@ -1293,9 +1387,25 @@ int ts_program_run_gang(const ts_job* jobs, int32_t count, const ts_step* steps,
       t.rec[nt] = rec_dev + sg.begin;
       t.arena[nt] = ar;
       t.kind[nt] = dot_shaped(st) ? 1 : 0;
-      const uint64_t b = t.kind[nt] ? (1ull << st.out_rank) : (uint64_t)grid_for(1ull << st.out_rank);
+      const uint64_t pk = ((uint64_t)jobs[g].arena_elems + 255) & ~uint64_t(255);
+      if (!t.kind[nt] && packable(st) && pk + packed_bytes(st) <= (uint64_t)arena_stride) {
+        const int k = st.sum_rank;
+        const uint32_t lrows = 1u << (st.lhs_rank - k), rrows = 1u << (st.rhs_rank - k);
+        pkd.src[npk] = ar + st.lhs_off;
+        pkd.dst[npk] = ar + pk;
+        pkd.rows[npk] = lrows;
+        pkd.k[npk++] = (uint8_t)k;
+        pkd.src[npk] = ar + st.rhs_off;
+        pkd.dst[npk] = ar + pk + packed_rhs_at(st);
+        pkd.rows[npk] = rrows;
+        pkd.k[npk++] = (uint8_t)k;
+        pk_rows = std::max(pk_rows, std::max(lrows, rrows));
+        t.kind[nt] = 2;
+        t.pk[nt] = (uint32_t)pk;
+      }
+      const uint64_t b = t.kind[nt] == 1 ? (1ull << st.out_rank) : (uint64_t)grid_for(1ull << st.out_rank);
       wants.push_back(b);
-      if (!t.kind[nt]) want += b;
+      if (t.kind[nt] != 1) want += b;
       if (++nt == GANG) {
         const int rc = flush_step();
         if (rc) return rc;

@@ -111,8 +111,31 @@ def run_job_fused(dp: DeviceProgram, j: int, mode: str = "GT0", *, arena=None, s
 
 
 def gang_stride(dp: DeviceProgram) -> int:
-    """Bytes per job arena in a gang: the largest job arena, rounded up to 256."""
-    return (max(16, int(dp.jobs_host["arena_elems"].max())) + 255) & ~255
+    """Bytes per job arena in a gang: the largest job arena (rounded up to 256) plus room
+    for the bit masks of the rows of its largest packable step (summed bits innermost in
+    both operands, 4..10 of them, at least 2^12 outputs, at least two out bits held by one
+    operand only: ``ts_program_run_gang`` packs those rows when the stride has room)."""
+    cached = getattr(dp, "_gang_stride", None)
+    if cached is not None:
+        return cached
+    jobs, st = dp.jobs_host, dp.steps_host
+    base = (np.maximum(16, jobs["arena_elems"].astype(np.int64)) + 255) & ~255
+    extra = np.zeros(len(jobs), dtype=np.int64)
+    if len(st):
+        ro = st["out_rank"].astype(np.int64)
+        k = st["sum_rank"].astype(np.int64)
+        bit = np.arange(PG.MAX_BITS)[None, :] < ro[:, None]
+        one_side = (bit & ((st["o_lpos"] == PG.NOPOS) != (st["o_rpos"] == PG.NOPOS))).sum(axis=1)
+        ok = (((st["flags"] & PG.STEP_KINNER) != 0) & ((st["flags"] & PG.STEP_NORHS) == 0) & (k >= 4) & (k <= 10)
+              & (ro >= 12) & (one_side >= 2))
+        need = np.where(ok, ((np.left_shift(1, np.maximum(st["lhs_rank"].astype(np.int64) - 3, 0)) + 15) & ~15)
+                        + np.left_shift(1, np.maximum(st["rhs_rank"].astype(np.int64) - 3, 0)), 0)
+        for j in range(len(jobs)):  # once per program (cached on it)
+            b, n = int(jobs["step_begin"][j]), int(jobs["num_steps"][j])
+            if n:
+                extra[j] = int(need[b : b + n].max())
+    dp._gang_stride = int((int((base + extra).max()) + 255) & ~255)
+    return dp._gang_stride
 
 
 def run_gang(dp: DeviceProgram, first: int, count: int, mode: str = "GT0", *, arena=None, results=None,

@@ -89,3 +89,33 @@ def test_gang_rejects_bad_arguments():
         run_gang(dp, 2, 10)
     with pytest.raises(ValueError):
         run_gang(dp, 0, 2, arena=torch.empty(16, dtype=torch.uint8, device="cuda"))
+
+
+@pytest.mark.parametrize("seed,k", [(0, 10), (1, 5)])
+def test_gang_packed_rows(seed, k):
+    """n=40 slices whose largest steps have 2^k-byte rows (k = 10 and 5) read by many
+    outputs: gang execution packs those rows into bit masks first; every result equals
+    the unpacked single-job path, in both epilogues."""
+    from tensorsat_amd.contractions import PlannedContraction
+    from tensorsat_amd.executor import DeviceProgram, gang_stride, run_gang, run_job_fused
+    from tensorsat_amd.rel import DeviceRel
+
+    import torch
+
+    cnf = plain_cpu_3sat(seed)
+    d = (cnf.inputs(None) >> cnf.diagram()).flatten()
+    prog = PlannedContraction(DeviceRel, d.wiring, optimize="greedy", slicing_target=26).contract_program(d)
+    st = prog.steps[: int(prog.jobs["num_steps"][0])]
+    assert np.any((st["sum_rank"] == k) & (st["out_rank"] >= 12) & ((st["flags"] & 1) != 0))
+    dp = DeviceProgram(prog)
+    stride = gang_stride(dp)
+    assert stride > int(prog.jobs["arena_elems"].max()) + 255  # room for the masks
+    for mode in ("GT0", "MOD256"):
+        arena = torch.empty(4 * stride, dtype=torch.uint8, device="cuda")
+        flags = run_gang(dp, 0, 4, mode, arena=arena).cpu().numpy()
+        for g in range(4):
+            r0, rr = int(prog.jobs["result_off"][g]), int(prog.jobs["result_rank"][g])
+            got = arena[g * stride + r0 : g * stride + r0 + (1 << rr)].cpu().numpy()
+            want = run_job_fused(dp, g, mode).cpu().numpy()
+            np.testing.assert_array_equal(got, want, err_msg=f"slice {g} {mode}")
+            assert flags[g] == int(want.any())
