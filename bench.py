@@ -126,9 +126,9 @@ def main() -> int:
     ap.add_argument("--budget-nodes", default="auto", help="batched: nodes a search job runs before it hands "
                     "its remaining subtrees over, 'none', or 'auto'")
     ap.add_argument("--seeds", type=int, default=3, help="single / count workloads: instances (seeds 0..)")
-    ap.add_argument("--gang", type=int, default=32,
-                    help="single / plain: largest gang of cubes (slices) advanced together (0: one at a time on "
-                         "--streams streams)")
+    ap.add_argument("--gang", type=int, default=None,
+                    help="single / plain: largest gang of cubes (slices) advanced together (default 16 for the "
+                         "n=120 cubes, 32 for the n=40 slices; 0: one at a time on --streams streams)")
     ap.add_argument("--streams", type=int, default=4, help="single / plain with --gang 0: streams")
     ap.add_argument("--min-sum-bits", type=int, default=6,
                     help="dense steps with at least this many summed bits take the int8 MFMA route (0..27)")
@@ -667,6 +667,7 @@ def run_single(args, seeds, log) -> dict:
 
     from tensorsat_amd.single import decide
 
+    gang = 16 if args.gang is None else args.gang  # the first satisfiable cube comes early: small gangs
     rows = []
     tot_bytes = tot_gpu = tot_plan = 0.0
     mism = 0
@@ -677,7 +678,7 @@ def run_single(args, seeds, log) -> dict:
         prog = PG.sat_batch([cnf], width=26, plan_trials=2)
         plan_s = time.perf_counter() - t0
         progs.append(prog)
-        kw = dict(program=prog, mode=args.mode, streams=args.streams, gang=args.gang)
+        kw = dict(program=prog, mode=args.mode, streams=args.streams, gang=gang)
         decide(cnf, **kw)  # warm-up: code objects, arenas
         r = decide(cnf, **kw)
         gpu_s = r.gpu_s
@@ -717,7 +718,7 @@ def run_single(args, seeds, log) -> dict:
         "value": (tot_plan + tot_gpu) / len(rows), "unit": "s", "steps": len(rows),
         "ms_per_step": tot_gpu * 1000.0 / len(rows), "higher_is_better": False, "dtype": "u8",
         "config": {"workload": "single random 3-SAT n=120 m=510 (seeds 0..), cubes of width <= 26 in gangs of up to "
-                               f"{args.gang} advanced together (0: one at a time on {args.streams} streams)",
+                               f"{gang} advanced together (0: one at a time on {args.streams} streams)",
                    "plan_s_mean": tot_plan / len(rows), "gpu_s_mean": tot_gpu / len(rows), "rows": rows},
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
                      "traffic": None, "kernel": "k_gang_step / k_gang_seq: algorithmic bytes of every step of the "
@@ -745,6 +746,7 @@ def run_plain(args, seeds, log) -> dict:
     from tensorsat_amd.single import decide
     from tensorsat_amd.workloads import plain_cpu_3sat
 
+    gang = 32 if args.gang is None else args.gang  # one structure in every slice: larger gangs
     rows, progs = [], []
     mism = 0
     for seed in seeds:
@@ -757,7 +759,7 @@ def run_plain(args, seeds, log) -> dict:
         prog = c.contract_program(d)
         compile_s = time.perf_counter() - t0
         progs.append(prog)
-        kw = dict(program=prog, mode=args.mode, streams=args.streams, gang=args.gang)
+        kw = dict(program=prog, mode=args.mode, streams=args.streams, gang=gang)
         decide(cnf, **kw)  # warm-up
         r = decide(cnf, **kw)
         launched = r.stats.get("cubes_launched", r.cubes_run)
@@ -802,7 +804,7 @@ def run_plain(args, seeds, log) -> dict:
         "value": host + gpu, "unit": "s", "steps": len(rows), "ms_per_step": gpu * 1000.0,
         "higher_is_better": False, "dtype": "u8",
         "config": {"workload": "plain random 3-SAT n=40 m=170 (seeds 0..), bintree network, greedy tree sliced to "
-                               f"2^26 entries, one compiled job per slice, gangs of up to {args.gang} slices, "
+                               f"2^26 entries, one compiled job per slice, gangs of up to {gang} slices, "
                                "early exit",
                    "host_s_mean": host, "gpu_s_mean": gpu, "rows": rows},
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
