@@ -142,21 +142,51 @@ __device__ __forceinline__ const T& kernarg() {
 // the next step reads them (workgroup barriers between steps).  Saves one launch per
 // step where steps are too small to fill the device.
 // The next record is loaded while the current step runs.
+// Step records are double-buffered in LDS: record i + 1 is stored while step i runs, so
+// the barrier that ends step i also publishes the next record.  Steps of at most 8 out
+// bits whose summed bits are innermost (or absent) compute their offsets directly, one
+// output per thread, without chunk tables: one barrier per step.  Larger steps build
+// their tables first (one more barrier).
 __device__ void seq_run(uint8_t* __restrict__ arena, const ts_step* __restrict__ steps, int n, int mode, Lut& L,
-                        ts_step& s_st) {
+                        ts_step* s_st /* [2] in LDS */) {
   constexpr int RW = (int)(sizeof(ts_step) / 16);
   const int tid = threadIdx.x;
   uint4 r = {0u, 0u, 0u, 0u};
-  if (tid < RW && n > 0) r = reinterpret_cast<const uint4*>(steps)[tid];
+  if (tid < RW && n > 0) reinterpret_cast<uint4*>(&s_st[0])[tid] = reinterpret_cast<const uint4*>(steps)[tid];
+  if (tid < RW && n > 1) r = reinterpret_cast<const uint4*>(steps + 1)[tid];
+  __syncthreads();
   for (int i = 0; i < n; ++i) {
-    if (tid < RW) reinterpret_cast<uint4*>(&s_st)[tid] = r;
-    __syncthreads();
-    if (tid < RW && i + 1 < n) r = reinterpret_cast<const uint4*>(steps + i + 1)[tid];
-    const bool need_c = !(s_st.flags & TS_STEP_KINNER) && s_st.sum_rank > 0;
-    build_lut(s_st, L, need_c, tid);
-    __syncthreads();
-    const uint8_t* rhs = arena + ((s_st.flags & TS_STEP_NORHS) ? s_st.lhs_off : s_st.rhs_off);
-    step_u8(s_st, L, arena + s_st.lhs_off, rhs, arena + s_st.out_off, (uint64_t)tid, (uint64_t)WG, mode);
+    if (tid < RW && i + 1 < n) {
+      reinterpret_cast<uint4*>(&s_st[(i + 1) & 1])[tid] = r;
+      if (i + 2 < n) r = reinterpret_cast<const uint4*>(steps + i + 2)[tid];
+    }
+    const ts_step& st = s_st[i & 1];
+    const bool norhs = st.flags & TS_STEP_NORHS, kinner = st.flags & TS_STEP_KINNER;
+    const int ro = st.out_rank, nc = st.sum_rank;
+    const uint8_t* lhs = arena + st.lhs_off;
+    const uint8_t* rhs = arena + (norhs ? st.lhs_off : st.rhs_off);
+    if (ro <= 8 && (kinner || nc == 0)) {
+      if (tid < (1 << ro)) {
+        uint32_t offl = 0, offr = 0;
+        for (int j = 0; j < ro; ++j)
+          if ((tid >> j) & 1) {
+            if (st.o_lpos[j] != TS_NOPOS) offl |= 1u << st.o_lpos[j];
+            if (st.o_rpos[j] != TS_NOPOS) offr |= 1u << st.o_rpos[j];
+          }
+        uint32_t acc;
+        if (nc == 0) {
+          acc = lhs[offl];
+          if (!norhs) acc &= rhs[offr];
+        } else {
+          acc = popc_and(lhs + offl, rhs + offr, nc, norhs);
+        }
+        arena[st.out_off + tid] = (uint8_t)(mode == TS_MODE_MOD256 ? ((acc & 255u) != 0) : (acc != 0));
+      }
+    } else {
+      build_lut(st, L, !kinner && nc > 0, tid);
+      __syncthreads();
+      step_u8(st, L, lhs, rhs, arena + st.out_off, (uint64_t)tid, (uint64_t)WG, mode);
+    }
     __syncthreads();
   }
 }
@@ -164,7 +194,7 @@ __device__ void seq_run(uint8_t* __restrict__ arena, const ts_step* __restrict__
 __global__ __launch_bounds__(WG) void k_step_seq(uint8_t* __restrict__ arena, const ts_step* __restrict__ steps,
                                                  int begin, int end, int mode) {
   __shared__ Lut L;
-  __shared__ ts_step s_st;
+  __shared__ ts_step s_st[2];
   seq_run(arena, steps + begin, end - begin, mode, L, s_st);
 }
 
@@ -218,7 +248,7 @@ struct GangAny {
 
 __global__ __launch_bounds__(WG) void k_gang_seq(GangSeq) {
   __shared__ Lut L;
-  __shared__ ts_step s_st;
+  __shared__ ts_step s_st[2];
   const GangSeq& g = kernarg<GangSeq>();
   const int j = blockIdx.x;
   seq_run(g.arena[j], g.steps[j], g.count[j], g.mode, L, s_st);
