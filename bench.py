@@ -1290,9 +1290,7 @@ def bench_dense(args) -> int:
         "dtype": "i8", "data": "synthetic",
         "config": {"workload": f"dense composition 2^{mb} x 2^{kb} by 2^{kb} x 2^{nb}, K innermost, "
                                f"density 2^-{kb / 2:g}, {args.mode}",
-                   "kernel_ms": kernel_ms,
-            "k_search_launch_avg_us": launch_avg_us,
-            "k_search_round_us": round_us, "valu_ms": valu_ms, "bytes": nbytes,
+                   "kernel_ms": kernel_ms, "valu_ms": valu_ms, "bytes": nbytes,
                    "hbm_gbs": nbytes / (kernel_ms / 1000.0) / 1e9,
                    "tile_autotune": None if tune is None else {
                        "chosen": "x".join(map(str, DT.TILES[tune["tile"]])),
