@@ -310,6 +310,15 @@ typedef struct {
                                       v & 63 of word v >> 6; variables the leaf leaves free
                                       are 0, its relation's entry the lexicographically
                                       smallest), written by each job that finds one */
+  uint64_t* counts;                /* device, per instance (optional, zeroed by the caller):
+                                      with exhaustive = 1 and early_exit = 0, the exact model
+                                      count -- each owned leaf adds its relation's entry count
+                                      times 2 to the number of free variables outside it (no
+                                      counterpart in the reference package; the #SAT
+                                      prototype's counting einsum, notebooks/prototypes/
+                                      2025-02-13-#sat-tensor-contraction.ipynb cell 12) */
+  int32_t* count_overflow;         /* device int32 (required with counts, zeroed by the
+                                      caller): set to 1 when a count does not fit in 64 bits */
 } ts_search;
 
 /* Branching orders.  SCORE: the free variable of largest (P + 1) * (N + 1), P and N its

@@ -271,7 +271,8 @@ struct Shared {
 struct JobResult {
   int64_t nodes, leaves, conflicts;
   uint64_t digest;
-  int status, max_depth;
+  uint64_t count;  // models under the owned leaves (with p.counts)
+  int status, max_depth, count_overflow;
 };
 
 // Queue mode (ts_search_run_queue): the continuation list is the queue itself; `state`
@@ -501,6 +502,19 @@ __device__ __forceinline__ void search_job(const ts_search& p, const ts_search_j
       if (own) {
         ++r.leaves;
         r.digest += splitmix(splitmix(bits) + (uint64_t)rank);
+        if (p.counts) {
+          // models below this leaf: the relation's entries times 2^(free variables that
+          // no residual clause holds); the assignment so far is propagated and consistent
+          const uint64_t c = (uint64_t)__popcll(bits);
+          const int freev = inst.num_vars - (__popcll(s.ae) + __popcll(s.ao)) - rank;
+          if (c && (freev >= 64 || (freev > 0 && (c >> (64 - freev)) != 0))) {
+            r.count_overflow = 1;
+          } else if (c) {
+            const uint64_t add = c << freev;
+            if (r.count + add < r.count) r.count_overflow = 1;
+            r.count += add;
+          }
+        }
         if (bits != 0) {
           uint64_t w0 = 0, w1 = 0;  // the leaf's assignment's values, natural layout
           if (p.witness) to_natural(s.ve, s.vo, w0, w1, lane);
@@ -616,6 +630,14 @@ __device__ __forceinline__ void run_record(const ts_search& p, const ts_search_j
     search_variant<V...>(need, p, job, inst, sh, lane, r, q);
   }
   if (r.status == 2) atomicOr(&p.verdicts[job.instance], lane == 0 ? TS_SEARCH_INCOMPLETE : 0u);
+  if (p.counts && (r.count || r.count_overflow)) {
+    // whole-wave atomics, lane 0 carrying the value
+    const unsigned long long add = lane == 0 ? (unsigned long long)r.count : 0ull;
+    const unsigned long long old =
+        atomicAdd(reinterpret_cast<unsigned long long*>(&p.counts[job.instance]), add);
+    const bool wrapped = lane == 0 && old + add < old;
+    atomicOr(p.count_overflow, (lane == 0 && (wrapped || r.count_overflow)) ? 1 : 0);
+  }
 }
 
 // A job's statistics, written by lanes 0..9 (one dword each).

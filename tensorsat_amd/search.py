@@ -56,7 +56,8 @@ class _Search(ctypes.Structure):
         ("stats", ctypes.c_void_p), ("width", ctypes.c_int32), ("exhaustive", ctypes.c_int32),
         ("early_exit", ctypes.c_int32), ("spill_capacity", ctypes.c_int32), ("max_nodes", ctypes.c_int64),
         ("spill", ctypes.c_void_p), ("spill_count", ctypes.c_void_p), ("max_clauses", ctypes.c_int32),
-        ("order", ctypes.c_int32), ("witness", ctypes.c_void_p),
+        ("order", ctypes.c_int32), ("witness", ctypes.c_void_p), ("counts", ctypes.c_void_p),
+        ("count_overflow", ctypes.c_void_p),
     ]
 
 
@@ -240,9 +241,10 @@ class SearchSolver:
                  budget: int | None = None, exhaustive: bool = False, max_nodes: int = 1 << 26,
                  stats: bool = False, device: str = "cuda", spill_capacity: int | None = None,
                  max_rounds: int = 100_000, order: str = "score", witness: bool = False, queue: bool = False,
-                 workers: int = 0):
+                 workers: int = 0, count: bool = False):
         """``order``: "score" (the decision default) or "lex" (smallest residual variable,
-        false first); ``witness``: keep each satisfiable instance's assignment
+        false first); ``count``: exact model counts (:meth:`counts`; implies exhaustive);
+        ``witness``: keep each satisfiable instance's assignment
         (:meth:`witnesses`).  "lex" with one job per instance (``split_target`` None,
         ``budget`` None) makes that assignment the lexicographically smallest."""
         if not 0 <= width <= MAX_WIDTH:
@@ -253,6 +255,8 @@ class SearchSolver:
             raise ValueError("order must be 'score' or 'lex'")
         self.order = order
         self.want_witness = witness
+        self.want_count = bool(count)
+        exhaustive = exhaustive or self.want_count
         self.queue = bool(queue) and budget is not None
         self.workers = int(workers)
         self.width = width
@@ -353,6 +357,8 @@ class SearchSolver:
         self.d_verdicts = torch.zeros(max(1, len(inst)), dtype=torch.int32, device=dev)
         self.d_witness = torch.zeros(2 * max(1, len(inst)), dtype=torch.int64, device=dev) \
             if self.want_witness else None
+        self.d_counts = torch.zeros(max(1, len(inst)), dtype=torch.int64, device=dev) if self.want_count else None
+        self.d_count_ovf = torch.zeros(1, dtype=torch.int32, device=dev) if self.want_count else None
         self.num_vars = inst["num_vars"].copy()
         self._queue_checked = True
         if self.queue:
@@ -382,6 +388,8 @@ class SearchSolver:
             N.dptr(spill) if spill is not None else None, N.dptr(self.d_count) if spill is not None else None,
             self.max_clauses, 1 if self.order == "lex" else 0,
             N.dptr(self.d_witness) if self.d_witness is not None else None,
+            N.dptr(self.d_counts) if self.d_counts is not None else None,
+            N.dptr(self.d_count_ovf) if self.d_counts is not None else None,
         )
 
     def launch(self, events: list | None = None) -> None:
@@ -395,6 +403,9 @@ class SearchSolver:
         self.d_verdicts.zero_()
         if self.d_witness is not None:
             self.d_witness.zero_()
+        if self.d_counts is not None:
+            self.d_counts.zero_()
+            self.d_count_ovf.zero_()
         self._stats_rounds = []
         if self.queue:
             self._launch_queue(events)
@@ -468,6 +479,20 @@ class SearchSolver:
         out = np.where(v & SAT, 1, 0)
         out[((v & INCOMPLETE) != 0) & ((v & SAT) == 0)] = -1
         return out
+
+    def counts(self) -> list[int]:
+        """Per instance: the exact number of satisfying assignments over its variables
+        (construct with ``count=True``).  Raises OverflowError when a count does not fit in
+        64 bits and RuntimeError when a job ran out of its node budget."""
+        if self.d_counts is None:
+            raise ValueError("construct the solver with count=True")
+        if int(self.d_count_ovf.item()):
+            raise OverflowError("a model count does not fit in 64 bits")
+        v = self.d_verdicts.cpu().numpy()[: self.num_instances]
+        if np.any(v & INCOMPLETE):
+            raise RuntimeError("a search job ran out of its node budget: counts are incomplete")
+        c = self.d_counts.cpu().numpy().view(np.uint64)[: self.num_instances]
+        return [int(x) for x in c]
 
     def witnesses(self) -> list:
         """Per instance: the assignment of the satisfiable leaf found ("0"/"1" per
