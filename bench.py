@@ -128,7 +128,7 @@ def main() -> int:
                     "its remaining subtrees over, 'none', or 'auto'")
     ap.add_argument("--seeds", type=int, default=3, help="single / count workloads: instances (seeds 0..)")
     ap.add_argument("--gang", type=int, default=None,
-                    help="single / plain: largest gang of cubes (slices) advanced together (default 16 for the "
+                    help="single / plain: largest gang of cubes (slices) advanced together (default 8 for the "
                          "n=120 cubes, 32 for the n=40 slices; 0: one at a time on --streams streams)")
     ap.add_argument("--streams", type=int, default=4, help="single / plain with --gang 0: streams")
     ap.add_argument("--min-sum-bits", type=int, default=6,
@@ -652,7 +652,7 @@ def run_single(args, seeds, log) -> dict:
     """
     One large random 3-SAT instance on the whole GPU (n=120, m=510, seeds 0..): conditioned
     slicing into cubes of width up to 26 (tensors up to 2^26 bytes), cubes in depth-first
-    order in gangs of 16 then 32 advanced together (``ts_program_run_gang``: the runs of
+    order in gangs of 8 advanced together (``ts_program_run_gang``: the runs of
     small steps of a gang's cubes in one launch, one workgroup per cube, their large steps
     in one grid-wide launch; ``single.decide``), until the first satisfiable cube.  Reported: plan and GPU seconds per instance, and the step
     kernels' algorithmic bytes over the GPU time (HBM bound).  Verdicts are checked against
@@ -668,7 +668,7 @@ def run_single(args, seeds, log) -> dict:
 
     from tensorsat_amd.single import decide
 
-    gang = 16 if args.gang is None else args.gang  # the first satisfiable cube comes early: small gangs
+    gang = 8 if args.gang is None else args.gang  # the first satisfiable cube comes early: small gangs
     rows = []
     tot_bytes = tot_gpu = tot_plan = 0.0
     mism = 0
