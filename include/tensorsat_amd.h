@@ -146,8 +146,11 @@ int ts_program_run_fused(const ts_job* job_host, const ts_step* steps_host, cons
  * job's p-th segment, a run of small steps or one large step -- with one launch per
  * phase for all the runs (one workgroup per job), one for the large steps (blocks shared
  * out among the jobs; steps with few outputs and many summed terms get one workgroup
- * per output) and one per MFMA-routed step.  results[g] (device) = 1 if any entry of job
- * g's result is nonzero, else 0.  The slices of a sliced tree (cotengra.py:100-128) and
+ * per output) and one per MFMA-routed step.  Where the stride leaves room after a job's
+ * arena (rounded up to 256 bytes), a large step whose rows of 2^4..2^10 bytes (summed
+ * wires innermost in both operands) are read by many outputs has both operands' rows
+ * packed into bit masks there first and runs on the masks (the same counts).  results[g]
+ * (device) = 1 if any entry of job g's result is nonzero, else 0.  The slices of a sliced tree (cotengra.py:100-128) and
  * the cubes of one instance are gangs: their small steps, one workgroup each when run
  * alone, share the device.  Enqueued on `stream`; no allocation. */
 int ts_program_run_gang(const ts_job* jobs_host, int32_t count, const ts_step* steps_host,
