@@ -153,69 +153,94 @@ __device__ __forceinline__ uint64_t load_row(const uint8_t* p, int nc) {
 
 // ---- vector path for short rows ------------------------------------------------------
 //
-// When the low t out bits of a step are, in each operand, either the bits right above
-// its summed bits in order (a run: the rows of 2^t consecutive outputs are 2^t
-// consecutive rows) or not held at all (one row for all of them), a lane computes 2^t
-// consecutive outputs from one load per operand and one store.
+// When, in each operand, the low t out bits it holds sit right above its summed bits in
+// order (the others not held at all), the rows of 2^t consecutive outputs lie in one
+// chunk of 2^(sum + held) bytes: a lane loads each operand's chunk once (at most 32
+// bytes), picks the row of each of its 2^t outputs out of it, and stores the 2^t results
+// at once.
 
-// t (0 = no vector path): at most 4 bits, at most 32 row bytes per operand and lane.
-__host__ __device__ inline int vec_bits(const ts_step& st) {
+// t (0 = no vector path, at most 4) and the held low out bits of each operand.
+__host__ __device__ inline int vec_plan(const ts_step& st, uint32_t& ml, uint32_t& mr) {
+  ml = mr = 0;
   const int nc = st.sum_rank;
   if (nc > 3 || !(nc == 0 || (st.flags & TS_STEP_KINNER))) return 0;
   const bool norhs = st.flags & TS_STEP_NORHS;
-  int tmax = 5 - nc < 4 ? 5 - nc : 4;
-  if (tmax > st.out_rank) tmax = st.out_rank;
-  const bool lrun = st.o_lpos[0] != TS_NOPOS, rrun = !norhs && st.o_rpos[0] != TS_NOPOS;
-  int t = 0;
-  for (; t < tmax; ++t) {
-    const bool lok = lrun ? st.o_lpos[t] == nc + t : st.o_lpos[t] == TS_NOPOS;
-    const bool rok = norhs || (rrun ? st.o_rpos[t] == nc + t : st.o_rpos[t] == TS_NOPOS);
-    if (!lok || !rok) break;
-  }
-  return t;
-}
-
-template <int BYTES>
-__device__ __forceinline__ void load_bytes(uint64_t* w, const uint8_t* p) {
-  if constexpr (BYTES >= 16) {
-#pragma unroll
-    for (int i = 0; i < BYTES / 16; ++i) {
-      const uint4 v = reinterpret_cast<const uint4*>(p)[i];
-      w[2 * i] = ((uint64_t)v.y << 32) | v.x;
-      w[2 * i + 1] = ((uint64_t)v.w << 32) | v.z;
+  const int tmax = st.out_rank < 4 ? st.out_rank : 4;
+  int best = 0, hl = 0, hr = 0;
+  uint32_t cl = 0, cr = 0;
+  for (int t = 1; t <= tmax; ++t) {
+    const int j = t - 1;
+    if (st.o_lpos[j] != TS_NOPOS) {
+      if (st.o_lpos[j] != nc + hl) break;
+      cl |= 1u << j;
+      ++hl;
     }
-  } else if constexpr (BYTES == 8) {
-    w[0] = *reinterpret_cast<const uint64_t*>(p);
-  } else if constexpr (BYTES == 4) {
-    w[0] = *reinterpret_cast<const uint32_t*>(p);
-  } else if constexpr (BYTES == 2) {
-    w[0] = *reinterpret_cast<const uint16_t*>(p);
-  } else {
-    w[0] = *p;
+    if (!norhs && st.o_rpos[j] != TS_NOPOS) {
+      if (st.o_rpos[j] != nc + hr) break;
+      cr |= 1u << j;
+      ++hr;
+    }
+    if (nc + hl > 5 || nc + hr > 5) break;  // chunks of at most 32 bytes
+    best = t;
+    ml = cl;
+    mr = cr;
+  }
+  return best;
+}
+
+// 2^lg bytes (lg <= 5) into 64-bit words.
+__device__ __forceinline__ void load_chunk(uint64_t (&w)[4], const uint8_t* p, int lg) {
+  switch (lg) {
+    case 0: w[0] = *p; break;
+    case 1: w[0] = *reinterpret_cast<const uint16_t*>(p); break;
+    case 2: w[0] = *reinterpret_cast<const uint32_t*>(p); break;
+    case 3: w[0] = *reinterpret_cast<const uint64_t*>(p); break;
+    case 4: {
+      const uint4 v = *reinterpret_cast<const uint4*>(p);
+      w[0] = ((uint64_t)v.y << 32) | v.x;
+      w[1] = ((uint64_t)v.w << 32) | v.z;
+      break;
+    }
+    default: {
+      const uint4 v = reinterpret_cast<const uint4*>(p)[0], u = reinterpret_cast<const uint4*>(p)[1];
+      w[0] = ((uint64_t)v.y << 32) | v.x;
+      w[1] = ((uint64_t)v.w << 32) | v.z;
+      w[2] = ((uint64_t)u.y << 32) | u.x;
+      w[3] = ((uint64_t)u.w << 32) | u.z;
+    }
   }
 }
 
-// Row i (2^K bytes) of a chunk held as 64-bit words.
+// Row idx (2^K bytes) of a chunk held as 64-bit words (word picked by selects: no
+// indexed register access).
 template <int K>
-__device__ __forceinline__ uint64_t row_of(const uint64_t* w, int i) {
+__device__ __forceinline__ uint64_t chunk_row(const uint64_t (&w)[4], uint32_t idx) {
+  const uint32_t off = idx << K, q = off >> 3;
+  const uint64_t word = q == 0 ? w[0] : q == 1 ? w[1] : q == 2 ? w[2] : w[3];
   if constexpr (K == 3) {
-    return w[i];
+    return word;
   } else {
-    constexpr int PER = 8 >> K;  // rows per word
     constexpr uint64_t MASK = (1ull << (8 << K)) - 1;
-    return (w[i / PER] >> (8 * ((i % PER) << K))) & MASK;
+    return (word >> (8 * (off & 7))) & MASK;
   }
+}
+
+// Bits of i (< 16) at the set positions of m, packed low.
+__device__ __forceinline__ uint32_t pext4(uint32_t i, uint32_t m) {
+  uint32_t r = 0, b = 0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+    if ((m >> j) & 1u) r |= ((i >> j) & 1u) << b++;
+  return r;
 }
 
 template <int K, int V, int NQ>
 __device__ void step_vec_kv(const ts_step& st, const LutT<NQ>& L, const uint8_t* __restrict__ lhs,
                             const uint8_t* __restrict__ rhs, uint8_t* __restrict__ out, uint64_t first,
-                            uint64_t stride) {
+                            uint64_t stride, uint32_t ml, uint32_t mr) {
   constexpr int T = V == 2 ? 1 : V == 4 ? 2 : V == 8 ? 3 : 4;
-  constexpr int RUNB = V << K, ROWB = 1 << K;
-  constexpr int NW = (RUNB + 7) / 8;
   const bool norhs = st.flags & TS_STEP_NORHS;
-  const bool lrun = st.o_lpos[0] != TS_NOPOS, rrun = !norhs && st.o_rpos[0] != TS_NOPOS;
+  const int lgl = K + __popc(ml), lgr = K + __popc(mr);
   const int nqo = nchunks(st.out_rank);
   const uint64_t nq = 1ull << (st.out_rank - T);
   if (first >= nq) return;
@@ -224,22 +249,16 @@ __device__ void step_vec_kv(const ts_step& st, const LutT<NQ>& L, const uint8_t*
   for (uint64_t q = first; q < nq; q += stride) {
     const uint64_t o = q << T;
     const uint32_t hi = (uint32_t)(o >> 8);
-    const uint8_t* pl = lhs + lo_l + hi_off<NQ>(L.ol, hi, nqo);
-    uint64_t a[NW], b[NW];
-    if (lrun) load_bytes<RUNB>(a, pl);
-    else load_bytes<ROWB>(a, pl);
-    if (!norhs) {
-      const uint8_t* pr = rhs + lo_r + hi_off<NQ>(L.orr, hi, nqo);
-      if (rrun) load_bytes<RUNB>(b, pr);
-      else load_bytes<ROWB>(b, pr);
-    }
+    uint64_t a[4] = {0, 0, 0, 0}, b[4] = {0, 0, 0, 0};
+    load_chunk(a, lhs + lo_l + hi_off<NQ>(L.ol, hi, nqo), lgl);
+    if (!norhs) load_chunk(b, rhs + lo_r + hi_off<NQ>(L.orr, hi, nqo), lgr);
     uint32_t res[(V + 3) / 4];
 #pragma unroll
     for (int i = 0; i < (V + 3) / 4; ++i) res[i] = 0;
 #pragma unroll
     for (int i = 0; i < V; ++i) {
-      uint64_t x = row_of<K>(a, lrun ? i : 0);
-      if (!norhs) x &= row_of<K>(b, rrun ? i : 0);
+      uint64_t x = chunk_row<K>(a, pext4((uint32_t)i, ml));
+      if (!norhs) x &= chunk_row<K>(b, pext4((uint32_t)i, mr));
       res[i / 4] |= (uint32_t)(x != 0) << (8 * (i % 4));
     }
     uint8_t* po = out + o;
@@ -250,13 +269,13 @@ __device__ void step_vec_kv(const ts_step& st, const LutT<NQ>& L, const uint8_t*
   }
 }
 
-// Grid-stride over groups of 2^vec_bits(st) outputs (first / stride as for step_u8).
-// Counts are at most 8, so both epilogues reduce to count != 0.
+// Grid-stride over groups of 2^t outputs, t = vec_plan(st, ml, mr) (first / stride as
+// for step_u8).  Counts are at most 8, so both epilogues reduce to count != 0.
 template <int NQ>
 __device__ void step_u8_vec(const ts_step& st, const LutT<NQ>& L, const uint8_t* __restrict__ lhs,
                             const uint8_t* __restrict__ rhs, uint8_t* __restrict__ out, uint64_t first,
-                            uint64_t stride, int t) {
-#define TS_VEC(K, V) step_vec_kv<K, V, NQ>(st, L, lhs, rhs, out, first, stride)
+                            uint64_t stride, int t, uint32_t ml, uint32_t mr) {
+#define TS_VEC(K, V) step_vec_kv<K, V, NQ>(st, L, lhs, rhs, out, first, stride, ml, mr)
   switch (st.sum_rank * 8 + t) {
     case 0 * 8 + 1: TS_VEC(0, 2); break;
     case 0 * 8 + 2: TS_VEC(0, 4); break;
@@ -269,8 +288,11 @@ __device__ void step_u8_vec(const ts_step& st, const LutT<NQ>& L, const uint8_t*
     case 2 * 8 + 1: TS_VEC(2, 2); break;
     case 2 * 8 + 2: TS_VEC(2, 4); break;
     case 2 * 8 + 3: TS_VEC(2, 8); break;
+    case 2 * 8 + 4: TS_VEC(2, 16); break;
     case 3 * 8 + 1: TS_VEC(3, 2); break;
     case 3 * 8 + 2: TS_VEC(3, 4); break;
+    case 3 * 8 + 3: TS_VEC(3, 8); break;
+    case 3 * 8 + 4: TS_VEC(3, 16); break;
     default: break;
   }
 #undef TS_VEC

@@ -37,9 +37,10 @@ __global__ __launch_bounds__(WG) void k_step_u8(const uint8_t* __restrict__ lhs,
   const bool need_c = !(st.flags & TS_STEP_KINNER) && st.sum_rank > 0;
   build_lut(st, L, need_c, threadIdx.x);
   __syncthreads();
-  const int t = vec_bits(st);
+  uint32_t ml, mr;
+  const int t = vec_plan(st, ml, mr);
   if (t > 0)
-    step_u8_vec(st, L, lhs, rhs, out, (uint64_t)blockIdx.x * WG + threadIdx.x, (uint64_t)gridDim.x * WG, t);
+    step_u8_vec(st, L, lhs, rhs, out, (uint64_t)blockIdx.x * WG + threadIdx.x, (uint64_t)gridDim.x * WG, t, ml, mr);
   else
     step_u8(st, L, lhs, rhs, out, (uint64_t)blockIdx.x * WG + threadIdx.x, (uint64_t)gridDim.x * WG, mode);
 }
@@ -295,10 +296,11 @@ __global__ __launch_bounds__(WG) void k_gang_step(GangStep) {
   }
   build_lut(s_st, L, !kinner && s_st.sum_rank > 0, tid);
   __syncthreads();
-  const int t = vec_bits(s_st);
+  uint32_t ml, mr;
+  const int t = vec_plan(s_st, ml, mr);
   if (t > 0)
     step_u8_vec(s_st, L, arena + s_st.lhs_off, rhs, arena + s_st.out_off, (uint64_t)first * WG + tid,
-                (uint64_t)nb * WG, t);
+                (uint64_t)nb * WG, t, ml, mr);
   else
     step_u8(s_st, L, arena + s_st.lhs_off, rhs, arena + s_st.out_off, (uint64_t)first * WG + tid, (uint64_t)nb * WG,
             g.mode);
