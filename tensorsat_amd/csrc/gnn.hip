@@ -200,25 +200,40 @@ __global__ __launch_bounds__(256) void k_segsum(const int32_t* __restrict__ rowp
                                                 const float* __restrict__ weight, int nrows,
                                                 const void* __restrict__ X, int64_t ldx, int xoff_pos,
                                                 int xoff_neg, int x_f32, void* __restrict__ out, int64_t ldo,
-                                                int off_pos, int off_neg, int F, int out_f32_accumulate) {
+                                                int off_pos, int off_neg, int F, int out_f32_accumulate,
+                                                int pairs) {
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   if (wave >= nrows) return;
   const int r = wave;
   float p0 = 0.f, p1 = 0.f, q0 = 0.f, q1 = 0.f;
-  const int f0 = lane, f1 = lane + 64;
+  // pairs: lane l holds features 2l and 2l + 1 (one 4- or 8-byte load per incidence, a
+  // wave reading one contiguous row); otherwise features l and l + 64.
+  const int f0 = pairs ? 2 * lane : lane, f1 = pairs ? 2 * lane + 1 : lane + 64;
   auto load = [&](int e, float& v0, float& v1, float& w) {
     w = weight[e];
     const int64_t base = (int64_t)col[e] * ldx + (w > 0.f ? xoff_pos : xoff_neg);
     v0 = v1 = 0.f;
     if (x_f32) {
       const float* x = reinterpret_cast<const float*>(X) + base;
-      if (f0 < F) v0 = x[f0];
-      if (f1 < F) v1 = x[f1];
+      if (pairs && f1 < F) {
+        const float2 v = *reinterpret_cast<const float2*>(x + f0);
+        v0 = v.x;
+        v1 = v.y;
+      } else {
+        if (f0 < F) v0 = x[f0];
+        if (f1 < F) v1 = x[f1];
+      }
     } else {
       const __bf16* x = reinterpret_cast<const __bf16*>(X) + base;
-      if (f0 < F) v0 = bf2f(x[f0]);
-      if (f1 < F) v1 = bf2f(x[f1]);
+      if (pairs && f1 < F) {
+        const uint32_t u = *reinterpret_cast<const uint32_t*>(x + f0);
+        v0 = bf2f(__builtin_bit_cast(__bf16, (uint16_t)(u & 0xFFFFu)));
+        v1 = bf2f(__builtin_bit_cast(__bf16, (uint16_t)(u >> 16)));
+      } else {
+        if (f0 < F) v0 = bf2f(x[f0]);
+        if (f1 < F) v1 = bf2f(x[f1]);
+      }
     }
   };
   auto add = [&](float v0, float v1, float w) {
@@ -249,17 +264,31 @@ __global__ __launch_bounds__(256) void k_segsum(const int32_t* __restrict__ rowp
   }
   if (out_f32_accumulate) {
     float* o = reinterpret_cast<float*>(out) + (int64_t)r * ldo;
-    if (f0 < F) o[off_pos + f0] += p0 + q0;
-    if (f1 < F) o[off_pos + f1] += p1 + q1;
+    if (pairs && f1 < F) {
+      float2* d = reinterpret_cast<float2*>(o + off_pos + f0);
+      const float2 v = *d;
+      *d = make_float2(v.x + p0 + q0, v.y + p1 + q1);
+    } else {
+      if (f0 < F) o[off_pos + f0] += p0 + q0;
+      if (f1 < F) o[off_pos + f1] += p1 + q1;
+    }
   } else {
     __bf16* o = reinterpret_cast<__bf16*>(out) + (int64_t)r * ldo;
-    if (f0 < F) {
-      o[off_pos + f0] = f2bf(p0);
-      o[off_neg + f0] = f2bf(q0);
-    }
-    if (f1 < F) {
-      o[off_pos + f1] = f2bf(p1);
-      o[off_neg + f1] = f2bf(q1);
+    if (pairs && f1 < F) {
+      const auto pack = [](float a, float b) {
+        return (uint32_t)__builtin_bit_cast(uint16_t, f2bf(a)) | ((uint32_t)__builtin_bit_cast(uint16_t, f2bf(b)) << 16);
+      };
+      *reinterpret_cast<uint32_t*>(o + off_pos + f0) = pack(p0, p1);
+      *reinterpret_cast<uint32_t*>(o + off_neg + f0) = pack(q0, q1);
+    } else {
+      if (f0 < F) {
+        o[off_pos + f0] = f2bf(p0);
+        o[off_neg + f0] = f2bf(q0);
+      }
+      if (f1 < F) {
+        o[off_pos + f1] = f2bf(p1);
+        o[off_neg + f1] = f2bf(q1);
+      }
     }
   }
 }
@@ -383,9 +412,13 @@ int ts_gnn_segsum(const int32_t* rowptr, const int32_t* col, const float* weight
   if (!rowptr || !col || !weight || !X || !out) return TS_EINVAL;
   const int threads = 256;
   const int64_t waves = nrows;
+  // Feature pairs need every pair's first element at an even element offset (8- or
+  // 4-byte aligned pair loads and stores).
+  const int pairs = ((reinterpret_cast<uintptr_t>(X) | reinterpret_cast<uintptr_t>(out)) & 7) == 0 &&
+                    ((ldx | xoff_pos | xoff_neg | ldo | off_pos | off_neg) & 1) == 0;
   hipLaunchKernelGGL(k_segsum, dim3(blocks_for(waves * 64, threads)), dim3(threads), 0,
                      reinterpret_cast<hipStream_t>(stream), rowptr, col, weight, nrows, X, ldx, xoff_pos, xoff_neg,
-                     x_f32, out, ldo, off_pos, off_neg, F, out_f32_accumulate);
+                     x_f32, out, ldo, off_pos, off_neg, F, out_f32_accumulate, pairs);
   return hipGetLastError() == hipSuccess ? TS_OK : TS_EHIP;
 }
 
