@@ -668,7 +668,7 @@ def run_single(args, seeds, log) -> dict:
 
     from tensorsat_amd.single import decide
 
-    gang = 8 if args.gang is None else args.gang  # the first satisfiable cube comes early: small gangs
+    gang = args.gang  # None: single.decide's choice (8 for cubes of different shapes)
     rows = []
     tot_bytes = tot_gpu = tot_plan = 0.0
     mism = 0
@@ -719,7 +719,7 @@ def run_single(args, seeds, log) -> dict:
         "value": (tot_plan + tot_gpu) / len(rows), "unit": "s", "steps": len(rows),
         "ms_per_step": tot_gpu * 1000.0 / len(rows), "higher_is_better": False, "dtype": "u8",
         "config": {"workload": "single random 3-SAT n=120 m=510 (seeds 0..), cubes of width <= 26 in gangs of up to "
-                               f"{gang} advanced together (0: one at a time on {args.streams} streams)",
+                               f"{gang or 8} advanced together (0: one at a time on {args.streams} streams)",
                    "plan_s_mean": tot_plan / len(rows), "gpu_s_mean": tot_gpu / len(rows), "rows": rows},
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
                      "traffic": None, "kernel": "k_gang_step / k_gang_seq: algorithmic bytes of every step of the "
@@ -747,7 +747,7 @@ def run_plain(args, seeds, log) -> dict:
     from tensorsat_amd.single import decide
     from tensorsat_amd.workloads import plain_cpu_3sat
 
-    gang = 32 if args.gang is None else args.gang  # one structure in every slice: larger gangs
+    gang = args.gang  # None: single.decide's choice (32 for slices of one structure)
     rows, progs = [], []
     mism = 0
     for seed in seeds:
@@ -805,7 +805,7 @@ def run_plain(args, seeds, log) -> dict:
         "value": host + gpu, "unit": "s", "steps": len(rows), "ms_per_step": gpu * 1000.0,
         "higher_is_better": False, "dtype": "u8",
         "config": {"workload": "plain random 3-SAT n=40 m=170 (seeds 0..), bintree network, greedy tree sliced to "
-                               f"2^26 entries, one compiled job per slice, gangs of up to {gang} slices, "
+                               f"2^26 entries, one compiled job per slice, gangs of up to {gang or 32} slices, "
                                "early exit",
                    "host_s_mean": host, "gpu_s_mean": gpu, "rows": rows},
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,

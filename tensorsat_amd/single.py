@@ -3,8 +3,8 @@ One large instance on the whole GPU.
 
 * :func:`decide` — a SAT decision for one instance whose intermediates need the whole
   device (the n=120, m=510 workload: cubes of width up to 26, i.e. 2^26-entry tensors):
-  cubes in depth-first order, in gangs (16 cubes, then 32) that advance together phase
-  by phase: the runs of small steps of all the gang's cubes in one launch (one
+  cubes in depth-first order, in gangs (of 8 cubes, or 16 then 32 slices of one sliced
+  tree) that advance together phase by phase: the runs of small steps of all the gang's cubes in one launch (one
   workgroup per cube), their large steps in one grid-wide launch; the flags are read
   after each gang and the first satisfiable cube stops the submission.
 * :func:`decide_budget` — a SAT decision under a wall-clock budget for instances whose
@@ -51,10 +51,14 @@ class SingleResult:
 
 
 def decide(cnf: CNFInstance, *, width: int = 26, mode: str = "GT0", plan_trials: int = 2,
-           program: PG.Program | None = None, streams: int = 4, small_bits: int = 12, gang: int = 32,
+           program: PG.Program | None = None, streams: int = 4, small_bits: int = 12, gang: int | None = None,
            first_gang: int = 16, lookahead: bool = False) -> SingleResult:
     """See the module docstring.  With ``gang`` > 0 the cubes (or slices) run in gangs
-    (``ts_program_run_gang``) of ``first_gang`` jobs, doubling up to ``gang``; with
+    (``ts_program_run_gang``) of ``first_gang`` jobs, doubling up to ``gang`` (None: 32
+    when every job has the same steps -- the slices of one sliced tree, whose shared
+    small-step runs cost the same for 1 or 32 jobs -- else 8, cubes of different shapes
+    where launching past the first satisfiable one costs most; measured on the n=40
+    slices and the n=120 cubes); with
     ``lookahead`` each gang is enqueued before the previous one's flags are read (the
     device never waits for the host, at the price of one more gang past the deciding
     one), else the flags are read between gangs.  ``gang=0`` runs the cubes one by one on
@@ -68,6 +72,11 @@ def decide(cnf: CNFInstance, *, width: int = 26, mode: str = "GT0", plan_trials:
     plan_s = time.perf_counter() - t0
     if len(prog.jobs) == 0:
         return SingleResult(0, plan_s, 0.0, 0, 0, {"host_unsat": True})
+    if gang is None:
+        jobs = prog.jobs
+        same = bool(np.all(jobs["num_steps"] == jobs["num_steps"][0]) and
+                    np.all(jobs["arena_elems"] == jobs["arena_elems"][0]))
+        gang = 32 if same else 8
     if streams < 1 or gang < 0 or first_gang < 1:
         raise ValueError("streams and first_gang must be >= 1, gang >= 0")
     dp = DeviceProgram(prog)
