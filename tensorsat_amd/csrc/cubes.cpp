@@ -26,10 +26,13 @@ static_assert(sizeof(ts_job) == 56, "ts_job layout");
 
 struct ts_program {
   std::vector<ts_job> jobs;
-  std::vector<ts_step> steps;
-  std::vector<uint8_t> leaves;
+  // Job c's steps and leaf bytes, concatenated only by tsc_program_copy (straight into
+  // the caller's arrays at jobs[c].step_begin / leaf_begin, on several threads).
+  std::vector<tsp::Job> parts;
   std::vector<int32_t> stats;  // TSC_NSTAT per instance
   int64_t num_instances = 0;
+  int64_t num_steps = 0, num_leaves = 0;
+  int threads = 1;
 };
 
 namespace tsp {
@@ -105,6 +108,13 @@ struct Enumerator {
   std::vector<int> s_rc, s_rv, s_score, s_pos, s_neg, s_vmap;
   std::vector<std::vector<int>> s_cl;
   int64_t learned_trees = 0, fallback_trees = 0;
+  // emit() scratch.
+  BitNet e_net;
+  std::vector<int> e_vmap;
+  std::vector<uint8_t> e_leaf;
+  std::vector<size_t> e_leaf_off;
+  std::vector<const uint8_t*> e_leaves;
+  std::vector<int32_t> e_path;
 
   Enumerator(const Formula& f_, const tsc_sat_opts& o_, uint64_t sd, std::vector<Job>* jobs,
              std::vector<int32_t>* fr, const float* pr)
@@ -199,28 +209,32 @@ struct Enumerator {
   // step does not fit a record.  Cubes before cube_begin are planned and counted but not
   // kept, so every pass numbers the cubes identically.
   bool emit(const std::vector<int>& rclauses_idx) {
-    // Residual: unsatisfied clauses restricted to free variables.
-    std::vector<int> vmap(f.n, -1);
+    // Residual: unsatisfied clauses restricted to free variables.  The network, the leaf
+    // tables and the path live in members reused by every cube of this enumerator.
+    std::vector<int>& vmap = e_vmap;
+    vmap.assign(f.n, -1);
     int nv = 0;
-    BitNet net;
-    std::vector<std::vector<uint8_t>> leaf_data;
-    for (int c : rclauses_idx) {
-      std::vector<int> ws;
-      std::vector<uint8_t> neg;
-      for (int l : f.clauses[c]) {
+    BitNet& net = e_net;
+    std::vector<uint8_t>& leaf_data = e_leaf;
+    std::vector<size_t>& leaf_off = e_leaf_off;
+    leaf_data.clear();
+    leaf_off.clear();
+    if (net.slot_wires.size() != rclauses_idx.size()) net.slot_wires.resize(rclauses_idx.size());
+    for (size_t r = 0; r < rclauses_idx.size(); ++r) {
+      std::vector<int>& ws = net.slot_wires[r];
+      ws.clear();
+      uint64_t zero = 0;
+      for (int l : f.clauses[rclauses_idx[r]]) {
         int v = std::abs(l) - 1;
         if (val[v] != 0) continue;
         if (vmap[v] < 0) vmap[v] = nv++;
         ws.push_back(vmap[v]);
-        neg.push_back(l < 0);
+        zero = (zero << 1) | (l < 0 ? 1u : 0u);  // the one falsifying assignment, first wire highest
       }
-      std::vector<uint8_t> t((size_t)1 << ws.size(), 1);
-      uint32_t zero = 0;
-      for (size_t i = 0; i < ws.size(); ++i)
-        if (neg[i]) zero |= 1u << (ws.size() - 1 - i);
-      t[zero] = 0;
-      net.slot_wires.push_back(ws);
-      leaf_data.push_back(std::move(t));
+      const size_t base = leaf_data.size();
+      leaf_off.push_back(base);
+      leaf_data.resize(base + ((size_t)1 << ws.size()), 1);
+      leaf_data[base + zero] = 0;
     }
     int free_vars = 0;
     for (int v = 0; v < f.n; ++v)
@@ -228,7 +242,8 @@ struct Enumerator {
     net.S = (int)net.slot_wires.size();
     net.W = nv;
     net.is_out.assign(nv, 0);
-    std::vector<int32_t> path;
+    std::vector<int32_t>& path = e_path;
+    path.clear();
     TreeCost tc;
     if (prio) {
       // Learned proposal: one min-fill pass over the residual steered by the priorities,
@@ -254,8 +269,9 @@ struct Enumerator {
       elim_plan(net, o.plan_trials, seed + (uint64_t)seen * 7919u, path, &tc);
     }
     if (!rclauses_idx.empty() && tc.log2_max > std::min(o.width, TS_MAX_BITS)) return false;
-    std::vector<const uint8_t*> leaves;
-    for (auto& t : leaf_data) leaves.push_back(t.data());
+    std::vector<const uint8_t*>& leaves = e_leaves;
+    leaves.clear();
+    for (size_t o : leaf_off) leaves.push_back(leaf_data.data() + o);
     Job job;
     if (compile_job(net, path, leaves, job) != TSP_OK) return false;
     if (seen++ < o.cube_begin) return true;  // kept by an earlier pass
@@ -577,8 +593,9 @@ int tsc_compile(const tsp_network* n, const int32_t* path, const uint8_t* const*
     j.arena_elems = job.arena_elems;
     j.max_rank = job.max_rank;
     prog->jobs.push_back(j);
-    prog->steps = std::move(job.steps);
-    prog->leaves = std::move(job.leaf_bytes);
+    prog->num_steps = (int64_t)job.steps.size();
+    prog->num_leaves = (int64_t)job.leaf_bytes.size();
+    prog->parts.push_back(std::move(job));
     prog->num_instances = 1;
     prog->stats.assign(TSC_NSTAT, 0);
     prog->stats[0] = 1;
@@ -737,6 +754,12 @@ int tsc_sat_batch(int32_t num_inst, const int32_t* nvars, const int64_t* clause_
   try {
     prog->num_instances = num_inst;
     prog->stats.resize((size_t)num_inst * TSC_NSTAT);
+    prog->threads = threads;
+    size_t tot_jobs = 0;
+    for (const auto& r : res) tot_jobs += r.jobs.size();
+    prog->jobs.reserve(tot_jobs);
+    prog->parts.reserve(tot_jobs);
+    int64_t sb = 0, lb = 0;
     for (int i = 0; i < num_inst; ++i) {
       std::memcpy(&prog->stats[(size_t)i * TSC_NSTAT], res[i].stats, sizeof(res[i].stats));
       for (size_t c = 0; c < res[i].jobs.size(); ++c) {
@@ -744,9 +767,9 @@ int tsc_sat_batch(int32_t num_inst, const int32_t* nvars, const int64_t* clause_
         ts_job j{};
         j.instance = i;
         j.cube = (int32_t)(opts->cube_begin + (int)c);
-        j.step_begin = (int32_t)prog->steps.size();
+        j.step_begin = (int32_t)sb;
         j.num_steps = (int32_t)jb.steps.size();
-        j.leaf_begin = (int64_t)prog->leaves.size();
+        j.leaf_begin = lb;
         j.num_leaf_bytes = (int32_t)jb.leaf_bytes.size();
         j.result_off = jb.result_off;
         j.result_rank = jb.result_rank;
@@ -754,12 +777,15 @@ int tsc_sat_batch(int32_t num_inst, const int32_t* nvars, const int64_t* clause_
         j.arena_elems = jb.arena_elems;
         j.max_rank = jb.max_rank;
         prog->jobs.push_back(j);
-        prog->steps.insert(prog->steps.end(), jb.steps.begin(), jb.steps.end());
-        prog->leaves.insert(prog->leaves.end(), jb.leaf_bytes.begin(), jb.leaf_bytes.end());
+        sb += (int64_t)jb.steps.size();
+        lb += (int64_t)jb.leaf_bytes.size();
+        prog->parts.push_back(std::move(jb));
       }
       res[i].jobs.clear();
       res[i].jobs.shrink_to_fit();
     }
+    prog->num_steps = sb;
+    prog->num_leaves = lb;
   } catch (...) {
     delete prog;
     return TSP_ENOMEM;
@@ -772,8 +798,8 @@ int tsc_program_sizes(const ts_program* p, int64_t* num_jobs, int64_t* num_steps
                       int64_t* num_leaf_bytes, int64_t* num_instances) {
   if (!p) return TSP_EINVAL;
   if (num_jobs) *num_jobs = (int64_t)p->jobs.size();
-  if (num_steps) *num_steps = (int64_t)p->steps.size();
-  if (num_leaf_bytes) *num_leaf_bytes = (int64_t)p->leaves.size();
+  if (num_steps) *num_steps = p->num_steps;
+  if (num_leaf_bytes) *num_leaf_bytes = p->num_leaves;
   if (num_instances) *num_instances = p->num_instances;
   return TSP_OK;
 }
@@ -782,8 +808,41 @@ int tsc_program_copy(const ts_program* p, ts_job* jobs, ts_step* steps, uint8_t*
                      int32_t* inst_stats) {
   if (!p) return TSP_EINVAL;
   if (jobs && !p->jobs.empty()) std::memcpy(jobs, p->jobs.data(), p->jobs.size() * sizeof(ts_job));
-  if (steps && !p->steps.empty()) std::memcpy(steps, p->steps.data(), p->steps.size() * sizeof(ts_step));
-  if (leaf_bytes && !p->leaves.empty()) std::memcpy(leaf_bytes, p->leaves.data(), p->leaves.size());
+  if (steps || leaf_bytes) {
+    const int64_t nj = (int64_t)p->parts.size();
+    auto copy_jobs = [&](int64_t b, int64_t e) {
+      for (int64_t c = b; c < e; ++c) {
+        const tsp::Job& jb = p->parts[c];
+        const ts_job& j = p->jobs[c];
+        if (steps && !jb.steps.empty())
+          std::memcpy(steps + j.step_begin, jb.steps.data(), jb.steps.size() * sizeof(ts_step));
+        if (leaf_bytes && !jb.leaf_bytes.empty())
+          std::memcpy(leaf_bytes + j.leaf_begin, jb.leaf_bytes.data(), jb.leaf_bytes.size());
+      }
+    };
+    // At least 16 MiB per thread: a small program is copied on the calling thread.
+    const int64_t bytes = p->num_steps * (int64_t)sizeof(ts_step) + p->num_leaves;
+    const int nt = (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)p->threads, bytes >> 24, nj}));
+    if (nt <= 1) {
+      copy_jobs(0, nj);
+    } else {
+      std::vector<std::thread> pool;
+      bool spawned = true;
+      try {
+        for (int t = 1; t < nt; ++t) pool.emplace_back(copy_jobs, nj * t / nt, nj * (t + 1) / nt);
+      } catch (...) {
+        spawned = false;
+      }
+      if (!spawned) {  // no threads left: wait for the ones started, then copy everything here
+        for (auto& th : pool) th.join();
+        pool.clear();
+        copy_jobs(0, nj);
+      } else {
+        copy_jobs(0, nj / nt);
+      }
+      for (auto& th : pool) th.join();
+    }
+  }
   if (inst_stats && !p->stats.empty())
     std::memcpy(inst_stats, p->stats.data(), p->stats.size() * sizeof(int32_t));
   return TSP_OK;

@@ -50,14 +50,23 @@ inline void merged(const std::vector<int>& a, const std::vector<int>& b,
   }
 }
 
+// Resize a list of wire lists to n empty lists, keeping the inner lists' capacity (the
+// planner's per-thread scratch is reused for every cube, so steady state allocates nothing).
+inline void reset_lists(std::vector<std::vector<int>>& v, size_t n) {
+  if (v.size() != n) v.resize(n);
+  for (auto& x : v) x.clear();
+}
+
 }  // namespace
 
 bool path_legs(const BitNet& net, const std::vector<int32_t>& path,
                std::vector<std::vector<int>>& legs) {
   const int S = net.S;
-  legs.assign(std::max(1, 2 * S - 1), {});
-  std::vector<int> count(net.W, 0);
-  std::vector<char> alive(legs.size(), 0);
+  reset_lists(legs, (size_t)std::max(1, 2 * S - 1));
+  thread_local std::vector<int> count;
+  thread_local std::vector<char> alive;
+  count.assign(net.W, 0);
+  alive.assign(legs.size(), 0);
   for (int s = 0; s < S; ++s) {
     legs[s] = net.slot_wires[s];
     std::sort(legs[s].begin(), legs[s].end());
@@ -84,8 +93,10 @@ bool path_legs(const BitNet& net, const std::vector<int32_t>& path,
 TreeCost tree_cost(const BitNet& net, const std::vector<int32_t>& path) {
   TreeCost tc;
   const int S = net.S;
-  std::vector<std::vector<int>> legs(std::max(1, 2 * S - 1));
-  std::vector<int> count(net.W, 0);
+  thread_local std::vector<std::vector<int>> legs;
+  thread_local std::vector<int> count;
+  reset_lists(legs, (size_t)std::max(1, 2 * S - 1));
+  count.assign(net.W, 0);
   double live = 0;
   for (int s = 0; s < S; ++s) {
     legs[s] = net.slot_wires[s];
@@ -140,14 +151,16 @@ struct Bits {
 int elim_order(int nv, const std::vector<std::vector<int>>& cliques, const std::vector<char>& keep,
                bool min_fill, std::mt19937_64* rng, std::vector<int>& order, const float* prior = nullptr,
                float lam = 0.f, int stop_above = INT_MAX) {
-  Bits adj;
+  thread_local Bits adj;
+  thread_local std::vector<char> gone;
+  thread_local std::vector<int> deg, cand, nb;
   adj.init(nv, nv);
   for (const auto& c : cliques)
     for (int u : c)
       for (int v : c)
         if (u != v) adj.row(u)[v >> 6] |= 1ull << (v & 63);
-  std::vector<char> gone(nv, 0);
-  std::vector<int> deg(nv, 0);
+  gone.assign(nv, 0);
+  deg.assign(nv, 0);
   for (int v = 0; v < nv; ++v) {
     int d = 0;
     for (int i = 0; i < adj.nw; ++i) d += __builtin_popcountll(adj.row(v)[i]);
@@ -158,7 +171,6 @@ int elim_order(int nv, const std::vector<std::vector<int>>& cliques, const std::
   int remaining = 0;
   for (int v = 0; v < nv; ++v)
     if (!keep[v]) ++remaining;
-  std::vector<int> cand, nb;
   while (remaining > 0) {
     long best = -1;
     double bestp = 0.0;
@@ -241,10 +253,13 @@ void order_to_path(const BitNet& net, const std::vector<int>& order, std::vector
   const int S = net.S;
   path.clear();
   if (S <= 1) return;
-  std::vector<std::vector<int>> legs(2 * S - 1);
-  std::vector<int> count(net.W, 0);
-  std::vector<char> alive(2 * S - 1, 0);
-  std::vector<std::vector<int>> holders(net.W);
+  thread_local std::vector<std::vector<int>> legs, holders;
+  thread_local std::vector<int> count, group;
+  thread_local std::vector<char> alive;
+  reset_lists(legs, (size_t)(2 * S - 1));
+  reset_lists(holders, (size_t)net.W);
+  count.assign(net.W, 0);
+  alive.assign(2 * S - 1, 0);
   for (int s = 0; s < S; ++s) {
     legs[s] = net.slot_wires[s];
     std::sort(legs[s].begin(), legs[s].end());
@@ -277,7 +292,7 @@ void order_to_path(const BitNet& net, const std::vector<int>& order, std::vector
     if (legs[x].size() != legs[y].size()) return legs[x].size() > legs[y].size();
     return x > y;
   };
-  std::vector<int> group;
+  group.clear();
   for (int v : order) {
     group.clear();
     for (int t : holders[v])
@@ -341,11 +356,11 @@ void elim_plan(const BitNet& net, int trials, uint64_t seed, std::vector<int32_t
     }
     return;
   }
-  std::vector<char> keep(net.W, 0);
-  for (int w = 0; w < net.W; ++w) keep[w] = net.is_out[w];
+  thread_local std::vector<char> keep;
+  thread_local std::vector<int> order;
+  thread_local std::vector<int32_t> cur;
+  keep.assign(net.is_out.begin(), net.is_out.end());
   std::mt19937_64 rng(seed);
-  std::vector<int> order;
-  std::vector<int32_t> cur;
   TreeCost best{};
   bool have = false;
   const bool fill_ok = net.W <= 600;
@@ -376,45 +391,52 @@ void prior_plan(const BitNet& net, const float* prior, float lam, std::vector<in
 }
 
 int mindeg_width(int nv, const std::vector<std::vector<int>>& clauses_vars, int stop_above) {
-  std::vector<char> keep(nv, 0);
-  std::vector<int> order;
+  thread_local std::vector<char> keep;
+  thread_local std::vector<int> order;
+  keep.assign(nv, 0);
   return elim_order(nv, clauses_vars, keep, false, nullptr, order, nullptr, 0.f, stop_above);
 }
 
 namespace {
 
 struct Allocator {
-  // First-fit over a free list of [off, off+len) blocks sorted by offset.
-  std::map<int64_t, int64_t> free_;  // off -> len
+  // First-fit over a free list of [off, off+len) blocks sorted by offset (a few dozen
+  // blocks at most: a sorted array beats a tree and is reused across jobs).
+  std::vector<std::pair<int64_t, int64_t>> free_;  // (off, len), ascending off
   int64_t top = 0;
+  void reset() {
+    free_.clear();
+    top = 0;
+  }
   static int64_t align(int64_t n) { return (n + 15) & ~int64_t(15); }
   int64_t alloc(int64_t n) {
     n = align(n);
-    for (auto it = free_.begin(); it != free_.end(); ++it) {
-      if (it->second >= n) {
-        int64_t off = it->first, len = it->second;
-        free_.erase(it);
-        if (len > n) free_[off + n] = len - n;
+    for (size_t i = 0; i < free_.size(); ++i) {
+      if (free_[i].second >= n) {
+        const int64_t off = free_[i].first, len = free_[i].second;
+        if (len > n) free_[i] = {off + n, len - n};  // stays below the next block
+        else free_.erase(free_.begin() + (std::ptrdiff_t)i);
         return off;
       }
     }
     // Extend the top, merging with a free block that ends at the top.
-    if (!free_.empty()) {
-      auto last = std::prev(free_.end());
-      if (last->first + last->second == top) {
-        int64_t off = last->first;
-        free_.erase(last);
-        top = off + n;
-        return off;
-      }
+    if (!free_.empty() && free_.back().first + free_.back().second == top) {
+      const int64_t off = free_.back().first;
+      free_.pop_back();
+      top = off + n;
+      return off;
     }
-    int64_t off = top;
+    const int64_t off = top;
     top += n;
     return off;
   }
   void release(int64_t off, int64_t n) {
     n = align(n);
-    auto it = free_.emplace(off, n).first;
+    auto it = std::lower_bound(free_.begin(), free_.end(), std::make_pair(off, int64_t(0)),
+                               [](const std::pair<int64_t, int64_t>& a, const std::pair<int64_t, int64_t>& b) {
+                                 return a.first < b.first;
+                               });
+    it = free_.insert(it, {off, n});
     auto nx = std::next(it);
     if (nx != free_.end() && it->first + it->second == nx->first) {
       it->second += nx->second;
@@ -459,20 +481,34 @@ void permute_leaf(const uint8_t* src, const std::vector<int>& src_wires,
 int compile_job(const BitNet& net_in, const std::vector<int32_t>& path_in,
                 const std::vector<const uint8_t*>& leaves_in, Job& job) {
   // Dangling out wires get an all-ones rank-1 leaf merged at the end
-  // (SimpleContraction's spider(t, 1) products, simple.py:281-285).
-  BitNet net = net_in;
-  std::vector<int32_t> path = path_in;
-  std::vector<const uint8_t*> leaves = leaves_in;
+  // (SimpleContraction's spider(t, 1) products, simple.py:281-285).  The network, path
+  // and leaves are copied only then.
+  BitNet net_copy;
+  std::vector<int32_t> path_copy;
+  std::vector<const uint8_t*> leaves_copy;
+  const BitNet* netp = &net_in;
+  const std::vector<int32_t>* pathp = &path_in;
+  const std::vector<const uint8_t*>* leavesp = &leaves_in;
   static const uint8_t kOnes[2] = {1, 1};
   static const uint8_t kOne[1] = {1};
   {
-    std::vector<char> used(net.W, 0);
-    for (auto& ws : net.slot_wires)
+    thread_local std::vector<char> used;
+    used.assign(net_in.W, 0);
+    for (auto& ws : net_in.slot_wires)
       for (int w : ws) used[w] = 1;
     std::vector<int> dangling;
-    for (int w : net.out_wires)
+    for (int w : net_in.out_wires)
       if (!used[w]) dangling.push_back(w);
     if (!dangling.empty()) {
+      net_copy = net_in;
+      path_copy = path_in;
+      leaves_copy = leaves_in;
+      netp = &net_copy;
+      pathp = &path_copy;
+      leavesp = &leaves_copy;
+      BitNet& net = net_copy;
+      std::vector<int32_t>& path = path_copy;
+      std::vector<const uint8_t*>& leaves = leaves_copy;
       const int S0 = net.S;
       int root = S0 == 0 ? -1 : (S0 == 1 ? 0 : 2 * S0 - 2);
       // Re-number merges: existing merge ids shift by the number of new leaves.
@@ -498,13 +534,18 @@ int compile_job(const BitNet& net_in, const std::vector<int32_t>& path_in,
       }
     }
   }
+  const BitNet& net = *netp;
+  const std::vector<int32_t>& path = *pathp;
+  const std::vector<const uint8_t*>& leaves = *leavesp;
   const int S = net.S;
   job = Job();
+  job.steps.reserve((size_t)std::max(1, S));
   for (auto& ws : net.slot_wires) {
-    std::vector<int> u = ws;
-    std::sort(u.begin(), u.end());
-    if (std::adjacent_find(u.begin(), u.end()) != u.end()) return TSP_EINVAL;  // diagonals are extracted by the caller
     if ((int)ws.size() > TS_MAX_BITS) return TSP_EINVAL;
+    int u[TS_MAX_BITS];
+    std::copy(ws.begin(), ws.end(), u);
+    std::sort(u, u + ws.size());
+    if (std::adjacent_find(u, u + ws.size()) != u + ws.size()) return TSP_EINVAL;  // diagonals are extracted by the caller
   }
   if (S == 0) {
     job.leaf_bytes.assign(16, 0);
@@ -514,7 +555,7 @@ int compile_job(const BitNet& net_in, const std::vector<int32_t>& path_in,
     job.result_rank = 0;
     return TSP_OK;
   }
-  std::vector<std::vector<int>> legs;
+  thread_local std::vector<std::vector<int>> legs;
   if (!path_legs(net, path, legs)) return TSP_EINVAL;
   const int root = S == 1 ? 0 : 2 * S - 2;
   // The root must hold exactly the out wires (dangling ones were added above).
@@ -525,7 +566,8 @@ int compile_job(const BitNet& net_in, const std::vector<int32_t>& path_in,
     if (S > 1 && o != legs[root]) return TSP_EINVAL;
   }
   const int nodes = std::max(1, 2 * S - 1);
-  std::vector<std::vector<int>> layout(nodes);
+  thread_local std::vector<std::vector<int>> layout;
+  reset_lists(layout, (size_t)nodes);
   // Unary root fix-up when a single leaf is not already the out tensor.
   bool unary = false;
   if (S == 1) {
@@ -535,11 +577,18 @@ int compile_job(const BitNet& net_in, const std::vector<int32_t>& path_in,
   } else {
     layout[root] = net.out_wires;
   }
-  std::vector<std::array<std::vector<int>, 3>> sums(std::max(0, S - 1));  // Kc, La, Lb
+  thread_local std::vector<std::array<std::vector<int>, 3>> sums;  // Kc, La, Lb
+  sums.resize(std::max(0, S - 1));
+  thread_local std::vector<int> common, kc, la, lb, lay_a, lay_b, csum;
   for (int i = S - 2; i >= 0; --i) {
     const int a = path[2 * i], b = path[2 * i + 1], k = S + i;
     const auto &A = legs[a], &B = legs[b], &K = legs[k];
-    std::vector<int> common, kc, la, lb;
+    common.clear();
+    kc.clear();
+    la.clear();
+    lb.clear();
+    lay_a.clear();
+    lay_b.clear();
     std::set_intersection(A.begin(), A.end(), B.begin(), B.end(), std::back_inserter(common));
     for (int w : common)
       if (!std::binary_search(K.begin(), K.end(), w)) kc.push_back(w);
@@ -549,7 +598,6 @@ int compile_job(const BitNet& net_in, const std::vector<int32_t>& path_in,
     for (int w : B)
       if (!std::binary_search(K.begin(), K.end(), w) && !std::binary_search(A.begin(), A.end(), w))
         lb.push_back(w);
-    std::vector<int> lay_a, lay_b;
     for (int w : layout[k]) {
       if (std::binary_search(A.begin(), A.end(), w)) lay_a.push_back(w);
       if (std::binary_search(B.begin(), B.end(), w)) lay_b.push_back(w);
@@ -558,13 +606,17 @@ int compile_job(const BitNet& net_in, const std::vector<int32_t>& path_in,
     lay_a.insert(lay_a.end(), kc.begin(), kc.end());
     lay_b.insert(lay_b.end(), lb.begin(), lb.end());
     lay_b.insert(lay_b.end(), kc.begin(), kc.end());
-    layout[a] = lay_a;
-    layout[b] = lay_b;
-    sums[i] = {kc, la, lb};
+    layout[a].swap(lay_a);
+    layout[b].swap(lay_b);
+    sums[i][0].assign(kc.begin(), kc.end());
+    sums[i][1].assign(la.begin(), la.end());
+    sums[i][2].assign(lb.begin(), lb.end());
   }
   // Leaves in their layouts, packed at the start of the arena.
-  Allocator al;
-  std::vector<int64_t> off(nodes, -1);
+  thread_local Allocator al;
+  al.reset();
+  thread_local std::vector<int64_t> off;
+  off.assign(nodes, -1);
   int64_t leaf_end = 0;
   for (int s = 0; s < S; ++s) {
     off[s] = al.alloc(int64_t(1) << layout[s].size());
@@ -632,7 +684,7 @@ int compile_job(const BitNet& net_in, const std::vector<int32_t>& path_in,
   if (S == 1) {
     if (unary) {
       // out = sum over the leaf's non-out wires, in out order.
-      std::vector<int> csum;
+      csum.clear();
       for (int w : layout[0])
         if (std::find(net.out_wires.begin(), net.out_wires.end(), w) == net.out_wires.end())
           csum.push_back(w);
@@ -658,7 +710,7 @@ int compile_job(const BitNet& net_in, const std::vector<int32_t>& path_in,
     const int a = path[2 * i], b = path[2 * i + 1], k = S + i;
     off[k] = al.alloc(int64_t(1) << layout[k].size());
     // Summed bits: contracted (innermost, LSB first), then lhs-only, then rhs-only.
-    std::vector<int> csum;
+    csum.clear();
     const auto& kc = sums[i][0];
     for (int j = (int)kc.size() - 1; j >= 0; --j) csum.push_back(kc[j]);
     for (int w : sums[i][1]) csum.push_back(w);

@@ -129,9 +129,9 @@ def _take(handle) -> Program:
     try:
         _plan._check(L.tsc_program_sizes(handle, ctypes.byref(nj), ctypes.byref(ns), ctypes.byref(nl),
                                          ctypes.byref(ni)), "tsc_program_sizes")
-        jobs = np.zeros(nj.value, dtype=JOB_DTYPE)
-        steps = np.zeros(ns.value, dtype=STEP_DTYPE)
-        leaves = np.zeros(nl.value, dtype=np.uint8)
+        jobs = np.empty(nj.value, dtype=JOB_DTYPE)  # every byte written by tsc_program_copy
+        steps = np.empty(ns.value, dtype=STEP_DTYPE)
+        leaves = np.empty(nl.value, dtype=np.uint8)
         stats = np.zeros((ni.value, NSTAT), dtype=np.int32)
         _plan._check(L.tsc_program_copy(handle, jobs.ctypes.data, steps.ctypes.data, leaves.ctypes.data,
                                         stats.ctypes.data), "tsc_program_copy")
