@@ -182,7 +182,14 @@ int elim_order(int nv, const std::vector<std::vector<int>>& cliques, const std::
       if (min_fill) {
         long fill = 0;
         const uint64_t* rv = adj.row(v);
-        for (int i = 0; i < adj.nw; ++i) {
+        if (adj.nw == 1) {  // at most 64 vertices: one word per row
+          const uint64_t r0 = rv[0];
+          for (uint64_t m = r0; m; m &= m - 1) {
+            const int u = __builtin_ctzll(m);
+            fill += __builtin_popcountll(r0 & ~adj.d[u] & ~(1ull << u));
+          }
+        }
+        for (int i = 0; i < adj.nw && adj.nw > 1; ++i) {
           uint64_t m = rv[i];
           while (m) {
             int u = i * 64 + __builtin_ctzll(m);
@@ -452,12 +459,6 @@ struct Allocator {
   }
 };
 
-inline uint8_t pos_in(const std::vector<int>& layout, int w) {
-  for (size_t i = 0; i < layout.size(); ++i)
-    if (layout[i] == w) return (uint8_t)(layout.size() - 1 - i);
-  return TS_NOPOS;
-}
-
 void permute_leaf(const uint8_t* src, const std::vector<int>& src_wires,
                   const std::vector<int>& dst_wires, uint8_t* dst) {
   const int r = (int)dst_wires.size();
@@ -629,6 +630,9 @@ int compile_job(const BitNet& net_in, const std::vector<int32_t>& path_in,
   }
   int max_rank = 0;
   for (int s = 0; s < S; ++s) max_rank = std::max(max_rank, (int)layout[s].size());
+  thread_local std::vector<uint8_t> pa, pb;  // wire -> bit position in lhs / rhs (TS_NOPOS: absent)
+  pa.assign(net.W, TS_NOPOS);
+  pb.assign(net.W, TS_NOPOS);
   auto emit = [&](int a, int b, int k, const std::vector<int>& csum) -> int {
     ts_step st;
     std::memset(&st, 0, sizeof(st));
@@ -646,18 +650,28 @@ int compile_job(const BitNet& net_in, const std::vector<int32_t>& path_in,
     st.out_rank = (uint8_t)LO.size();
     st.sum_rank = (uint8_t)csum.size();
     const int ro = (int)LO.size();
+    // Bit position of each wire in the operands' layouts (first occurrence, counted from
+    // the innermost axis), through per-wire tables cleared again right after use.
+    const auto& LA = layout[a];
+    const std::vector<int>* LB = b >= 0 ? &layout[b] : nullptr;
+    for (int i = (int)LA.size() - 1; i >= 0; --i) pa[LA[i]] = (uint8_t)(LA.size() - 1 - i);
+    if (LB)
+      for (int i = (int)LB->size() - 1; i >= 0; --i) pb[(*LB)[i]] = (uint8_t)(LB->size() - 1 - i);
     for (int j = 0; j < ro; ++j) {
       int w = LO[ro - 1 - j];
-      st.o_lpos[j] = pos_in(layout[a], w);
-      st.o_rpos[j] = b >= 0 ? pos_in(layout[b], w) : TS_NOPOS;
+      st.o_lpos[j] = pa[w];
+      st.o_rpos[j] = pb[w];
     }
     bool kinner = true;
     for (size_t j = 0; j < csum.size(); ++j) {
       int w = csum[j];
-      st.c_lpos[j] = pos_in(layout[a], w);
-      st.c_rpos[j] = b >= 0 ? pos_in(layout[b], w) : TS_NOPOS;
+      st.c_lpos[j] = pa[w];
+      st.c_rpos[j] = pb[w];
       if (st.c_lpos[j] != j || (b >= 0 && st.c_rpos[j] != j)) kinner = false;
     }
+    for (int w : LA) pa[w] = TS_NOPOS;
+    if (LB)
+      for (int w : *LB) pb[w] = TS_NOPOS;
     // Every operand bit must be covered exactly once.
     for (int side = 0; side < (b >= 0 ? 2 : 1); ++side) {
       int rank = side ? st.rhs_rank : st.lhs_rank;
