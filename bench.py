@@ -347,6 +347,22 @@ def bench_search(args) -> int:
     verdicts = solver.verdicts()
     value = len(cnfs) / (ms_per_step / 1000.0)
     jobs = solver.host_jobs
+    # Streamed end to end: the batch submitted 16 times in a row from the CNF list; two
+    # solvers alternate so the host packs, plans and stages submission k+1 while the GPU
+    # searches submission k (one launch per batch: no hand-over budget).
+    stream = None
+    if budget is None:
+        reps = 16
+        list(SE.decide_stream([cnfs] * 2, width=width, split_target=split))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        outs = list(SE.decide_stream([cnfs] * reps, width=width, split_target=split))
+        torch.cuda.synchronize()
+        per = (time.perf_counter() - t0) / reps
+        stream = {"s_per_batch": per, "instances_per_s": len(cnfs) / per, "batches": reps,
+                  "mismatches": int(sum(int((v != verdicts).sum()) for v in outs))}
+        log(f"streamed end to end: {per * 1000:.2f} ms per batch = {len(cnfs) / per / 1e6:.2f} M inst/s "
+            f"({reps} submissions, {stream['mismatches']} verdict mismatches)")
     # Critical path: the statistics of one more launch (per job branch nodes).
     st = SE.SearchSolver(width=width, split_target=split, budget=budget, stats=True)
     st.prepare(cnfs)
@@ -398,6 +414,7 @@ def bench_search(args) -> int:
         parity["dpll_mismatches"] = int((dp != verdicts).sum())
         log(f"independent DPLL cross-check: {parity['dpll_mismatches']} mismatches over {len(cnfs)}")
     parity["end_to_end_mismatches"] = int((v_e2e != verdicts).sum())
+    parity["streamed_mismatches"] = stream["mismatches"] if stream else None
     parity["undecided"] = int((verdicts < 0).sum())
 
     nested = {}
@@ -444,6 +461,7 @@ def bench_search(args) -> int:
             "end_to_end_instances_per_s": len(cnfs) / e2e_s,
             "end_to_end": "CNF list -> packed clauses -> job list -> upload -> launch -> verdicts on the host "
                           "(median of 5)",
+            "end_to_end_streamed": stream,
             "host_s": host,
             "branch_nodes": nodes_total,
             "leaves": int(js["leaves"].sum()),
@@ -475,7 +493,7 @@ def bench_search(args) -> int:
     }
     print(json.dumps(out), flush=True)
     bad = [parity[k] for k in ("oracle_search_mismatches", "reference_path_mismatches", "dpll_mismatches",
-                               "end_to_end_mismatches", "undecided") if parity.get(k)]
+                               "end_to_end_mismatches", "streamed_mismatches", "undecided") if parity.get(k)]
     nested_bad = any(v.get("status") == "mismatch" for v in nested.values())
     return 1 if bad or nested_bad else 0
 

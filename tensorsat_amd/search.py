@@ -105,6 +105,14 @@ def _align(n: int) -> int:
     return (n + 255) & ~255
 
 
+def _decode(v: np.ndarray) -> np.ndarray:
+    """Verdict words -> 1 SAT, 0 UNSAT, -1 undecided."""
+    v = v.astype(np.int64)
+    out = np.where(v & SAT, 1, 0)
+    out[((v & INCOMPLETE) != 0) & ((v & SAT) == 0)] = -1
+    return out
+
+
 def log2_search_size(num_vars: np.ndarray, num_clauses: np.ndarray) -> np.ndarray:
     """Estimated log2 of the branch nodes and conflicts one search needs with early exit,
     for random 3-SAT-like instances: linear in n with a slope that grows with the clause
@@ -271,13 +279,15 @@ class SearchSolver:
         self.max_rounds = max_rounds
         self.timing: dict = {}
         self._pin = self._dev_stage = self._staged = None
+        self._vpin = self._vdone = None
         self.rounds = 0
         self.round_jobs: list[int] = []
 
-    def prepare(self, cnfs) -> None:
+    def prepare(self, cnfs, sync: bool = True) -> None:
         """Pack, plan the jobs and upload (host work, timed in ``self.timing``).  Clauses,
         instance records and jobs are written straight into one pinned staging buffer
-        (kept by the solver, grown as needed) and sent with one host-to-device copy."""
+        (kept by the solver, grown as needed) and sent with one host-to-device copy.
+        ``sync`` False returns once the copy is enqueued on the current stream."""
         import torch
 
         if self._staged is not None:
@@ -316,7 +326,8 @@ class SearchSolver:
         validate(words, inst, None, self.width, literals=False)  # packer and planner checked the rest
         t2 = time.perf_counter()
         self.upload(words, inst, jobs, staged=layout)
-        torch.cuda.synchronize()
+        if sync:
+            torch.cuda.synchronize()
         self.timing.update(pack_s=t1 - t0, plan_s=t2 - t1, upload_s=time.perf_counter() - t2)
 
     def _stage(self, nbytes: int) -> None:
@@ -475,10 +486,26 @@ class SearchSolver:
         """Per instance: 1 SAT, 0 UNSAT, -1 undecided (a job ran out of its node budget)."""
         if self.queue and not self._queue_checked:
             self.queue_records()
-        v = self.d_verdicts.cpu().numpy()[: self.num_instances].astype(np.int64)
-        out = np.where(v & SAT, 1, 0)
-        out[((v & INCOMPLETE) != 0) & ((v & SAT) == 0)] = -1
-        return out
+        return _decode(self.d_verdicts.cpu().numpy()[: self.num_instances])
+
+    def fetch_verdicts(self) -> None:
+        """Enqueue the copy of the verdicts into pinned host memory on the current stream
+        (after the search launched last); :meth:`fetched_verdicts` waits for it."""
+        import torch
+
+        n = max(1, self.num_instances)
+        if self._vpin is None or self._vpin.numel() < n:
+            self._vpin = torch.empty(max(n, 4096), dtype=torch.int32, pin_memory=True)
+        self._vpin[:n].copy_(self.d_verdicts[:n], non_blocking=True)
+        self._vdone = torch.cuda.Event()
+        self._vdone.record()
+
+    def fetched_verdicts(self) -> np.ndarray:
+        """The verdicts copied by the last :meth:`fetch_verdicts`, once that copy is done."""
+        if self._vdone is None:
+            raise RuntimeError("fetch_verdicts() was not called")
+        self._vdone.synchronize()
+        return _decode(self._vpin.numpy()[: self.num_instances])
 
     def counts(self) -> list[int]:
         """Per instance: the exact number of satisfying assignments over its variables
@@ -546,6 +573,29 @@ class SearchSolver:
         self.launch()
         torch.cuda.synchronize()
         return self.verdicts()
+
+
+def decide_stream(batches, **solver_kw):
+    """Verdicts of a stream of batches (an iterable of CNF lists), one array per batch, in
+    order.  Two solvers alternate so that the host packs, plans and stages batch k+1
+    while the GPU searches batch k: each batch's upload, search and verdict copy are
+    enqueued on the current stream without waiting, and its verdicts are read once the
+    next batch is enqueued behind it.  One launch per batch, so ``budget`` must be None
+    (hand-over rounds read a job count on the host) and ``queue`` False."""
+    if solver_kw.get("budget") is not None or solver_kw.get("queue"):
+        raise ValueError("decide_stream needs budget=None and queue=False")
+    solvers = (SearchSolver(**solver_kw), SearchSolver(**solver_kw))
+    pending = None
+    for k, cnfs in enumerate(batches):
+        s = solvers[k % 2]
+        s.prepare(cnfs, sync=False)  # waits only for this solver's previous staging copy
+        s.launch()
+        s.fetch_verdicts()
+        if pending is not None:
+            yield pending.fetched_verdicts()
+        pending = s
+    if pending is not None:
+        yield pending.fetched_verdicts()
 
 
 def autotune(cnfs, *, widths=(4, 6), split_targets=(None, 32.0, 8.0), budgets=(None, 64, 16),

@@ -44,6 +44,24 @@ def test_batched_workload_verdicts(batch):
     np.testing.assert_array_equal(v, want)
 
 
+def test_streamed_batches_match_one_at_a_time(batch):
+    """decide_stream over batches of different sizes (the third needs a larger staging
+    buffer than the first of its solver, the last is smaller again), each searched while
+    the next is prepared: every batch's verdicts equal an unstreamed solve and DPLL."""
+    from tensorsat_amd.search import decide_stream
+
+    parts = [batch[:300], batch[300:317], batch[317:3317], batch[3317:3400], batch[:1]]
+    got = list(decide_stream(parts, width=4, split_target=8.0))
+    assert len(got) == len(parts)
+    for k, (p, v) in enumerate(zip(parts, got)):
+        _, want = _solve(p, width=4, split_target=8.0)
+        print(f"batch {k}: {len(p)} instances, sat {int(v.sum())}", flush=True)
+        np.testing.assert_array_equal(v, want)
+        np.testing.assert_array_equal(v, np.array([dpll.solve(c.num_vars, c.clauses) for c in p], dtype=np.int64))
+    with pytest.raises(ValueError):
+        list(decide_stream(parts, width=4, budget=8))
+
+
 @pytest.mark.parametrize("width", [0, 3, 6])
 def test_unsplit_early_exit_statistics_exact(batch, width):
     """One job per instance: a sequential search, so every statistic equals the oracle's."""
