@@ -23,7 +23,7 @@ Clause = tuple[int, ...]
 class CNFInstance:
     """A SAT instance in conjunctive normal form (``lib/sat.py:40``)."""
 
-    __slots__ = ("_num_vars", "_clauses", "_arrays", "_addr", "_head")
+    __slots__ = ("_num_vars", "_clauses", "_arrays", "_addr", "_head", "_hb")
 
     def __init__(self, num_vars: int, clauses: Sequence[Sequence[int]]) -> None:
         """Public constructor, validating literals (``lib/sat.py:111-121``)."""
@@ -38,6 +38,7 @@ class CNFInstance:
         self._arrays = None
         self._addr = None
         self._head = None
+        self._hb = None
 
     @classmethod
     def _new(cls, num_vars: int, clauses: tuple[Clause, ...], arrays=None) -> "CNFInstance":
@@ -47,6 +48,7 @@ class CNFInstance:
         self._arrays = arrays
         self._addr = None
         self._head = None
+        self._hb = None
         return self
 
     @classmethod
@@ -179,6 +181,13 @@ class CNFInstance:
         if self._head is None:
             self._head = (self._num_vars, *self.literal_addresses())
         return self._head
+
+    def pack_header_bytes(self) -> bytes:
+        """:meth:`pack_header` as four native int64s (cached in ``_hb``, so a batch's
+        headers are gathered by one ``bytes.join`` without a Python call per instance)."""
+        if self._hb is None:
+            self._hb = np.asarray(self.pack_header(), dtype=np.int64).tobytes()
+        return self._hb
 
     def clause_matrix(self) -> np.ndarray:
         """Clauses as a dense ``(m, k_max)`` int32 array, zero-padded on the right."""

@@ -24,8 +24,8 @@ raise, and :func:`supported` tells them apart beforehand.
 from __future__ import annotations
 
 import ctypes
-import itertools
 import math
+import operator
 import time
 
 import numpy as np
@@ -78,8 +78,11 @@ def pack_batch(cnfs, alloc=None) -> tuple[np.ndarray, np.ndarray]:
 
     L = PG._lib()
     num = len(cnfs)
-    head = np.fromiter(itertools.chain.from_iterable(c.pack_header() for c in cnfs), dtype=np.int64,
-                       count=4 * num).reshape(num, 4)
+    try:  # every header already cached: gathered at C speed
+        hb = b"".join(map(_HEADER_BYTES, cnfs))
+    except (AttributeError, TypeError):
+        hb = b"".join(c.pack_header_bytes() for c in cnfs)
+    head = np.frombuffer(hb, dtype=np.int64).reshape(num, 4)
     nv = head[:, 0].astype(np.int32)
     ncl = np.ascontiguousarray(head[:, 1])
     lens_p = np.ascontiguousarray(head[:, 2])
@@ -99,6 +102,9 @@ def pack_batch(cnfs, alloc=None) -> tuple[np.ndarray, np.ndarray]:
     if rc != 0:
         raise ValueError(f"instance {bad.value}: literal out of range")
     return words[: total.value], inst
+
+
+_HEADER_BYTES = operator.attrgetter("_hb")
 
 
 def _align(n: int) -> int:
