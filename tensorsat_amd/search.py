@@ -293,9 +293,22 @@ class SearchSolver:
         """Pack, plan the jobs and upload (host work, timed in ``self.timing``).  Clauses,
         instance records and jobs are written straight into one pinned staging buffer
         (kept by the solver, grown as needed) and sent with one host-to-device copy.
-        ``sync`` False returns once the copy is enqueued on the current stream."""
+        ``sync`` False returns once the copy is enqueued on the current stream.  The two
+        halves are :meth:`stage` and :meth:`submit`."""
         import torch
 
+        st = self.stage(cnfs)
+        t2 = time.perf_counter()
+        self.submit(st)
+        if sync:
+            torch.cuda.synchronize()
+            self.timing["upload_s"] = time.perf_counter() - t2
+
+    def stage(self, cnfs) -> dict:
+        """Host half of :meth:`prepare`: pack and plan into the pinned staging buffer (after
+        this solver's previous staging copy has completed).  It reads and writes only the
+        staging buffer, so it may run on another thread while this solver's previous batch
+        is searched or its verdicts are read; no GPU work is enqueued."""
         if self._staged is not None:
             self._staged.synchronize()  # the previous batch's copy has read the staging buffer
         t0 = time.perf_counter()
@@ -304,7 +317,7 @@ class SearchSolver:
         def alloc(nwords, ninst):
             layout["words"] = (0, nwords * 4)
             layout["inst"] = (_align(nwords * 4), ninst * INST_DTYPE.itemsize)
-            self._stage(layout["inst"][0] + ninst * INST_DTYPE.itemsize)
+            self._stage_pin(layout["inst"][0] + ninst * INST_DTYPE.itemsize)
             h = self._pin.numpy()
             return (h[: nwords * 4].view(np.uint32),
                     h[layout["inst"][0] : layout["inst"][0] + layout["inst"][1]].view(INST_DTYPE))
@@ -319,7 +332,7 @@ class SearchSolver:
         if room <= 0 or not np.may_share_memory(jobs, h):  # grown: stage the jobs too
             need = jo + jobs.nbytes
             old = (words.copy(), inst.copy())
-            self._stage(need)
+            self._stage_pin(need)
             h = self._pin.numpy()
             words = h[: old[0].nbytes].view(np.uint32)
             words[:] = old[0]
@@ -330,24 +343,29 @@ class SearchSolver:
             jobs = staged
         layout["jobs"] = (jo, jobs.nbytes)
         validate(words, inst, None, self.width, literals=False)  # packer and planner checked the rest
-        t2 = time.perf_counter()
-        self.upload(words, inst, jobs, staged=layout)
-        if sync:
-            torch.cuda.synchronize()
-        self.timing.update(pack_s=t1 - t0, plan_s=t2 - t1, upload_s=time.perf_counter() - t2)
+        return {"words": words, "inst": inst, "jobs": jobs, "layout": layout, "pin": self._pin,
+                "pack_s": t1 - t0, "plan_s": time.perf_counter() - t1}
 
-    def _stage(self, nbytes: int) -> None:
-        """Pinned staging buffer (and its device twin) of at least ``nbytes``."""
+    def submit(self, st: dict) -> None:
+        """Device half of :meth:`prepare`: the staged records to the device with one copy
+        enqueued on the current stream (call from the thread that launches)."""
+        t2 = time.perf_counter()
+        self.upload(st["words"], st["inst"], st["jobs"], staged=st["layout"], pin=st["pin"])
+        self.timing.update(pack_s=st["pack_s"], plan_s=st["plan_s"], upload_s=time.perf_counter() - t2)
+
+    def _stage_pin(self, nbytes: int) -> None:
+        """Pinned staging buffer of at least ``nbytes`` (its device twin is sized by
+        :meth:`upload`)."""
         import torch
 
         if self._pin is None or self._pin.numel() < nbytes:
             size = max(int(nbytes * 1.25), 1 << 20)
             self._pin = torch.empty(size, dtype=torch.uint8, pin_memory=True)
-            self._dev_stage = torch.empty(size, dtype=torch.uint8, device=self.device)
 
-    def upload(self, words: np.ndarray, inst: np.ndarray, jobs: np.ndarray, staged: dict | None = None) -> None:
-        """Device copies of the records; ``staged`` (from :meth:`prepare`): their byte
-        ranges in the pinned staging buffer, sent with one copy."""
+    def upload(self, words: np.ndarray, inst: np.ndarray, jobs: np.ndarray, staged: dict | None = None,
+               pin=None) -> None:
+        """Device copies of the records; ``staged`` (from :meth:`stage`): their byte
+        ranges in the pinned staging buffer ``pin``, sent with one copy."""
         import torch
 
         dev = self.device
@@ -356,7 +374,10 @@ class SearchSolver:
         self.host_jobs = jobs
         if staged is not None and len(inst) and len(jobs) and words.size:
             end = staged["jobs"][0] + staged["jobs"][1]
-            self._dev_stage[:end].copy_(self._pin[:end], non_blocking=True)
+            if self._dev_stage is None or self._dev_stage.numel() < end:
+                self._dev_stage = torch.empty(max(int(end * 1.25), 1 << 20), dtype=torch.uint8, device=dev)
+            pin = self._pin if pin is None else pin
+            self._dev_stage[:end].copy_(pin[:end], non_blocking=True)
             self._staged = torch.cuda.Event()
             self._staged.record()
 
@@ -581,27 +602,54 @@ class SearchSolver:
         return self.verdicts()
 
 
-def decide_stream(batches, **solver_kw):
+def decide_stream(batches, *, ahead: int = 3, **solver_kw):
     """Verdicts of a stream of batches (an iterable of CNF lists), one array per batch, in
-    order.  Two solvers alternate so that the host packs, plans and stages batch k+1
-    while the GPU searches batch k: each batch's upload, search and verdict copy are
-    enqueued on the current stream without waiting, and its verdicts are read once the
-    next batch is enqueued behind it.  One launch per batch, so ``budget`` must be None
+    order.  ``ahead`` batches are packed and planned on worker threads (:meth:`SearchSolver.stage`;
+    the native packer and job planner release the interpreter lock) while the calling thread
+    uploads, launches and reads back (:meth:`SearchSolver.submit`, ``launch``,
+    ``fetch_verdicts``): each batch's copy, search and verdict copy are enqueued on the
+    current stream without waiting, and its verdicts are read once the next batch is
+    enqueued behind it.  ``ahead=0`` stages on the calling thread (4096-instance batches
+    of random 3-SAT on one MI355X: 1.14 ms per batch; ahead 1: 1.49 ms, the one worker
+    stages while the caller waits; 2: 0.99 ms; 3: 0.94 ms, the Python glue of staging and
+    submission holding the interpreter lock is what remains).  The solvers form a ring
+    of ``ahead + 2``: the one that stages batch k + ahead searched batch k - 2, whose
+    verdicts were already returned.  One launch per batch, so ``budget`` must be None
     (hand-over rounds read a job count on the host) and ``queue`` False."""
+    import collections
+    from concurrent.futures import ThreadPoolExecutor
+
     if solver_kw.get("budget") is not None or solver_kw.get("queue"):
         raise ValueError("decide_stream needs budget=None and queue=False")
-    solvers = (SearchSolver(**solver_kw), SearchSolver(**solver_kw))
-    pending = None
-    for k, cnfs in enumerate(batches):
-        s = solvers[k % 2]
-        s.prepare(cnfs, sync=False)  # waits only for this solver's previous staging copy
-        s.launch()
-        s.fetch_verdicts()
+    ahead = max(0, int(ahead))
+    solvers = [SearchSolver(**solver_kw) for _ in range(ahead + 2)]
+    it = iter(batches)
+    staged = collections.deque()
+    k = 0
+    with ThreadPoolExecutor(max_workers=max(1, ahead)) as ex:
+
+        def schedule() -> None:
+            nonlocal k
+            for cnfs in it:
+                s = solvers[k % len(solvers)]
+                k += 1
+                staged.append((s, ex.submit(s.stage, cnfs) if ahead else s.stage(cnfs)))
+                return
+
+        for _ in range(max(1, ahead)):
+            schedule()
+        pending = None
+        while staged:
+            s, st = staged.popleft()
+            s.submit(st.result() if ahead else st)
+            s.launch()
+            s.fetch_verdicts()
+            schedule()
+            if pending is not None:
+                yield pending.fetched_verdicts()
+            pending = s
         if pending is not None:
             yield pending.fetched_verdicts()
-        pending = s
-    if pending is not None:
-        yield pending.fetched_verdicts()
 
 
 def autotune(cnfs, *, widths=(4, 6), split_targets=(None, 32.0, 8.0), budgets=(None, 64, 16),

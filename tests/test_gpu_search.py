@@ -44,14 +44,16 @@ def test_batched_workload_verdicts(batch):
     np.testing.assert_array_equal(v, want)
 
 
-def test_streamed_batches_match_one_at_a_time(batch):
+@pytest.mark.parametrize("ahead", [0, 1, 2])
+def test_streamed_batches_match_one_at_a_time(batch, ahead):
     """decide_stream over batches of different sizes (the third needs a larger staging
-    buffer than the first of its solver, the last is smaller again), each searched while
-    the next is prepared: every batch's verdicts equal an unstreamed solve and DPLL."""
+    buffer than the first of its solver, the last ones are smaller again, and more
+    batches than solvers in the ring), each searched while the next ones are staged on
+    worker threads: every batch's verdicts equal an unstreamed solve and DPLL."""
     from tensorsat_amd.search import decide_stream
 
-    parts = [batch[:300], batch[300:317], batch[317:3317], batch[3317:3400], batch[:1]]
-    got = list(decide_stream(parts, width=4, split_target=8.0))
+    parts = [batch[:300], batch[300:317], batch[317:3317], batch[3317:3400], batch[:1], batch[3400:4096]]
+    got = list(decide_stream(parts, ahead=ahead, width=4, split_target=8.0))
     assert len(got) == len(parts)
     for k, (p, v) in enumerate(zip(parts, got)):
         _, want = _solve(p, width=4, split_target=8.0)
