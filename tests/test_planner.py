@@ -155,3 +155,29 @@ def test_concat_programs_keeps_every_verdict():
     np.testing.assert_array_equal(m.stats, p1.stats)
     with pytest.raises(ValueError):
         PG.concat([p2], [sub + 40], 40)
+
+
+def test_planner_scratch_reuse_leaves_no_state():
+    """The planner and the step compiler keep per-thread scratch from cube to cube and
+    call to call: a program compiled after larger programs on the same thread, or inside
+    a batch whose other instances run on the same worker threads, is byte-identical to
+    the same program compiled first and alone."""
+    from tensorsat_amd.workloads import single_3sat
+
+    a = CNFInstance.random(3, 60, 200, rng=3)
+    b = single_3sat(0)
+    first = PG.sat_batch([a], width=8, plan_trials=1, threads=1)
+    big = PG.sat_batch([b], width=20, plan_trials=1, threads=1)  # wider residuals, longer jobs
+    assert int(big.jobs["num_steps"].max()) > int(first.jobs["num_steps"].max())
+    again = PG.sat_batch([a], width=8, plan_trials=1, threads=1)
+    assert len(first.jobs) == len(again.jobs) > 16
+    np.testing.assert_array_equal(first.jobs, again.jobs)
+    np.testing.assert_array_equal(first.steps.view(np.uint8), again.steps.view(np.uint8))
+    np.testing.assert_array_equal(first.leaves, again.leaves)
+    mixed = PG.sat_batch([b, a, b, a], width=8, plan_trials=1, threads=2)
+    for inst in (1, 3):
+        js = np.flatnonzero(mixed.jobs["instance"] == inst)
+        assert len(js) == len(first.jobs)
+        for k, j in enumerate(js):
+            np.testing.assert_array_equal(mixed.job_steps(j).view(np.uint8), first.job_steps(k).view(np.uint8))
+            np.testing.assert_array_equal(mixed.job_leaves(j), first.job_leaves(k))
