@@ -185,15 +185,7 @@ class PlannedContraction(Contraction):
         sliced = list(self._tree.sliced)
         if not sliced:
             return _prog.compile_network(self.wiring, leaves, self._tree)
-        w = self.wiring
-        slots = [tuple(x for x in ws if x not in set(sliced)) for ws in w.slot_wires_list]
-        sub = Wiring(w.wire_dims, slots, ())
-        progs = []
-        for vals in itertools.product(range(2), repeat=len(sliced)):
-            fixed = dict(zip(sliced, vals))
-            cut = [np.ascontiguousarray(t[tuple(fixed.get(x, slice(None)) for x in ws)])
-                   for t, ws in zip(leaves, w.slot_wires_list)]
-            progs.append(_prog.compile_network(sub, cut, self._tree))
+        progs = _prog.compile_sliced(self.wiring, leaves, self._tree, sliced)
         return _prog.concat(progs, [np.zeros(1, np.int64)] * len(progs), 1)
 
     def __repr__(self) -> str:

@@ -216,14 +216,58 @@ def compile_network(wiring: Wiring, leaves, plan: _plan.Plan) -> Program:
         if a.size != 1 << len(ws):
             raise ValueError(f"leaf {s} has {a.size} entries, expected {1 << len(ws)}")
     ptrs = (ctypes.c_void_p * max(1, len(arrs)))(*[a.ctypes.data for a in arrs])
-    h = _plan._NetHolder(wiring)
     path = np.ascontiguousarray(plan.path.reshape(-1), dtype=np.int32)
+    return _compile_ptrs(_plan._NetHolder(wiring), path, ptrs, plan)
+
+
+def _compile_ptrs(h, path: np.ndarray, ptrs, plan: _plan.Plan) -> Program:
+    """``tsc_compile`` of a network descriptor, a flat path and leaf pointers (the leaf
+    bytes are copied during the call)."""
     handle = ctypes.c_void_p()
-    L = _lib()
-    _plan._check(L.tsc_compile(ctypes.byref(h.net), _plan._ptr(path), ptrs, ctypes.byref(handle)), "tsc_compile")
+    _plan._check(_lib().tsc_compile(ctypes.byref(h.net), _plan._ptr(path), ptrs, ctypes.byref(handle)), "tsc_compile")
     prog = _take(handle)
     prog.meta["plan"] = plan
     return prog
+
+
+def compile_sliced(wiring: Wiring, leaves, plan: _plan.Plan, sliced) -> list[Program]:
+    """
+    One compiled program per assignment of the ``sliced`` wires (in
+    ``itertools.product(range(2), repeat=len(sliced))`` order): ``compile_network`` of the
+    wiring without the sliced wires, with every leaf that holds one of them fixed at
+    that assignment (``Box.__getitem__``).  The network, the path and the pointers of the
+    leaves that hold no sliced wire are built once; per slice only the fixed leaves are
+    cut and their pointers replaced.
+    """
+    import itertools
+
+    sliced = list(sliced)
+    drop = set(sliced)
+    full = wiring.slot_wires_list
+    if any(d != 2 for d in wiring.wire_dims):
+        raise ValueError("compile_sliced needs extent-2 wires")
+    if len(leaves) != wiring.num_slots:
+        raise ValueError("one leaf per slot is required")
+    arrs = [np.ascontiguousarray(np.asarray(t, dtype=np.uint8)) for t in leaves]
+    for s, (a, ws) in enumerate(zip(arrs, full)):
+        if a.size != 1 << len(ws):
+            raise ValueError(f"leaf {s} has {a.size} entries, expected {1 << len(ws)}")
+    sub = Wiring(wiring.wire_dims, [tuple(x for x in ws if x not in drop) for ws in full], ())
+    hot = [s for s, ws in enumerate(full) if drop.intersection(ws)]
+    shaped = {s: arrs[s].reshape((2,) * len(full[s])) for s in hot}
+    ptrs = (ctypes.c_void_p * max(1, len(arrs)))(*[a.ctypes.data for a in arrs])
+    h = _plan._NetHolder(sub)
+    path = np.ascontiguousarray(plan.path.reshape(-1), dtype=np.int32)
+    progs = []
+    for vals in itertools.product(range(2), repeat=len(sliced)):
+        fixed = dict(zip(sliced, vals))
+        cut = []
+        for s in hot:
+            c = np.ascontiguousarray(shaped[s][tuple(fixed.get(x, slice(None)) for x in full[s])])
+            cut.append(c)  # alive until tsc_compile has copied it
+            ptrs[s] = c.ctypes.data
+        progs.append(_compile_ptrs(h, path, ptrs, plan))
+    return progs
 
 
 def sat_batch(

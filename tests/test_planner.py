@@ -181,3 +181,31 @@ def test_planner_scratch_reuse_leaves_no_state():
         for k, j in enumerate(js):
             np.testing.assert_array_equal(mixed.job_steps(j).view(np.uint8), first.job_steps(k).view(np.uint8))
             np.testing.assert_array_equal(mixed.job_leaves(j), first.job_leaves(k))
+
+
+def test_sliced_compilation_equals_per_slice_compile_network():
+    """PlannedContraction.contract_program compiles one job per slice with the network,
+    path and untouched leaves shared (program.compile_sliced): every job equals
+    compile_network of that slice's sub-wiring and cut leaves, in product order."""
+    from tensorsat_amd.contractions import PlannedContraction
+    from tensorsat_amd.rel import DeviceRel
+
+    cnf = CNFInstance.random(3, 20, 80, rng=7)
+    d = (cnf.inputs(None) >> cnf.diagram()).flatten()
+    c = PlannedContraction(DeviceRel, d.wiring, optimize="greedy", slicing_target=9)
+    sliced = c.tree.sliced
+    assert 2 <= len(sliced) <= 8
+    prog = c.contract_program(d)
+    assert len(prog.jobs) == 1 << len(sliced)
+    w = d.wiring
+    leaves = [np.asarray(getattr(b, "tensor", b), dtype=np.uint8) for b in d.blocks]
+    sub = Wiring(w.wire_dims, [tuple(x for x in ws if x not in set(sliced)) for ws in w.slot_wires_list], ())
+    for k, vals in enumerate(itertools.product(range(2), repeat=len(sliced))):
+        fixed = dict(zip(sliced, vals))
+        cut = [np.ascontiguousarray(t[tuple(fixed.get(x, slice(None)) for x in ws)])
+               for t, ws in zip(leaves, w.slot_wires_list)]
+        one = PG.compile_network(sub, cut, c.tree)
+        np.testing.assert_array_equal(prog.job_steps(k).view(np.uint8), one.steps.view(np.uint8))
+        np.testing.assert_array_equal(prog.job_leaves(k), one.leaves)
+        for f in ("num_steps", "result_off", "result_rank", "arena_elems", "max_rank"):
+            assert prog.jobs[f][k] == one.jobs[f][0]
