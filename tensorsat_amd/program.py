@@ -230,16 +230,19 @@ def _compile_ptrs(h, path: np.ndarray, ptrs, plan: _plan.Plan) -> Program:
     return prog
 
 
-def compile_sliced(wiring: Wiring, leaves, plan: _plan.Plan, sliced) -> list[Program]:
+def compile_sliced(wiring: Wiring, leaves, plan: _plan.Plan, sliced, threads: int = 0) -> list[Program]:
     """
     One compiled program per assignment of the ``sliced`` wires (in
     ``itertools.product(range(2), repeat=len(sliced))`` order): ``compile_network`` of the
     wiring without the sliced wires, with every leaf that holds one of them fixed at
     that assignment (``Box.__getitem__``).  The network, the path and the pointers of the
     leaves that hold no sliced wire are built once; per slice only the fixed leaves are
-    cut and their pointers replaced.
+    cut and their pointers replaced.  Slices compile on ``threads`` threads (0: up to 8;
+    the native compiler releases the interpreter lock and keeps its scratch per thread).
     """
     import itertools
+    import os
+    from concurrent.futures import ThreadPoolExecutor
 
     sliced = list(sliced)
     drop = set(sliced)
@@ -254,20 +257,46 @@ def compile_sliced(wiring: Wiring, leaves, plan: _plan.Plan, sliced) -> list[Pro
             raise ValueError(f"leaf {s} has {a.size} entries, expected {1 << len(ws)}")
     sub = Wiring(wiring.wire_dims, [tuple(x for x in ws if x not in drop) for ws in full], ())
     hot = [s for s, ws in enumerate(full) if drop.intersection(ws)]
-    shaped = {s: arrs[s].reshape((2,) * len(full[s])) for s in hot}
-    ptrs = (ctypes.c_void_p * max(1, len(arrs)))(*[a.ctypes.data for a in arrs])
+    # Every cut of every leaf that holds sliced wires, once, in one buffer: leaf r's cut
+    # for local assignment a (bit j = the value of its j-th sliced wire, in slot order)
+    # starts at var_off[r] + a * var_size[r].
+    sidx = {w: i for i, w in enumerate(sliced)}
+    weights = np.zeros((len(hot), len(sliced)), dtype=np.int64)
+    var_off, var_size, chunks, off = [], [], [], 0
+    for r, s in enumerate(hot):
+        ws = full[s]
+        sw = [x for x in ws if x in drop]
+        for j, x in enumerate(sw):
+            weights[r, sidx[x]] = 1 << j
+        shaped = arrs[s].reshape((2,) * len(ws))
+        size = 1 << (len(ws) - len(sw))
+        var_off.append(off)
+        var_size.append(size)
+        for a in range(1 << len(sw)):
+            fixed = {x: (a >> j) & 1 for j, x in enumerate(sw)}
+            chunks.append(np.ascontiguousarray(shaped[tuple(fixed.get(x, slice(None)) for x in ws)]).reshape(-1))
+            off += size
+    variants = np.concatenate(chunks) if chunks else np.zeros(1, dtype=np.uint8)
+    hot_idx = np.asarray(hot, dtype=np.int64)
+    hot_base = np.uint64(variants.ctypes.data) + np.asarray(var_off, dtype=np.uint64)
+    hot_size = np.asarray(var_size, dtype=np.uint64)
+    base = np.asarray([a.ctypes.data for a in arrs] or [0], dtype=np.uint64)
+    PtrArray = ctypes.c_void_p * len(base)
     h = _plan._NetHolder(sub)
     path = np.ascontiguousarray(plan.path.reshape(-1), dtype=np.int32)
-    progs = []
-    for vals in itertools.product(range(2), repeat=len(sliced)):
-        fixed = dict(zip(sliced, vals))
-        cut = []
-        for s in hot:
-            c = np.ascontiguousarray(shaped[s][tuple(fixed.get(x, slice(None)) for x in full[s])])
-            cut.append(c)  # alive until tsc_compile has copied it
-            ptrs[s] = c.ctypes.data
-        progs.append(_compile_ptrs(h, path, ptrs, plan))
-    return progs
+
+    def one(vals) -> Program:
+        local = (weights @ np.asarray(vals, dtype=np.int64)).astype(np.uint64)
+        p = base.copy()
+        p[hot_idx] = hot_base + local * hot_size
+        return _compile_ptrs(h, path, PtrArray.from_buffer_copy(p), plan)  # variants outlive the call
+
+    assignments = list(itertools.product(range(2), repeat=len(sliced)))
+    nt = threads if threads > 0 else min(8, os.cpu_count() or 1)
+    if nt <= 1 or len(assignments) < 8:
+        return [one(v) for v in assignments]
+    with ThreadPoolExecutor(max_workers=nt) as ex:
+        return list(ex.map(one, assignments))
 
 
 def sat_batch(
