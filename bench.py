@@ -444,6 +444,7 @@ def bench_search(args) -> int:
             "instances": len(cnfs),
             "width": width,
             "split_target": split,
+            "tail": {"fraction": solver.tail_fraction, "boost": solver.tail_boost},
             "budget": budget,
             "rounds": solver.rounds,
             "jobs_per_round": solver.round_jobs,

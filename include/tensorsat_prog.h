@@ -208,11 +208,16 @@ int tsc_search_pack(int32_t num, const int32_t* nvars, const int64_t* nclauses, 
  * estimates log2 of its branch nodes (slope 0.024 at clause ratio <= 3.5 rising linearly
  * to 0.054 at 4.26, then 0.054 * 4.26 / ratio); split_target <= 0: s_i = 0.  Jobs in
  * increasing -(e_i - s_i), ties in instance order, an instance's jobs in prefix order;
+ * with split_target > 0, tail_boost > 0 and tail_fraction > 0, an instance whose e_i is at
+ * least the k-th largest estimate of the batch, k = clamp(ceil(tail_fraction * num), 1,
+ * num), gets s_i = min(s_i + tail_boost, min(max_split, 31)) (the longest searches set a
+ * batch's time); tail_fraction outside [0, 1] or tail_boost outside 0..31: TSP_EINVAL;
  * own_from = 1 + the index of the prefix's highest set bit (0 for prefix 0).  inst:
  * `num` records of 4 int32 (clause_begin, num_clauses, num_vars, reserved), as
  * tsc_search_pack writes them.  *njobs_out = the job count; TSP_ENOMEM (nothing written)
  * when it exceeds `capacity`.  No counterpart in the reference. */
-int tsc_search_plan(int32_t num, const int32_t* inst, double split_target, int32_t max_split, void* jobs_out,
+int tsc_search_plan(int32_t num, const int32_t* inst, double split_target, int32_t max_split,
+                    double tail_fraction, int32_t tail_boost, void* jobs_out,
                     int64_t capacity, int64_t* njobs_out);
 
 #ifdef __cplusplus

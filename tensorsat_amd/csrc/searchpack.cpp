@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <thread>
 #include <vector>
 
@@ -151,20 +152,35 @@ double log2_search_size(int32_t num_vars, int32_t num_clauses) {
 }  // namespace
 
 extern "C" int tsc_search_plan(int32_t num, const int32_t* inst, double split_target, int32_t max_split,
-                               void* jobs_out, int64_t capacity, int64_t* njobs_out) {
+                               double tail_fraction, int32_t tail_boost, void* jobs_out, int64_t capacity,
+                               int64_t* njobs_out) {
   if (num < 0 || (num > 0 && !inst) || !njobs_out || max_split < 0 || capacity < 0) return TSP_EINVAL;
+  if (!(tail_fraction >= 0.0 && tail_fraction <= 1.0) || tail_boost < 0 || tail_boost > 31) return TSP_EINVAL;
   const int32_t cap_split = std::min<int32_t>(max_split, 31);
   std::vector<int32_t> split((size_t)num, 0);
-  std::vector<double> key((size_t)num);
+  std::vector<double> key((size_t)num), est((size_t)num);
   const double lt = split_target > 0 ? std::log2(split_target) : 0.0;
+  for (int32_t i = 0; i < num; ++i) est[i] = log2_search_size(inst[4 * i + 2], inst[4 * i + 1]);
+  // The tail: instances whose estimate is at least the k-th largest, k = ceil(tail_fraction
+  // * num), get tail_boost more split levels -- the longest searches of a batch set its
+  // time, and splitting only them shortens the critical path without multiplying the
+  // work of the rest.
+  double thr = std::numeric_limits<double>::infinity();
+  if (split_target > 0 && tail_boost > 0 && tail_fraction > 0 && num > 0) {
+    const int64_t k = std::min<int64_t>(num, std::max<int64_t>(1, (int64_t)std::ceil(tail_fraction * (double)num)));
+    std::vector<double> sorted(est);
+    std::nth_element(sorted.begin(), sorted.begin() + (num - k), sorted.end());
+    thr = sorted[(size_t)(num - k)];
+  }
   int64_t total = 0;
   for (int32_t i = 0; i < num; ++i) {
-    const double est = log2_search_size(inst[4 * i + 2], inst[4 * i + 1]);
     if (split_target > 0) {
-      const double s = std::ceil(est - lt);
-      split[i] = s <= 0 ? 0 : (s >= cap_split ? cap_split : (int32_t)s);
+      const double s = std::ceil(est[i] - lt);
+      int32_t v = s <= 0 ? 0 : (s >= cap_split ? cap_split : (int32_t)s);
+      if (est[i] >= thr) v = std::min<int32_t>(v + tail_boost, cap_split);
+      split[i] = v;
     }
-    key[i] = -(est - (double)split[i]);
+    key[i] = -(est[i] - (double)split[i]);
     total += int64_t(1) << split[i];
   }
   *njobs_out = total;

@@ -89,8 +89,8 @@ def _lib():
         L.tsc_search_pack.argtypes = [ctypes.c_int32, vp, vp, vp, vp, vp, vp, P(ctypes.c_int64), P(ctypes.c_int32),
                                       ctypes.c_int32]
         L.tsc_search_pack.restype = ctypes.c_int
-        L.tsc_search_plan.argtypes = [ctypes.c_int32, vp, ctypes.c_double, ctypes.c_int32, vp, ctypes.c_int64,
-                                      P(ctypes.c_int64)]
+        L.tsc_search_plan.argtypes = [ctypes.c_int32, vp, ctypes.c_double, ctypes.c_int32, ctypes.c_double,
+                                      ctypes.c_int32, vp, ctypes.c_int64, P(ctypes.c_int64)]
         L.tsc_search_plan.restype = ctypes.c_int
         L.tsc_lanes.argtypes = [vp, ctypes.c_int64, vp, ctypes.c_int64, vp, ctypes.c_int64, vp, ctypes.c_int32]
         for f in (L.tsc_compile, L.tsc_sat_batch, L.tsc_program_sizes, L.tsc_program_copy, L.tsp_elim,

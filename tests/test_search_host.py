@@ -119,14 +119,19 @@ def test_validate_checks_job_start_assignments():
         S.validate(words, inst, bad, 6)
 
 
-def _plan_jobs_numpy(inst, split_target, max_split):
+def _plan_jobs_numpy(inst, split_target, max_split, tail_fraction=0.0, tail_boost=0):
     """Restatement of the start-job rule in numpy (the C planner's specification)."""
     num = len(inst)
     est = S.log2_search_size(inst["num_vars"], inst["num_clauses"])
     if split_target is None:
         split = np.zeros(num, dtype=np.int64)
     else:
-        split = np.clip(np.ceil(est - np.log2(split_target)), 0, min(max_split, S.MAX_SPLIT)).astype(np.int64)
+        cap = min(max_split, S.MAX_SPLIT)
+        split = np.clip(np.ceil(est - np.log2(split_target)), 0, cap).astype(np.int64)
+        if tail_boost > 0 and tail_fraction > 0 and num:
+            k = min(num, max(1, int(np.ceil(tail_fraction * num))))
+            thr = np.sort(est)[num - k]  # the k-th largest estimate
+            split = np.minimum(split + np.where(est >= thr, tail_boost, 0), cap)
     order = np.argsort(-(est - split), kind="stable")
     nj = (np.int64(1) << split)[order]
     jobs = np.zeros(int(nj.sum()), dtype=S.JOB_DTYPE)
@@ -148,11 +153,18 @@ def test_plan_jobs_match_the_numpy_rule():
     cnfs = batched_3sat(300) + [CNFInstance.random(3, n, int(r * n), rng=7 + n) for n in (20, 60, 100)
                                for r in (3.0, 4.26, 5.5)]
     _, inst = S.pack_batch(cnfs)
-    for target, ms in ((None, 8), (512.0, 8), (64.0, 6), (4.0, 8), (1.0, 3), (0.25, 40)):
-        got = S.plan_jobs(inst, split_target=target, max_split=ms)
-        want = _plan_jobs_numpy(inst, target, ms)
-        assert got.tobytes() == want.tobytes(), (target, ms)
+    for target, ms, tf, tb in ((None, 8, 0.0, 0), (512.0, 8, 0.0, 0), (64.0, 6, 0.0, 0), (4.0, 8, 0.0, 0),
+                               (1.0, 3, 0.0, 0), (0.25, 40, 0.0, 0), (8.0, 8, 0.01, 3), (8.0, 8, 0.2, 2),
+                               (64.0, 5, 1.0, 4), (8.0, 6, 0.001, 31), (None, 8, 0.5, 3)):
+        got = S.plan_jobs(inst, split_target=target, max_split=ms, tail_fraction=tf, tail_boost=tb)
+        want = _plan_jobs_numpy(inst, target, ms, tf, tb)
+        assert got.tobytes() == want.tobytes(), (target, ms, tf, tb)
+        if tb and target is not None and tf < 1.0:  # the tail got more jobs than the rule without it
+            assert len(got) > len(_plan_jobs_numpy(inst, target, ms))
     out = np.zeros(4, dtype=S.JOB_DTYPE)  # too small: a new array comes back
     assert S.plan_jobs(inst, split_target=64.0, out=out).tobytes() == _plan_jobs_numpy(inst, 64.0, 8).tobytes()
     with pytest.raises(ValueError):
         S.plan_jobs(inst, split_target=0.0)
+    for tf, tb in ((-0.1, 3), (1.5, 3), (0.01, -1), (0.01, 32)):
+        with pytest.raises(ValueError):
+            S.plan_jobs(inst, split_target=8.0, tail_fraction=tf, tail_boost=tb)
