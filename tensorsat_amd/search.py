@@ -620,9 +620,10 @@ def decide_stream(batches, *, ahead: int = 3, **solver_kw):
     ``fetch_verdicts``): each batch's copy, search and verdict copy are enqueued on the
     current stream without waiting, and its verdicts are read once the next batch is
     enqueued behind it.  ``ahead=0`` stages on the calling thread (4096-instance batches
-    of random 3-SAT on one MI355X: 1.14 ms per batch; ahead 1: 1.49 ms, the one worker
-    stages while the caller waits; 2: 0.99 ms; 3: 0.94 ms, the Python glue of staging and
-    submission holding the interpreter lock is what remains).  The solvers form a ring
+    of random 3-SAT on one MI355X, 21 780 jobs each: 1.14 ms per batch; ahead 1: 1.49 ms,
+    the one worker stages while the caller waits; 2: 0.99 ms; 3: 0.94 ms, the Python glue
+    of staging and submission holding the interpreter lock is what remains).  The solvers
+    form a ring
     of ``ahead + 2``: the one that stages batch k + ahead searched batch k - 2, whose
     verdicts were already returned.  One launch per batch, so ``budget`` must be None
     (hand-over rounds read a job count on the host) and ``queue`` False."""
