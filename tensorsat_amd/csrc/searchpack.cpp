@@ -188,20 +188,20 @@ extern "C" int tsc_search_plan(int32_t num, const int32_t* inst, double split_ta
   std::vector<int32_t> order((size_t)num);
   for (int32_t i = 0; i < num; ++i) order[i] = i;
   std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return key[a] < key[b]; });
-  StartJob* out = static_cast<StartJob*>(jobs_out);
-  int64_t j = 0;
+  StartJob* r = static_cast<StartJob*>(jobs_out);
+  // One thread: starting threads costs more than writing these 1-2 MB.
   for (int32_t i : order) {
     const uint32_t nj = 1u << split[i];
-    for (uint32_t p = 0; p < nj; ++p, ++j) {
-      StartJob& r = out[j];
-      std::memset(&r, 0, sizeof(r));
-      r.instance = i;
-      r.split = split[i];
-      r.prefix = p;
+    std::memset(static_cast<void*>(r), 0, (size_t)nj * sizeof(StartJob));
+    for (uint32_t p = 0; p < nj; ++p) {
+      r[p].instance = i;
+      r[p].split = split[i];
+      r[p].prefix = p;
       // A node on the prefix levels belongs to the job whose remaining prefix bits are
       // all 0: the job counts from one level below its highest set prefix bit.
-      r.own_from = p ? 32 - __builtin_clz(p) : 0;
+      r[p].own_from = p ? 32 - __builtin_clz(p) : 0;
     }
+    r += nj;
   }
   return TSP_OK;
 }
