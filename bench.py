@@ -301,14 +301,17 @@ def bench_search(args) -> int:
             ((None,) if args.budget_nodes == "none" else (int(args.budget_nodes),))
         choice, table = SE.autotune(cnfs, widths=widths, split_targets=splits, budgets=budgets)
         width, split, budget = choice["width"], choice["split_target"], choice["budget"]
+        boost = choice["tail_boost"]
         tune = {"chosen": choice, "seconds": time.perf_counter() - t0, "candidates": list(table.values())}
-        log(f"autotuned width {width}, split target {split}, budget {budget} in {tune['seconds']:.1f}s: " +
-            ", ".join(f"W{r['width']}/{r['split_target']}/{r['budget']}: {r['ms']:.2f}ms" for r in table.values()))
+        log(f"autotuned width {width}, split target {split}, tail boost {boost}, budget {budget} in "
+            f"{tune['seconds']:.1f}s: " + ", ".join(f"W{r['width']}/{r['split_target']}/+{r['tail_boost']}/{r['budget']}: "
+                                                    f"{r['ms']:.2f}ms" for r in table.values()))
     else:
         width = int(args.width)
         split = None if args.split == "none" else float(args.split)
         budget = None if args.budget_nodes in ("none", "auto") else int(args.budget_nodes)
-    solver = SE.SearchSolver(width=width, split_target=split, budget=budget)
+        boost = 3
+    solver = SE.SearchSolver(width=width, split_target=split, budget=budget, tail_boost=boost)
     # End to end from the CNF list: pack, plan the jobs, upload, launch, verdicts back.
     e2e = []
     for _ in range(5):
@@ -352,10 +355,10 @@ def bench_search(args) -> int:
     stream = None
     if budget is None:
         reps = 16
-        list(SE.decide_stream([cnfs] * 2, width=width, split_target=split))
+        list(SE.decide_stream([cnfs] * 2, width=width, split_target=split, tail_boost=boost))
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        outs = list(SE.decide_stream([cnfs] * reps, width=width, split_target=split))
+        outs = list(SE.decide_stream([cnfs] * reps, width=width, split_target=split, tail_boost=boost))
         torch.cuda.synchronize()
         per = (time.perf_counter() - t0) / reps
         stream = {"s_per_batch": per, "instances_per_s": len(cnfs) / per, "batches": reps,
@@ -363,7 +366,7 @@ def bench_search(args) -> int:
         log(f"streamed end to end: {per * 1000:.2f} ms per batch = {len(cnfs) / per / 1e6:.2f} M inst/s "
             f"({reps} submissions, {stream['mismatches']} verdict mismatches)")
     # Critical path: the statistics of one more launch (per job branch nodes).
-    st = SE.SearchSolver(width=width, split_target=split, budget=budget, stats=True)
+    st = SE.SearchSolver(width=width, split_target=split, budget=budget, stats=True, tail_boost=boost)
     st.prepare(cnfs)
     st.launch()
     all_jobs, js = st.job_stats()

@@ -24,6 +24,7 @@ raise, and :func:`supported` tells them apart beforehand.
 from __future__ import annotations
 
 import ctypes
+import itertools
 import math
 import operator
 import time
@@ -664,42 +665,47 @@ def decide_stream(batches, *, ahead: int = 3, **solver_kw):
 
 
 def autotune(cnfs, *, widths=(4, 6), split_targets=(None, 32.0, 8.0), budgets=(None, 64, 16),
-             max_split: int = 8, reps: int = 3, tail_fraction: float = 0.01, tail_boost: int = 3) -> tuple[dict, dict]:
-    """Pick the residual width, the split target and the hand-over budget for a batch:
-    every candidate decides the batch ``reps`` times on the GPU (device-resident inputs,
-    wall time of each complete decision, hand-over rounds included); the fastest mean
-    wins.  Verdicts of every candidate must agree (they are the same decisions), else
-    ``RuntimeError``.  Returns (choice, table)."""
+             max_split: int = 8, reps: int = 3, tail_fraction: float = 0.01,
+             tail_boosts=(3, 0)) -> tuple[dict, dict]:
+    """Pick the residual width, the split target, the tail boost (extra split levels for
+    the ``tail_fraction`` of instances with the largest estimated searches) and the
+    hand-over budget for a batch: every candidate decides the batch ``reps`` times on the
+    GPU (device-resident inputs, wall time of each complete decision, hand-over rounds
+    included); the fastest mean wins.  Verdicts of every candidate must agree (they are
+    the same decisions), else ``RuntimeError``.  Returns (choice, table)."""
     import torch
 
     words, inst = pack_batch(cnfs)
     table = {}
     ref = None
-    for w in widths:
-        for tgt in split_targets:
-            jobs = plan_jobs(inst, split_target=tgt, max_split=max_split, tail_fraction=tail_fraction,
-                             tail_boost=tail_boost)
-            validate(words, inst, jobs, w)
-            for bud in budgets:
-                s = SearchSolver(width=w, split_target=tgt, max_split=max_split, budget=bud,
-                                 tail_fraction=tail_fraction, tail_boost=tail_boost)
-                s.upload(words, inst, jobs)
+    for w, tgt, tb in itertools.product(widths, split_targets, tail_boosts):
+        if tgt is None and tb != tail_boosts[0]:
+            continue  # no split: the boost has nothing to add to
+        jobs = plan_jobs(inst, split_target=tgt, max_split=max_split, tail_fraction=tail_fraction,
+                         tail_boost=tb)
+        validate(words, inst, jobs, w)
+        for bud in budgets:
+            s = SearchSolver(width=w, split_target=tgt, max_split=max_split, budget=bud,
+                             tail_fraction=tail_fraction, tail_boost=tb)
+            s.upload(words, inst, jobs)
+            s.launch()
+            torch.cuda.synchronize()
+            v = s.verdicts()
+            if ref is None:
+                ref = v
+            elif not np.array_equal(v, ref):
+                raise RuntimeError(f"verdicts differ between search configurations ({w}, {tgt}, {tb}, {bud})")
+            ts = []
+            for _ in range(reps):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
                 s.launch()
                 torch.cuda.synchronize()
-                v = s.verdicts()
-                if ref is None:
-                    ref = v
-                elif not np.array_equal(v, ref):
-                    raise RuntimeError(f"verdicts differ between search configurations ({w}, {tgt}, {bud})")
-                ts = []
-                for _ in range(reps):
-                    torch.cuda.synchronize()
-                    t0 = time.perf_counter()
-                    s.launch()
-                    torch.cuda.synchronize()
-                    ts.append(time.perf_counter() - t0)
-                table[(w, tgt, bud)] = {"width": w, "split_target": tgt, "budget": bud, "jobs": int(len(jobs)),
-                                        "rounds": s.rounds, "max_split": int(jobs["split"].max()) if len(jobs) else 0,
+                ts.append(time.perf_counter() - t0)
+            table[(w, tgt, tb, bud)] = {"width": w, "split_target": tgt, "tail_boost": tb, "budget": bud,
+                                        "jobs": int(len(jobs)), "rounds": s.rounds,
+                                        "max_split": int(jobs["split"].max()) if len(jobs) else 0,
                                         "ms": float(np.mean(ts)) * 1000.0}
     best = min(table.values(), key=lambda r: r["ms"])
-    return {"width": best["width"], "split_target": best["split_target"], "budget": best["budget"]}, table
+    return {"width": best["width"], "split_target": best["split_target"], "tail_boost": best["tail_boost"],
+            "budget": best["budget"]}, table

@@ -43,10 +43,10 @@ def test_search_autotuner(batch):
     cnfs, want = batch
     choice, table = autotune(cnfs, widths=(4, 6), split_targets=(None, 8.0), budgets=(None, 32), reps=2)
     print(choice, {k: round(v["ms"], 3) for k, v in table.items()}, flush=True)
-    assert len(table) == 2 * 2 * 2
+    assert len(table) == 2 * (1 + 2) * 2  # the tail boost (3 or 0) only with a split target
     best = min(table.values(), key=lambda r: r["ms"])
-    assert (choice["width"], choice["split_target"], choice["budget"]) == \
-        (best["width"], best["split_target"], best["budget"])
+    assert (choice["width"], choice["split_target"], choice["tail_boost"], choice["budget"]) == \
+        (best["width"], best["split_target"], best["tail_boost"], best["budget"])
     for r in table.values():
         assert r["ms"] > 0 and r["rounds"] >= 1
         if r["budget"] is None:
