@@ -13,8 +13,11 @@ search).  The host only packs the clauses and lists the jobs.
 Hard instances are split: a job owns the subtree below a fixed prefix of the first
 ``split`` branch choices, so 2^split waves search one instance at once, and the jobs are
 launched hardest first (an estimate of the search size from the instance's size and
-clause ratio).  With early exit, the jobs of an instance stop once one of them finds a
-satisfiable leaf.  Verdicts do not depend on the split or the order.
+clause ratio); the few instances with the largest estimates get more split levels, since
+the longest search sets the batch's time.  With early exit, the jobs of an instance stop
+once one of them finds a satisfiable leaf.  Verdicts do not depend on the split or the
+order.  :func:`decide_stream` overlaps the host work of the next batches with the search
+of the current one.
 
 :class:`SearchSolver` decides instances the kernel's limits admit (at most 128
 variables, 768 clauses after normalisation, 3 distinct literals per clause); the others
