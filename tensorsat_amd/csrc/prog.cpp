@@ -215,6 +215,13 @@ int elim_order(int nv, const std::vector<std::vector<int>>& cliques, const std::
         }
         continue;
       }
+      if (!rng) {  // deterministic: the first minimum in index order, no tie list
+        if (best < 0 || score < best) {
+          best = score;
+          bestv = v;
+        }
+        continue;
+      }
       if (best < 0 || score < best) {
         best = score;
         cand.clear();
@@ -223,9 +230,11 @@ int elim_order(int nv, const std::vector<std::vector<int>>& cliques, const std::
         cand.push_back(v);
       }
     }
-    if (prior) cand.assign(1, bestv);
-    int v = cand[0];
-    if (rng && cand.size() > 1) v = cand[(*rng)() % cand.size()];
+    int v = bestv;
+    if (!prior && rng) {
+      v = cand[0];
+      if (cand.size() > 1) v = cand[(*rng)() % cand.size()];
+    }
     width = std::max(width, deg[v]);
     if (width > stop_above) return width;
     order.push_back(v);
