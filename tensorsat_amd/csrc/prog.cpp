@@ -171,6 +171,49 @@ int elim_order(int nv, const std::vector<std::vector<int>>& cliques, const std::
   int remaining = 0;
   for (int v = 0; v < nv; ++v)
     if (!keep[v]) ++remaining;
+  if (!min_fill && !prior && !rng) {
+    // Deterministic min-degree: vertices in one bit row per degree, so a pick is the
+    // lowest set bit of the first non-empty row -- the first minimum in index order, as
+    // the scan below finds it.
+    thread_local Bits bucket;
+    bucket.init(std::max(1, nv), nv);
+    for (int v = 0; v < nv; ++v)
+      if (!keep[v]) bucket.row(deg[v])[v >> 6] |= 1ull << (v & 63);
+    while (remaining > 0) {
+      int v = -1, d = 0;
+      for (; d < nv && v < 0; ++d) {
+        const uint64_t* b = bucket.row(d);
+        for (int i = 0; i < adj.nw; ++i)
+          if (b[i]) {
+            v = i * 64 + __builtin_ctzll(b[i]);
+            break;
+          }
+      }
+      width = std::max(width, deg[v]);
+      if (width > stop_above) return width;
+      order.push_back(v);
+      gone[v] = 1;
+      --remaining;
+      bucket.row(deg[v])[v >> 6] &= ~(1ull << (v & 63));
+      uint64_t* rv = adj.row(v);
+      nb.clear();
+      for (int i = 0; i < adj.nw; ++i)
+        for (uint64_t m = rv[i]; m; m &= m - 1) nb.push_back(i * 64 + __builtin_ctzll(m));
+      for (int u : nb) {
+        uint64_t* ru = adj.row(u);
+        const bool listed = !keep[u] && !gone[u];
+        if (listed) bucket.row(deg[u])[u >> 6] &= ~(1ull << (u & 63));
+        ru[v >> 6] &= ~(1ull << (v & 63));
+        for (int i = 0; i < adj.nw; ++i) ru[i] |= rv[i];
+        ru[u >> 6] &= ~(1ull << (u & 63));
+        int dd = 0;
+        for (int i = 0; i < adj.nw; ++i) dd += __builtin_popcountll(ru[i]);
+        deg[u] = dd;
+        if (listed) bucket.row(dd)[u >> 6] |= 1ull << (u & 63);
+      }
+    }
+    return width;
+  }
   while (remaining > 0) {
     long best = -1;
     double bestp = 0.0;
